@@ -1,0 +1,391 @@
+// C ABI of libgdx (declared in include/gdx.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "gdx_internal.h"
+
+namespace gdx {
+int launch_coefficients(gdx_problem *p, hipStream_t st);
+int launch_rhs(gdx_problem *p, hipStream_t st);
+int launch_matrix(gdx_problem *p, hipStream_t st);
+bool matrix_kernel_supported(int dim, int K, int NQ);
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace gdx
+
+using gdx::set_error;
+
+#define GDX_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                            \
+      return e_ == hipErrorOutOfMemory ? GDX_ERR_ALLOC : GDX_ERR_HIP;                          \
+    }                                                                                          \
+  } while (0)
+
+#define GDX_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) {                   \
+      set_error(msg);                \
+      return code;                   \
+    }                                \
+  } while (0)
+
+namespace {
+
+template <class T>
+int upload(T **dst, const std::vector<T> &src) {
+  if (*dst) {
+    (void)hipFree(*dst);
+    *dst = nullptr;
+  }
+  size_t n = src.size() ? src.size() : 1;
+  GDX_HIP(hipMalloc(reinterpret_cast<void **>(dst), n * sizeof(T)));
+  if (!src.empty()) GDX_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return GDX_OK;
+}
+
+void free_device(gdx_problem *p) {
+  gdx::Device &d = p->dev;
+  void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_len, d.pat_dir,
+                  d.tiles};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  d = gdx::Device{};
+}
+
+int check_desc(const gdx_problem_desc *d) {
+  GDX_REQUIRE(d, GDX_ERR_ARG, "problem descriptor is NULL");
+  GDX_REQUIRE(d->dim == 2 || d->dim == 3, GDX_ERR_ARG, "dim must be 2 or 3");
+  GDX_REQUIRE(d->order == 1 || d->order == 2, GDX_ERR_UNSUPPORTED, "only Lagrangian order 1 and 2 are supported");
+  GDX_REQUIRE(d->qdeg >= 0, GDX_ERR_ARG, "quadrature degree must be >= 0");
+  GDX_REQUIRE(d->qdeg / 2 + 1 <= gdx::kMaxQ, GDX_ERR_UNSUPPORTED, "quadrature degree above 7 is not supported");
+  GDX_REQUIRE(d->strategy == GDX_SUB_ASSEMBLED_ROWS || d->strategy == GDX_FULLY_ASSEMBLED_ROWS, GDX_ERR_ARG,
+              "unknown assembly strategy");
+  for (int k = 0; k < d->dim; ++k) {
+    GDX_REQUIRE(d->parts[k] >= 1, GDX_ERR_ARG, "parts must be >= 1 in every direction");
+    GDX_REQUIRE(d->cells[k] >= d->parts[k], GDX_ERR_ARG, "each part needs at least one cell per direction");
+    GDX_REQUIRE(d->cells[k] * d->order < (int64_t)1 << 30, GDX_ERR_ARG, "mesh too large for 32-bit node coordinates");
+    GDX_REQUIRE(d->domain[2 * k + 1] > d->domain[2 * k], GDX_ERR_ARG, "empty domain interval");
+  }
+  GDX_REQUIRE(d->nmodes >= 0 && d->nmodes <= 1024, GDX_ERR_ARG, "nmodes must be in [0,1024]");
+  GDX_REQUIRE(d->nmodes == 0 || d->modes, GDX_ERR_ARG, "modes is NULL with nmodes > 0");
+  GDX_REQUIRE(d->rhs_kind == GDX_RHS_CONST || d->rhs_kind == GDX_RHS_MANUFACTURED, GDX_ERR_ARG, "unknown rhs kind");
+  GDX_REQUIRE(d->dir_kind == 0 || d->dir_kind == 1, GDX_ERR_ARG, "unknown Dirichlet kind");
+  GDX_REQUIRE(gdx::matrix_kernel_supported(d->dim, d->order, d->qdeg / 2 + 1), GDX_ERR_UNSUPPORTED,
+              "no matrix kernel for this (dim, order, quadrature) combination");
+  return GDX_OK;
+}
+
+int part_check(gdx_problem *p, int32_t part) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(part >= 0 && part < p->nparts, GDX_ERR_ARG, "part index out of range");
+  return GDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gdx_version(int32_t *major, int32_t *minor, int32_t *patch) {
+  GDX_REQUIRE(major && minor && patch, GDX_ERR_ARG, "NULL output");
+  *major = GDX_VERSION_MAJOR;
+  *minor = GDX_VERSION_MINOR;
+  *patch = GDX_VERSION_PATCH;
+  return GDX_OK;
+}
+
+const char *gdx_last_error(void) { return gdx::g_last_error.c_str(); }
+
+int gdx_init(int32_t device, gdx_ctx **out) {
+  GDX_REQUIRE(out, GDX_ERR_ARG, "NULL output");
+  *out = nullptr;
+  int n = 0;
+  GDX_HIP(hipGetDeviceCount(&n));
+  GDX_REQUIRE(device >= 0 && device < n, GDX_ERR_ARG, "device index out of range");
+  GDX_HIP(hipSetDevice(device));
+  gdx_ctx *c = new (std::nothrow) gdx_ctx;
+  GDX_REQUIRE(c, GDX_ERR_ALLOC, "host allocation failed");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return GDX_ERR_HIP;
+  }
+  *out = c;
+  return GDX_OK;
+}
+
+int gdx_finalize(gdx_ctx *ctx) {
+  GDX_REQUIRE(ctx, GDX_ERR_ARG, "context is NULL");
+  GDX_HIP(hipSetDevice(ctx->device));
+  GDX_HIP(hipStreamSynchronize(ctx->stream));
+  GDX_HIP(hipStreamDestroy(ctx->stream));
+  delete ctx;
+  return GDX_OK;
+}
+
+int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem **out) {
+  // ctx == NULL creates a host-only problem: the symbolic phase and the
+  // structure exports work without a device, gdx_numeric is refused
+  GDX_REQUIRE(out, GDX_ERR_ARG, "NULL output");
+  *out = nullptr;
+  int rc = check_desc(desc);
+  if (rc) return rc;
+  gdx_problem *p = new (std::nothrow) gdx_problem;
+  GDX_REQUIRE(p, GDX_ERR_ALLOC, "host allocation failed");
+  p->ctx = ctx;
+  p->desc = *desc;
+  p->modes.assign(desc->modes, desc->modes + 5 * desc->nmodes);
+  p->desc.modes = p->modes.data();
+  for (int k = desc->dim; k < 3; ++k) {
+    p->desc.cells[k] = 1;
+    p->desc.parts[k] = 1;
+    p->desc.domain[2 * k] = 0.0;
+    p->desc.domain[2 * k + 1] = 1.0;
+  }
+  p->D = desc->dim;
+  p->K = desc->order;
+  p->NQ = desc->qdeg / 2 + 1;
+  p->nparts = p->desc.parts[0] * p->desc.parts[1] * p->desc.parts[2];
+  *out = p;
+  return GDX_OK;
+}
+
+int gdx_problem_destroy(gdx_problem *p) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  if (p->ctx) {
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    free_device(p);
+    for (hipEvent_t &e : p->ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  delete p;
+  return GDX_OK;
+}
+
+int gdx_symbolic(gdx_problem *p) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  if (p->ctx) GDX_HIP(hipSetDevice(p->ctx->device));
+  try {
+    int rc = gdx::symbolic(p);
+    if (rc) return rc;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed in the symbolic phase");
+    return GDX_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return GDX_ERR_ARG;
+  }
+  gdx::fill_kparams(p);
+  if (!p->ctx) {
+    p->symbolic_done = true;
+    return GDX_OK;
+  }
+  free_device(p);
+  gdx::Device &d = p->dev;
+  const size_t nq = (size_t)p->kp.qn[0] * p->kp.qn[1] * p->kp.qn[2];
+  int qnmax = std::max(p->kp.qn[0], std::max(p->kp.qn[1], p->kp.qn[2]));
+  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.W), nq * sizeof(double)));
+  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.F), nq * sizeof(double)));
+  if (p->desc.nmodes > 0) {
+    GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.E), (size_t)2 * 3 * p->desc.nmodes * qnmax * sizeof(double)));
+    int rc = upload(&d.modes, p->modes);
+    if (rc) return rc;
+  }
+  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.values), std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
+  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.b), std::max<int64_t>(p->rows_total, 1) * sizeof(double)));
+  // ghost rows are never written by the numeric phase: they stay zero
+  GDX_HIP(hipMemset(d.values, 0, std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
+  GDX_HIP(hipMemset(d.b, 0, std::max<int64_t>(p->rows_total, 1) * sizeof(double)));
+  int rc;
+  if ((rc = upload(&d.rowinfo, p->rowinfo))) return rc;
+  if ((rc = upload(&d.pat_fwd, p->pat_fwd))) return rc;
+  if ((rc = upload(&d.pat_inv, p->pat_inv))) return rc;
+  if ((rc = upload(&d.pat_len, p->pat_len))) return rc;
+  if ((rc = upload(&d.pat_dir, p->pat_dir))) return rc;
+  if ((rc = upload(&d.tiles, p->tiles))) return rc;
+  for (hipEvent_t &e : p->ev)
+    if (!e) GDX_HIP(hipEventCreate(&e));
+  p->symbolic_done = true;
+  return GDX_OK;
+}
+
+int gdx_numeric(gdx_problem *p, int32_t precision) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(precision == GDX_PREC_FP64, GDX_ERR_UNSUPPORTED, "unsupported precision");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t st = p->ctx->stream;
+  GDX_HIP(hipEventRecord(p->ev[0], st));
+  int rc = gdx::launch_coefficients(p, st);
+  if (rc) {
+    set_error("coefficient kernel launch failed");
+    return rc;
+  }
+  GDX_HIP(hipEventRecord(p->ev[1], st));
+  if ((rc = gdx::launch_rhs(p, st))) {
+    set_error("rhs kernel launch failed");
+    return rc;
+  }
+  GDX_HIP(hipEventRecord(p->ev[2], st));
+  if ((rc = gdx::launch_matrix(p, st))) {
+    set_error("matrix kernel launch failed");
+    return rc;
+  }
+  GDX_HIP(hipEventRecord(p->ev[3], st));
+  return GDX_OK;
+}
+
+int gdx_sync(gdx_problem *p) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  if (!p->ctx) return GDX_OK;
+  GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+  return GDX_OK;
+}
+
+int gdx_stream(gdx_problem *p, void **stream) {
+  GDX_REQUIRE(p && stream, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  *stream = reinterpret_cast<void *>(p->ctx->stream);
+  return GDX_OK;
+}
+
+int gdx_phase_times(gdx_problem *p, double *ms, int32_t n) {
+  GDX_REQUIRE(p && ms && n >= 0, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_HIP(hipEventSynchronize(p->ev[3]));
+  float t[4] = {0, 0, 0, 0};
+  GDX_HIP(hipEventElapsedTime(&t[0], p->ev[0], p->ev[1]));
+  GDX_HIP(hipEventElapsedTime(&t[1], p->ev[1], p->ev[2]));
+  GDX_HIP(hipEventElapsedTime(&t[2], p->ev[2], p->ev[3]));
+  GDX_HIP(hipEventElapsedTime(&t[3], p->ev[0], p->ev[3]));
+  for (int i = 0; i < n && i < 4; ++i) ms[i] = t[i];
+  return GDX_OK;
+}
+
+int gdx_num_parts(gdx_problem *p, int32_t *nparts) {
+  GDX_REQUIRE(p && nparts, GDX_ERR_ARG, "NULL argument");
+  *nparts = p->nparts;
+  return GDX_OK;
+}
+
+int gdx_num_global_dofs(gdx_problem *p, int64_t *ngdofs) {
+  GDX_REQUIRE(p && ngdofs, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  *ngdofs = p->ngdofs;
+  return GDX_OK;
+}
+
+int gdx_owned_nnz(gdx_problem *p, int64_t *nnz) {
+  GDX_REQUIRE(p && nnz, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  *nnz = p->owned_nnz;
+  return GDX_OK;
+}
+
+int gdx_part_sizes(gdx_problem *p, int32_t part, int64_t *s) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(s, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  s[GDX_SZ_NLDOFS] = P.n_ldofs;
+  s[GDX_SZ_NOWN] = P.n_own;
+  s[GDX_SZ_NGHOST_ROWS] = P.n_ghost_rows;
+  s[GDX_SZ_NGHOST_COLS] = P.n_ghost_cols;
+  s[GDX_SZ_NNZ] = P.rowptr[P.nrows];
+  return GDX_OK;
+}
+
+int gdx_export_fe(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner, int64_t *gnode) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(gid && owner && gnode, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  for (int64_t i = 0; i < P.n_ldofs; ++i) {
+    gid[i] = P.ldof_gid[i];
+    owner[i] = P.ldof_owner[i];
+    const int64_t l = P.ldof_lnode[i];
+    const int64_t g0 = p->K * P.loc_lo[0] + l % P.lnn[0];
+    const int64_t g1 = p->K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
+    const int64_t g2 = p->K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1]);
+    gnode[i] = g0 + (int64_t)p->kp.nn[0] * (g1 + (int64_t)p->kp.nn[1] * g2);
+  }
+  return GDX_OK;
+}
+
+int gdx_export_rows(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(gid && owner, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  std::memcpy(gid, P.row_gid.data(), P.nrows * sizeof(int64_t));
+  std::memcpy(owner, P.row_owner.data(), P.nrows * sizeof(int32_t));
+  return GDX_OK;
+}
+
+int gdx_export_cols(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(gid && owner, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  std::memcpy(gid, P.col_gid.data(), P.ncols * sizeof(int64_t));
+  std::memcpy(owner, P.col_owner.data(), P.ncols * sizeof(int32_t));
+  return GDX_OK;
+}
+
+int gdx_export_csr(gdx_problem *p, int32_t part, int64_t *rowptr, int32_t *colidx, double *values) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(rowptr && colidx && values, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  std::memcpy(rowptr, P.rowptr.data(), (P.nrows + 1) * sizeof(int64_t));
+  std::memcpy(colidx, P.colidx.data(), P.rowptr[P.nrows] * sizeof(int32_t));
+  if (!p->ctx) {
+    std::memset(values, 0, P.rowptr[P.nrows] * sizeof(double));
+    return GDX_OK;
+  }
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+  if (P.rowptr[P.nrows] > 0)
+    GDX_HIP(hipMemcpy(values, p->dev.values + P.val_base, P.rowptr[P.nrows] * sizeof(double), hipMemcpyDeviceToHost));
+  return GDX_OK;
+}
+
+int gdx_export_rhs(gdx_problem *p, int32_t part, double *b) {
+  int rc = part_check(p, part);
+  if (rc) return rc;
+  GDX_REQUIRE(b, GDX_ERR_ARG, "NULL argument");
+  const gdx::Part &P = p->P[part];
+  if (!p->ctx) {
+    std::memset(b, 0, P.nrows * sizeof(double));
+    return GDX_OK;
+  }
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+  if (P.nrows > 0) GDX_HIP(hipMemcpy(b, p->dev.b + P.row_base, P.nrows * sizeof(double), hipMemcpyDeviceToHost));
+  return GDX_OK;
+}
+
+int gdx_device_values(gdx_problem *p, double **values, int64_t *nnz_total) {
+  GDX_REQUIRE(p && values && nnz_total, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  *values = p->dev.values;
+  *nnz_total = p->nnz_total;
+  return GDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Internal data structures of libgdx (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gdx.h"
+
+namespace gdx {
+
+constexpr int kMaxQ = 4;     // Gauss points per direction
+constexpr int kMaxK = 2;     // polynomial order
+constexpr uint8_t kSlotAbsent = 0xFF;
+constexpr uint8_t kSlotDirichlet = 0xFE;
+
+// One part (subdomain) of the Cartesian partition; all parts live on one GPU.
+struct Part {
+  int rank = 0, pc[3] = {0, 0, 0};
+  int64_t own_lo[3], own_hi[3];  // owned cells [lo,hi)
+  int64_t loc_lo[3], loc_hi[3];  // owned + one ghost layer
+  int64_t lnn[3];                // local node counts
+  std::vector<int64_t> lnode_dof;  // local node -> free ldof (>=0) or -1
+  std::vector<int64_t> ldof_lnode;
+  std::vector<int32_t> ldof_owner;
+  std::vector<int64_t> ldof_gid;
+  int64_t n_ldofs = 0, n_own = 0;
+  std::vector<int64_t> own_ldofs;
+  std::vector<int64_t> ldof_col;  // ldof -> local col id (or -1)
+  // matrix PRanges
+  std::vector<int64_t> row_gid, col_gid;
+  std::vector<int32_t> row_owner, col_owner;
+  int64_t nrows = 0, ncols = 0, n_ghost_rows = 0, n_ghost_cols = 0;
+  std::vector<int64_t> row_ldof;  // local row -> ldof
+  // local CSR pattern
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> colidx;
+  int64_t val_base = 0, row_base = 0;
+  // owned node box (global node coords)
+  int64_t onb_lo[3], onb_n[3];
+  int64_t rowinfo_base = 0;
+};
+
+// Per owned node of a part's owned node box (device layout).
+struct RowInfo {
+  long long rowstart;  // offset of the row's first value in the global values buffer
+  long long bidx;      // offset of the row in the global RHS buffer
+  int pid;             // position pattern id, -1 when the node is not a free row
+  int pad;
+};
+
+// A tile of owned rows processed by one workgroup.
+struct Tile {
+  int part;
+  int org[3];     // global node coords of the tile origin (multiple of order)
+  int ext[3];     // rows of the tile inside the owned node box
+  int onb_lo[3];  // owned node box of the part
+  int onb_n[3];
+  long long rowinfo_base;
+};
+
+// Problem-wide constants passed to the kernels by value.
+struct KParams {
+  int dim, K, NQ, S;             // S = (2K+1)^dim stencil slots
+  int ncells[3];
+  int qn[3];                     // qp lattice extents NQ*ncells
+  int nn[3];                     // node lattice extents K*ncells+1
+  double x0[3], h[3];
+  double phi[kMaxK + 1][kMaxQ], dphi[kMaxK + 1][kMaxQ], qw[kMaxQ], qx[kMaxQ];
+  double coef_const;
+  int nmodes;
+  int rhs_kind;
+  double rhs_const;
+  int dir_kind;
+  double dir_coef[4];
+};
+
+struct Device {
+  double *W = nullptr;       // kappa at quadrature points (global qp lattice)
+  double *F = nullptr;       // f at quadrature points
+  double *E = nullptr;       // separable Fourier tables [3][nmodes][qn][2]
+  double *modes = nullptr;   // [nmodes][5]
+  double *values = nullptr;  // concatenated local CSR values of all parts
+  double *b = nullptr;       // concatenated RHS (rows of every part)
+  RowInfo *rowinfo = nullptr;
+  uint8_t *pat_fwd = nullptr;  // [npat][S]: slot -> offset | absent | Dirichlet
+  uint8_t *pat_inv = nullptr;  // [npat][S]: offset -> slot
+  int *pat_len = nullptr;
+  int *pat_dir = nullptr;      // pattern has Dirichlet slots
+  Tile *tiles = nullptr;
+};
+
+}  // namespace gdx
+
+struct gdx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct gdx_problem {
+  gdx_ctx *ctx = nullptr;
+  gdx_problem_desc desc{};
+  std::vector<double> modes;
+  int nparts = 0;
+  int K = 1, D = 2, NQ = 2;
+  int64_t nn[3] = {1, 1, 1};
+  int64_t ngdofs = 0;
+  std::vector<gdx::Part> P;
+  bool symbolic_done = false;
+  // structured-kernel data (host copies)
+  std::vector<gdx::RowInfo> rowinfo;
+  std::vector<uint8_t> pat_fwd, pat_inv;
+  std::vector<int> pat_len, pat_dir;
+  int npat = 0;
+  std::vector<gdx::Tile> tiles;
+  int tileR[3] = {1, 1, 1};
+  int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
+  gdx::KParams kp{};
+  gdx::Device dev;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double phase_ms[4] = {0, 0, 0, 0};
+};
+
+namespace gdx {
+void set_error(const std::string &msg);
+int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
+void fill_kparams(gdx_problem *p);
+void tile_shape(int dim, int K, int R[3]);
+}  // namespace gdx

@@ -1,0 +1,555 @@
+// Host symbolic phase of libgdx: partition, conforming dof numbering, global
+// ids, matrix row/col PRanges, local CSR pattern and the position patterns the
+// numeric kernels scatter through.  Runs once per problem; its outputs are
+// uploaded to HBM by gdx_capi.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "gdx_internal.h"
+
+namespace gdx {
+
+namespace {
+
+// src/PArraysExtras.jl:72-84 (_local_range), 0-based and half-open: the last
+// `rem` parts get one extra cell, `ghost` adds one layer on interior sides.
+void local_range(int pi, int np, int64_t n, int ghost, int64_t &lo, int64_t &hi) {
+  int64_t l = n / np, rem = n % np, p = pi + 1, offset = l * (p - 1);
+  if (rem >= np - p + 1) {
+    l += 1;
+    offset += p - (np - rem) - 1;
+  }
+  lo = std::max<int64_t>(offset - ghost, 0);
+  hi = std::min<int64_t>(l + offset + ghost, n);
+}
+
+// Local faces of Gridap's QUAD/HEX polytopes as points of the doubled lattice
+// of a cell (vertices, then edges, then faces, in the polytope's order).
+const int kHexV[8][3] = {{0, 0, 0}, {2, 0, 0}, {0, 2, 0}, {2, 2, 0}, {0, 0, 2}, {2, 0, 2}, {0, 2, 2}, {2, 2, 2}};
+const int kHexE[12][3] = {{1, 0, 0}, {1, 2, 0}, {1, 0, 2}, {1, 2, 2}, {0, 1, 0}, {2, 1, 0},
+                          {0, 1, 2}, {2, 1, 2}, {0, 0, 1}, {2, 0, 1}, {0, 2, 1}, {2, 2, 1}};
+const int kHexF[6][3] = {{1, 1, 0}, {1, 1, 2}, {1, 0, 1}, {1, 2, 1}, {0, 1, 1}, {2, 1, 1}};
+const int kQuadV[4][3] = {{0, 0, 0}, {2, 0, 0}, {0, 2, 0}, {2, 2, 0}};
+const int kQuadE[4][3] = {{1, 0, 0}, {1, 2, 0}, {0, 1, 0}, {2, 1, 0}};
+
+// Cell dofs of LagrangianRefFE(order K) in Gridap's local order, as offsets in
+// the order-K node lattice of the cell.
+int cell_dof_offsets(int D, int K, int off[27][3]) {
+  int n = 0;
+  const int(*V)[3] = D == 3 ? kHexV : kQuadV;
+  for (int v = 0; v < (D == 3 ? 8 : 4); ++v, ++n)
+    for (int d = 0; d < 3; ++d) off[n][d] = V[v][d] * K / 2;
+  if (K == 1) return n;
+  const int(*E)[3] = D == 3 ? kHexE : kQuadE;
+  for (int e = 0; e < (D == 3 ? 12 : 4); ++e, ++n)
+    for (int d = 0; d < 3; ++d) off[n][d] = E[e][d];
+  if (D == 3)
+    for (int f = 0; f < 6; ++f, ++n)
+      for (int d = 0; d < 3; ++d) off[n][d] = kHexF[f][d];
+  off[n][0] = 1;
+  off[n][1] = 1;
+  off[n][2] = D == 3 ? 1 : 0;
+  return n + 1;
+}
+
+struct Ctx {
+  gdx_problem *p;
+  int D, K;
+  int64_t n[3], nn[3];
+  std::vector<int> node_owner_dim[3];  // part index along d owning global node N
+  std::vector<int64_t> own_lo_dim[3], own_hi_dim[3];
+};
+
+inline int64_t lin(const int64_t e[3], int64_t x, int64_t y, int64_t z) { return x + e[0] * (y + e[1] * z); }
+
+bool is_dirichlet(const Ctx &c, const int64_t g[3]) {
+  if (c.p->desc.dir_kind == 0) return false;
+  for (int d = 0; d < c.D; ++d)
+    if (g[d] == 0 || g[d] == c.nn[d] - 1) return true;
+  return false;
+}
+
+// Conforming numbering of the part's local model (Gridap: faces dimension-major,
+// each face's own dofs consecutive; edges and faces numbered at their first
+// appearance when cells are visited in order and local faces in polytope
+// order; Dirichlet dofs skipped).  The first cell of a doubled-lattice point
+// is the lexicographically smallest local cell containing it.
+void number_part(const Ctx &c, Part &P) {
+  const int D = c.D, K = c.K;
+  const int64_t *lnn = P.lnn;
+  const int64_t nl = lnn[0] * lnn[1] * lnn[2];
+  int64_t lc[3] = {P.loc_hi[0] - P.loc_lo[0], P.loc_hi[1] - P.loc_lo[1], D == 3 ? P.loc_hi[2] - P.loc_lo[2] : 1};
+  std::vector<int64_t> order;
+  order.reserve(nl);
+  const int64_t nz = D == 3 ? lnn[2] : 1;
+  for (int64_t z = 0; z < nz; ++z)
+    for (int64_t y = 0; y < lnn[1]; ++y)
+      for (int64_t x = 0; x < lnn[0]; ++x)
+        if (K == 1 || !((x | y | z) & 1)) order.push_back(lin(lnn, x, y, z));
+  if (K == 2) {
+    auto first_cell = [](int64_t pt) { return (pt & 1) ? (pt - 1) / 2 : std::max<int64_t>(pt / 2 - 1, 0); };
+    for (int fdim = 1; fdim < D; ++fdim) {
+      const int(*F)[3] = D == 3 ? (fdim == 1 ? kHexE : kHexF) : kQuadE;
+      const int nf = D == 3 ? (fdim == 1 ? 12 : 6) : 4;
+      for (int64_t cz = 0; cz < lc[2]; ++cz)
+        for (int64_t cy = 0; cy < lc[1]; ++cy)
+          for (int64_t cx = 0; cx < lc[0]; ++cx)
+            for (int f = 0; f < nf; ++f) {
+              int64_t pt[3] = {2 * cx + F[f][0], 2 * cy + F[f][1], D == 3 ? 2 * cz + F[f][2] : 0};
+              bool first = first_cell(pt[0]) == cx && first_cell(pt[1]) == cy && (D == 2 || first_cell(pt[2]) == cz);
+              if (first) order.push_back(lin(lnn, pt[0], pt[1], pt[2]));
+            }
+    }
+    for (int64_t cz = 0; cz < lc[2]; ++cz)
+      for (int64_t cy = 0; cy < lc[1]; ++cy)
+        for (int64_t cx = 0; cx < lc[0]; ++cx) order.push_back(lin(lnn, 2 * cx + 1, 2 * cy + 1, D == 3 ? 2 * cz + 1 : 0));
+  }
+  if ((int64_t)order.size() != nl) throw std::runtime_error("internal: local numbering does not cover all nodes");
+  P.lnode_dof.assign(nl, -1);
+  P.ldof_lnode.clear();
+  for (int64_t l : order) {
+    int64_t g[3] = {K * P.loc_lo[0] + l % lnn[0], K * P.loc_lo[1] + (l / lnn[0]) % lnn[1],
+                    K * P.loc_lo[2] + l / (lnn[0] * lnn[1])};
+    if (!is_dirichlet(c, g)) {
+      P.lnode_dof[l] = (int64_t)P.ldof_lnode.size();
+      P.ldof_lnode.push_back(l);
+    }
+  }
+  P.n_ldofs = (int64_t)P.ldof_lnode.size();
+}
+
+inline void gnode_of(const Ctx &c, const Part &P, int64_t l, int64_t g[3]) {
+  g[0] = c.K * P.loc_lo[0] + l % P.lnn[0];
+  g[1] = c.K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
+  g[2] = c.K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1]);
+}
+
+inline int64_t lnode_in(const Ctx &c, const Part &P, const int64_t g[3]) {
+  int64_t l[3];
+  for (int d = 0; d < 3; ++d) {
+    l[d] = g[d] - c.K * P.loc_lo[d];
+    if (l[d] < 0 || l[d] >= P.lnn[d]) return -1;
+  }
+  return lin(P.lnn, l[0], l[1], l[2]);
+}
+
+inline int node_owner(const Ctx &c, const int64_t g[3]) {
+  const int *np = c.p->desc.parts;
+  int r = c.node_owner_dim[2][g[2]];
+  r = r * np[1] + c.node_owner_dim[1][g[1]];
+  return r * np[0] + c.node_owner_dim[0][g[0]];
+}
+
+// Global nodes sharing a cell with node i along one dimension, restricted to
+// cells [clo, chi): [jlo, jhi] (empty when jlo > jhi).
+inline void shared_range(int K, int64_t i, int64_t clo, int64_t chi, int64_t &jlo, int64_t &jhi) {
+  int64_t cmin = std::max<int64_t>((i + K - 1) / K - 1, clo), cmax = std::min<int64_t>(i / K, chi - 1);
+  jlo = K * cmin;
+  jhi = K * cmax + K;
+  if (cmin > cmax) jlo = 1, jhi = 0;
+}
+
+}  // namespace
+
+void tile_shape(int dim, int K, int R[3]) {
+  // must match the template instances launched by launch_matrix
+  if (dim == 3) {
+    R[0] = 8; R[1] = 8; R[2] = K == 2 ? 8 : 4;
+  } else {
+    R[0] = 16; R[1] = 16; R[2] = 1;
+  }
+}
+
+int symbolic(gdx_problem *p) {
+  Ctx c;
+  c.p = p;
+  c.D = p->D;
+  c.K = p->K;
+  const gdx_problem_desc &d = p->desc;
+  for (int k = 0; k < 3; ++k) {
+    c.n[k] = k < c.D ? d.cells[k] : 1;
+    c.nn[k] = k < c.D ? c.K * c.n[k] + 1 : 1;
+    int np = k < c.D ? d.parts[k] : 1;
+    c.own_lo_dim[k].resize(np);
+    c.own_hi_dim[k].resize(np);
+    for (int q = 0; q < np; ++q) local_range(q, np, c.n[k], 0, c.own_lo_dim[k][q], c.own_hi_dim[k][q]);
+    // owner of node N along k: the highest part whose first node is <= N
+    // (generate_gids: owner = max owner of the cells touching the dof)
+    c.node_owner_dim[k].assign(c.nn[k], 0);
+    for (int64_t N = 0; N < c.nn[k]; ++N) {
+      int o = 0;
+      for (int q = 0; q < np; ++q)
+        if (c.K * c.own_lo_dim[k][q] <= N) o = q;
+      c.node_owner_dim[k][N] = o;
+    }
+  }
+  const int nparts = p->nparts;
+  p->P.assign(nparts, Part{});
+  const bool FA = d.strategy == GDX_FULLY_ASSEMBLED_ROWS;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.rank = r;
+    P.pc[0] = r % d.parts[0];
+    P.pc[1] = (r / d.parts[0]) % d.parts[1];
+    P.pc[2] = c.D == 3 ? r / (d.parts[0] * d.parts[1]) : 0;
+    for (int k = 0; k < 3; ++k) {
+      if (k < c.D) {
+        local_range(P.pc[k], d.parts[k], c.n[k], 0, P.own_lo[k], P.own_hi[k]);
+        local_range(P.pc[k], d.parts[k], c.n[k], 1, P.loc_lo[k], P.loc_hi[k]);
+        P.lnn[k] = c.K * (P.loc_hi[k] - P.loc_lo[k]) + 1;
+        P.onb_lo[k] = c.K * P.own_lo[k];
+        P.onb_n[k] = c.K * (P.own_hi[k] - P.own_lo[k]) + (P.pc[k] == d.parts[k] - 1 ? 1 : 0);
+      } else {
+        P.own_lo[k] = P.loc_lo[k] = 0;
+        P.own_hi[k] = P.loc_hi[k] = 1;
+        P.lnn[k] = 1;
+        P.onb_lo[k] = 0;
+        P.onb_n[k] = 1;
+      }
+    }
+    number_part(c, P);
+  }
+  // owners and rank-blocked global ids (src/FESpaces.jl:139-261)
+  int64_t first = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.ldof_owner.resize(P.n_ldofs);
+    P.ldof_gid.assign(P.n_ldofs, -1);
+    P.own_ldofs.clear();
+    for (int64_t i = 0; i < P.n_ldofs; ++i) {
+      int64_t g[3];
+      gnode_of(c, P, P.ldof_lnode[i], g);
+      P.ldof_owner[i] = node_owner(c, g);
+      if (P.ldof_owner[i] == r) {
+        P.ldof_gid[i] = first + (int64_t)P.own_ldofs.size();
+        P.own_ldofs.push_back(i);
+      }
+    }
+    P.n_own = (int64_t)P.own_ldofs.size();
+    first += P.n_own;
+  }
+  p->ngdofs = first;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    for (int64_t i = 0; i < P.n_ldofs; ++i) {
+      if (P.ldof_gid[i] >= 0) continue;
+      int64_t g[3];
+      gnode_of(c, P, P.ldof_lnode[i], g);
+      const Part &Q = p->P[P.ldof_owner[i]];
+      int64_t lq = lnode_in(c, Q, g);
+      if (lq < 0 || Q.lnode_dof[lq] < 0) throw std::runtime_error("internal: ghost dof missing in its owner");
+      P.ldof_gid[i] = Q.ldof_gid[Q.lnode_dof[lq]];
+    }
+  }
+
+  int off[27][3];
+  const int ncd = cell_dof_offsets(c.D, c.K, off);
+  const int K = c.K;
+  auto cell_ldof = [&](const Part &P, int64_t cx, int64_t cy, int64_t cz, int a) -> int64_t {
+    int64_t l = lin(P.lnn, K * (cx - P.loc_lo[0]) + off[a][0], K * (cy - P.loc_lo[1]) + off[a][1],
+                    K * (cz - P.loc_lo[2]) + off[a][2]);
+    return P.lnode_dof[l];
+  };
+
+  // row and col PRanges (src/Algebra.jl:593-646, 1136-1257)
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.row_gid.clear();
+    P.row_owner.clear();
+    P.row_ldof.clear();
+    for (int64_t o = 0; o < P.n_own; ++o) {
+      P.row_gid.push_back(P.ldof_gid[P.own_ldofs[o]]);
+      P.row_owner.push_back(r);
+      P.row_ldof.push_back(P.own_ldofs[o]);
+    }
+    if (!FA) {
+      // ghost rows touched by owned cells, in FE local id order
+      for (int64_t i = 0; i < P.n_ldofs; ++i) {
+        if (P.ldof_owner[i] == r) continue;
+        int64_t g[3];
+        gnode_of(c, P, P.ldof_lnode[i], g);
+        bool in = true;
+        for (int k = 0; k < c.D; ++k) in = in && g[k] >= K * P.own_lo[k] && g[k] <= K * P.own_hi[k];
+        if (in) {
+          P.row_gid.push_back(P.ldof_gid[i]);
+          P.row_owner.push_back(P.ldof_owner[i]);
+          P.row_ldof.push_back(i);
+        }
+      }
+    }
+    P.nrows = (int64_t)P.row_gid.size();
+    P.n_ghost_rows = P.nrows - P.n_own;
+
+    P.ldof_col.assign(P.n_ldofs, -1);
+    P.col_gid.clear();
+    P.col_owner.clear();
+    for (int64_t o = 0; o < P.n_own; ++o) {
+      P.ldof_col[P.own_ldofs[o]] = o;
+      P.col_gid.push_back(P.ldof_gid[P.own_ldofs[o]]);
+      P.col_owner.push_back(r);
+    }
+    std::vector<int64_t> ghosts;  // ldofs of ghost cols in PRange order
+    auto visit_cell_cols = [&](int64_t cx, int64_t cy, int64_t cz) {
+      for (int b = 0; b < ncd; ++b) {
+        int64_t j = cell_ldof(P, cx, cy, cz, b);
+        if (j >= 0 && P.ldof_owner[j] != r && P.ldof_col[j] == -1) {
+          P.ldof_col[j] = -2;  // seen
+          ghosts.push_back(j);
+        }
+      }
+    };
+    auto cell_has_row = [&](const Part &Q, int64_t cx, int64_t cy, int64_t cz, int want_owner) {
+      for (int a = 0; a < ncd; ++a) {
+        int64_t i = cell_ldof(Q, cx, cy, cz, a);
+        if (i >= 0 && (want_owner < 0 || Q.ldof_owner[i] == want_owner)) return true;
+      }
+      return false;
+    };
+    if (!FA) {
+      // first encounter over the local COO: own cells (columns outer), then the
+      // triplets received from each sender in ascending rank order
+      for (int64_t cz = P.own_lo[2]; cz < P.own_hi[2]; ++cz)
+        for (int64_t cy = P.own_lo[1]; cy < P.own_hi[1]; ++cy)
+          for (int64_t cx = P.own_lo[0]; cx < P.own_hi[0]; ++cx)
+            if (cell_has_row(P, cx, cy, cz, -1)) visit_cell_cols(cx, cy, cz);
+      for (int s = 0; s < nparts; ++s) {
+        if (s == r) continue;
+        const Part &S = p->P[s];
+        int64_t lo[3], hi[3];
+        bool empty = false;
+        for (int k = 0; k < 3; ++k) {
+          // sender cells whose closure meets this part's owned node box
+          int64_t a = std::max<int64_t>(S.own_lo[k], (P.onb_lo[k] + K - 1) / K - 1);
+          int64_t b = std::min<int64_t>(S.own_hi[k], (P.onb_lo[k] + P.onb_n[k] - 1) / K + 1);
+          if (k >= c.D) a = 0, b = 1;
+          lo[k] = a;
+          hi[k] = b;
+          if (a >= b) empty = true;
+        }
+        if (empty) continue;
+        for (int64_t cz = lo[2]; cz < hi[2]; ++cz)
+          for (int64_t cy = lo[1]; cy < hi[1]; ++cy)
+            for (int64_t cx = lo[0]; cx < hi[0]; ++cx)
+              if (cell_has_row(S, cx, cy, cz, r)) visit_cell_cols(cx, cy, cz);
+      }
+    } else {
+      // touched ghost columns of cells holding an owned row, sorted by FE local id
+      for (int64_t cz = P.loc_lo[2]; cz < P.loc_hi[2]; ++cz)
+        for (int64_t cy = P.loc_lo[1]; cy < P.loc_hi[1]; ++cy)
+          for (int64_t cx = P.loc_lo[0]; cx < P.loc_hi[0]; ++cx)
+            if (cell_has_row(P, cx, cy, cz, r)) visit_cell_cols(cx, cy, cz);
+      std::sort(ghosts.begin(), ghosts.end());
+    }
+    for (size_t g = 0; g < ghosts.size(); ++g) {
+      P.ldof_col[ghosts[g]] = P.n_own + (int64_t)g;
+      P.col_gid.push_back(P.ldof_gid[ghosts[g]]);
+      P.col_owner.push_back(P.ldof_owner[ghosts[g]]);
+    }
+    P.ncols = (int64_t)P.col_gid.size();
+    P.n_ghost_cols = P.ncols - P.n_own;
+  }
+
+  // local CSR patterns and the position patterns of owned rows
+  const int S = (c.D == 3) ? (2 * K + 1) * (2 * K + 1) * (2 * K + 1) : (2 * K + 1) * (2 * K + 1);
+  std::unordered_map<std::string, int> pat_ids;
+  p->pat_fwd.clear();
+  p->pat_inv.clear();
+  p->pat_len.clear();
+  p->pat_dir.clear();
+  p->rowinfo.clear();
+  int64_t val_base = 0, row_base = 0, owned_nnz = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.val_base = val_base;
+    P.row_base = row_base;
+    P.rowptr.assign(P.nrows + 1, 0);
+    P.colidx.clear();
+    std::vector<std::pair<int32_t, int>> ents;
+    std::vector<int> own_pid(P.n_own, -1);
+    std::string key(S, (char)kSlotAbsent);
+    for (int64_t row = 0; row < P.nrows; ++row) {
+      int64_t g[3];
+      gnode_of(c, P, P.ldof_lnode[P.row_ldof[row]], g);
+      const bool own = row < P.n_own;
+      int64_t jlo[3] = {0, 0, 0}, jhi[3] = {0, 0, 0};
+      for (int k = 0; k < c.D; ++k) {
+        if (own) shared_range(K, g[k], 0, c.n[k], jlo[k], jhi[k]);
+        else shared_range(K, g[k], P.own_lo[k], P.own_hi[k], jlo[k], jhi[k]);
+      }
+      ents.clear();
+      std::fill(key.begin(), key.end(), (char)kSlotAbsent);
+      for (int64_t jz = jlo[2]; jz <= jhi[2]; ++jz)
+        for (int64_t jy = jlo[1]; jy <= jhi[1]; ++jy)
+          for (int64_t jx = jlo[0]; jx <= jhi[0]; ++jx) {
+            int64_t j[3] = {jx, jy, jz};
+            int slot = (int)(jx - g[0] + K);
+            slot += (2 * K + 1) * (int)(jy - g[1] + K);
+            if (c.D == 3) slot += (2 * K + 1) * (2 * K + 1) * (int)(jz - g[2] + K);
+            if (is_dirichlet(c, j)) {
+              key[slot] = (char)kSlotDirichlet;
+              continue;
+            }
+            int64_t lj = lnode_in(c, P, j);
+            int64_t col = lj >= 0 ? P.ldof_col[P.lnode_dof[lj]] : -1;
+            if (col < 0) throw std::runtime_error("internal: stencil column missing from the col PRange");
+            ents.emplace_back((int32_t)col, slot);
+          }
+      std::sort(ents.begin(), ents.end());
+      for (size_t e = 0; e < ents.size(); ++e) {
+        P.colidx.push_back(ents[e].first);
+        key[ents[e].second] = (char)e;
+      }
+      P.rowptr[row + 1] = (int64_t)P.colidx.size();
+      if (own) {
+        auto it = pat_ids.find(key);
+        int pid;
+        if (it == pat_ids.end()) {
+          pid = (int)p->pat_len.size();
+          pat_ids.emplace(key, pid);
+          std::vector<uint8_t> inv(S, 0);
+          bool has_dir = false;
+          for (int s = 0; s < S; ++s) {
+            uint8_t v = (uint8_t)key[s];
+            if (v == kSlotDirichlet) has_dir = true;
+            else if (v != kSlotAbsent) inv[v] = (uint8_t)s;
+            p->pat_fwd.push_back(v);
+          }
+          p->pat_inv.insert(p->pat_inv.end(), inv.begin(), inv.end());
+          p->pat_len.push_back((int)ents.size());
+          p->pat_dir.push_back(has_dir ? 1 : 0);
+        } else {
+          pid = it->second;
+        }
+        own_pid[row] = pid;
+      }
+    }
+    owned_nnz += P.rowptr[P.n_own];
+    val_base += P.rowptr[P.nrows];
+    row_base += P.nrows;
+    // row table over the owned node box
+    P.rowinfo_base = (int64_t)p->rowinfo.size();
+    std::vector<int64_t> ldof_own(P.n_ldofs, -1);
+    for (int64_t o = 0; o < P.n_own; ++o) ldof_own[P.own_ldofs[o]] = o;
+    for (int64_t z = 0; z < P.onb_n[2]; ++z)
+      for (int64_t y = 0; y < P.onb_n[1]; ++y)
+        for (int64_t x = 0; x < P.onb_n[0]; ++x) {
+          int64_t g[3] = {P.onb_lo[0] + x, P.onb_lo[1] + y, P.onb_lo[2] + z};
+          RowInfo ri{0, 0, -1, 0};
+          int64_t l = lnode_in(c, P, g);
+          int64_t ld = l >= 0 ? P.lnode_dof[l] : -1;
+          if (ld >= 0) {
+            int64_t o = ldof_own[ld];
+            if (o < 0) throw std::runtime_error("internal: owned node box holds a non-owned dof");
+            ri.rowstart = P.val_base + P.rowptr[o];
+            ri.bidx = P.row_base + o;
+            ri.pid = own_pid[o];
+          }
+          p->rowinfo.push_back(ri);
+        }
+  }
+  p->npat = (int)p->pat_len.size();
+  p->nnz_total = val_base;
+  p->rows_total = row_base;
+  p->owned_nnz = owned_nnz;
+
+  // tiles of owned rows
+  tile_shape(c.D, K, p->tileR);
+  p->tiles.clear();
+  for (int r = 0; r < nparts; ++r) {
+    const Part &P = p->P[r];
+    int64_t nt[3];
+    for (int k = 0; k < 3; ++k) nt[k] = (P.onb_n[k] + p->tileR[k] - 1) / p->tileR[k];
+    for (int64_t tz = 0; tz < nt[2]; ++tz)
+      for (int64_t ty = 0; ty < nt[1]; ++ty)
+        for (int64_t tx = 0; tx < nt[0]; ++tx) {
+          Tile t{};
+          t.part = r;
+          int64_t tt[3] = {tx, ty, tz};
+          bool any = false;
+          for (int k = 0; k < 3; ++k) {
+            t.org[k] = (int)(P.onb_lo[k] + tt[k] * p->tileR[k]);
+            t.ext[k] = (int)std::min<int64_t>(p->tileR[k], P.onb_n[k] - tt[k] * p->tileR[k]);
+            t.onb_lo[k] = (int)P.onb_lo[k];
+            t.onb_n[k] = (int)P.onb_n[k];
+          }
+          t.rowinfo_base = P.rowinfo_base;
+          for (int z = 0; z < t.ext[2] && !any; ++z)
+            for (int y = 0; y < t.ext[1] && !any; ++y)
+              for (int x = 0; x < t.ext[0] && !any; ++x) {
+                int64_t li = ((int64_t)(t.org[2] - t.onb_lo[2] + z) * P.onb_n[1] + (t.org[1] - t.onb_lo[1] + y)) *
+                                 P.onb_n[0] +
+                             (t.org[0] - t.onb_lo[0] + x);
+                any = p->rowinfo[P.rowinfo_base + li].pid >= 0;
+              }
+          if (any) p->tiles.push_back(t);
+        }
+  }
+  return GDX_OK;
+}
+
+// Reference-cell tables: Gauss-Legendre points/weights on [0,1] and the 1D
+// Lagrange basis on equispaced nodes j/K (tensor index order).
+void fill_kparams(gdx_problem *p) {
+  KParams &kp = p->kp;
+  std::memset(&kp, 0, sizeof(kp));
+  const gdx_problem_desc &d = p->desc;
+  kp.dim = p->D;
+  kp.K = p->K;
+  kp.NQ = p->NQ;
+  kp.S = p->D == 3 ? (2 * p->K + 1) * (2 * p->K + 1) * (2 * p->K + 1) : (2 * p->K + 1) * (2 * p->K + 1);
+  for (int k = 0; k < 3; ++k) {
+    kp.ncells[k] = k < p->D ? (int)d.cells[k] : 1;
+    kp.qn[k] = k < p->D ? p->NQ * kp.ncells[k] : 1;
+    kp.nn[k] = k < p->D ? p->K * kp.ncells[k] + 1 : 1;
+    kp.x0[k] = k < p->D ? d.domain[2 * k] : 0.0;
+    kp.h[k] = k < p->D ? (d.domain[2 * k + 1] - d.domain[2 * k]) / (double)kp.ncells[k] : 1.0;
+  }
+  // Gauss-Legendre on [-1,1] by Newton on P_n, mapped to [0,1]
+  const int n = p->NQ;
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p1 = z, p0 = 1.0;
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    kp.qx[n - 1 - i] = 0.5 * (1.0 + z);
+    kp.qw[n - 1 - i] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+  const int K = p->K;
+  for (int a = 0; a <= K; ++a)
+    for (int q = 0; q < n; ++q) {
+      double x = kp.qx[q], v = 1.0, dv = 0.0;
+      for (int j = 0; j <= K; ++j)
+        if (j != a) v *= (x - (double)j / K) / ((double)(a - j) / K);
+      for (int m = 0; m <= K; ++m) {
+        if (m == a) continue;
+        double t = 1.0 / ((double)(a - m) / K);
+        for (int j = 0; j <= K; ++j)
+          if (j != a && j != m) t *= (x - (double)j / K) / ((double)(a - j) / K);
+        dv += t;
+      }
+      kp.phi[a][q] = v;
+      kp.dphi[a][q] = dv;
+    }
+  kp.coef_const = d.coef_const;
+  kp.nmodes = d.nmodes;
+  kp.rhs_kind = d.rhs_kind;
+  kp.rhs_const = d.rhs_const;
+  kp.dir_kind = d.dir_kind;
+  for (int k = 0; k < 4; ++k) kp.dir_coef[k] = d.dir_coef[k];
+}
+
+}  // namespace gdx

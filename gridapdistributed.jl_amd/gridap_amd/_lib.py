@@ -1,0 +1,100 @@
+"""ctypes binding of libgdx.so (the C ABI declared in include/gdx.h).
+
+The library must have been built in-tree (``python gridapdistributed.jl_amd/build.py``);
+there is no CPU fallback: every entry point fails loudly when the library or
+the GPU is missing.
+"""
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "libgdx.so")
+
+GDX_OK = 0
+ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ERR_UNSUPPORTED", -5: "GDX_ERR_ALLOC"}
+SUB_ASSEMBLED_ROWS, FULLY_ASSEMBLED_ROWS = 0, 1
+PREC_FP64 = 0
+RHS_CONST, RHS_MANUFACTURED = 0, 1
+SZ_NLDOFS, SZ_NOWN, SZ_NGHOST_ROWS, SZ_NGHOST_COLS, SZ_NNZ, SZ_COUNT = range(6)
+
+# every function declared in include/gdx.h
+EXPORTED = [
+    "gdx_version", "gdx_last_error", "gdx_init", "gdx_finalize", "gdx_problem_create", "gdx_problem_destroy",
+    "gdx_symbolic", "gdx_numeric", "gdx_num_parts", "gdx_num_global_dofs", "gdx_part_sizes", "gdx_export_fe",
+    "gdx_export_rows", "gdx_export_cols", "gdx_export_csr", "gdx_export_rhs", "gdx_device_values", "gdx_stream",
+    "gdx_sync", "gdx_phase_times", "gdx_owned_nnz",
+]
+
+
+class GdxError(RuntimeError):
+    def __init__(self, code, func, msg):
+        super().__init__(f"{func} failed with {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("order", ctypes.c_int32), ("qdeg", ctypes.c_int32), ("strategy", ctypes.c_int32),
+        ("cells", ctypes.c_int64 * 3), ("parts", ctypes.c_int32 * 3), ("domain", ctypes.c_double * 6),
+        ("coef_const", ctypes.c_double), ("nmodes", ctypes.c_int32), ("modes", ctypes.POINTER(ctypes.c_double)),
+        ("rhs_kind", ctypes.c_int32), ("rhs_const", ctypes.c_double), ("dir_kind", ctypes.c_int32),
+        ("dir_coef", ctypes.c_double * 4),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libgdx.so; raises if it was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python gridapdistributed.jl_amd/build.py` "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    P = ctypes.POINTER
+    sig = {
+        "gdx_version": [P(i32), P(i32), P(i32)],
+        "gdx_init": [i32, P(vp)],
+        "gdx_finalize": [vp],
+        "gdx_problem_create": [vp, P(ProblemDesc), P(vp)],
+        "gdx_problem_destroy": [vp],
+        "gdx_symbolic": [vp],
+        "gdx_numeric": [vp, i32],
+        "gdx_num_parts": [vp, P(i32)],
+        "gdx_num_global_dofs": [vp, P(i64)],
+        "gdx_part_sizes": [vp, i32, P(i64)],
+        "gdx_export_fe": [vp, i32, vp, vp, vp],
+        "gdx_export_rows": [vp, i32, vp, vp],
+        "gdx_export_cols": [vp, i32, vp, vp],
+        "gdx_export_csr": [vp, i32, vp, vp, vp],
+        "gdx_export_rhs": [vp, i32, vp],
+        "gdx_device_values": [vp, P(vp), P(i64)],
+        "gdx_stream": [vp, P(vp)],
+        "gdx_sync": [vp],
+        "gdx_phase_times": [vp, P(ctypes.c_double), i32],
+        "gdx_owned_nnz": [vp, P(i64)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.gdx_last_error.argtypes = []
+    L.gdx_last_error.restype = ctypes.c_char_p
+    _LIB = L
+    return L
+
+
+def check(rc, func):
+    if rc != GDX_OK:
+        msg = lib().gdx_last_error()
+        raise GdxError(rc, func, msg.decode() if msg else "")
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc

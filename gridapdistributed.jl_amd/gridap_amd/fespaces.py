@@ -1,0 +1,327 @@
+"""FE spaces, weak forms and the assembler (mirror of src/FESpaces.jl:570-862).
+
+The GPU path lowers a restricted weak-form vocabulary:
+
+    a(u,v) = int( kappa * grad(v) . grad(u) ) dOmega      -> Diffusion(kappa)
+    l(v)   = int( f * v ) dOmega                          -> Source(f)
+
+with kappa a ``ConstantCoefficient`` or ``FourierExpCoefficient`` and f a
+``ConstantSource`` or ``ManufacturedSource`` (f = -div(kappa grad uD)).
+``SparseMatrixAssembler(U, V, strategy)`` owns one device problem (gdx_problem):
+the symbolic phase (numbering, gids, PRanges, CSR pattern, position map) runs
+once, every numeric assembly refills the values in HBM.
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from .algebra import FullyAssembledRows, LocalIndices, PSparseMatrix, PVector, SubAssembledRows
+
+
+class _Lagrangian:
+    pass
+
+
+lagrangian = _Lagrangian()
+Float64 = np.float64
+
+
+class ReferenceFE:
+    """ReferenceFE(lagrangian, Float64, order)."""
+
+    def __init__(self, kind, T, order):
+        if not isinstance(kind, _Lagrangian) or T is not np.float64:
+            raise NotImplementedError("only scalar Lagrangian Float64 reference FEs are lowered to the GPU")
+        if order not in (1, 2):
+            raise NotImplementedError("only order 1 and 2 are supported")
+        self.order = int(order)
+
+
+class LinearFunction:
+    """u(x) = c0 + c . x (Dirichlet data of the supported vocabulary)."""
+
+    def __init__(self, c0=0.0, c=(0.0, 0.0, 0.0)):
+        c = tuple(float(v) for v in c) + (0.0,) * (3 - len(c))
+        self.coef = (float(c0),) + c[:3]
+
+    def __call__(self, x):
+        x = np.atleast_2d(x)
+        return self.coef[0] + sum(self.coef[1 + d] * x[:, d] for d in range(x.shape[1]))
+
+
+class FESpace:
+    """TestFESpace(model, reffe; dirichlet_tags="boundary") (src/FESpaces.jl:570-614)."""
+
+    def __init__(self, model, reffe, dirichlet_tags=None):
+        if dirichlet_tags not in (None, "boundary"):
+            raise NotImplementedError("only dirichlet_tags='boundary' (or none) is lowered to the GPU")
+        self.model, self.reffe = model, reffe
+        self.dirichlet = dirichlet_tags == "boundary"
+
+
+TestFESpace = FESpace
+
+
+class TrialFESpace:
+    """TrialFESpace(V, uD) / TrialFESpace(uD, V) (src/FESpaces.jl:396-401)."""
+
+    def __init__(self, V, uD=None):
+        if not isinstance(V, FESpace):
+            V, uD = uD, V
+        self.space = V
+        self.uD = uD if uD is not None else LinearFunction()
+        if not isinstance(self.uD, LinearFunction):
+            raise NotImplementedError("Dirichlet data must be a LinearFunction on the GPU path")
+
+    @property
+    def model(self):
+        return self.space.model
+
+
+class ConstantCoefficient:
+    def __init__(self, value=1.0):
+        self.value = float(value)
+
+
+class FourierExpCoefficient:
+    """kappa(x) = scale * exp(sum_m a_m cos(2 pi k_m . x + phi_m)); modes rows (k1,k2,k3,a,phi)."""
+
+    def __init__(self, modes, scale=1.0):
+        self.modes = np.ascontiguousarray(np.asarray(modes, np.float64).reshape(-1, 5))
+        self.scale = float(scale)
+
+
+class ConstantSource:
+    def __init__(self, value=0.0):
+        self.value = float(value)
+
+
+class ManufacturedSource:
+    """f = -div(kappa grad uD) for the linear Dirichlet data uD."""
+
+
+class Diffusion:
+    def __init__(self, kappa=None):
+        self.kappa = kappa if kappa is not None else ConstantCoefficient(1.0)
+
+
+class Source:
+    def __init__(self, f=None):
+        self.f = f if f is not None else ConstantSource(0.0)
+
+
+class Integral:
+    """int(integrand) dOmega -- a DistributedDomainContribution in the reference."""
+
+    def __init__(self, integrand, measure):
+        self.integrand, self.measure = integrand, measure
+
+
+def collect_cell_matrix_and_vector(U, V, a, l, uhd=None):
+    """Collect the (lazy) cell data of the bilinear and linear forms
+    (src/FESpaces.jl:703-715).  Here the cell data stays symbolic; it is lowered
+    to a problem descriptor and integrated on the GPU."""
+    if not (isinstance(a, Integral) and isinstance(a.integrand, Diffusion)):
+        raise NotImplementedError("bilinear form outside the GPU vocabulary")
+    if not (isinstance(l, Integral) and isinstance(l.integrand, Source)):
+        raise NotImplementedError("linear form outside the GPU vocabulary")
+    if a.measure.degree != l.measure.degree:
+        raise NotImplementedError("both forms must use the same Measure")
+    return dict(U=U, V=V, a=a, l=l)
+
+
+def problem_desc(data, strategy):
+    """Lower collected data to the C descriptor (gdx_problem_desc)."""
+    U, V, a, l = data["U"], data["V"], data["a"], data["l"]
+    m = V.model
+    d = _lib.ProblemDesc()
+    d.dim, d.order, d.qdeg = m.dim, V.reffe.order, a.measure.degree
+    d.strategy = strategy.code
+    for k in range(3):
+        d.cells[k] = m.cells[k] if k < m.dim else 1
+        d.parts[k] = m.parts[k] if k < m.dim else 1
+    dom = list(m.domain) + [0.0, 1.0] * (3 - m.dim)
+    for k in range(6):
+        d.domain[k] = dom[k]
+    kap = a.integrand.kappa
+    keep = None
+    if isinstance(kap, ConstantCoefficient):
+        d.coef_const, d.nmodes, d.modes = kap.value, 0, None
+    elif isinstance(kap, FourierExpCoefficient):
+        keep = kap.modes
+        d.coef_const, d.nmodes = kap.scale, keep.shape[0]
+        d.modes = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if keep.size else None
+    else:
+        raise NotImplementedError("coefficient outside the GPU vocabulary")
+    f = l.integrand.f
+    if isinstance(f, ConstantSource):
+        d.rhs_kind, d.rhs_const = _lib.RHS_CONST, f.value
+    elif isinstance(f, ManufacturedSource):
+        d.rhs_kind, d.rhs_const = _lib.RHS_MANUFACTURED, 0.0
+    else:
+        raise NotImplementedError("source outside the GPU vocabulary")
+    d.dir_kind = 1 if V.dirichlet else 0
+    for k in range(4):
+        d.dir_coef[k] = U.uD.coef[k]
+    return d, keep
+
+
+class GpuProblem:
+    """Owner of one gdx_problem handle."""
+
+    def __init__(self, ctx, desc, keep=None):
+        self._keep = keep
+        h = ctypes.c_void_p()
+        _lib.call("gdx_problem_create", ctx.handle, ctypes.byref(desc), ctypes.byref(h))
+        self.handle = h
+        self.symbolic_done = False
+
+    def symbolic(self):
+        _lib.call("gdx_symbolic", self.handle)
+        self.symbolic_done = True
+        n = ctypes.c_int32()
+        _lib.call("gdx_num_parts", self.handle, ctypes.byref(n))
+        g = ctypes.c_int64()
+        _lib.call("gdx_num_global_dofs", self.handle, ctypes.byref(g))
+        self.nparts, self.ngdofs = n.value, g.value
+
+    def numeric(self, precision=_lib.PREC_FP64):
+        _lib.call("gdx_numeric", self.handle, int(precision))
+
+    def sync(self):
+        _lib.call("gdx_sync", self.handle)
+
+    def phase_times(self):
+        t = (ctypes.c_double * 4)()
+        _lib.call("gdx_phase_times", self.handle, t, 4)
+        return list(t)
+
+    def owned_nnz(self):
+        n = ctypes.c_int64()
+        _lib.call("gdx_owned_nnz", self.handle, ctypes.byref(n))
+        return n.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _lib.call("gdx_stream", self.handle, ctypes.byref(s))
+        return s.value
+
+    def sizes(self, part):
+        s = (ctypes.c_int64 * _lib.SZ_COUNT)()
+        _lib.call("gdx_part_sizes", self.handle, int(part), s)
+        return list(s)
+
+    def export_part(self, part):
+        nl, no, ngr, ngc, nnz = self.sizes(part)
+        P = dict(n_ldofs=nl, n_own=no, n_ghost_rows=ngr, n_ghost_cols=ngc, nnz=nnz)
+        P["ldof_gid"] = np.empty(nl, np.int64)
+        P["ldof_owner"] = np.empty(nl, np.int32)
+        P["ldof_gnode"] = np.empty(nl, np.int64)
+        _lib.call("gdx_export_fe", self.handle, int(part), P["ldof_gid"].ctypes.data, P["ldof_owner"].ctypes.data,
+                  P["ldof_gnode"].ctypes.data)
+        nr, nc = no + ngr, no + ngc
+        for k, n, t in (("row_gid", nr, np.int64), ("row_owner", nr, np.int32), ("col_gid", nc, np.int64),
+                        ("col_owner", nc, np.int32), ("rowptr", nr + 1, np.int64), ("colidx", nnz, np.int32),
+                        ("val", nnz, np.float64), ("b", nr, np.float64)):
+            P[k] = np.empty(n, t)
+        _lib.call("gdx_export_rows", self.handle, int(part), P["row_gid"].ctypes.data, P["row_owner"].ctypes.data)
+        _lib.call("gdx_export_cols", self.handle, int(part), P["col_gid"].ctypes.data, P["col_owner"].ctypes.data)
+        _lib.call("gdx_export_csr", self.handle, int(part), P["rowptr"].ctypes.data, P["colidx"].ctypes.data,
+                  P["val"].ctypes.data)
+        _lib.call("gdx_export_rhs", self.handle, int(part), P["b"].ctypes.data)
+        return P
+
+    def close(self):
+        if self.handle:
+            _lib.call("gdx_problem_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SparseMatrixAssembler:
+    """SparseMatrixAssembler(U, V[, strategy]) (src/FESpaces.jl:819-862).
+
+    Default strategy SubAssembledRows; local matrices are CSR (0-based)."""
+
+    def __init__(self, U, V, strategy=None):
+        self.U, self.V = U, V
+        self.strategy = strategy if strategy is not None else SubAssembledRows()
+        if not isinstance(self.strategy, (SubAssembledRows, FullyAssembledRows)):
+            raise ValueError("strategy must be SubAssembledRows() or FullyAssembledRows()")
+        self.problem = None
+        self._key = None
+
+    def _problem_for(self, data):
+        desc, keep = problem_desc(data, self.strategy)
+        key = (bytes(desc), None if keep is None else keep.tobytes())
+        if self.problem is None or key != self._key:
+            if self.problem is not None:
+                self.problem.close()
+            self.problem = GpuProblem(self.V.model.ctx, desc, keep)
+            self.problem.symbolic()
+            self._key = key
+        return self.problem
+
+    def export(self):
+        prob = self.problem
+        mats, rows, cols, bs = [], [], [], []
+        for r in range(prob.nparts):
+            P = prob.export_part(r)
+            no = P["n_own"]
+            rows.append(LocalIndices(r, prob.ngdofs, P["row_gid"][:no], P["row_gid"][no:], P["row_owner"][no:]))
+            cols.append(LocalIndices(r, prob.ngdofs, P["col_gid"][:no], P["col_gid"][no:], P["col_owner"][no:]))
+            nr, nc = len(P["row_gid"]), len(P["col_gid"])
+            mats.append(sp.csr_matrix((P["val"], P["colidx"], P["rowptr"]), shape=(nr, nc)))
+            bs.append(P["b"])
+        return PSparseMatrix(mats, rows, cols), PVector(bs, rows)
+
+
+def assemble_matrix_and_vector(assem, data, export=True):
+    """Fresh assembly: symbolic (once per assembler) + numeric on the GPU."""
+    prob = assem._problem_for(data)
+    prob.numeric()
+    prob.sync()
+    return assem.export() if export else None
+
+
+def allocate_matrix_and_vector(assem, data):
+    """Symbolic phase only: matrix/vector with the final structure, zero values."""
+    assem._problem_for(data)
+    return assem.export()
+
+
+def assemble_matrix_and_vector_(A, b, assem, data):
+    """In-place numeric re-assembly (assemble_matrix_and_vector! in the
+    reference, src/Algebra.jl:323-393,530-535): refills the device values and
+    updates A and b's local values."""
+    prob = assem._problem_for(data)
+    prob.numeric()
+    prob.sync()
+    for r in range(prob.nparts):
+        P = prob.export_part(r)
+        A.matrix_partition[r].data[:] = P["val"]
+        b.partition[r][:] = P["b"]
+    return A, b
+
+
+class AffineFEOperator:
+    """AffineFEOperator(a, l, U, V[, assem]): assembles A and b on construction."""
+
+    def __init__(self, a, l, U, V, assem=None):
+        self.assem = assem if assem is not None else SparseMatrixAssembler(U, V)
+        data = collect_cell_matrix_and_vector(U, V, a, l, None)
+        self.matrix, self.vector = assemble_matrix_and_vector(self.assem, data)
+        self.trial, self.test = U, V
+
+    def get_matrix(self):
+        return self.matrix
+
+    def get_vector(self):
+        return self.vector

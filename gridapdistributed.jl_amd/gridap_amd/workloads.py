@@ -1,0 +1,69 @@
+"""Problem descriptors for the assembly workloads.
+
+Each descriptor is a plain dict describing a Poisson problem on a partitioned
+Cartesian mesh, in the vocabulary of the reference's drivers
+(test/PoissonTests.jl:10-45, test/FESpacesTests.jl:73-88):
+
+  dim, order       Lagrangian Q_order reference FE (ReferenceFE(lagrangian,Float64,order))
+  qdeg             Measure(Omega, qdeg): div(qdeg,2)+1 Gauss points per direction
+  cells, domain    CartesianDiscreteModel(ranks, parts, domain, cells)
+  parts            mesh partition (x-fastest rank numbering, src/PArraysExtras.jl:23)
+  strategy         "sub" (SubAssembledRows) or "full" (FullyAssembledRows)
+  coef_const,modes kappa(x) = coef_const * exp(sum_m a_m cos(2 pi k_m.x + phi_m))
+  rhs, rhs_const   "const": f = rhs_const; "manufactured": f = -div(kappa grad uD)
+  dirichlet, uD    Dirichlet on the whole boundary with uD = c0 + c1 x + c2 y + c3 z
+"""
+import numpy as np
+
+
+def fourier_modes(seed, nmodes=16, kmax=8, sigma=0.5):
+    """Seeded random Fourier modes (k1,k2,k3,a,phi): k_i uniform in 1..kmax,
+    a ~ N(0, sigma^2), phi ~ U[0, 2 pi)."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, kmax + 1, size=(nmodes, 3)).astype(np.float64)
+    a = rng.normal(0.0, sigma, size=nmodes)
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=nmodes)
+    return np.column_stack([k, a, phi])
+
+
+def poisson2d_q1(n=256, parts=(2, 2), strategy="sub"):
+    """2D Poisson on (0,1)^2, Q1, quadrature degree 2, u = x + y, f = 0."""
+    return dict(dim=2, order=1, qdeg=2, cells=(n, n), parts=tuple(parts), domain=(0.0, 1.0, 0.0, 1.0),
+                strategy=strategy, coef_const=1.0, modes=np.zeros((0, 5)), rhs="const", rhs_const=0.0,
+                dirichlet=True, uD=(0.0, 1.0, 1.0, 0.0))
+
+
+def poisson3d_q2_fourier(n=64, parts=(2, 2, 2), seed=2024, strategy="sub"):
+    """3D Poisson on (0,1)^3, Q2, quadrature degree 4, kappa = exp(g) with g a
+    seeded sum of 16 Fourier modes, uD = x + y + z, f = -div(kappa grad uD)."""
+    return dict(dim=3, order=2, qdeg=4, cells=(n, n, n), parts=tuple(parts),
+                domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy=strategy, coef_const=1.0,
+                modes=fourier_modes(seed), rhs="manufactured", rhs_const=0.0, dirichlet=True,
+                uD=(0.0, 1.0, 1.0, 1.0))
+
+
+def poisson3d_q1(n, parts=(1, 1, 1), strategy="sub", modes=None, rhs_const=1.0):
+    """3D Poisson Q1 on (0,1)^3, quadrature degree 2, homogeneous Dirichlet, f = rhs_const."""
+    return dict(dim=3, order=1, qdeg=2, cells=(n, n, n) if np.isscalar(n) else tuple(n), parts=tuple(parts),
+                domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy=strategy, coef_const=1.0,
+                modes=np.zeros((0, 5)) if modes is None else modes, rhs="const", rhs_const=rhs_const,
+                dirichlet=True, uD=(0.0, 0.0, 0.0, 0.0))
+
+
+def free_dofs(p):
+    """Closed form for the number of free dofs with Dirichlet on the whole boundary."""
+    K = p["order"]
+    n = 1
+    for c in p["cells"]:
+        n *= K * c - 1
+    return n
+
+
+def nnz_free(p):
+    """Closed form for nnz(free x free) with Dirichlet on the whole boundary:
+    Q1: prod(3 n_d - 5), Q2: prod(8 n_d - 9)."""
+    K = p["order"]
+    n = 1
+    for c in p["cells"]:
+        n *= (3 * c - 5) if K == 1 else (8 * c - 9)
+    return n

@@ -1,0 +1,152 @@
+/*
+ * gdx.h -- C ABI of the MI355X-native assembly library (libgdx.so).
+ *
+ * The library replaces, for Lagrangian Q1/Q2 problems on a partitioned
+ * CartesianDiscreteModel, the reference's assembly hot path:
+ *
+ *   SparseMatrixAssembler(U,V,strategy)       src/FESpaces.jl:819-862
+ *   assemble_matrix_and_vector(assem,data)    Gridap generic loop driven by
+ *     symbolic_loop_matrix_and_vector!        src/FESpaces.jl:785-792
+ *     numeric_loop_matrix_and_vector!         src/FESpaces.jl:794-798
+ *     create_from_nz (COO exchange+compress)  src/Algebra.jl:593-646,778-793
+ *   assemble_matrix_and_vector!(A,b,...)      src/Algebra.jl:323-393,530-535
+ *   generate_gids                             src/FESpaces.jl:139-261
+ *   _uniform_partition                        src/PArraysExtras.jl:5-84
+ *
+ * All parts of the partition live on one GPU.  The library owns every device
+ * buffer; host arrays passed in are caller-owned and copied.  Indices in the
+ * ABI are 0-based.  Every entry point returns GDX_OK (0) or a negative error
+ * class; gdx_last_error() returns a thread-local message.  A context and the
+ * problems created on it must be used from one host thread at a time.
+ */
+#ifndef GDX_H
+#define GDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDX_VERSION_MAJOR 0
+#define GDX_VERSION_MINOR 1
+#define GDX_VERSION_PATCH 0
+
+/* status codes */
+#define GDX_OK 0
+#define GDX_ERR_ARG (-1)         /* invalid argument (the reference's @check / @assert) */
+#define GDX_ERR_HIP (-2)         /* HIP runtime error */
+#define GDX_ERR_STATE (-3)       /* call out of order (e.g. numeric before symbolic) */
+#define GDX_ERR_UNSUPPORTED (-4) /* valid request outside the supported vocabulary (@notimplemented) */
+#define GDX_ERR_ALLOC (-5)       /* device or host allocation failed */
+
+/* assembly strategies: src/Algebra.jl:397-398 */
+#define GDX_SUB_ASSEMBLED_ROWS 0
+#define GDX_FULLY_ASSEMBLED_ROWS 1
+
+/* numeric precision of the cell contractions; the CSR accumulation is FP64 */
+#define GDX_PREC_FP64 0
+
+/* right-hand side kinds */
+#define GDX_RHS_CONST 0        /* f = rhs_const */
+#define GDX_RHS_MANUFACTURED 1 /* f = -div(kappa grad uD) for the linear uD below */
+
+typedef struct gdx_ctx gdx_ctx;
+typedef struct gdx_problem gdx_problem;
+
+/*
+ * Problem descriptor: the restricted weak-form vocabulary the GPU path lowers.
+ *   a(u,v) = int( kappa * grad(v) . grad(u) ) dOmega
+ *   l(v)   = int( f * v ) dOmega
+ * on CartesianDiscreteModel(ranks, parts, domain, cells) with
+ * ReferenceFE(lagrangian, Float64, order), Measure(Omega, qdeg) and
+ * TrialFESpace(uD, V) with V = TestFESpace(model, reffe; dirichlet_tags="boundary")
+ * (test/FESpacesTests.jl:12-27, test/PoissonTests.jl:14-45).
+ *   kappa(x) = coef_const * exp( sum_m a_m cos(2 pi k_m . x + phi_m) )
+ *   modes: nmodes rows of (k1, k2, k3, a, phi)
+ *   uD(x)  = dir_coef[0] + dir_coef[1] x + dir_coef[2] y + dir_coef[3] z
+ */
+typedef struct gdx_problem_desc {
+  int32_t dim;      /* 2 or 3 */
+  int32_t order;    /* 1 or 2 */
+  int32_t qdeg;     /* quadrature degree: qdeg/2+1 Gauss points per direction (<= 4) */
+  int32_t strategy; /* GDX_SUB_ASSEMBLED_ROWS or GDX_FULLY_ASSEMBLED_ROWS */
+  int64_t cells[3];
+  int32_t parts[3];
+  double domain[6]; /* x0 x1 y0 y1 z0 z1 */
+  double coef_const;
+  int32_t nmodes;
+  const double *modes;
+  int32_t rhs_kind;
+  double rhs_const;
+  int32_t dir_kind; /* 0: no Dirichlet boundary, 1: Dirichlet on the whole boundary */
+  double dir_coef[4];
+} gdx_problem_desc;
+
+/* sizes reported per part by gdx_part_sizes (index into the int64 array) */
+#define GDX_SZ_NLDOFS 0      /* free local dofs of the FE space (own + ghost) */
+#define GDX_SZ_NOWN 1        /* owned dofs = owned rows = owned cols */
+#define GDX_SZ_NGHOST_ROWS 2 /* ghost rows of the matrix row PRange */
+#define GDX_SZ_NGHOST_COLS 3 /* ghost cols of the matrix col PRange */
+#define GDX_SZ_NNZ 4         /* stored entries of the local CSR */
+#define GDX_SZ_COUNT 5
+
+/* library */
+int gdx_version(int32_t *major, int32_t *minor, int32_t *patch);
+const char *gdx_last_error(void);
+/* replaces with_debug()/distribute: one context per GPU device */
+int gdx_init(int32_t device, gdx_ctx **out);
+int gdx_finalize(gdx_ctx *ctx);
+
+/* FE space + assembler construction.
+   Replaces CartesianDiscreteModel(ranks,parts,...) (src/Geometry.jl:378-411),
+   FESpace/TrialFESpace (src/FESpaces.jl:570-614,396-401) and
+   SparseMatrixAssembler(U,V,strategy) (src/FESpaces.jl:819-862).
+   ctx == NULL makes a host-only problem: gdx_symbolic and the structure
+   exports work (values and RHS export as zeros, like allocate_matrix_and_vector),
+   gdx_numeric returns GDX_ERR_STATE. */
+int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem **out);
+int gdx_problem_destroy(gdx_problem *prob);
+
+/* Symbolic phase, run once: partition, dof numbering and generate_gids
+   (src/FESpaces.jl:139-261), row/col PRanges (src/Algebra.jl:1136-1257),
+   local CSR pattern and the cell-dof -> CSR position map
+   (src/Algebra.jl:410-418,849-857 and the compression of :636). */
+int gdx_symbolic(gdx_problem *prob);
+
+/* Numeric phase: fills the CSR values and the RHS on the device.  Replaces
+   numeric_loop_matrix_and_vector! (src/FESpaces.jl:794-798) followed by the
+   owner/ghost exchange and compression (src/Algebra.jl:593-646, :1048-1131)
+   or, on an allocated matrix, assemble_matrix_and_vector! + assemble!
+   (src/Algebra.jl:530-535,755-758).  Ghost rows end with zero values. */
+int gdx_numeric(gdx_problem *prob, int32_t precision);
+
+/* queries / exports (copies device data into caller arrays) */
+int gdx_num_parts(gdx_problem *prob, int32_t *nparts);
+int gdx_num_global_dofs(gdx_problem *prob, int64_t *ngdofs);
+int gdx_part_sizes(gdx_problem *prob, int32_t part, int64_t *sizes /* [GDX_SZ_COUNT] */);
+/* FE dof PRange of a part (LocalIndices of src/FESpaces.jl:249-255) plus the
+   global lattice node of each free local dof */
+int gdx_export_fe(gdx_problem *prob, int32_t part, int64_t *ldof_gid, int32_t *ldof_owner, int64_t *ldof_gnode);
+/* row / col PRanges: own then ghost (OwnAndGhostIndices, src/Algebra.jl:1240-1252) */
+int gdx_export_rows(gdx_problem *prob, int32_t part, int64_t *gid, int32_t *owner);
+int gdx_export_cols(gdx_problem *prob, int32_t part, int64_t *gid, int32_t *owner);
+/* local CSR (SparseMatrixCSR{0,Float64,Int} layout, test/PoissonTests.jl:40) */
+int gdx_export_csr(gdx_problem *prob, int32_t part, int64_t *rowptr, int32_t *colidx, double *values);
+/* RHS on the row PRange of A (src/Algebra.jl:720-753) */
+int gdx_export_rhs(gdx_problem *prob, int32_t part, double *b);
+
+/* device views and timing */
+int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);
+int gdx_stream(gdx_problem *prob, void **stream);
+int gdx_sync(gdx_problem *prob);
+/* per-phase device times (ms) of the last gdx_numeric, measured with hipEvents
+   on the library's stream: [0] coefficient, [1] rhs, [2] matrix, [3] total */
+int gdx_phase_times(gdx_problem *prob, double *ms, int32_t n);
+/* number of owned free x free stored entries (the Gnnz/s numerator) */
+int gdx_owned_nnz(gdx_problem *prob, int64_t *nnz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDX_H */

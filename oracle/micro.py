@@ -1,0 +1,175 @@
+"""Brute-force numpy micro-oracle (test infrastructure only).
+
+A second, independent CPU restatement of the cell integrals of the assembly
+path, used to pin the C oracle (gdx_oracle.c) and, row by row, the GPU result
+at full size.  It shares no code with the C oracle: Gauss points come from
+numpy.polynomial.legendre.leggauss, the Lagrange basis from numpy polynomials,
+the geometry from the affine box map, and assembly is keyed by global lattice
+node (no partitioning, no dof numbering), so it is insensitive to Gridap's
+numbering.
+
+It follows the integrand of the reference's drivers
+(test/FESpacesTests.jl:16-17, test/PoissonTests.jl:34-37):
+  a(u,v) = int kappa grad(u).grad(v),  l(v) = int f v,  b -= A u_D (lifting,
+  src/FESpaces.jl:703-715).
+Only tests/ may import it.
+"""
+import numpy as np
+from numpy.polynomial import legendre as npleg
+from numpy.polynomial import polynomial as nppoly
+
+
+def gauss01(n):
+    x, w = npleg.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def lagrange01(K):
+    """Coefficient arrays of the 1D Lagrange basis on nodes j/K (tensor order)."""
+    nodes = np.arange(K + 1) / K
+    basis = []
+    for a in range(K + 1):
+        c = np.array([1.0])
+        for j in range(K + 1):
+            if j != a:
+                c = nppoly.polymul(c, np.array([-nodes[j], 1.0]) / (nodes[a] - nodes[j]))
+        basis.append(c)
+    return basis
+
+
+def kappa_f(p, X):
+    """kappa and f at points X (n, 3)."""
+    modes = np.asarray(p.get("modes", np.zeros((0, 5))), np.float64).reshape(-1, 5)
+    th = 2 * np.pi * X @ modes[:, :3].T + modes[:, 4]
+    g = np.cos(th) @ modes[:, 3] if len(modes) else np.zeros(len(X))
+    kap = p.get("coef_const", 1.0) * np.exp(g)
+    if p.get("rhs") == "manufactured":
+        c = np.asarray(p["uD"][1:4])
+        # f = -div(kappa grad uD) = -kappa grad(g).c
+        dg = -(np.sin(th) * modes[:, 3]) * 2 * np.pi
+        f = -kap * (dg @ (modes[:, :3] @ c)) if len(modes) else np.zeros(len(X))
+    else:
+        f = np.full(len(X), float(p.get("rhs_const", 0.0)))
+    return kap, f
+
+
+def _geometry(p):
+    D, K = p["dim"], p["order"]
+    cells = np.array(list(p["cells"]) + [1] * (3 - D))
+    dom = np.array(list(p["domain"]) + [0.0, 1.0] * (3 - D)).reshape(3, 2)
+    h = (dom[:, 1] - dom[:, 0]) / cells
+    nn = np.where(np.arange(3) < D, K * cells + 1, 1)
+    return D, K, cells, dom, h, nn
+
+
+def element(p, cell):
+    """Element matrix/vector of one cell in tensor node order (x fastest),
+    plus the global node ids of the cell nodes."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    nq = p["qdeg"] // 2 + 1
+    xq, wq = gauss01(nq)
+    B = lagrange01(K)
+    phi = np.array([nppoly.polyval(xq, c) for c in B])             # [a, q]
+    dphi = np.array([nppoly.polyval(xq, nppoly.polyder(c)) for c in B])
+    n1 = K + 1
+    # tensor node list
+    idx = np.array(np.meshgrid(*[np.arange(n1) if d < D else np.array([0]) for d in range(3)], indexing="ij"))
+    idx = idx.reshape(3, -1).T  # (i,j,k) with i fastest after reordering below
+    order = np.lexsort((idx[:, 0], idx[:, 1], idx[:, 2]))
+    idx = idx[order]
+    qidx = np.array(np.meshgrid(*[np.arange(nq) if d < D else np.array([0]) for d in range(3)], indexing="ij"))
+    qidx = qidx.reshape(3, -1).T
+    qidx = qidx[np.lexsort((qidx[:, 0], qidx[:, 1], qidx[:, 2]))]
+    # quadrature points in physical space and weights (affine box map)
+    X = np.zeros((len(qidx), 3))
+    w = np.ones(len(qidx))
+    for d in range(D):
+        X[:, d] = dom[d, 0] + h[d] * (cell[d] + xq[qidx[:, d]])
+        w *= wq[qidx[:, d]] * h[d]
+    kap, f = kappa_f(p, X)
+    # basis values and physical gradients at quadrature points
+    Phi = np.ones((len(idx), len(qidx)))
+    G = np.ones((D, len(idx), len(qidx)))
+    for d in range(D):
+        Phi *= phi[idx[:, d]][:, qidx[:, d]]
+        for e in range(D):
+            G[e] *= (dphi[idx[:, d]][:, qidx[:, d]] / h[d]) if d == e else phi[idx[:, d]][:, qidx[:, d]]
+    A = np.einsum("eiq,ejq,q->ij", G, G, kap * w)
+    bvec = Phi @ (f * w)
+    gnode = (K * cell[0] + idx[:, 0]) + nn[0] * ((K * cell[1] + idx[:, 1]) + nn[1] * (K * cell[2] + idx[:, 2]))
+    return A, bvec, gnode
+
+
+def node_coords(p, gnode):
+    D, K, cells, dom, h, nn = _geometry(p)
+    g = np.atleast_1d(gnode)
+    ijk = np.stack([g % nn[0], (g // nn[0]) % nn[1], g // (nn[0] * nn[1])], 1)
+    X = np.zeros((len(g), 3))
+    for d in range(D):
+        X[:, d] = dom[d, 0] + h[d] * ijk[:, d] / K
+    return ijk, X
+
+
+def is_dirichlet(p, gnode):
+    D, K, cells, dom, h, nn = _geometry(p)
+    if not p.get("dirichlet", True):
+        return np.zeros(len(np.atleast_1d(gnode)), bool)
+    ijk, _ = node_coords(p, gnode)
+    out = np.zeros(len(ijk), bool)
+    for d in range(D):
+        out |= (ijk[:, d] == 0) | (ijk[:, d] == nn[d] - 1)
+    return out
+
+
+def uD(p, gnode):
+    _, X = node_coords(p, gnode)
+    c = p.get("uD", (0, 0, 0, 0))
+    return c[0] + X @ np.asarray(c[1:4], np.float64)
+
+
+def row(p, gnode_i):
+    """Assembled row of the free node gnode_i: ({gnode_j: A_ij} over free j, b_i)."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    ijk, _ = node_coords(p, gnode_i)
+    ijk = ijk[0]
+    ranges = []
+    for d in range(3):
+        if d >= D:
+            ranges.append([0])
+            continue
+        cmin, cmax = max(-(-ijk[d] // K) - 1, 0), min(ijk[d] // K, cells[d] - 1)
+        ranges.append(list(range(cmin, cmax + 1)))
+    vals, bi = {}, 0.0
+    for cz in ranges[2]:
+        for cy in ranges[1]:
+            for cx in ranges[0]:
+                A, bv, gn = element(p, (cx, cy, cz))
+                a = int(np.nonzero(gn == gnode_i)[0][0])
+                dirm = is_dirichlet(p, gn)
+                ud = np.where(dirm, uD(p, gn), 0.0)
+                bi += bv[a] - A[a] @ ud
+                for j, g in enumerate(gn):
+                    if not dirm[j]:
+                        vals[int(g)] = vals.get(int(g), 0.0) + A[a, j]
+    return vals, bi
+
+
+def assemble_global(p):
+    """Global free x free matrix keyed by lattice node (dense dict of rows) and RHS."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    I, J, V, bI, bV = [], [], [], [], []
+    for cz in range(cells[2]):
+        for cy in range(cells[1]):
+            for cx in range(cells[0]):
+                A, bv, gn = element(p, (cx, cy, cz))
+                dirm = is_dirichlet(p, gn)
+                ud = np.where(dirm, uD(p, gn), 0.0)
+                free = ~dirm
+                bv = bv - A @ ud
+                ii, jj = np.meshgrid(gn[free], gn[free], indexing="ij")
+                I.append(ii.ravel())
+                J.append(jj.ravel())
+                V.append(A[np.ix_(free, free)].ravel())
+                bI.append(gn[free])
+                bV.append(bv[free])
+    return np.concatenate(I), np.concatenate(J), np.concatenate(V), np.concatenate(bI), np.concatenate(bV)
