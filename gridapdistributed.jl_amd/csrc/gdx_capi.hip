@@ -1,6 +1,7 @@
 // C ABI of libgdx (declared in include/gdx.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -14,6 +15,7 @@ int launch_coefficients(gdx_problem *p, hipStream_t st);
 int launch_rhs(gdx_problem *p, hipStream_t st);
 int launch_matrix(gdx_problem *p, hipStream_t st);
 bool matrix_kernel_supported(int dim, int K, int NQ);
+int pattern_capacity_3d(int K, int NQ);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -55,7 +57,7 @@ int upload(T **dst, const std::vector<T> &src) {
 void free_device(gdx_problem *p) {
   gdx::Device &d = p->dev;
   void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_len, d.pat_dir,
-                  d.tiles};
+                  d.tiles, d.tile_pats};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   d = gdx::Device{};
@@ -188,6 +190,13 @@ int gdx_symbolic(gdx_problem *p) {
     return GDX_ERR_ARG;
   }
   gdx::fill_kparams(p);
+  {
+    // tiles whose position patterns fit in the matrix kernel's LDS go first
+    const int cap = p->D == 3 ? gdx::pattern_capacity_3d(p->K, p->NQ) : 0;
+    std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
+    p->n_fast_tiles = 0;
+    for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += t.npat <= cap ? 1 : 0;
+  }
   if (!p->ctx) {
     p->symbolic_done = true;
     return GDX_OK;
@@ -215,6 +224,7 @@ int gdx_symbolic(gdx_problem *p) {
   if ((rc = upload(&d.pat_len, p->pat_len))) return rc;
   if ((rc = upload(&d.pat_dir, p->pat_dir))) return rc;
   if ((rc = upload(&d.tiles, p->tiles))) return rc;
+  if ((rc = upload(&d.tile_pats, p->tile_pats))) return rc;
   for (hipEvent_t &e : p->ev)
     if (!e) GDX_HIP(hipEventCreate(&e));
   p->symbolic_done = true;

@@ -47,8 +47,11 @@ struct RowInfo {
   long long rowstart;  // offset of the row's first value in the global values buffer
   long long bidx;      // offset of the row in the global RHS buffer
   int pid;             // position pattern id, -1 when the node is not a free row
-  int pad;
+  int lenf;            // row length | has-Dirichlet-neighbours << 16 | tile-local pattern index << 17
 };
+
+// Pattern rows are padded to whole dwords (stride of pat_fwd / pat_inv).
+constexpr int pattern_stride(int S) { return (S + 3) & ~3; }
 
 // A tile of owned rows processed by one workgroup.
 struct Tile {
@@ -57,7 +60,9 @@ struct Tile {
   int ext[3];     // rows of the tile inside the owned node box
   int onb_lo[3];  // owned node box of the part
   int onb_n[3];
+  int npat;       // distinct position patterns of the tile's rows
   long long rowinfo_base;
+  long long pat_off;  // offset of the tile's pattern id list in tile_pats
 };
 
 // Problem-wide constants passed to the kernels by value.
@@ -66,8 +71,10 @@ struct KParams {
   int ncells[3];
   int qn[3];                     // qp lattice extents NQ*ncells
   int nn[3];                     // node lattice extents K*ncells+1
-  double x0[3], h[3];
+  double x0[3], h[3], hK[3];  // hK = h / K: node spacing
   double phi[kMaxK + 1][kMaxQ], dphi[kMaxK + 1][kMaxQ], qw[kMaxQ], qx[kMaxQ];
+  // 1D pair factors per dimension: G = w dphi_a dphi_b / h, M = w h phi_a phi_b
+  double fG[3][kMaxK + 1][kMaxK + 1][kMaxQ], fM[3][kMaxK + 1][kMaxK + 1][kMaxQ];
   double coef_const;
   int nmodes;
   int rhs_kind;
@@ -89,6 +96,7 @@ struct Device {
   int *pat_len = nullptr;
   int *pat_dir = nullptr;      // pattern has Dirichlet slots
   Tile *tiles = nullptr;
+  int *tile_pats = nullptr;    // per tile: its distinct pattern ids
 };
 
 }  // namespace gdx
@@ -114,6 +122,8 @@ struct gdx_problem {
   std::vector<int> pat_len, pat_dir;
   int npat = 0;
   std::vector<gdx::Tile> tiles;
+  std::vector<int> tile_pats;
+  int n_fast_tiles = 0;  // leading tiles whose position patterns fit in LDS
   int tileR[3] = {1, 1, 1};
   int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
   gdx::KParams kp{};

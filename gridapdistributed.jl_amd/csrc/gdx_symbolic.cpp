@@ -158,7 +158,11 @@ inline void shared_range(int K, int64_t i, int64_t clo, int64_t chi, int64_t &jl
 void tile_shape(int dim, int K, int R[3]) {
   // must match the template instances launched by launch_matrix
   if (dim == 3) {
-    R[0] = 8; R[1] = 8; R[2] = K == 2 ? 8 : 4;
+    if (K == 2) {
+      R[0] = 4; R[1] = 8; R[2] = 8;
+    } else {
+      R[0] = 8; R[1] = 8; R[2] = 2;
+    }
   } else {
     R[0] = 16; R[1] = 16; R[2] = 1;
   }
@@ -410,14 +414,16 @@ int symbolic(gdx_problem *p) {
         if (it == pat_ids.end()) {
           pid = (int)p->pat_len.size();
           pat_ids.emplace(key, pid);
-          std::vector<uint8_t> inv(S, 0);
+          const int SP = pattern_stride(S);
+          std::vector<uint8_t> inv(SP, 0), fwd(SP, kSlotAbsent);
           bool has_dir = false;
           for (int s = 0; s < S; ++s) {
             uint8_t v = (uint8_t)key[s];
             if (v == kSlotDirichlet) has_dir = true;
             else if (v != kSlotAbsent) inv[v] = (uint8_t)s;
-            p->pat_fwd.push_back(v);
+            fwd[s] = v;
           }
+          p->pat_fwd.insert(p->pat_fwd.end(), fwd.begin(), fwd.end());
           p->pat_inv.insert(p->pat_inv.end(), inv.begin(), inv.end());
           p->pat_len.push_back((int)ents.size());
           p->pat_dir.push_back(has_dir ? 1 : 0);
@@ -447,6 +453,7 @@ int symbolic(gdx_problem *p) {
             ri.rowstart = P.val_base + P.rowptr[o];
             ri.bidx = P.row_base + o;
             ri.pid = own_pid[o];
+            ri.lenf = p->pat_len[ri.pid] | (p->pat_dir[ri.pid] << 16);
           }
           p->rowinfo.push_back(ri);
         }
@@ -459,6 +466,7 @@ int symbolic(gdx_problem *p) {
   // tiles of owned rows
   tile_shape(c.D, K, p->tileR);
   p->tiles.clear();
+  p->tile_pats.clear();
   for (int r = 0; r < nparts; ++r) {
     const Part &P = p->P[r];
     int64_t nt[3];
@@ -477,15 +485,28 @@ int symbolic(gdx_problem *p) {
             t.onb_n[k] = (int)P.onb_n[k];
           }
           t.rowinfo_base = P.rowinfo_base;
-          for (int z = 0; z < t.ext[2] && !any; ++z)
-            for (int y = 0; y < t.ext[1] && !any; ++y)
-              for (int x = 0; x < t.ext[0] && !any; ++x) {
+          // distinct patterns of the tile; rows get their tile-local index
+          std::vector<int> pids;
+          t.pat_off = (long long)p->tile_pats.size();
+          for (int z = 0; z < t.ext[2]; ++z)
+            for (int y = 0; y < t.ext[1]; ++y)
+              for (int x = 0; x < t.ext[0]; ++x) {
                 int64_t li = ((int64_t)(t.org[2] - t.onb_lo[2] + z) * P.onb_n[1] + (t.org[1] - t.onb_lo[1] + y)) *
                                  P.onb_n[0] +
                              (t.org[0] - t.onb_lo[0] + x);
-                any = p->rowinfo[P.rowinfo_base + li].pid >= 0;
+                RowInfo &ri = p->rowinfo[P.rowinfo_base + li];
+                if (ri.pid < 0) continue;
+                any = true;
+                int l = (int)(std::find(pids.begin(), pids.end(), ri.pid) - pids.begin());
+                if (l == (int)pids.size()) pids.push_back(ri.pid);
+                if (l >= (1 << 14)) throw std::runtime_error("internal: too many patterns in one tile");
+                ri.lenf = (ri.lenf & 0x1FFFF) | (l << 17);
               }
-          if (any) p->tiles.push_back(t);
+          t.npat = (int)pids.size();
+          if (any) {
+            p->tile_pats.insert(p->tile_pats.end(), pids.begin(), pids.end());
+            p->tiles.push_back(t);
+          }
         }
   }
   return GDX_OK;
@@ -507,6 +528,7 @@ void fill_kparams(gdx_problem *p) {
     kp.nn[k] = k < p->D ? p->K * kp.ncells[k] + 1 : 1;
     kp.x0[k] = k < p->D ? d.domain[2 * k] : 0.0;
     kp.h[k] = k < p->D ? (d.domain[2 * k + 1] - d.domain[2 * k]) / (double)kp.ncells[k] : 1.0;
+    kp.hK[k] = kp.h[k] / p->K;
   }
   // Gauss-Legendre on [-1,1] by Newton on P_n, mapped to [0,1]
   const int n = p->NQ;
@@ -544,6 +566,13 @@ void fill_kparams(gdx_problem *p) {
       kp.phi[a][q] = v;
       kp.dphi[a][q] = dv;
     }
+  for (int d = 0; d < 3; ++d)
+    for (int a = 0; a <= K; ++a)
+      for (int b = 0; b <= K; ++b)
+        for (int q = 0; q < n; ++q) {
+          kp.fG[d][a][b][q] = kp.qw[q] * kp.dphi[a][q] * kp.dphi[b][q] / kp.h[d];
+          kp.fM[d][a][b][q] = kp.qw[q] * kp.h[d] * kp.phi[a][q] * kp.phi[b][q];
+        }
   kp.coef_const = d.coef_const;
   kp.nmodes = d.nmodes;
   kp.rhs_kind = d.rhs_kind;
