@@ -168,6 +168,7 @@ int gdx_problem_destroy(gdx_problem *p) {
   if (p->ctx) {
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    gdx::solve_free(p);
     free_device(p);
     for (hipEvent_t &e : p->ev)
       if (e) (void)hipEventDestroy(e);
@@ -387,6 +388,29 @@ int gdx_export_rhs(gdx_problem *p, int32_t part, double *b) {
   GDX_HIP(hipStreamSynchronize(p->ctx->stream));
   if (P.nrows > 0) GDX_HIP(hipMemcpy(b, p->dev.b + P.row_base, P.nrows * sizeof(double), hipMemcpyDeviceToHost));
   return GDX_OK;
+}
+
+int gdx_spmv(gdx_problem *p, const double *x, double *y) {
+  GDX_REQUIRE(p && x && y, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int rc = gdx::solve_spmv(p, x, y);
+  if (rc) set_error("gdx_spmv failed");
+  return rc;
+}
+
+int gdx_pcg(gdx_problem *p, double rtol, int32_t maxiter, double *x, int32_t *iters, double *relres, double *ms) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(rtol > 0.0 && maxiter >= 0, GDX_ERR_ARG, "rtol must be > 0 and maxiter >= 0");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int it = 0;
+  int rc = gdx::solve_pcg(p, rtol, maxiter, x, &it, relres, ms);
+  if (rc) set_error("gdx_pcg failed (singular diagonal or device error)");
+  if (iters) *iters = it;
+  return rc;
 }
 
 int gdx_device_values(gdx_problem *p, double **values, int64_t *nnz_total) {

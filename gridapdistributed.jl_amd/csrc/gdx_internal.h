@@ -130,6 +130,7 @@ struct gdx_problem {
   gdx::Device dev;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double phase_ms[4] = {0, 0, 0, 0};
+  void *solve = nullptr;  // solver state (gdx_solve.hip), built on the first solve
 };
 
 namespace gdx {
@@ -137,4 +138,7 @@ void set_error(const std::string &msg);
 int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
 void fill_kparams(gdx_problem *p);
 void tile_shape(int dim, int K, int R[3]);
+void solve_free(gdx_problem *p);
+int solve_spmv(gdx_problem *p, const double *x, double *y);
+int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *x, int *iters, double *relres, double *ms);
 }  // namespace gdx

@@ -1,0 +1,305 @@
+// Solve side of libgdx: FP64 CSR SpMV over the owned rows of all parts and a
+// Jacobi-preconditioned conjugate gradient, operating in place on the values
+// the numeric assembly left in HBM.  The reference solves with A\b (sparse LU
+// of the gathered system, README.md:76); these kernels replace that step.
+//
+// Global view: owned rows in global-id order.  Because global ids are
+// rank-blocked and each part lists its owned rows in id order, row g of the
+// global system is owned row (g - first_p) of part p; its values sit at
+// rstart[g] in the concatenated values buffer and its column global ids in
+// cgid (int32), built once.  Reductions are two-pass (per-block partials then
+// one block in fixed order), so every solve is bit-for-bit repeatable.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "gdx_internal.h"
+
+namespace gdx {
+
+constexpr int kRedBlocks = 1024;  // partial sums per reduction
+constexpr int kBlk = 256;
+
+// y[g] = sum_k val[rstart[g]+k] * x[cgid[rstart[g]+k]] ; LPR lanes per row
+template <int LPR>
+__global__ __launch_bounds__(kBlk) void k_spmv(int n, const long long *__restrict__ rstart,
+                                               const int *__restrict__ rlen, const int *__restrict__ cgid,
+                                               const double *__restrict__ val, const double *__restrict__ x,
+                                               double *__restrict__ y) {
+  const int lane = threadIdx.x % LPR;
+  const int row = (blockIdx.x * kBlk + threadIdx.x) / LPR;
+  double acc = 0.0;
+  if (row < n) {
+    const long long s = rstart[row];
+    const int len = rlen[row];
+    for (int k = lane; k < len; k += LPR) acc = fma(val[s + k], x[cgid[s + k]], acc);
+  }
+#pragma unroll
+  for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+  if (row < n && lane == 0) y[row] = acc;
+}
+
+// partial sums of up to two dot products: block i sums a grid-stride slice
+__global__ __launch_bounds__(kBlk) void k_dot2(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                               const double *__restrict__ c, const double *__restrict__ d,
+                                               double *__restrict__ part) {
+  __shared__ double s1[kBlk], s2[kBlk];
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
+    p1 = fma(a[i], b[i], p1);
+    if (c) p2 = fma(c[i], d[i], p2);
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  __syncthreads();
+  for (int o = kBlk / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[kRedBlocks + blockIdx.x] = s2[0];
+  }
+}
+
+// finish the two reductions into out[0], out[1] (one block, fixed order)
+__global__ __launch_bounds__(kBlk) void k_finish2(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  __shared__ double s1[kBlk], s2[kBlk];
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlk) {
+    p1 += part[i];
+    p2 += part[kRedBlocks + i];
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  __syncthreads();
+  for (int o = kBlk / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s1[0];
+    out[1] = s2[0];
+  }
+}
+
+__global__ void k_gather_b(int n, const long long *__restrict__ bidx, const double *__restrict__ bsrc,
+                           double *__restrict__ b) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) b[i] = bsrc[bidx[i]];
+}
+
+__global__ void k_inv_diag(int n, const long long *__restrict__ dpos, const double *__restrict__ val,
+                           double *__restrict__ dinv) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) dinv[i] = 1.0 / val[dpos[i]];
+}
+
+// r = b - Ax (Ax in r on entry), z = dinv r, p = z
+__global__ void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ dinv,
+                          double *__restrict__ r, double *__restrict__ z, double *__restrict__ p) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) {
+    const double ri = b[i] - r[i];
+    r[i] = ri;
+    z[i] = dinv[i] * ri;
+    p[i] = z[i];
+  }
+}
+
+// alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = dinv r
+__global__ void k_cg_update(int n, const double *__restrict__ sc, double *__restrict__ x, double *__restrict__ r,
+                            double *__restrict__ z, const double *__restrict__ p, const double *__restrict__ q,
+                            const double *__restrict__ dinv) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) {
+    const double alpha = sc[0] / sc[2];
+    x[i] = fma(alpha, p[i], x[i]);
+    const double ri = fma(-alpha, q[i], r[i]);
+    r[i] = ri;
+    z[i] = dinv[i] * ri;
+  }
+}
+
+// p = z + (rz_new / rz_old) p
+__global__ void k_cg_dir(int n, const double *__restrict__ sc, const double *__restrict__ z, double *__restrict__ p) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) p[i] = fma(sc[4] / sc[0], p[i], z[i]);
+}
+
+// scalar bookkeeping on the device: sc[0]=rz, sc[1]=rr, sc[2]=pq, sc[4]=rz_new, sc[5]=rr_new
+__global__ void k_cg_roll(double *sc) {
+  sc[0] = sc[4];
+  sc[1] = sc[5];
+}
+
+namespace {
+
+struct SolveData {
+  int n = 0, lpr = 8;
+  long long *rstart = nullptr, *bidx = nullptr, *dpos = nullptr;
+  int *rlen = nullptr, *cgid = nullptr;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+  double *part = nullptr, *sc = nullptr;
+};
+
+}  // namespace
+
+static SolveData *solve_data(gdx_problem *p) { return reinterpret_cast<SolveData *>(p->solve); }
+
+void solve_free(gdx_problem *p) {
+  SolveData *s = solve_data(p);
+  if (!s) return;
+  void *ptrs[] = {s->rstart, s->bidx, s->dpos, s->rlen, s->cgid, s->x, s->b, s->r, s->z, s->p, s->q, s->dinv,
+                  s->part, s->sc};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  delete s;
+  p->solve = nullptr;
+}
+
+static int solve_setup(gdx_problem *p) {
+  if (p->solve) return GDX_OK;
+  const int64_t n = p->ngdofs;
+  if (n >= (int64_t)1 << 31) return GDX_ERR_UNSUPPORTED;
+  std::vector<long long> rstart(n), bidx(n), dpos(n);
+  std::vector<int> rlen(n), cgid(std::max<int64_t>(p->nnz_total, 1), 0);
+  int64_t maxlen = 0;
+  for (const Part &P : p->P) {
+    for (int64_t o = 0; o < P.n_own; ++o) {
+      const int64_t g = P.row_gid[o];
+      rstart[g] = P.val_base + P.rowptr[o];
+      rlen[g] = (int)(P.rowptr[o + 1] - P.rowptr[o]);
+      bidx[g] = P.row_base + o;
+      maxlen = std::max<int64_t>(maxlen, rlen[g]);
+      dpos[g] = -1;
+      for (int64_t k = P.rowptr[o]; k < P.rowptr[o + 1]; ++k) {
+        const int64_t cg = P.col_gid[P.colidx[k]];
+        cgid[P.val_base + k] = (int)cg;
+        if (cg == g) dpos[g] = P.val_base + k;
+      }
+      if (dpos[g] < 0) return GDX_ERR_STATE;
+    }
+  }
+  SolveData *s = new SolveData;
+  p->solve = s;
+  s->n = (int)n;
+  s->lpr = maxlen > 64 ? 32 : (maxlen > 32 ? 16 : 8);
+  auto up = [](auto **dst, const auto &v) -> hipError_t {
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(v.size(), 1) * sizeof(v[0]));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+  };
+  hipError_t e = hipSuccess;
+  if ((e = up(&s->rstart, rstart)) || (e = up(&s->bidx, bidx)) || (e = up(&s->dpos, dpos)) ||
+      (e = up(&s->rlen, rlen)) || (e = up(&s->cgid, cgid))) {
+    solve_free(p);
+    return GDX_ERR_ALLOC;
+  }
+  double **vecs[] = {&s->x, &s->b, &s->r, &s->z, &s->p, &s->q, &s->dinv};
+  for (double **v : vecs)
+    if (hipMalloc(reinterpret_cast<void **>(v), std::max<int64_t>(n, 1) * sizeof(double)) != hipSuccess) {
+      solve_free(p);
+      return GDX_ERR_ALLOC;
+    }
+  if (hipMalloc(reinterpret_cast<void **>(&s->part), 2 * kRedBlocks * sizeof(double)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&s->sc), 8 * sizeof(double)) != hipSuccess) {
+    solve_free(p);
+    return GDX_ERR_ALLOC;
+  }
+  return GDX_OK;
+}
+
+static void spmv(gdx_problem *p, SolveData *s, const double *x, double *y, hipStream_t st) {
+  const int rows_per_block = kBlk / s->lpr;
+  const dim3 g((s->n + rows_per_block - 1) / rows_per_block);
+  if (s->lpr == 8)
+    hipLaunchKernelGGL(k_spmv<8>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
+  else if (s->lpr == 16)
+    hipLaunchKernelGGL(k_spmv<16>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
+  else
+    hipLaunchKernelGGL(k_spmv<32>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
+}
+
+static void dot2(SolveData *s, const double *a, const double *b, const double *c, const double *d, double *out,
+                 hipStream_t st) {
+  const int nb = std::min(kRedBlocks, std::max(1, (s->n + kBlk - 1) / kBlk));
+  hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kBlk), 0, st, s->n, a, b, c, d, s->part);
+  hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlk), 0, st, s->part, nb, out);
+}
+
+int solve_spmv(gdx_problem *p, const double *xh, double *yh) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  if (hipMemcpyAsync(s->x, xh, s->n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  spmv(p, s, s->x, s->q, st);
+  if (hipMemcpyAsync(yh, s->q, s->n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
+  return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+// Jacobi-PCG from x0 = 0; stops when the recursive residual meets rtol, then
+// reports the true FP64 residual ||b - A x|| / ||b|| recomputed explicitly.
+int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, double *relres, double *ms) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  const int n = s->n;
+  const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
+  auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+  hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
+  (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
+  (void)hipMemsetAsync(s->r, 0, n * sizeof(double), st);  // A x0 = 0
+  hipLaunchKernelGGL(k_cg_init, g, b, 0, st, n, s->b, s->dinv, s->r, s->z, s->p);
+  dot2(s, s->r, s->z, s->b, s->b, s->sc, st);  // sc[0] = rz, sc[1] = bb
+  double h[2];
+  if (hipMemcpyAsync(h, s->sc, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GDX_ERR_HIP;
+  const double bnorm = std::sqrt(h[1]);
+  int it = 0;
+  const int check_every = 5;
+  if (bnorm > 0.0) {
+    for (it = 1; it <= maxiter; ++it) {
+      spmv(p, s, s->p, s->q, st);
+      dot2(s, s->p, s->q, nullptr, nullptr, s->sc + 2, st);  // sc[2] = pq
+      hipLaunchKernelGGL(k_cg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->z, s->p, s->q, s->dinv);
+      dot2(s, s->r, s->z, s->r, s->r, s->sc + 4, st);  // sc[4] = rz_new, sc[5] = rr_new
+      hipLaunchKernelGGL(k_cg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+      hipLaunchKernelGGL(k_cg_roll, dim3(1), dim3(1), 0, st, s->sc);
+      if (it % check_every == 0 || it == maxiter) {
+        if (hipMemcpyAsync(h, s->sc, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return GDX_ERR_HIP;
+        if (std::sqrt(h[1]) <= rtol * bnorm) break;
+      }
+    }
+    if (it > maxiter) it = maxiter;
+  }
+  // true residual
+  spmv(p, s, s->x, s->q, st);
+  hipLaunchKernelGGL(k_cg_init, g, b, 0, st, n, s->b, s->dinv, s->q, s->z, s->r);  // q = b - Ax
+  dot2(s, s->q, s->q, nullptr, nullptr, s->sc + 6, st);
+  if (hipMemcpyAsync(h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
+  if (xh && hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
+  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
+  auto t1 = std::chrono::steady_clock::now();
+  if (iters) *iters = it;
+  if (relres) *relres = bnorm > 0.0 ? std::sqrt(h[0]) / bnorm : 0.0;
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return GDX_OK;
+}
+
+}  // namespace gdx

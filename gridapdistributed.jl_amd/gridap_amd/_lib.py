@@ -8,7 +8,7 @@ import ctypes
 import os
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_ROOT, "libgdx.so")
+LIB_PATH = os.environ.get("GDX_LIBRARY", os.path.join(_PKG_ROOT, "libgdx.so"))
 
 GDX_OK = 0
 ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ERR_UNSUPPORTED", -5: "GDX_ERR_ALLOC"}
@@ -22,7 +22,7 @@ EXPORTED = [
     "gdx_version", "gdx_last_error", "gdx_init", "gdx_finalize", "gdx_problem_create", "gdx_problem_destroy",
     "gdx_symbolic", "gdx_numeric", "gdx_num_parts", "gdx_num_global_dofs", "gdx_part_sizes", "gdx_export_fe",
     "gdx_export_rows", "gdx_export_cols", "gdx_export_csr", "gdx_export_rhs", "gdx_device_values", "gdx_stream",
-    "gdx_sync", "gdx_phase_times", "gdx_owned_nnz",
+    "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg",
 ]
 
 
@@ -77,6 +77,8 @@ def lib():
         "gdx_sync": [vp],
         "gdx_phase_times": [vp, P(ctypes.c_double), i32],
         "gdx_owned_nnz": [vp, P(i64)],
+        "gdx_spmv": [vp, vp, vp],
+        "gdx_pcg": [vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -203,6 +203,21 @@ class GpuProblem:
         _lib.call("gdx_owned_nnz", self.handle, ctypes.byref(n))
         return n.value
 
+    def spmv(self, x):
+        """y = A x over the owned rows (global-id order), on the device."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.ngdofs)
+        _lib.call("gdx_spmv", self.handle, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def pcg(self, rtol=1e-8, maxiter=10000, want_x=True):
+        """Jacobi-PCG on the assembled system; returns (x, iterations, true relative residual, ms)."""
+        x = np.empty(self.ngdofs) if want_x else None
+        it, rr, ms = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        _lib.call("gdx_pcg", self.handle, float(rtol), int(maxiter), None if x is None else x.ctypes.data,
+                  ctypes.byref(it), ctypes.byref(rr), ctypes.byref(ms))
+        return x, it.value, rr.value, ms.value
+
     def stream(self):
         s = ctypes.c_void_p()
         _lib.call("gdx_stream", self.handle, ctypes.byref(s))

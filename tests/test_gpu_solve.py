@@ -1,0 +1,60 @@
+"""Solve side on the GPU: CSR SpMV over the owned rows and Jacobi-PCG, on the
+matrix the device assembled (the reference applies A*x and solves A\\b,
+test/FESpacesTests.jl:21-27, README.md:76)."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle
+from _helpers import gid_to_gnode, owned_global, rel_err
+from gridap_amd import problem_from_workload
+from gridap_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _prob(ctx, p):
+    assem, data = problem_from_workload(ctx, p)
+    prob = assem._problem_for(data)
+    prob.numeric()
+    prob.sync()
+    return prob
+
+
+@pytest.mark.parametrize("p", [W.poisson3d_q2_fourier(6, (2, 2, 1)), W.poisson3d_q1(9, (1, 2, 3)),
+                               W.poisson2d_q1(20, (2, 2))], ids=["q2", "q1", "2d"])
+def test_spmv_matches_oracle_matrix(gpu_ctx, p):
+    r = oracle.run(p)
+    A, b = owned_global(r.parts, r.ngdofs)
+    prob = _prob(gpu_ctx, p)
+    x = np.random.default_rng(1).standard_normal(r.ngdofs)
+    y = prob.spmv(x)
+    assert rel_err(y, A @ x) < 1e-13
+    prob.close()
+
+
+def test_pcg_linear_solution_2d(gpu_ctx):
+    # u = x + y is in the Q1 space: the solve reproduces it at the nodes
+    p = W.poisson2d_q1(64, (2, 2))
+    prob = _prob(gpu_ctx, p)
+    x, it, rr, ms = prob.pcg(rtol=1e-12, maxiter=5000)
+    assert rr <= 1e-11
+    parts = [prob.export_part(k) for k in range(prob.nparts)]
+    g2n = gid_to_gnode(parts, prob.ngdofs)
+    u = (g2n % 65) / 64 + (g2n // 65) / 64
+    assert np.abs(x - u).max() < 1e-9
+    prob.close()
+
+
+def test_pcg_matches_direct_solve_and_repeats(gpu_ctx):
+    p = W.poisson3d_q2_fourier(10, (2, 1, 2))
+    prob = _prob(gpu_ctx, p)
+    parts = [prob.export_part(k) for k in range(prob.nparts)]
+    A, b = owned_global(parts, prob.ngdofs)
+    x, it, rr, ms = prob.pcg(rtol=1e-10, maxiter=5000)
+    assert rr <= 1e-10 and it > 0
+    xd = spla.spsolve(A.tocsc(), b)
+    assert rel_err(x, xd) < 1e-7
+    x2, it2, rr2, _ = prob.pcg(rtol=1e-10, maxiter=5000)
+    assert it2 == it and np.array_equal(x.view(np.int64), x2.view(np.int64))
+    prob.close()
