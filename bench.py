@@ -148,7 +148,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_assemble3d<2,3,8,8,8>",
+            "kernel": "k_march3d<2,3,4,8>",
             "alg_bytes_per_launch": int(alg_bytes),
         },
     }

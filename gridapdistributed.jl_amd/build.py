@@ -9,7 +9,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = ["csrc/gdx_capi.hip", "csrc/gdx_kernels.hip", "csrc/gdx_assemble3d.hip", "csrc/gdx_solve.hip",
+SRC = ["csrc/gdx_capi.hip", "csrc/gdx_kernels.hip", "csrc/gdx_march3d.hip", "csrc/gdx_solve.hip",
        "csrc/gdx_symbolic.cpp"]
 OUT = os.path.join(HERE, "libgdx.so")
 

@@ -56,7 +56,8 @@ int upload(T **dst, const std::vector<T> &src) {
 
 void free_device(gdx_problem *p) {
   gdx::Device &d = p->dev;
-  void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_len, d.pat_dir,
+  void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_invr, d.pat_len,
+                  d.pat_dir,
                   d.tiles, d.tile_pats};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -222,6 +223,7 @@ int gdx_symbolic(gdx_problem *p) {
   if ((rc = upload(&d.rowinfo, p->rowinfo))) return rc;
   if ((rc = upload(&d.pat_fwd, p->pat_fwd))) return rc;
   if ((rc = upload(&d.pat_inv, p->pat_inv))) return rc;
+  if ((rc = upload(&d.pat_invr, p->pat_invr))) return rc;
   if ((rc = upload(&d.pat_len, p->pat_len))) return rc;
   if ((rc = upload(&d.pat_dir, p->pat_dir))) return rc;
   if ((rc = upload(&d.tiles, p->tiles))) return rc;

@@ -137,4 +137,11 @@ __device__ __forceinline__ void write_row(const double *__restrict__ row, long l
 }
 
 
+// row table entry staged in LDS
+struct RowT {
+  long long rowstart;
+  int bidx;
+  int lenf;  // 0 when the node is not a free row of the tile
+};
+
 }  // namespace gdx

@@ -50,8 +50,15 @@ struct RowInfo {
   int lenf;            // row length | has-Dirichlet-neighbours << 16 | tile-local pattern index << 17
 };
 
-// Pattern rows are padded to whole dwords (stride of pat_fwd / pat_inv).
-constexpr int pattern_stride(int S) { return (S + 3) & ~3; }
+// Pattern rows are padded to whole dwords (stride of pat_fwd / pat_inv), with
+// at least one spare byte (pat_invr keeps the Dirichlet count there).
+constexpr int pattern_stride(int S) { return (S + 4) & ~3; }
+
+// Reduced stencil numbering: along a dimension a node with i % K == 0 couples
+// to offsets -K..K (2K+1 slots), any other node to -1..1 (3 slots).  A row's
+// reduced slot is x-fastest over its own box of coupled offsets.
+__host__ __device__ constexpr int stencil_width(int K, int i) { return (i % K == 0) ? 2 * K + 1 : 3; }
+__host__ __device__ constexpr int stencil_lo(int K, int i) { return (i % K == 0) ? -K : -1; }
 
 // A tile of owned rows processed by one workgroup.
 struct Tile {
@@ -93,6 +100,7 @@ struct Device {
   RowInfo *rowinfo = nullptr;
   uint8_t *pat_fwd = nullptr;  // [npat][S]: slot -> offset | absent | Dirichlet
   uint8_t *pat_inv = nullptr;  // [npat][S]: offset -> slot
+  uint8_t *pat_invr = nullptr; // [npat][SP]: offset -> reduced slot; Dirichlet slots after len; last byte = their count
   int *pat_len = nullptr;
   int *pat_dir = nullptr;      // pattern has Dirichlet slots
   Tile *tiles = nullptr;
@@ -118,7 +126,7 @@ struct gdx_problem {
   bool symbolic_done = false;
   // structured-kernel data (host copies)
   std::vector<gdx::RowInfo> rowinfo;
-  std::vector<uint8_t> pat_fwd, pat_inv;
+  std::vector<uint8_t> pat_fwd, pat_inv, pat_invr;
   std::vector<int> pat_len, pat_dir;
   int npat = 0;
   std::vector<gdx::Tile> tiles;

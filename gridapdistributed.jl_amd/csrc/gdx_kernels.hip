@@ -57,45 +57,61 @@ __global__ void k_fourier_tables(const double *__restrict__ modes, KParams kp, i
 }
 
 // One block = LINES consecutive x-lines (q2 .. q2+LINES-1 at fixed q3) of the
-// quadrature lattice; thread <-> q1.  The x factors of a chunk of CH modes stay
-// in registers across the lines; the (y,z) factor product is per line.
-constexpr int kCoefLines = 8, kCoefModes = 16;
+// quadrature lattice; thread <-> q1.  The x factors of a chunk of modes stay
+// in registers across the lines; the per-line (y,z) products b = e2 e3 and
+// (k.c) b are tabulated in LDS, so each (point, mode) costs four FMAs:
+//   g += Re(e1 b),  s += Im(e1 (k.c) b).
+constexpr int kCoefLines = 16, kCoefModes = 16;
 
 __global__ __launch_bounds__(256) void k_coefficients(const double *__restrict__ modes,
                                                       const double *__restrict__ E, KParams kp, int qnmax,
                                                       double *__restrict__ W, double *__restrict__ F) {
+  __shared__ double tb[kCoefLines][kCoefModes][4];  // br, bi, (k.c) br, (k.c) bi
   const int q1 = blockIdx.x * blockDim.x + threadIdx.x;
   const int q2b = blockIdx.y * kCoefLines, q3 = blockIdx.z;
-  if (q1 >= kp.qn[0]) return;
+  const bool active = q1 < kp.qn[0];
   const int nm = kp.nmodes;
   double g[kCoefLines], s[kCoefLines];
 #pragma unroll
   for (int l = 0; l < kCoefLines; ++l) g[l] = s[l] = 0.0;
   for (int m0 = 0; m0 < nm; m0 += kCoefModes) {
-    double e1r[kCoefModes], e1i[kCoefModes];
-#pragma unroll
-    for (int j = 0; j < kCoefModes; ++j) {
-      const int m = m0 + j < nm ? m0 + j : nm - 1;
-      const double *e1 = E + 2 * ((size_t)m * qnmax + q1);
-      e1r[j] = m0 + j < nm ? e1[0] : 0.0;
-      e1i[j] = m0 + j < nm ? e1[1] : 0.0;
-    }
-#pragma unroll
-    for (int l = 0; l < kCoefLines; ++l) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCoefLines * kCoefModes; i += blockDim.x) {
+      const int l = i / kCoefModes, j = i % kCoefModes, m = m0 + j;
       const int q2 = q2b + l < kp.qn[1] ? q2b + l : kp.qn[1] - 1;
-#pragma unroll
-      for (int j = 0; j < kCoefModes; ++j) {
-        const int m = m0 + j < nm ? m0 + j : nm - 1;
+      double br = 0.0, bi = 0.0, kc = 0.0;
+      if (m < nm) {
         const double *M = modes + 5 * m;
         const double *e2 = E + 2 * ((size_t)(1 * nm + m) * qnmax + q2);
         const double *e3 = E + 2 * ((size_t)(2 * nm + m) * qnmax + q3);
-        const double br = e2[0] * e3[0] - e2[1] * e3[1], bi = e2[0] * e3[1] + e2[1] * e3[0];
-        const double kc = M[0] * kp.dir_coef[1] + M[1] * kp.dir_coef[2] + M[2] * kp.dir_coef[3];
-        g[l] = fma(e1r[j], br, fma(-e1i[j], bi, g[l]));
-        s[l] = fma(kc, fma(e1r[j], bi, e1i[j] * br), s[l]);
+        br = e2[0] * e3[0] - e2[1] * e3[1];
+        bi = e2[0] * e3[1] + e2[1] * e3[0];
+        kc = M[0] * kp.dir_coef[1] + M[1] * kp.dir_coef[2] + M[2] * kp.dir_coef[3];
+      }
+      tb[l][j][0] = br;
+      tb[l][j][1] = bi;
+      tb[l][j][2] = kc * br;
+      tb[l][j][3] = kc * bi;
+    }
+    __syncthreads();
+    double e1r[kCoefModes], e1i[kCoefModes];
+#pragma unroll
+    for (int j = 0; j < kCoefModes; ++j) {
+      const bool ok = active && m0 + j < nm;
+      const double *e1 = E + 2 * ((size_t)(ok ? m0 + j : 0) * qnmax + (active ? q1 : 0));
+      e1r[j] = ok ? e1[0] : 0.0;
+      e1i[j] = ok ? e1[1] : 0.0;
+    }
+#pragma unroll
+    for (int l = 0; l < kCoefLines; ++l) {
+#pragma unroll
+      for (int j = 0; j < kCoefModes; ++j) {
+        g[l] = fma(e1r[j], tb[l][j][0], fma(-e1i[j], tb[l][j][1], g[l]));
+        s[l] = fma(e1r[j], tb[l][j][3], fma(e1i[j], tb[l][j][2], s[l]));
       }
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int l = 0; l < kCoefLines; ++l) {
     const int q2 = q2b + l;
@@ -326,10 +342,7 @@ int launch_rhs(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    if (K == 2 && NQ == 3) rhs_run<3, 2, 3, 4, 8, 8>(p, st);
-    else if (K == 2 && NQ == 2) rhs_run<3, 2, 2, 4, 8, 8>(p, st);
-    else if (K == 1 && NQ == 2) rhs_run<3, 1, 2, 8, 8, 2>(p, st);
-    else return GDX_ERR_UNSUPPORTED;
+    return GDX_OK;  // the z-marching matrix kernel writes b itself
   } else {
     if (K == 2 && NQ == 3) rhs_run<2, 2, 3, 16, 16, 1>(p, st);
     else if (K == 2 && NQ == 2) rhs_run<2, 2, 2, 16, 16, 1>(p, st);

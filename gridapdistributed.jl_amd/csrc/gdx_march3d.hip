@@ -1,0 +1,361 @@
+// 3D assembly kernel of libgdx, z-marching form (formulation in gdx_kernels.hip).
+//
+// One workgroup owns an R1 x R2 column of owned rows of one part and walks
+// along z one cell per step.  Per step c (cell c of the column):
+//   stage 1  kappa(x_q) and f(x_q) of cell c (prefetched one step ahead) are
+//            contracted over x quadrature points: X[x pair][z point][y point]
+//   stage 2  thread (x pair, y pair) contracts y points: Y[z point] of cell c
+//            (registers; cell c-1's values are kept, the z halo is never
+//            recomputed); the f part likewise per (x row, y row)
+//   stage 3  the K z rows whose support is cells c-1 and c are completed into
+//            an LDS slab (each row's stencil box, sized by node parity), then
+//            one wave per CSR row stores it contiguously through the row's
+//            position pattern, and writes b = f part - sum_j A_ij uD_j.
+// Two barriers per step; no float atomics; every value is written once.
+#include <hip/hip_runtime.h>
+
+#include "gdx_device.h"
+
+namespace gdx {
+
+template <int K>
+__host__ __device__ constexpr int width_prefix(int u) {
+  // sum of stencil widths of tile rows 0..u-1 (tile origins are multiples of K)
+  return K == 1 ? 3 * u : 8 * (u >> 1) + 5 * (u & 1);
+}
+
+template <int K, int NQ, int R1, int R2>
+struct March {
+  static constexpr int C1 = R1 / K + 1, C2 = R2 / K + 1;
+  static constexpr int Q1 = C1 * NQ, Q2 = C2 * NQ;
+  static constexpr int NP1 = pair_count<K>(R1), NP2 = pair_count<K>(R2);
+  static constexpr int NT = NP1 * NP2, NW = NT / 64;
+  static constexpr int CS = Q1 * Q2 * NQ;      // quadrature points of one z cell of the column
+  static constexpr int XS = (NQ * Q2) | 1;     // X stride per x pair (odd: conflict-free)
+  static constexpr int W = 2 * K + 1, S = W * W * W, SP = pattern_stride(S);
+  static constexpr int T1 = width_prefix<K>(R1), T2 = width_prefix<K>(R2);
+  static constexpr int ZW0 = 2 * K + 1, ZW1 = 3;  // widths of the step's z rows
+  static constexpr int SLAB = T1 * T2 * (K == 2 ? ZW0 + ZW1 : ZW0);
+  static constexpr int RS = R1 * R2 * K;          // rows completed per step
+  static constexpr int PMAX = 96;  // tile patterns kept in LDS (3D Q2 64^3 needs at most 72)
+  static constexpr int NPF = (2 * CS + NT - 1) / NT;  // prefetched doubles per thread
+  // LDS (in doubles)
+  static constexpr int O_KF = 0;                        // [2 buffers][kappa, f][CS]
+  static constexpr int O_X = O_KF + 4 * CS;             // XG, XM: [NP1][XS] each
+  static constexpr int O_Z1 = O_X + 2 * NP1 * XS;       // f part after x: [R1][NQ][Q2]
+  static constexpr int O_SLAB = O_Z1 + R1 * NQ * Q2;
+  static constexpr int O_BF = O_SLAB + SLAB;            // f part of the step's rows
+  static constexpr int O_RT = O_BF + RS;                // row tables [2][RS] x 16 B
+  static constexpr int O_FY = O_RT + 2 * 2 * RS;        // y pair factors: [NP2][G | M][2 NQ]
+  static constexpr int O_FZ = O_FY + NP2 * 4 * NQ;      // f part after y: [2 cells][R1 R2][NQ]
+  static constexpr int O_PAT = O_FZ + 2 * R1 * R2 * NQ; // PMAX patterns x SP bytes
+  static constexpr int LDS_DOUBLES = O_PAT + PMAX * SP / 8;
+  static_assert(NT % 64 == 0, "whole waves");
+  static_assert(RS <= NT, "one row-table entry per thread");
+};
+
+// row of the step's slab: offset of row (u1,u2) of z row o, and its reduced slot
+template <int K, int R1, int R2>
+__device__ __forceinline__ int slab_row_offset(int u1, int u2, int o) {
+  constexpr int T1 = width_prefix<K>(R1), T2 = width_prefix<K>(R2);
+  const int w3 = o == 0 ? 2 * K + 1 : 3;
+  const int zoff = o == 0 ? 0 : T1 * T2 * (2 * K + 1);
+  return zoff + w3 * (width_prefix<K>(u2) * T1 + stencil_width(K, u2) * width_prefix<K>(u1));
+}
+
+template <int K, int NQ, int R1, int R2, bool PATG>
+// two workgroups per CU: at most 128 VGPRs (4 waves per SIMD)
+__global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_waves_per_eu(4))) void k_march3d(
+    const Tile *__restrict__ tiles, int tile0, int ntiles, const RowInfo *__restrict__ rowinfo,
+    const double *__restrict__ Wq, const double *__restrict__ Fq, const uint8_t *__restrict__ pat_invr,
+    const int *__restrict__ tile_pats, double *__restrict__ values, double *__restrict__ b, KParams kp) {
+  using T = March<K, NQ, R1, R2>;
+  __shared__ double lds[T::LDS_DOUBLES];
+  double *XG = lds + T::O_X, *XM = XG + T::NP1 * T::XS;
+  double *Z1 = lds + T::O_Z1, *slab = lds + T::O_SLAB, *BF = lds + T::O_BF;
+  RowT *RT = reinterpret_cast<RowT *>(lds + T::O_RT);
+  uint8_t *PAT = reinterpret_cast<uint8_t *>(lds + T::O_PAT);
+  const int tb = tile_of_block(ntiles);
+  if (tb >= ntiles) return;
+  const Tile t = tiles[tile0 + tb];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int c0 = t.org[2] / K;                      // first cell whose left node is a tile row
+  const int cl = (t.org[2] + t.ext[2] - 1) / K;      // last cell touching a tile row
+  const int q0 = (t.org[0] / K - 1) * NQ, q1 = (t.org[1] / K - 1) * NQ;
+
+  // quadrature data of z cell c into buffer c & 1 (zero outside the mesh)
+  auto kf_load = [&](int c, double (&reg)[T::NPF]) {
+#pragma unroll
+    for (int w = 0; w < T::NPF; ++w) {
+      const int i = tid + w * T::NT;
+      double v = 0.0;
+      if (i < 2 * T::CS) {
+        const int j = i % T::CS, a = j % T::Q1, bq = (j / T::Q1) % T::Q2, tz = j / (T::Q1 * T::Q2);
+        const int qa = q0 + a, qb = q1 + bq, qc = c * NQ + tz;
+        if (c >= 0 && c < kp.ncells[2] && qa >= 0 && qa < kp.qn[0] && qb >= 0 && qb < kp.qn[1]) {
+          const size_t idx = qa + (size_t)kp.qn[0] * (qb + (size_t)kp.qn[1] * qc);
+          v = i < T::CS ? Wq[idx] : Fq[idx];
+        }
+      }
+      reg[w] = v;
+    }
+  };
+  auto kf_store = [&](int c, const double (&reg)[T::NPF]) {
+    double *buf = lds + T::O_KF + (c & 1) * 2 * T::CS;
+#pragma unroll
+    for (int w = 0; w < T::NPF; ++w) {
+      const int i = tid + w * T::NT;
+      if (i < 2 * T::CS) buf[i] = reg[w];
+    }
+  };
+  // row table of step c (rows z = K c + o) into buffer c & 1
+  auto rt_load = [&](int c) -> RowT {
+    RowT e{0, 0, 0};
+    if (tid < T::RS) {
+      const int u1 = tid % R1, u2 = (tid / R1) % R2, o = tid / (R1 * R2);
+      const int z = K * c + o, uz = z - t.org[2];
+      if (u1 < t.ext[0] && u2 < t.ext[1] && uz >= 0 && uz < t.ext[2]) {
+        const RowInfo ri = rowinfo[t.rowinfo_base + row_index(t, t.org[0] + u1, t.org[1] + u2, z)];
+        if (ri.pid >= 0) e = RowT{ri.rowstart, (int)ri.bidx, ri.lenf};
+      }
+    }
+    return e;
+  };
+
+  // prologue: patterns, first z cell, row table of the first emitting step
+  if (!PATG) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(pat_invr);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(PAT);
+    for (int i = tid; i < t.npat * (T::SP / 4); i += T::NT)
+      dst[i] = src[(size_t)tile_pats[t.pat_off + i / (T::SP / 4)] * (T::SP / 4) + i % (T::SP / 4)];
+  }
+  {
+    double reg[T::NPF];
+    kf_load(c0 - 1, reg);
+    kf_store(c0 - 1, reg);
+  }
+
+  // pair structure of this thread
+  const int p1 = tid % T::NP1, p2 = tid / T::NP1;
+  int u1, s1, u2, s2;
+  pair_natural<K>(p1, u1, s1);
+  pair_doubles_first<K, R2>(p2, u2, s2);
+  int lc2, nc2, a2, b2;
+  pair_cells<K>(u2, s2, lc2, nc2, a2, b2);
+  // y pair factors live in LDS (they vary per lane; registers are scarce)
+  double *FY = lds + T::O_FY;
+  if (tid < T::NP2) {
+    int uu, ss, l0, n0, aa, bb;
+    pair_doubles_first<K, R2>(tid, uu, ss);
+    pair_cells<K>(uu, ss, l0, n0, aa, bb);
+#pragma unroll
+    for (int tt = 0; tt < 2 * NQ; ++tt)
+      pair_factors<K, NQ>(kp, 1, aa, bb, tt, FY[tid * 4 * NQ + tt], FY[tid * 4 * NQ + 2 * NQ + tt]);
+  }
+  const double *G2 = FY + p2 * 4 * NQ, *M2 = G2 + 2 * NQ;
+  double *FZ = lds + T::O_FZ;  // f part of this thread's row after the y contraction, per cell parity
+  const int w1 = stencil_width(K, u1), w2 = stencil_width(K, u2);
+  const int rbase0 = slab_row_offset<K, R1, R2>(u1, u2, 0) + (s2 - stencil_lo(K, u2)) * w1 + (s1 - stencil_lo(K, u1));
+  const int rbase1 = K == 2 ? slab_row_offset<K, R1, R2>(u1, u2, 1) + (s2 - stencil_lo(K, u2)) * w1 +
+                                  (s1 - stencil_lo(K, u1))
+                            : 0;
+  // f part: thread tid < R1*R2 owns row (fu1, fu2) of the column
+  const int fu1 = tid % R1, fu2 = (tid / R1) % R2;
+
+  double Yp1[NQ], Yp2[NQ], Yc1[NQ], Yc2[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) Yc1[i] = Yc2[i] = 0.0;
+
+  __syncthreads();  // patterns and the first cell are in LDS
+
+  // Per step, barrier A (after stage 1) separates the previous step's row
+  // stores from this step's slab writes and frees the other quadrature buffer;
+  // barrier B (after stage 3) publishes the slab, the next cell's quadrature
+  // data and the next row table.
+#pragma unroll 1
+  for (int c = c0 - 1; c <= cl; ++c) {
+    const double *Kc = lds + T::O_KF + (c & 1) * 2 * T::CS, *Fl = Kc + T::CS;
+    // prefetch the next cell and the next step's row table (registers)
+    double reg[T::NPF];
+    kf_load(c + 1, reg);
+    const RowT rtn = rt_load(c + 1);
+
+    // stage 1: x contraction of kappa (per x pair) and of f (per x row)
+    for (int it = tid; it < T::NP1 * NQ * T::Q2 + R1 * NQ * T::Q2; it += T::NT) {
+      if (it < T::NP1 * NQ * T::Q2) {
+        const int pp = it / (NQ * T::Q2), r = it % (NQ * T::Q2);  // r = tz*Q2 + q2
+        int u, s, lc0, nc, a0, b0;
+        pair_natural<K>(pp, u, s);
+        pair_cells<K>(u, s, lc0, nc, a0, b0);
+        const double *kr = Kc + r * T::Q1 + lc0 * NQ;
+        double xg = 0.0, xm = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          if (tt >= nc * NQ) break;
+          double G, M;
+          pair_factors<K, NQ>(kp, 0, a0, b0, tt, G, M);
+          xg = fma(kr[tt], G, xg);
+          xm = fma(kr[tt], M, xm);
+        }
+        XG[pp * T::XS + r] = xg;
+        XM[pp * T::XS + r] = xm;
+      } else {
+        const int j = it - T::NP1 * NQ * T::Q2, u = j / (NQ * T::Q2), r = j % (NQ * T::Q2);
+        const double *fr = Fl + r * T::Q1;
+        const int v = u + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
+        double acc = 0.0;
+        for (int lc = cmin; lc <= cmax; ++lc)
+#pragma unroll
+          for (int tt = 0; tt < NQ; ++tt)
+            acc = fma(fr[lc * NQ + tt], kp.qw[tt] * kp.h[0] * kp.phi[v - K * lc][tt], acc);
+        Z1[j] = acc;
+      }
+    }
+    __syncthreads();
+    kf_store(c + 1, reg);
+    if (tid < T::RS) RT[((c + 1) & 1) * T::RS + tid] = rtn;
+
+    // stage 2: y contraction for this thread's (x pair, y pair), all z points of cell c
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      Yp1[i] = Yc1[i];
+      Yp2[i] = Yc2[i];
+    }
+    {
+      const double *xg = XG + p1 * T::XS + lc2 * NQ, *xm = XM + p1 * T::XS + lc2 * NQ;
+#pragma unroll
+      for (int tz = 0; tz < NQ; ++tz) {
+        double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          if (tt >= nc2 * NQ) break;
+          const double g = xg[tz * T::Q2 + tt], m = xm[tz * T::Q2 + tt];
+          y1 = fma(g, M2[tt], fma(m, G2[tt], y1));
+          y2 = fma(m, M2[tt], y2);
+        }
+        Yc1[tz] = y1;
+        Yc2[tz] = y2;
+      }
+    }
+    if (tid < R1 * R2) {
+      const int v = fu2 + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
+#pragma unroll
+      for (int tz = 0; tz < NQ; ++tz) {
+        const double *zr = Z1 + (fu1 * NQ + tz) * T::Q2;
+        double acc = 0.0;
+        for (int lc = cmin; lc <= cmax; ++lc)
+#pragma unroll
+          for (int tt = 0; tt < NQ; ++tt)
+            acc = fma(zr[lc * NQ + tt], kp.qw[tt] * kp.h[1] * kp.phi[v - K * lc][tt], acc);
+        FZ[((c & 1) * R1 * R2 + tid) * NQ + tz] = acc;
+      }
+    }
+
+    // stage 3: complete the K z rows of this step (supported on cells c-1, c)
+    if (c >= c0) {
+#pragma unroll
+      for (int o = 0; o < K; ++o) {
+#pragma unroll
+        for (int s3 = -K; s3 <= K; ++s3) {
+          if (o == 1 && (s3 < -1 || s3 > 1)) continue;
+          int lc0, nc, a0, b0;
+          pair_cells<K>(o, s3, lc0, nc, a0, b0);
+          double v = 0.0;
+#pragma unroll
+          for (int tt = 0; tt < 2 * NQ; ++tt) {
+            if (tt >= nc * NQ) break;
+            double G, M;
+            pair_factors<K, NQ>(kp, 2, a0, b0, tt, G, M);
+            const int ti = lc0 * NQ + tt;  // 0..NQ-1: cell c-1, NQ..2NQ-1: cell c
+            const double y1 = ti < NQ ? Yp1[ti % NQ] : Yc1[ti % NQ];
+            const double y2 = ti < NQ ? Yp2[ti % NQ] : Yc2[ti % NQ];
+            v = fma(y1, M, fma(y2, G, v));
+          }
+          const int lo3 = o == 0 ? -K : -1;
+          slab[(o == 0 ? rbase0 : rbase1) + (s3 - lo3) * w1 * w2] = v;
+        }
+        if (tid < R1 * R2) {
+          const int v3 = o + K, cmin = (v3 + K - 1) / K - 1, cmax = v3 / K;
+          double acc = 0.0;
+#pragma unroll
+          for (int lc = 0; lc < 2; ++lc) {
+            if (lc < cmin || lc > cmax) continue;
+#pragma unroll
+            for (int tt = 0; tt < NQ; ++tt)
+              acc = fma(FZ[(((c - 1 + lc) & 1) * R1 * R2 + tid) * NQ + tt],
+                        kp.qw[tt] * kp.h[2] * kp.phi[v3 - K * lc][tt], acc);
+          }
+          BF[o * R1 * R2 + fu2 * R1 + fu1] = acc;
+        }
+      }
+    }
+    __syncthreads();
+
+    // store the completed rows: one wave per row
+    if (c >= c0) {
+      const RowT *rt = RT + (c & 1) * T::RS;
+      for (int r = wave; r < T::RS; r += T::NW) {
+        const RowT e = rt[r];
+        if (e.lenf == 0) continue;
+        const int ru1 = r % R1, ru2 = (r / R1) % R2, o = r / (R1 * R2);
+        const int len = e.lenf & 0xFFFF, lpid = e.lenf >> 17;
+        const uint8_t *inv = PATG ? pat_invr + (size_t)tile_pats[t.pat_off + lpid] * T::SP : PAT + lpid * T::SP;
+        const double *row = slab + slab_row_offset<K, R1, R2>(ru1, ru2, o);
+        double *dst = values + e.rowstart;
+#pragma unroll
+        for (int kk = 0; kk < (T::S + 63) / 64; ++kk) {
+          const int k = lane + 64 * kk;
+          if (k < len) dst[k] = row[inv[k]];
+        }
+        // b = f part - sum over Dirichlet columns of A_ij uD_j
+        double acc = 0.0;
+        if (e.lenf & 0x10000) {
+          const int nd = inv[T::SP - 1];
+          const int rw1 = stencil_width(K, ru1), rw2 = stencil_width(K, ru2);
+          for (int j = lane; j < nd; j += 64) {
+            const int sl = inv[len + j];
+            const int d1 = sl % rw1 + stencil_lo(K, ru1), d2 = (sl / rw1) % rw2 + stencil_lo(K, ru2);
+            const int d3 = sl / (rw1 * rw2) + (o == 0 ? -K : -1);
+            acc += row[sl] * uD_at(kp, t.org[0] + ru1 + d1, t.org[1] + ru2 + d2, K * c + o + d3);
+          }
+          acc = wave_sum(acc);
+        }
+        if (lane == 0) b[e.bidx] = BF[r] - acc;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launcher.  Tile shapes must match tile_shape() in gdx_symbolic.cpp.
+// ---------------------------------------------------------------------------
+template <int K, int NQ, int R1, int R2>
+static void runm(gdx_problem *p, hipStream_t st) {
+  using T = March<K, NQ, R1, R2>;
+  const int nt = (int)p->tiles.size(), nf = p->n_fast_tiles;
+  if (nf > 0)
+    hipLaunchKernelGGL((k_march3d<K, NQ, R1, R2, false>), dim3(8u * ((nf + 7) / 8)), dim3(T::NT), 0, st,
+                       p->dev.tiles, 0, nf, p->dev.rowinfo, p->dev.W, p->dev.F, p->dev.pat_invr, p->dev.tile_pats,
+                       p->dev.values, p->dev.b, p->kp);
+  if (nt > nf)
+    hipLaunchKernelGGL((k_march3d<K, NQ, R1, R2, true>), dim3(8u * ((nt - nf + 7) / 8)), dim3(T::NT), 0, st,
+                       p->dev.tiles, nf, nt - nf, p->dev.rowinfo, p->dev.W, p->dev.F, p->dev.pat_invr,
+                       p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
+}
+
+int pattern_capacity_3d(int K, int NQ) {
+  (void)K;
+  (void)NQ;
+  return March<2, 3, 4, 8>::PMAX;
+}
+
+int launch_matrix3d(gdx_problem *p, hipStream_t st) {
+  const int K = p->K, NQ = p->NQ;
+  if (K == 2 && NQ == 3) runm<2, 3, 4, 8>(p, st);
+  else if (K == 2 && NQ == 2) runm<2, 2, 4, 8>(p, st);
+  else if (K == 1 && NQ == 2) runm<1, 2, 8, 8>(p, st);
+  else return GDX_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+}  // namespace gdx

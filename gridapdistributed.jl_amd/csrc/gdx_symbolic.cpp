@@ -158,11 +158,10 @@ inline void shared_range(int K, int64_t i, int64_t clo, int64_t chi, int64_t &jl
 void tile_shape(int dim, int K, int R[3]) {
   // must match the template instances launched by launch_matrix
   if (dim == 3) {
-    if (K == 2) {
-      R[0] = 4; R[1] = 8; R[2] = 8;
-    } else {
-      R[0] = 8; R[1] = 8; R[2] = 2;
-    }
+    // z-marching columns: R[2] rows along z per workgroup
+    R[0] = K == 2 ? 4 : 8;
+    R[1] = 8;
+    R[2] = 32;
   } else {
     R[0] = 16; R[1] = 16; R[2] = 1;
   }
@@ -362,6 +361,7 @@ int symbolic(gdx_problem *p) {
   std::unordered_map<std::string, int> pat_ids;
   p->pat_fwd.clear();
   p->pat_inv.clear();
+  p->pat_invr.clear();
   p->pat_len.clear();
   p->pat_dir.clear();
   p->rowinfo.clear();
@@ -425,6 +425,27 @@ int symbolic(gdx_problem *p) {
           }
           p->pat_fwd.insert(p->pat_fwd.end(), fwd.begin(), fwd.end());
           p->pat_inv.insert(p->pat_inv.end(), inv.begin(), inv.end());
+          // reduced numbering (the pattern fixes the node parity of its rows)
+          std::vector<uint8_t> invr(SP, 0);
+          int ndir = 0;
+          const int Wd = 2 * K + 1;
+          auto reduced = [&](int s) {
+            const int sx = s % Wd - K, sy = (s / Wd) % Wd - K, sz = c.D == 3 ? s / (Wd * Wd) - K : 0;
+            int r = sx - stencil_lo(K, (int)g[0]);
+            r += stencil_width(K, (int)g[0]) * (sy - stencil_lo(K, (int)g[1]));
+            if (c.D == 3)
+              r += stencil_width(K, (int)g[0]) * stencil_width(K, (int)g[1]) * (sz - stencil_lo(K, (int)g[2]));
+            return r;
+          };
+          for (int s = 0; s < S; ++s) {
+            const uint8_t v = fwd[s];
+            if (v == kSlotDirichlet) continue;
+            if (v != kSlotAbsent) invr[v] = (uint8_t)reduced(s);
+          }
+          for (int s = 0; s < S; ++s)
+            if (fwd[s] == kSlotDirichlet) invr[ents.size() + ndir++] = (uint8_t)reduced(s);
+          invr[SP - 1] = (uint8_t)ndir;
+          p->pat_invr.insert(p->pat_invr.end(), invr.begin(), invr.end());
           p->pat_len.push_back((int)ents.size());
           p->pat_dir.push_back(has_dir ? 1 : 0);
         } else {
