@@ -118,6 +118,13 @@ def main():
     value = nnz / (ms_per_step * 1e-3) / 1e9
     alg_bytes = 8.0 * nnz  # FP64 CSR values written per launch (8 B per stored owned entry)
     achieved = alg_bytes / (mat_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (copy-calibrated) on this workload; only valid at the default size
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_march3d_poisson3d_q2_64.json")
+    if args.cells == 64 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["traffic_bytes"]
     out = {
         "metric": "3D Q2 assembly Gnnz/s (FP64)",
         "value": round(value, 3),
@@ -147,7 +154,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_march3d<2,3,4,8>",
             "alg_bytes_per_launch": int(alg_bytes),
         },

@@ -12,6 +12,8 @@
 //            one wave per CSR row stores it contiguously through the row's
 //            position pattern, and writes b = f part - sum_j A_ij uD_j.
 // Two barriers per step; no float atomics; every value is written once.
+// Factors indexed per lane (x and y pairs, f weights) are tabulated in LDS at
+// kernel start; z factors are wave-uniform and stay in scalar registers.
 #include <hip/hip_runtime.h>
 
 #include "gdx_device.h"
@@ -48,7 +50,11 @@ struct March {
   static constexpr int O_RT = O_BF + RS;                // row tables [2][RS] x 16 B
   static constexpr int O_FY = O_RT + 2 * 2 * RS;        // y pair factors: [NP2][G | M][2 NQ]
   static constexpr int O_FZ = O_FY + NP2 * 4 * NQ;      // f part after y: [2 cells][R1 R2][NQ]
-  static constexpr int O_PAT = O_FZ + 2 * R1 * R2 * NQ; // PMAX patterns x SP bytes
+  static constexpr int O_FX = O_FZ + 2 * R1 * R2 * NQ; // x pair factors: [NP1][G | M][2 NQ]
+  static constexpr int O_PX = O_FX + NP1 * 4 * NQ;      // f weights of x rows: [R1][2 NQ]
+  static constexpr int O_PY = O_PX + R1 * 2 * NQ;       // f weights of y rows: [R2][2 NQ]
+  static constexpr int O_PZ = O_PY + R2 * 2 * NQ;       // f weights of the K z rows: [K][2 NQ]
+  static constexpr int O_PAT = O_PZ + K * 2 * NQ;       // PMAX patterns x SP bytes
   static constexpr int LDS_DOUBLES = O_PAT + PMAX * SP / 8;
   static_assert(NT % 64 == 0, "whole waves");
   static_assert(RS <= NT, "one row-table entry per thread");
@@ -152,6 +158,40 @@ __global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_w
     for (int tt = 0; tt < 2 * NQ; ++tt)
       pair_factors<K, NQ>(kp, 1, aa, bb, tt, FY[tid * 4 * NQ + tt], FY[tid * 4 * NQ + 2 * NQ + tt]);
   }
+  // x pair factors and the f weights of x and y rows: lane-varying indices
+  // into kernel arguments would become vector loads that queue behind the
+  // prefetch, so they are tabulated in LDS once
+  double *FX = lds + T::O_FX, *PX = lds + T::O_PX, *PY = lds + T::O_PY;
+  if (tid >= 64 && tid < 64 + T::NP1) {
+    const int pp = tid - 64;
+    int uu, ss, l0, n0, aa, bb;
+    pair_natural<K>(pp, uu, ss);
+    pair_cells<K>(uu, ss, l0, n0, aa, bb);
+#pragma unroll
+    for (int tt = 0; tt < 2 * NQ; ++tt)
+      pair_factors<K, NQ>(kp, 0, aa, bb, tt, FX[pp * 4 * NQ + tt], FX[pp * 4 * NQ + 2 * NQ + tt]);
+  }
+  if (tid >= 128 && tid < 128 + R1 + R2) {
+    const int d = tid < 128 + R1 ? 0 : 1, u = tid - 128 - d * R1, v = u + K;
+    const int cmin = (v + K - 1) / K - 1, cmax = v / K;
+    double *dst = (d == 0 ? PX : PY) + u * 2 * NQ;
+#pragma unroll
+    for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        dst[ci * NQ + q] = cmin + ci <= cmax ? kp.qw[q] * kp.h[d] * kp.phi[v - K * (cmin + ci)][q] : 0.0;
+  }
+  // f weights of the step's K z rows
+  double *PZ = lds + T::O_PZ;
+  if (tid >= 256 && tid < 256 + K) {
+    const int o = tid - 256, v3 = o + K, cmin = (v3 + K - 1) / K - 1, cmax = v3 / K;
+#pragma unroll
+    for (int lc = 0; lc < 2; ++lc)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        PZ[o * 2 * NQ + lc * NQ + q] =
+            lc >= cmin && lc <= cmax ? kp.qw[q] * kp.h[2] * kp.phi[v3 - K * lc][q] : 0.0;
+  }
   const double *G2 = FY + p2 * 4 * NQ, *M2 = G2 + 2 * NQ;
   double *FZ = lds + T::O_FZ;  // f part of this thread's row after the y contraction, per cell parity
   const int w1 = stencil_width(K, u1), w2 = stencil_width(K, u2);
@@ -187,33 +227,30 @@ __global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_w
         int u, s, lc0, nc, a0, b0;
         pair_natural<K>(pp, u, s);
         pair_cells<K>(u, s, lc0, nc, a0, b0);
-        const double *kr = Kc + r * T::Q1 + lc0 * NQ;
+        const double *kr = Kc + r * T::Q1 + lc0 * NQ, *fx = FX + pp * 4 * NQ;
         double xg = 0.0, xm = 0.0;
 #pragma unroll
         for (int tt = 0; tt < 2 * NQ; ++tt) {
           if (tt >= nc * NQ) break;
-          double G, M;
-          pair_factors<K, NQ>(kp, 0, a0, b0, tt, G, M);
-          xg = fma(kr[tt], G, xg);
-          xm = fma(kr[tt], M, xm);
+          xg = fma(kr[tt], fx[tt], xg);
+          xm = fma(kr[tt], fx[2 * NQ + tt], xm);
         }
         XG[pp * T::XS + r] = xg;
         XM[pp * T::XS + r] = xm;
       } else {
         const int j = it - T::NP1 * NQ * T::Q2, u = j / (NQ * T::Q2), r = j % (NQ * T::Q2);
-        const double *fr = Fl + r * T::Q1;
-        const int v = u + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
+        const int v = u + K, cmin = (v + K - 1) / K - 1, nc = v / K - cmin + 1;
+        const double *fr = Fl + r * T::Q1 + cmin * NQ, *px = PX + u * 2 * NQ;
         double acc = 0.0;
-        for (int lc = cmin; lc <= cmax; ++lc)
 #pragma unroll
-          for (int tt = 0; tt < NQ; ++tt)
-            acc = fma(fr[lc * NQ + tt], kp.qw[tt] * kp.h[0] * kp.phi[v - K * lc][tt], acc);
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          if (tt >= nc * NQ) break;
+          acc = fma(fr[tt], px[tt], acc);
+        }
         Z1[j] = acc;
       }
     }
     __syncthreads();
-    kf_store(c + 1, reg);
-    if (tid < T::RS) RT[((c + 1) & 1) * T::RS + tid] = rtn;
 
     // stage 2: y contraction for this thread's (x pair, y pair), all z points of cell c
 #pragma unroll
@@ -238,15 +275,17 @@ __global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_w
       }
     }
     if (tid < R1 * R2) {
-      const int v = fu2 + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
+      const int v = fu2 + K, cmin = (v + K - 1) / K - 1, nc = v / K - cmin + 1;
+      const double *py = PY + fu2 * 2 * NQ;
 #pragma unroll
       for (int tz = 0; tz < NQ; ++tz) {
-        const double *zr = Z1 + (fu1 * NQ + tz) * T::Q2;
+        const double *zr = Z1 + (fu1 * NQ + tz) * T::Q2 + cmin * NQ;
         double acc = 0.0;
-        for (int lc = cmin; lc <= cmax; ++lc)
 #pragma unroll
-          for (int tt = 0; tt < NQ; ++tt)
-            acc = fma(zr[lc * NQ + tt], kp.qw[tt] * kp.h[1] * kp.phi[v - K * lc][tt], acc);
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          if (tt >= nc * NQ) break;
+          acc = fma(zr[tt], py[tt], acc);
+        }
         FZ[((c & 1) * R1 * R2 + tid) * NQ + tz] = acc;
       }
     }
@@ -265,7 +304,7 @@ __global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_w
           for (int tt = 0; tt < 2 * NQ; ++tt) {
             if (tt >= nc * NQ) break;
             double G, M;
-            pair_factors<K, NQ>(kp, 2, a0, b0, tt, G, M);
+            pair_factors<K, NQ>(kp, 2, a0, b0, tt, G, M);  // wave-uniform: scalar loads
             const int ti = lc0 * NQ + tt;  // 0..NQ-1: cell c-1, NQ..2NQ-1: cell c
             const double y1 = ti < NQ ? Yp1[ti % NQ] : Yc1[ti % NQ];
             const double y2 = ti < NQ ? Yp2[ti % NQ] : Yc2[ti % NQ];
@@ -275,20 +314,18 @@ __global__ __launch_bounds__((March<K, NQ, R1, R2>::NT)) __attribute__((amdgpu_w
           slab[(o == 0 ? rbase0 : rbase1) + (s3 - lo3) * w1 * w2] = v;
         }
         if (tid < R1 * R2) {
-          const int v3 = o + K, cmin = (v3 + K - 1) / K - 1, cmax = v3 / K;
           double acc = 0.0;
 #pragma unroll
-          for (int lc = 0; lc < 2; ++lc) {
-            if (lc < cmin || lc > cmax) continue;
-#pragma unroll
-            for (int tt = 0; tt < NQ; ++tt)
-              acc = fma(FZ[(((c - 1 + lc) & 1) * R1 * R2 + tid) * NQ + tt],
-                        kp.qw[tt] * kp.h[2] * kp.phi[v3 - K * lc][tt], acc);
-          }
+          for (int ti = 0; ti < 2 * NQ; ++ti)
+            acc = fma(FZ[(((c - 1 + ti / NQ) & 1) * R1 * R2 + tid) * NQ + ti % NQ], PZ[o * 2 * NQ + ti], acc);
           BF[o * R1 * R2 + fu2 * R1 + fu1] = acc;
         }
       }
     }
+    // the prefetched cell and row table land last: their loads had all three
+    // stages to arrive
+    kf_store(c + 1, reg);
+    if (tid < T::RS) RT[((c + 1) & 1) * T::RS + tid] = rtn;
     __syncthreads();
 
     // store the completed rows: one wave per row
