@@ -82,6 +82,16 @@ int check_desc(const gdx_problem_desc *d) {
   GDX_REQUIRE(d->nmodes == 0 || d->modes, GDX_ERR_ARG, "modes is NULL with nmodes > 0");
   GDX_REQUIRE(d->rhs_kind == GDX_RHS_CONST || d->rhs_kind == GDX_RHS_MANUFACTURED, GDX_ERR_ARG, "unknown rhs kind");
   GDX_REQUIRE(d->dir_kind == 0 || d->dir_kind == 1, GDX_ERR_ARG, "unknown Dirichlet kind");
+  GDX_REQUIRE(d->op_kind == GDX_OP_DIFFUSION || d->op_kind == GDX_OP_FLUX, GDX_ERR_ARG, "unknown operator kind");
+  if (d->op_kind == GDX_OP_FLUX) {
+    GDX_REQUIRE(gdx::flux_supported(d->dim, d->order, d->qdeg / 2 + 1), GDX_ERR_UNSUPPORTED,
+                "the flux operator is built for Q1 with at most 3 Gauss points per direction");
+    GDX_REQUIRE(d->rhs_kind == GDX_RHS_CONST && d->nmodes == 0, GDX_ERR_UNSUPPORTED,
+                "the flux operator takes a constant source and no kappa modes");
+    GDX_REQUIRE(d->flux_p >= 1.0 && d->flux_alpha >= 0.0 && d->flux_beta >= 0.0, GDX_ERR_ARG,
+                "flux parameters need p >= 1, alpha >= 0, beta >= 0");
+    return GDX_OK;
+  }
   GDX_REQUIRE(gdx::matrix_kernel_supported(d->dim, d->order, d->qdeg / 2 + 1), GDX_ERR_UNSUPPORTED,
               "no matrix kernel for this (dim, order, quadrature) combination");
   return GDX_OK;
@@ -170,6 +180,7 @@ int gdx_problem_destroy(gdx_problem *p) {
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
     gdx::solve_free(p);
+    gdx::flux_free(p);
     free_device(p);
     for (hipEvent_t &e : p->ev)
       if (e) (void)hipEventDestroy(e);
@@ -242,6 +253,18 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_HIP(hipSetDevice(p->ctx->device));
   hipStream_t st = p->ctx->stream;
   GDX_HIP(hipEventRecord(p->ev[0], st));
+  if (p->desc.op_kind == GDX_OP_FLUX) {
+    // residual_and_jacobian at the stored state: no coefficient or separate RHS phase
+    GDX_HIP(hipEventRecord(p->ev[1], st));
+    GDX_HIP(hipEventRecord(p->ev[2], st));
+    int rcf = gdx::flux_assemble(p, nullptr, false, st);
+    if (rcf) {
+      set_error("flux operator launch failed");
+      return rcf;
+    }
+    GDX_HIP(hipEventRecord(p->ev[3], st));
+    return GDX_OK;
+  }
   int rc = gdx::launch_coefficients(p, st);
   if (rc) {
     set_error("coefficient kernel launch failed");
@@ -259,6 +282,32 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   }
   GDX_HIP(hipEventRecord(p->ev[3], st));
   return GDX_OK;
+}
+
+int gdx_set_state(gdx_problem *p, const double *x) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_FLUX, GDX_ERR_STATE, "the problem's operator does not depend on a state");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int rc = gdx::flux_set_state(p, x);
+  if (rc) set_error("setting the state failed");
+  return rc;
+}
+
+int gdx_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep) {
+  GDX_REQUIRE(p && o, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_FLUX, GDX_ERR_STATE, "Newton needs a GDX_OP_FLUX problem");
+  GDX_REQUIRE(o->maxiter >= 0 && o->lin_maxiter >= 1 && o->rtol >= 0.0 && o->atol >= 0.0 && o->lin_rtol > 0.0,
+              GDX_ERR_ARG, "invalid Newton options");
+  GDX_REQUIRE(o->init == GDX_NEWTON_INIT_GIVEN || o->init == GDX_NEWTON_INIT_POISSON, GDX_ERR_ARG,
+              "unknown Newton initial guess");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int rc = gdx::solve_newton(p, o, x, rep);
+  if (rc) set_error("Newton iteration failed");
+  return rc;
 }
 
 int gdx_sync(gdx_problem *p) {

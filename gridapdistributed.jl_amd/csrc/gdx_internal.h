@@ -139,6 +139,7 @@ struct gdx_problem {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double phase_ms[4] = {0, 0, 0, 0};
   void *solve = nullptr;  // solver state (gdx_solve.hip), built on the first solve
+  void *flux = nullptr;   // nonlinear operator state (gdx_flux.hip)
 };
 
 namespace gdx {
@@ -147,6 +148,13 @@ int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
 void fill_kparams(gdx_problem *p);
 void tile_shape(int dim, int K, int R[3]);
 void solve_free(gdx_problem *p);
+int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep);
+bool flux_supported(int dim, int order, int nq);
+int flux_setup(gdx_problem *p);
+void flux_free(gdx_problem *p);
+double *flux_state(gdx_problem *p);
+int flux_set_state(gdx_problem *p, const double *xh);
+int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st);
 int solve_spmv(gdx_problem *p, const double *x, double *y);
 int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *x, int *iters, double *relres, double *ms);
 }  // namespace gdx

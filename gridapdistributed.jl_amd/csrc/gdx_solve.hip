@@ -248,16 +248,13 @@ int solve_spmv(gdx_problem *p, const double *xh, double *yh) {
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
-// Jacobi-PCG from x0 = 0; stops when the recursive residual meets rtol, then
-// reports the true FP64 residual ||b - A x|| / ||b|| recomputed explicitly.
-int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, double *relres, double *ms) {
-  int rc = solve_setup(p);
-  if (rc) return rc;
-  SolveData *s = solve_data(p);
+// Jacobi-PCG from x0 = 0 on the assembled system, the solution left in s->x;
+// stops when the recursive residual meets rtol, then computes the true FP64
+// residual ||b - A x|| / ||b|| explicitly.
+static int pcg_run(gdx_problem *p, SolveData *s, double rtol, int maxiter, int *iters, double *relres) {
   hipStream_t st = p->ctx->stream;
   const int n = s->n;
   const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
-  auto t0 = std::chrono::steady_clock::now();
   hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
   hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
   (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
@@ -292,13 +289,119 @@ int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, 
   spmv(p, s, s->x, s->q, st);
   hipLaunchKernelGGL(k_cg_init, g, b, 0, st, n, s->b, s->dinv, s->q, s->z, s->r);  // q = b - Ax
   dot2(s, s->q, s->q, nullptr, nullptr, s->sc + 6, st);
-  if (hipMemcpyAsync(h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
-  if (xh && hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
-  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
-  auto t1 = std::chrono::steady_clock::now();
+  if (hipMemcpyAsync(h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GDX_ERR_HIP;
   if (iters) *iters = it;
   if (relres) *relres = bnorm > 0.0 ? std::sqrt(h[0]) / bnorm : 0.0;
+  return GDX_OK;
+}
+
+int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, double *relres, double *ms) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  auto t0 = std::chrono::steady_clock::now();
+  if ((rc = pcg_run(p, s, rtol, maxiter, iters, relres))) return rc;
+  if (xh && (hipMemcpyAsync(xh, s->x, s->n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess))
+    return GDX_ERR_HIP;
+  auto t1 = std::chrono::steady_clock::now();
   if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return GDX_OK;
+}
+
+// x -= d
+__global__ void k_sub(int n, double *__restrict__ x, const double *__restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] -= d[i];
+}
+
+// Newton on the flux operator: r = residual at x (into the RHS), J = Jacobian
+// (into the CSR values), J dx = r by Jacobi-PCG, x -= dx.  All vectors stay on
+// the device; the host reads one norm per iteration.
+int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newton_report *rep) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  if ((rc = flux_setup(p))) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  const int n = s->n;
+  const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
+  double *xs = flux_state(p);
+  using clk = std::chrono::steady_clock;
+  auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+  const auto t0 = clk::now();
+  double ms_asm = 0.0, ms_sol = 0.0;
+  if (xh) {
+    if (hipMemcpyAsync(xs, xh, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  } else if (n > 0 && hipMemsetAsync(xs, 0, n * sizeof(double), st) != hipSuccess) {
+    return GDX_ERR_HIP;
+  }
+  // assemble at xs and return ||r||
+  auto assemble = [&](bool linear, double *rnorm) -> int {
+    const auto ta = clk::now();
+    int r = flux_assemble(p, xs, linear, st);
+    if (r) return r;
+    hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+    dot2(s, s->b, s->b, nullptr, nullptr, s->sc + 6, st);
+    double h;
+    if (hipMemcpyAsync(&h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return GDX_ERR_HIP;
+    *rnorm = std::sqrt(h);
+    ms_asm += ms_since(ta);
+    return GDX_OK;
+  };
+  auto correct = [&](int *li) -> int {
+    const auto ts = clk::now();
+    double rr;
+    int r = pcg_run(p, s, o->lin_rtol, o->lin_maxiter, li, &rr);
+    if (r) return r;
+    hipLaunchKernelGGL(k_sub, g, b, 0, st, n, xs, s->x);
+    if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
+    ms_sol += ms_since(ts);
+    return GDX_OK;
+  };
+  double res_ref = 0.0, res = 0.0;
+  int lin_total = 0, li = 0;
+  if ((rc = assemble(false, &res_ref))) return rc;
+  if (o->init == GDX_NEWTON_INIT_POISSON) {
+    double r0;
+    if ((rc = assemble(true, &r0))) return rc;
+    if (r0 > 0.0) {
+      if ((rc = correct(&li))) return rc;
+      lin_total += li;
+    }
+  }
+  const double tol = std::max(o->atol, o->rtol * res_ref);
+  int it = 0;
+  bool conv = false;
+  for (;; ++it) {
+    if ((rc = assemble(false, &res))) return rc;
+    if (!(res == res)) break;  // NaN: diverged
+    if (res <= tol) {
+      conv = true;
+      break;
+    }
+    if (it == o->maxiter) break;
+    if ((rc = correct(&li))) return rc;
+    lin_total += li;
+  }
+  if (xh && (hipMemcpyAsync(xh, xs, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess))
+    return GDX_ERR_HIP;
+  if (rep) {
+    rep->iters = it;
+    rep->lin_iters = lin_total;
+    rep->converged = conv ? 1 : 0;
+    rep->res_ref = res_ref;
+    rep->res_final = res;
+    rep->ms_total = ms_since(t0);
+    rep->ms_assembly = ms_asm;
+    rep->ms_solve = ms_sol;
+  }
   return GDX_OK;
 }
 

@@ -13,6 +13,8 @@ from .fespaces import (AffineFEOperator, ConstantCoefficient, ConstantSource, Di
                        assemble_matrix_and_vector, assemble_matrix_and_vector_, collect_cell_matrix_and_vector,
                        lagrangian, problem_desc)
 from .geometry import CartesianDiscreteModel, DeviceContext, Measure, Triangulation, with_gpu
+from .nonlinear import (FEFunction, FEOperator, FESolver, FluxJacobian, FluxResidual, NLSolver, PowerLawFlux,
+                        jacobian, residual, residual_and_jacobian, residual_and_jacobian_, solve, zero)
 
 
 def problem_from_workload(ctx, p, strategy=None):
@@ -36,3 +38,20 @@ def problem_from_workload(ctx, p, strategy=None):
         strategy = FullyAssembledRows() if p.get("strategy") == "full" else SubAssembledRows()
     assem = SparseMatrixAssembler(U, V, strategy)
     return assem, collect_cell_matrix_and_vector(U, V, a, l, None)
+
+
+def flux_operator_from_workload(ctx, p, strategy=None):
+    """FEOperator(r, j, U, V, assem) of a flux workload (op == "flux")."""
+    model = CartesianDiscreteModel(ctx, p["parts"], p["domain"], p["cells"])
+    reffe = ReferenceFE(lagrangian, Float64, p["order"])
+    V = TestFESpace(model, reffe, dirichlet_tags="boundary" if p.get("dirichlet", True) else None)
+    uD = p.get("uD", (0.0, 0.0, 0.0, 0.0))
+    U = TrialFESpace(V, LinearFunction(uD[0], uD[1:]))
+    dO = Measure(Triangulation(model), p["qdeg"])
+    fp, fa, fb = p["flux"]
+    flux = PowerLawFlux(fp, fa, fb)
+    r = Integral(FluxResidual(flux, ConstantSource(p.get("rhs_const", 0.0))), dO)
+    j = Integral(FluxJacobian(flux), dO)
+    if strategy is None:
+        strategy = FullyAssembledRows() if p.get("strategy") == "full" else SubAssembledRows()
+    return FEOperator(r, j, U, V, SparseMatrixAssembler(U, V, strategy))

@@ -15,6 +15,8 @@ ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ER
 SUB_ASSEMBLED_ROWS, FULLY_ASSEMBLED_ROWS = 0, 1
 PREC_FP64 = 0
 RHS_CONST, RHS_MANUFACTURED = 0, 1
+OP_DIFFUSION, OP_FLUX = 0, 1
+NEWTON_INIT_GIVEN, NEWTON_INIT_POISSON = 0, 1
 SZ_NLDOFS, SZ_NOWN, SZ_NGHOST_ROWS, SZ_NGHOST_COLS, SZ_NNZ, SZ_COUNT = range(6)
 
 # every function declared in include/gdx.h
@@ -22,7 +24,7 @@ EXPORTED = [
     "gdx_version", "gdx_last_error", "gdx_init", "gdx_finalize", "gdx_problem_create", "gdx_problem_destroy",
     "gdx_symbolic", "gdx_numeric", "gdx_num_parts", "gdx_num_global_dofs", "gdx_part_sizes", "gdx_export_fe",
     "gdx_export_rows", "gdx_export_cols", "gdx_export_csr", "gdx_export_rhs", "gdx_device_values", "gdx_stream",
-    "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg",
+    "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg", "gdx_set_state", "gdx_newton",
 ]
 
 
@@ -38,8 +40,20 @@ class ProblemDesc(ctypes.Structure):
         ("cells", ctypes.c_int64 * 3), ("parts", ctypes.c_int32 * 3), ("domain", ctypes.c_double * 6),
         ("coef_const", ctypes.c_double), ("nmodes", ctypes.c_int32), ("modes", ctypes.POINTER(ctypes.c_double)),
         ("rhs_kind", ctypes.c_int32), ("rhs_const", ctypes.c_double), ("dir_kind", ctypes.c_int32),
-        ("dir_coef", ctypes.c_double * 4),
+        ("dir_coef", ctypes.c_double * 4), ("op_kind", ctypes.c_int32), ("flux_p", ctypes.c_double),
+        ("flux_alpha", ctypes.c_double), ("flux_beta", ctypes.c_double),
     ]
+
+
+class NewtonOpts(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiter", ctypes.c_int32),
+                ("lin_rtol", ctypes.c_double), ("lin_maxiter", ctypes.c_int32), ("init", ctypes.c_int32)]
+
+
+class NewtonReport(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int32), ("lin_iters", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("res_ref", ctypes.c_double), ("res_final", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("ms_assembly", ctypes.c_double), ("ms_solve", ctypes.c_double)]
 
 
 _LIB = None
@@ -79,6 +93,8 @@ def lib():
         "gdx_owned_nnz": [vp, P(i64)],
         "gdx_spmv": [vp, vp, vp],
         "gdx_pcg": [vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double)],
+        "gdx_set_state": [vp, vp],
+        "gdx_newton": [vp, P(NewtonOpts), vp, P(NewtonReport)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -86,6 +86,9 @@ class PVector:
     def __len__(self):
         return self.index_partition[0].ngids
 
+    def copy(self):
+        return PVector([v.copy() for v in self.partition], self.index_partition)
+
 
 class PSparseMatrix:
     """Per-part local CSR matrices on row/col index partitions."""
@@ -114,6 +117,10 @@ class PSparseMatrix:
 
     def to_csc(self):
         return [A.tocsc() for A in self.matrix_partition]
+
+    def copy(self):
+        """Base.copy(::PSparseMatrix) (src/Algebra.jl:34-38): values copied, partitions shared."""
+        return PSparseMatrix([A.copy() for A in self.matrix_partition], self.row_partition, self.col_partition)
 
     def __matmul__(self, x):
         """A*x for a global numpy vector or a PVector (owned rows)."""

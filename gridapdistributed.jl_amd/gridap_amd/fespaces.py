@@ -134,6 +134,10 @@ def collect_cell_matrix_and_vector(U, V, a, l, uhd=None):
 
 def problem_desc(data, strategy):
     """Lower collected data to the C descriptor (gdx_problem_desc)."""
+    if "flux_op" in data:
+        from .nonlinear import flux_problem_desc
+
+        return flux_problem_desc(data["flux_op"], strategy)
     U, V, a, l = data["U"], data["V"], data["a"], data["l"]
     m = V.model
     d = _lib.ProblemDesc()
@@ -217,6 +221,29 @@ class GpuProblem:
         _lib.call("gdx_pcg", self.handle, float(rtol), int(maxiter), None if x is None else x.ctypes.data,
                   ctypes.byref(it), ctypes.byref(rr), ctypes.byref(ms))
         return x, it.value, rr.value, ms.value
+
+    def set_state(self, x):
+        """State of a nonlinear operator: free values by global id (None = zero)."""
+        if x is None:
+            _lib.call("gdx_set_state", self.handle, None)
+            return
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.ngdofs,):
+            raise ValueError(f"state must have {self.ngdofs} free values")
+        _lib.call("gdx_set_state", self.handle, x.ctypes.data)
+
+    def newton(self, x0=None, rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8, lin_maxiter=20000,
+               init=_lib.NEWTON_INIT_GIVEN):
+        """Newton with Jacobi-PCG corrections on the device; returns (x, report dict)."""
+        x = np.zeros(self.ngdofs) if x0 is None else np.array(x0, dtype=np.float64)
+        if x.shape != (self.ngdofs,):
+            raise ValueError(f"initial guess must have {self.ngdofs} free values")
+        o = _lib.NewtonOpts(float(rtol), float(atol), int(maxiter), float(lin_rtol), int(lin_maxiter), int(init))
+        r = _lib.NewtonReport()
+        _lib.call("gdx_newton", self.handle, ctypes.byref(o), x.ctypes.data, ctypes.byref(r))
+        rep = {k: getattr(r, k) for k, _ in _lib.NewtonReport._fields_}
+        rep["converged"] = bool(rep["converged"])
+        return x, rep
 
     def stream(self):
         s = ctypes.c_void_p()

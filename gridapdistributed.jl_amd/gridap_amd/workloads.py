@@ -12,6 +12,8 @@ Cartesian mesh, in the vocabulary of the reference's drivers
   coef_const,modes kappa(x) = coef_const * exp(sum_m a_m cos(2 pi k_m.x + phi_m))
   rhs, rhs_const   "const": f = rhs_const; "manufactured": f = -div(kappa grad uD)
   dirichlet, uD    Dirichlet on the whole boundary with uD = c0 + c1 x + c2 y + c3 z
+  op, flux         "flux": the nonlinear operator of test/PLaplacianTests.jl:24-31 with
+                   sigma(g) = (alpha + beta |g|^2)^((p-2)/2) g, flux = (p, alpha, beta)
 """
 import numpy as np
 
@@ -48,6 +50,37 @@ def poisson3d_q1(n, parts=(1, 1, 1), strategy="sub", modes=None, rhs_const=1.0):
                 domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy=strategy, coef_const=1.0,
                 modes=np.zeros((0, 5)) if modes is None else modes, rhs="const", rhs_const=rhs_const,
                 dirichlet=True, uD=(0.0, 0.0, 0.0, 0.0))
+
+
+def plaplacian3d_q1(n=96, parts=(4, 2, 2), p=3.0, eps=1e-2, strategy="sub"):
+    """p-Laplacian on (0,1)^3, Q1, quadrature degree 2, f = 1, u = 0 on the boundary,
+    sigma(g) = (eps^2 + |g|^2)^((p-2)/2) g (regularised at grad u = 0)."""
+    return dict(dim=3, order=1, qdeg=2, cells=(n, n, n), parts=tuple(parts),
+                domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy=strategy, coef_const=1.0,
+                modes=np.zeros((0, 5)), rhs="const", rhs_const=1.0, dirichlet=True, uD=(0.0, 0.0, 0.0, 0.0),
+                op="flux", flux=(float(p), float(eps) ** 2, 1.0))
+
+
+def flux2d_reference_test(parts=(2, 2), strategy="sub"):
+    """The nonlinear problem of test/PLaplacianTests.jl:20-36: domain (0,4)^2,
+    4x4 cells, Q1, Measure degree 2, sigma(g) = (1 + |g|^2) g, u = x + y, f = 0."""
+    return dict(dim=2, order=1, qdeg=2, cells=(4, 4), parts=tuple(parts), domain=(0.0, 4.0, 0.0, 4.0),
+                strategy=strategy, coef_const=1.0, modes=np.zeros((0, 5)), rhs="const", rhs_const=0.0,
+                dirichlet=True, uD=(0.0, 1.0, 1.0, 0.0), op="flux", flux=(4.0, 1.0, 1.0))
+
+
+def lattice_shape(p):
+    """Global node lattice extents (x fastest): order * cells + 1 per direction."""
+    K = p["order"]
+    return tuple(K * c + 1 for c in p["cells"])
+
+
+def lattice_coords(p):
+    """Coordinates of every global lattice node, shape (nnodes, dim), x fastest."""
+    shp = lattice_shape(p)
+    axes = [np.linspace(p["domain"][2 * d], p["domain"][2 * d + 1], shp[d]) for d in range(p["dim"])]
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.ravel(order="F") for g in grids], axis=1)
 
 
 def free_dofs(p):

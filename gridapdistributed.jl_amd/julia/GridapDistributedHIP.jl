@@ -29,6 +29,16 @@ struct GdxProblemDesc
   cells::NTuple{3,Int64}; parts::NTuple{3,Int32}; domain::NTuple{6,Float64}
   coef_const::Float64; nmodes::Int32; modes::Ptr{Float64}
   rhs_kind::Int32; rhs_const::Float64; dir_kind::Int32; dir_coef::NTuple{4,Float64}
+  op_kind::Int32; flux_p::Float64; flux_alpha::Float64; flux_beta::Float64
+end
+
+# gdx_newton_opts / gdx_newton_report
+struct GdxNewtonOpts
+  rtol::Float64; atol::Float64; maxiter::Int32; lin_rtol::Float64; lin_maxiter::Int32; init::Int32
+end
+struct GdxNewtonReport
+  iters::Int32; lin_iters::Int32; converged::Int32; res_ref::Float64; res_final::Float64
+  ms_total::Float64; ms_assembly::Float64; ms_solve::Float64
 end
 
 check(rc, f) = rc == 0 || error("$f failed ($rc): ", unsafe_string(ccall((:gdx_last_error, libgdx), Cstring, ())))
@@ -103,6 +113,36 @@ function assemble_matrix_and_vector_hip(p::GdxProblem, ranks, ngdofs)
   A = PSparseMatrix(values, rows, cols)
   b = PVector(bs, rows)
   A, b
+end
+
+"""
+    residual_and_jacobian_hip(p, ranks, ngdofs, x) -> (b::PVector, A::PSparseMatrix)
+
+residual_and_jacobian(op, uh) of test/PLaplacianTests.jl:44 for a GDX_OP_FLUX
+problem: x holds uh's free values in global-id order (get_free_dof_values
+gathered by gid); Dirichlet values come from the descriptor.
+"""
+function residual_and_jacobian_hip(p::GdxProblem, ranks, ngdofs, x::Vector{Float64})
+  GC.@preserve x check(ccall((:gdx_set_state, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), p.h, x), :gdx_set_state)
+  A, b = assemble_matrix_and_vector_hip(p, ranks, ngdofs)
+  b, A
+end
+
+"""
+    newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false) -> (x, report)
+
+solve(FESolver(NLSolver(method=:newton)), op) of test/PLaplacianTests.jl:54-57
+with device Jacobi-PCG corrections; x is the solution's free values by gid.
+"""
+function newton_hip(p::GdxProblem, x0::Vector{Float64}; rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8,
+                    lin_maxiter=20000, poisson_init=false)
+  x = copy(x0)
+  o = GdxNewtonOpts(rtol, atol, maxiter, lin_rtol, lin_maxiter, poisson_init ? 1 : 0)
+  rep = Ref{GdxNewtonReport}()
+  GC.@preserve x check(ccall((:gdx_newton, libgdx), Cint,
+                             (Ptr{Cvoid}, Ref{GdxNewtonOpts}, Ptr{Float64}, Ref{GdxNewtonReport}), p.h, o, x, rep),
+                       :gdx_newton)
+  x, rep[]
 end
 
 end # module

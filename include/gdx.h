@@ -47,6 +47,10 @@ extern "C" {
 /* numeric precision of the cell contractions; the CSR accumulation is FP64 */
 #define GDX_PREC_FP64 0
 
+/* operator kinds */
+#define GDX_OP_DIFFUSION 0 /* affine: a(u,v) and l(v) below */
+#define GDX_OP_FLUX 1      /* nonlinear flux operator at a state, see gdx_set_state */
+
 /* right-hand side kinds */
 #define GDX_RHS_CONST 0        /* f = rhs_const */
 #define GDX_RHS_MANUFACTURED 1 /* f = -div(kappa grad uD) for the linear uD below */
@@ -65,6 +69,14 @@ typedef struct gdx_problem gdx_problem;
  *   kappa(x) = coef_const * exp( sum_m a_m cos(2 pi k_m . x + phi_m) )
  *   modes: nmodes rows of (k1, k2, k3, a, phi)
  *   uD(x)  = dir_coef[0] + dir_coef[1] x + dir_coef[2] y + dir_coef[3] z
+ *
+ * op_kind GDX_OP_FLUX (order 1, f = rhs_const, no kappa modes) replaces the
+ * FEOperator(r, j, U, V, assem) of test/PLaplacianTests.jl:24-42:
+ *   r(u,v)    = int( grad(v) . sigma(grad(u)) - v f ) dOmega
+ *   j(u,du,v) = int( grad(v) . dsigma(grad(du), grad(u)) ) dOmega
+ *   sigma(g)  = (flux_alpha + flux_beta |g|^2)^((flux_p - 2)/2) g
+ * (the reference test is flux_p = 4, flux_alpha = flux_beta = 1; the
+ * p-Laplacian is flux_p = p with flux_alpha = eps^2 > 0, flux_beta = 1).
  */
 typedef struct gdx_problem_desc {
   int32_t dim;      /* 2 or 3 */
@@ -81,6 +93,8 @@ typedef struct gdx_problem_desc {
   double rhs_const;
   int32_t dir_kind; /* 0: no Dirichlet boundary, 1: Dirichlet on the whole boundary */
   double dir_coef[4];
+  int32_t op_kind;  /* GDX_OP_DIFFUSION or GDX_OP_FLUX */
+  double flux_p, flux_alpha, flux_beta;
 } gdx_problem_desc;
 
 /* sizes reported per part by gdx_part_sizes (index into the int64 array) */
@@ -144,6 +158,40 @@ int gdx_spmv(gdx_problem *prob, const double *x, double *y);
    true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time */
 int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,
             double *ms);
+
+/* Nonlinear operators (GDX_OP_FLUX).  The state uh of
+   residual_and_jacobian(op, uh) (test/PLaplacianTests.jl:44-48) is given by its
+   free values x (host array of length ngdofs, global-id order; NULL = zero);
+   Dirichlet values come from the descriptor's uD.  After gdx_set_state,
+   gdx_numeric assembles the Jacobian into the CSR values and the residual into
+   the RHS (residual_and_jacobian; called again it is residual_and_jacobian!). */
+int gdx_set_state(gdx_problem *prob, const double *x);
+
+#define GDX_NEWTON_INIT_GIVEN 0   /* start from x as passed (NULL = zero) */
+#define GDX_NEWTON_INIT_POISSON 1 /* first solve the linear problem flux_p = 2, alpha = 1, beta = 0 */
+typedef struct gdx_newton_opts {
+  double rtol;         /* stop when ||r(x)|| <= max(atol, rtol ||r(x_given)||) */
+  double atol;
+  int32_t maxiter;     /* Newton iterations */
+  double lin_rtol;     /* Jacobi-PCG tolerance of each Newton correction */
+  int32_t lin_maxiter;
+  int32_t init;        /* GDX_NEWTON_INIT_* */
+} gdx_newton_opts;
+typedef struct gdx_newton_report {
+  int32_t iters;       /* Newton corrections applied */
+  int32_t lin_iters;   /* PCG iterations over all corrections (initial guess included) */
+  int32_t converged;
+  double res_ref;      /* ||r(x_given)|| (2-norm over free dofs) */
+  double res_final;    /* ||r(x)|| at the returned x */
+  double ms_total, ms_assembly, ms_solve;
+} gdx_newton_report;
+/* Newton's method on the flux operator: NLSolver(method=:newton) +
+   FESolver + solve(solver, op) (test/PLaplacianTests.jl:54-57), with the
+   Jacobian and residual re-assembled in place every iteration and each
+   correction solved by Jacobi-PCG on the device.  x: host array of length
+   ngdofs, in = initial guess (NULL with GDX_NEWTON_INIT_GIVEN means zero),
+   out = the solution's free values. */
+int gdx_newton(gdx_problem *prob, const gdx_newton_opts *opts, double *x, gdx_newton_report *report);
 
 /* device views and timing */
 int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);

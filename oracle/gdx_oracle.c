@@ -61,6 +61,14 @@ typedef struct {
   double rhs_const;
   int dir_kind;          /* 0: no Dirichlet; 1: whole boundary, uD = c0 + c1 x + c2 y + c3 z */
   double dir_coef[4];
+  /* op_kind 1: nonlinear flux operator at a state u (test/PLaplacianTests.jl:26-31 form)
+       residual r(u,v) = int( grad(v).sigma(grad(u)) - v f ) dOmega, f = rhs_const
+       jacobian j(u,du,v) = int( grad(v).dsigma(grad(du),grad(u)) ) dOmega
+       sigma(g) = (alpha + beta |g|^2)^((p-2)/2) g
+     state: u at every global lattice node (x fastest), NULL = 0; Dirichlet nodes take uD */
+  int op_kind;
+  double flux_p, flux_alpha, flux_beta;
+  const double *state;
 } orc_desc;
 
 typedef struct {
@@ -94,6 +102,7 @@ typedef struct {
 typedef struct {
   orc_desc d;
   double *modes;
+  double *state;
   int nparts, K, D, nq;
   i64 nn[3];       /* global node counts per dim */
   i64 ngdofs;
@@ -464,8 +473,19 @@ static double eval_uD(const orc_ctx *C, const double x[3]) {
 
 /* cell integration in Gridap's local node order.  Geometry through the Q1 map
    from the cell's vertices: J = sum_v x_v (x) grad N_v, detJ, J^{-1}. */
+/* flux operator: kappa(s) = s^((p-2)/2) and its derivative factor (p-2) beta s^((p-4)/2), with s = alpha + beta |g|^2
+   (written with pow on both exponents, independently of the library's kappa/s form) */
+static double flux_kappa(const orc_ctx *C, double s) { return pow(s, 0.5 * (C->d.flux_p - 2.0)); }
+static double flux_gamma(const orc_ctx *C, double s) {
+  return (C->d.flux_p - 2.0) * C->d.flux_beta * pow(s, 0.5 * (C->d.flux_p - 4.0));
+}
+
+/* ucell: u at the cell's nodes (op_kind 1).  energy != NULL: accumulate
+   int( (alpha + beta|grad u|^2)^(p/2) / (p beta) - f u ) over the cell instead of Ac/bc;
+   its gradient with respect to the free nodal values is the residual. */
 static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, int coff[27][3],
-                           double *Ac, double *bc, double uD[27], const int isdir[27]) {
+                           double *Ac, double *bc, double uD[27], const int isdir[27], const double ucell[27],
+                           double *energy) {
   int D = C->D, K = C->K, nq = C->nq;
   double vx[8][3];
   int nv = D == 3 ? 8 : 4;
@@ -533,6 +553,34 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
             for (int d = 0; d < D; d++) grad[a][e] += iJ[d][e] * ref[d];
           }
         }
+        if (C->d.op_kind == 1) {
+          double gu[3] = {0.0, 0.0, 0.0}, uq = 0.0;
+          for (int b = 0; b < ncn; b++) {
+            uq += ucell[b] * phi[b];
+            for (int e = 0; e < D; e++) gu[e] += ucell[b] * grad[b][e];
+          }
+          double s2 = C->d.flux_alpha + C->d.flux_beta * (gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2]);
+          double fq = C->d.rhs_const;
+          if (energy) {
+            *energy += (pow(s2, 0.5 * C->d.flux_p) / (C->d.flux_p * C->d.flux_beta) - fq * uq) * dV;
+            continue;
+          }
+          double kq = flux_kappa(C, s2), gq = flux_gamma(C, s2);
+          for (int a = 0; a < ncn; a++) {
+            double gua = 0.0;
+            for (int e = 0; e < D; e++) gua += gu[e] * grad[a][e];
+            bc[a] += (kq * gua - fq * phi[a]) * dV;
+            for (int b = 0; b < ncn; b++) {
+              double sab = 0.0, gub = 0.0;
+              for (int e = 0; e < D; e++) {
+                sab += grad[a][e] * grad[b][e];
+                gub += gu[e] * grad[b][e];
+              }
+              Ac[a * ncn + b] += (kq * sab + gq * gua * gub) * dV;
+            }
+          }
+          continue;
+        }
         for (int a = 0; a < ncn; a++) {
           bc[a] += f * phi[a] * dV;
           for (int b = 0; b < ncn; b++) {
@@ -542,10 +590,19 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
           }
         }
       }
-  /* Dirichlet lifting: b_c -= A_c u_D */
+  /* Dirichlet lifting: b_c -= A_c u_D (the flux operator's residual already
+     sees the Dirichlet values through u) */
+  if (C->d.op_kind == 1) return;
   for (int a = 0; a < ncn; a++)
     for (int b = 0; b < ncn; b++)
       if (isdir[b]) bc[a] -= Ac[a * ncn + b] * uD[b];
+}
+
+/* u at global lattice node g: the state, or uD on Dirichlet nodes */
+static double state_at(const orc_ctx *C, const i64 g[3], int isdir, double uDv) {
+  if (isdir) return uDv;
+  if (!C->state) return 0.0;
+  return C->state[g[0] + C->nn[0] * (g[1] + C->nn[1] * g[2])];
 }
 
 static void coo_push(part_t *P, i64 i, i64 j, double v) {
@@ -567,7 +624,7 @@ static void numeric_loop(orc_ctx *C, part_t *P, int mode, const i64 *fe_to_row, 
   int D = C->D, K = C->K;
   int coff[27][3];
   int ncn = cell_nodes(D, K, coff);
-  double Ac[27 * 27], bc[27], uD[27];
+  double Ac[27 * 27], bc[27], uD[27], ucell[27];
   int isdir[27];
   i64 row[27], col[27];
   int FA = C->d.strategy == 1;
@@ -586,8 +643,9 @@ static void numeric_loop(orc_ctx *C, part_t *P, int mode, const i64 *fe_to_row, 
       double x[3] = {0, 0, 0};
       for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
       uD[a] = isdir[a] ? eval_uD(C, x) : 0.0;
+      ucell[a] = state_at(C, g, isdir[a], uD[a]);
     }
-    integrate_cell(C, cx, cy, cz, ncn, coff, Ac, bc, uD, isdir);
+    integrate_cell(C, cx, cy, cz, ncn, coff, Ac, bc, uD, isdir, ucell, NULL);
     /* add_entries!: columns outer, rows inner, skipping non-positive ids */
     for (int b = 0; b < ncn; b++) {
       if (col[b] < 0) continue;
@@ -922,7 +980,15 @@ void *orc_run(const orc_desc *d, int inplace_reps) {
     memcpy(C->modes, d->modes, sizeof(double) * 5 * d->nmodes);
   }
   C->d.modes = C->modes;
+  C->state = NULL;
   gauss_legendre(C->nq, C->qx, C->qw);
+  for (int k = 0; k < 3; k++) C->nn[k] = k < d->dim ? d->order * d->cells[k] + 1 : 1;
+  if (d->state) {
+    i64 n = C->nn[0] * C->nn[1] * C->nn[2];
+    C->state = malloc(sizeof(double) * n);
+    memcpy(C->state, d->state, sizeof(double) * n);
+  }
+  C->d.state = C->state;
   double t0 = now_s();
   setup(C);
   double t1 = now_s();
@@ -996,5 +1062,35 @@ void orc_free(void *h) {
   }
   free(C->P);
   free(C->modes);
+  free(C->state);
   free(C);
+}
+
+/* Energy of the flux operator at the state (op_kind 1, flux_beta != 0), summed
+   over every cell of the mesh once:
+     E(u) = int( (alpha + beta |grad u|^2)^(p/2) / (p beta) - f u ) dOmega.
+   Its derivative along a free nodal direction is the residual, which pins the
+   residual independently of its own formula. */
+double orc_energy(void *h) {
+  orc_ctx *C = h;
+  if (!C || C->d.op_kind != 1 || C->d.flux_beta == 0.0) return NAN;
+  int D = C->D, K = C->K;
+  int coff[27][3];
+  int ncn = cell_nodes(D, K, coff);
+  double Ac[27 * 27], bc[27], uD[27], ucell[27];
+  int isdir[27];
+  double E = 0.0;
+  i64 lo[3] = {0, 0, 0}, hi[3] = {C->d.cells[0], D > 1 ? C->d.cells[1] : 1, D > 2 ? C->d.cells[2] : 1};
+  FOR_BOX(lo, hi, D, cx, cy, cz) {
+    for (int a = 0; a < ncn; a++) {
+      i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
+      double x[3] = {0, 0, 0};
+      for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
+      isdir[a] = C->d.dir_kind == 1 && is_dirichlet_gnode(C, g);
+      uD[a] = isdir[a] ? eval_uD(C, x) : 0.0;
+      ucell[a] = state_at(C, g, isdir[a], uD[a]);
+    }
+    integrate_cell(C, cx, cy, cz, ncn, coff, Ac, bc, uD, isdir, ucell, &E);
+  }
+  return E;
 }

@@ -25,6 +25,8 @@ class OrcDesc(ctypes.Structure):
         ("nmodes", ctypes.c_int), ("modes", ctypes.POINTER(ctypes.c_double)), ("coef_const", ctypes.c_double),
         ("rhs_kind", ctypes.c_int), ("rhs_const", ctypes.c_double),
         ("dir_kind", ctypes.c_int), ("dir_coef", ctypes.c_double * 4),
+        ("op_kind", ctypes.c_int), ("flux_p", ctypes.c_double), ("flux_alpha", ctypes.c_double),
+        ("flux_beta", ctypes.c_double), ("state", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -50,6 +52,8 @@ def lib():
         L.orc_part_fe.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.orc_part_matrix.argtypes = [vp, ctypes.c_int] + [vp] * 8
         L.orc_free.argtypes = [vp]
+        L.orc_energy.restype = ctypes.c_double
+        L.orc_energy.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -81,12 +85,34 @@ def make_desc(p):
     dc = list(p.get("uD", (0.0, 0.0, 0.0, 0.0)))
     for i in range(4):
         d.dir_coef[i] = dc[i]
+    # nonlinear flux operator sigma(g) = (alpha + beta|g|^2)^((p-2)/2) g at a lattice state
+    d.op_kind = 1 if p.get("op", "diffusion") == "flux" else 0
+    fp, fa, fb = p.get("flux", (2.0, 1.0, 0.0))
+    d.flux_p, d.flux_alpha, d.flux_beta = fp, fa, fb
+    st = p.get("state")
+    if st is not None:
+        st = np.ascontiguousarray(st, dtype=np.float64).ravel()
+        d._state_keep = st
+        d.state = st.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    else:
+        d.state = None
     return d
 
 
 class OracleResult:
-    def __init__(self, parts, ngdofs, times):
-        self.parts, self.ngdofs, self.times = parts, ngdofs, times
+    def __init__(self, parts, ngdofs, times, energy=None):
+        self.parts, self.ngdofs, self.times, self.energy = parts, ngdofs, times, energy
+
+
+def energy(p):
+    """Energy functional of the flux operator at p['state'] (see orc_energy)."""
+    L = lib()
+    d = make_desc(p)
+    h = L.orc_run(ctypes.byref(d), 0)
+    try:
+        return L.orc_energy(h)
+    finally:
+        L.orc_free(h)
 
 
 def run(p, inplace_reps=0):

@@ -103,3 +103,64 @@ def test_version_and_errors(host_ctx):
     with pytest.raises(_lib.GdxError):
         gp.sizes(99)
     gp.close()
+
+
+FLUX_CASES = [
+    W.plaplacian3d_q1(5, (2, 2, 1)),
+    W.plaplacian3d_q1(6, (1, 2, 2), strategy="full"),
+    W.flux2d_reference_test((2, 2)),
+    W.flux2d_reference_test((2, 1), "full"),
+]
+
+
+@pytest.mark.parametrize("p", FLUX_CASES, ids=lambda p: f"flux-{p['dim']}d-{p['cells']}-{p['parts']}-{p['strategy']}")
+def test_flux_structure_matches_oracle(host_ctx, p):
+    """The nonlinear operator shares the affine path's symbolic phase: same
+    PRanges and CSR pattern as the oracle's Jacobian."""
+    from gridap_amd import flux_operator_from_workload
+    from gridap_amd.nonlinear import flux_problem_desc
+
+    r = oracle.run(p)
+    op = flux_operator_from_workload(host_ctx, p)
+    desc, _ = flux_problem_desc(op, op.assem.strategy)
+    assert desc.op_kind == _lib.OP_FLUX
+    gp = fespaces.GpuProblem(host_ctx, desc)
+    gp.symbolic()
+    assert gp.ngdofs == r.ngdofs == W.free_dofs(p)
+    for k, P in enumerate(r.parts):
+        Q = gp.export_part(k)
+        for key in STRUCT_KEYS:
+            assert np.array_equal(P[key], Q[key]), (k, key)
+    # a host-only problem has no device state
+    with pytest.raises(_lib.GdxError) as e:
+        gp.set_state(None)
+    assert e.value.code == -3
+    gp.close()
+
+
+def test_flux_descriptor_errors(host_ctx):
+    from gridap_amd import flux_operator_from_workload
+    from gridap_amd.nonlinear import flux_problem_desc
+
+    L = _lib.lib()
+    op = flux_operator_from_workload(host_ctx, W.plaplacian3d_q1(4, (1, 1, 1)))
+    desc, _ = flux_problem_desc(op, op.assem.strategy)
+    h = ctypes.c_void_p()
+    bad = _lib.ProblemDesc.from_buffer_copy(desc)
+    bad.order = 2
+    assert L.gdx_problem_create(None, ctypes.byref(bad), ctypes.byref(h)) == -4
+    bad = _lib.ProblemDesc.from_buffer_copy(desc)
+    bad.flux_alpha = -1.0
+    assert L.gdx_problem_create(None, ctypes.byref(bad), ctypes.byref(h)) == -1
+    bad = _lib.ProblemDesc.from_buffer_copy(desc)
+    bad.op_kind = 7
+    assert L.gdx_problem_create(None, ctypes.byref(bad), ctypes.byref(h)) == -1
+    # Newton on an affine problem is refused
+    assem, data = problem_from_workload(host_ctx, W.poisson2d_q1(4, (2, 2)))
+    d2, _ = fespaces.problem_desc(data, assem.strategy)
+    gp = fespaces.GpuProblem(host_ctx, d2)
+    gp.symbolic()
+    o = _lib.NewtonOpts(1e-10, 0.0, 10, 1e-8, 100, 0)
+    rep = _lib.NewtonReport()
+    assert L.gdx_newton(gp.handle, ctypes.byref(o), None, ctypes.byref(rep)) == -3
+    gp.close()
