@@ -1,6 +1,8 @@
 // C ABI of libgdx (declared in include/gdx.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -57,8 +59,7 @@ int upload(T **dst, const std::vector<T> &src) {
 void free_device(gdx_problem *p) {
   gdx::Device &d = p->dev;
   void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_invr, d.pat_len,
-                  d.pat_dir,
-                  d.tiles, d.tile_pats};
+                  d.pat_dir, d.tiles, d.tile_pats, d.cellk};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   d = gdx::Device{};
@@ -293,6 +294,48 @@ int gdx_set_state(gdx_problem *p, const double *x) {
   int rc = gdx::flux_set_state(p, x);
   if (rc) set_error("setting the state failed");
   return rc;
+}
+
+int gdx_set_cell_coefficient(gdx_problem *p, const double *kappa) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
+              "cell-wise kappa needs the diffusion operator with a constant source");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  if (!kappa) {
+    p->coef_cell = false;
+    return GDX_OK;
+  }
+  const int64_t nc = gdx::num_cells(p);
+  for (int64_t i = 0; i < nc; ++i)
+    GDX_REQUIRE(kappa[i] > 0.0 && kappa[i] < INFINITY, GDX_ERR_ARG, "kappa must be positive and finite");
+  int rc = gdx::cell_coef_alloc(p);
+  if (rc) return rc;
+  GDX_HIP(hipMemcpy(p->dev.cellk, kappa, nc * sizeof(double), hipMemcpyHostToDevice));
+  p->coef_cell = true;
+  return GDX_OK;
+}
+
+int gdx_cell_coefficient_grf(gdx_problem *p, uint64_t seed, double corr_len, double log_scale) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
+              "cell-wise kappa needs the diffusion operator with a constant source");
+  GDX_REQUIRE(corr_len > 0.0 && log_scale >= 0.0 && log_scale < 50.0, GDX_ERR_ARG, "invalid field parameters");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int rc = gdx::grf_cell_coefficient(p, seed, corr_len, log_scale);
+  if (rc) set_error("random field generation failed");
+  return rc;
+}
+
+int gdx_export_cell_coefficient(gdx_problem *p, double *kappa) {
+  GDX_REQUIRE(p && kappa, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(p->coef_cell && p->dev.cellk, GDX_ERR_STATE, "no cell-wise coefficient is set");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  GDX_HIP(hipMemcpy(kappa, p->dev.cellk, gdx::num_cells(p) * sizeof(double), hipMemcpyDeviceToHost));
+  return GDX_OK;
 }
 
 int gdx_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep) {

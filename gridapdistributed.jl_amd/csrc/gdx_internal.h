@@ -105,6 +105,7 @@ struct Device {
   int *pat_dir = nullptr;      // pattern has Dirichlet slots
   Tile *tiles = nullptr;
   int *tile_pats = nullptr;    // per tile: its distinct pattern ids
+  double *cellk = nullptr;     // kappa per cell (cell-wise coefficient mode)
 };
 
 }  // namespace gdx
@@ -140,6 +141,7 @@ struct gdx_problem {
   double phase_ms[4] = {0, 0, 0, 0};
   void *solve = nullptr;  // solver state (gdx_solve.hip), built on the first solve
   void *flux = nullptr;   // nonlinear operator state (gdx_flux.hip)
+  bool coef_cell = false; // kappa from dev.cellk instead of the Fourier modes
 };
 
 namespace gdx {
@@ -149,7 +151,20 @@ void fill_kparams(gdx_problem *p);
 void tile_shape(int dim, int K, int R[3]);
 void solve_free(gdx_problem *p);
 int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep);
+// preconditioner callback of the flexible CG: z = M r (device vectors, global-id order)
+struct FpcgPrec {
+  int (*apply)(void *ctx, const double *r, double *z, hipStream_t st);
+  void *ctx;
+};
+int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double rtol, int maxiter, double *xh,
+               int *iters, double *relres, double *ms, int *fellback);
+int solve_dinv(gdx_problem *p, const double **dinv, int *n);
+void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st);
 bool flux_supported(int dim, int order, int nq);
+int64_t num_cells(const gdx_problem *p);
+int cell_coef_alloc(gdx_problem *p);
+int launch_cell_coefficients(gdx_problem *p, hipStream_t st);
+int grf_cell_coefficient(gdx_problem *p, uint64_t seed, double corr_len, double log_scale);
 int flux_setup(gdx_problem *p);
 void flux_free(gdx_problem *p);
 double *flux_state(gdx_problem *p);

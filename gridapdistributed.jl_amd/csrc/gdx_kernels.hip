@@ -304,6 +304,7 @@ __global__ __launch_bounds__(256) void k_assemble2d(const Tile *__restrict__ til
 static unsigned grid_for(int ntiles) { return 8u * (unsigned)((ntiles + 7) / 8); }
 
 int launch_coefficients(gdx_problem *p, hipStream_t st) {
+  if (p->coef_cell) return launch_cell_coefficients(p, st);
   const KParams &kp = p->kp;
   int qnmax = kp.qn[0];
   for (int d = 1; d < 3; ++d) qnmax = qnmax > kp.qn[d] ? qnmax : kp.qn[d];

@@ -1,0 +1,551 @@
+// Learned coarse solver of libgdx: a small convolutional multilevel network
+// (3 levels, channels 8/16/24, 3x3x3 convolutions, about 60k parameters)
+// trained and applied on the device with these kernels only.
+//
+//   x1 = relu(conv(in)) ; x1 = relu(conv(x1))                       level 0 (n^3)
+//   x2 = relu(conv(pool(x1))) ; x2 = relu(conv(x2))                  level 1
+//   x3 = relu(conv(pool(x2))) ; x3 = relu(conv(x3))                  level 2
+//   y2 = relu(conv(up(x3)) + x2) ; y2 = relu(conv(y2))               level 1
+//   y1 = relu(conv(up(y2)) + x1) ; out = conv(y1)                    level 0
+//
+// Storage: weights and activations in BF16 (round to nearest even), every
+// accumulation in FP32, gradients and the Adam state in FP32 with FP32 master
+// weights.  Tensors are [channel][z][y][x].  Convolutions are tiled 8x8x4
+// outputs per 256-thread workgroup with the input halo and the weights staged
+// in LDS; the weight gradient is reduced per workgroup and then across
+// workgroups in a fixed order, so training steps are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "gdx_nn.h"
+
+namespace gdx {
+namespace nn {
+
+namespace {
+
+constexpr int kT = 256;  // threads per workgroup
+constexpr int kTX = 8, kTY = 8, kTZ = 4, kHV = (kTX + 2) * (kTY + 2) * (kTZ + 2);
+constexpr int kPartialBlocks = 256;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline int tiles_of(int n) { return ((n + kTX - 1) / kTX) * ((n + kTY - 1) / kTY) * ((n + kTZ - 1) / kTZ); }
+
+__device__ __forceinline__ void tile_origin(int tile, int n, int &x0, int &y0, int &z0) {
+  const int ntx = (n + kTX - 1) / kTX, nty = (n + kTY - 1) / kTY;
+  x0 = (tile % ntx) * kTX;
+  y0 = ((tile / ntx) % nty) * kTY;
+  z0 = (tile / (ntx * nty)) * kTZ;
+}
+
+// MODE 0: forward, in = BF16 activations with KCI channels, weights W[KCO][KCI][27].
+// MODE 1: backward data, in = FP32 gradient with KCI (= the layer's outputs)
+//         channels, out = FP32 gradient of the layer's KCO inputs, weights
+//         transposed with the taps mirrored.
+template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM>
+__global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in_, const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, const uint16_t *__restrict__ skip,
+                                              void *__restrict__ out_) {
+  constexpr int CH = KCI < 8 ? KCI : 8;
+  __shared__ float sx[CH * kHV];
+  __shared__ float sw[KCO * CH * 27];
+  int x0, y0, z0;
+  tile_origin(blockIdx.x, n, x0, y0, z0);
+  const int tid = threadIdx.x, lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
+  const int64_t n3 = (int64_t)n * n * n;
+  float acc[KCO];
+#pragma unroll
+  for (int co = 0; co < KCO; ++co) acc[co] = MODE == 0 ? bias[co] : 0.0f;
+  for (int c0 = 0; c0 < KCI; c0 += CH) {
+    __syncthreads();
+    for (int i = tid; i < CH * kHV; i += kT) {
+      const int c = i / kHV, h = i % kHV, hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      float v = 0.0f;
+      if (c0 + c < KCI && gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n) {
+        const int64_t idx = (int64_t)(c0 + c) * n3 + ((int64_t)gz * n + gy) * n + gx;
+        v = MODE == 0 ? bf2f(reinterpret_cast<const uint16_t *>(in_)[idx]) : reinterpret_cast<const float *>(in_)[idx];
+      }
+      sx[i] = v;
+    }
+    for (int i = tid; i < KCO * CH * 27; i += kT) {
+      const int co = i / (CH * 27), r = i % (CH * 27), c = r / 27, t = r % 27;
+      float v = 0.0f;
+      if (c0 + c < KCI)
+        v = MODE == 0 ? bf2f(w[((int64_t)co * KCI + c0 + c) * 27 + t]) : bf2f(w[((int64_t)(c0 + c) * KCO + co) * 27 + 26 - t]);
+      sw[i] = v;
+    }
+    __syncthreads();
+    for (int c = 0; c < CH; ++c) {
+#pragma unroll
+      for (int t = 0; t < 27; ++t) {
+        const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+        const float v = sx[c * kHV + (lz + dz) * 100 + (ly + dy) * 10 + lx + dx];
+#pragma unroll
+        for (int co = 0; co < KCO; ++co) acc[co] = fmaf(v, sw[(co * CH + c) * 27 + t], acc[co]);
+      }
+    }
+  }
+  const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+  if (gx >= n || gy >= n || gz >= n) return;
+  const int64_t vi = ((int64_t)gz * n + gy) * n + gx;
+#pragma unroll
+  for (int co = 0; co < KCO; ++co) {
+    float v = acc[co];
+    if (SKIP) v += bf2f(skip[co * n3 + vi]);
+    if (RELU) v = fmaxf(v, 0.0f);
+    if (OUTF32) {
+      float *o = reinterpret_cast<float *>(out_) + co * n3 + vi;
+      *o = ACCUM ? *o + v : v;
+    } else {
+      reinterpret_cast<uint16_t *>(out_)[co * n3 + vi] = f2bf(v);
+    }
+  }
+}
+
+// weight and bias gradient of a layer: per-workgroup partial sums over its
+// tiles, partial[block][W (CO CI 27) | b (CO)]
+template <int CI, int CO>
+__global__ __launch_bounds__(kT) void k_conv_wgrad(int n, const uint16_t *__restrict__ x, const float *__restrict__ g,
+                                                   float *__restrict__ partial) {
+  constexpr int E = CO * CI * 27, EPT = (E + kT - 1) / kT, CH = CI < 8 ? CI : 8;
+  __shared__ float sx[CH * kHV];
+  __shared__ float sg[CO * kT];
+  const int tid = threadIdx.x;
+  const int64_t n3 = (int64_t)n * n * n;
+  const int ntiles = tiles_of(n);
+  float acc[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) acc[k] = 0.0f;
+  float bacc = 0.0f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int x0, y0, z0;
+    tile_origin(tile, n, x0, y0, z0);
+    __syncthreads();
+    for (int i = tid; i < CO * kT; i += kT) {
+      const int co = i / kT, v = i % kT;
+      const int gx = x0 + (v & 7), gy = y0 + ((v >> 3) & 7), gz = z0 + (v >> 6);
+      sg[i] = (gx < n && gy < n && gz < n) ? g[co * n3 + ((int64_t)gz * n + gy) * n + gx] : 0.0f;
+    }
+    for (int c0 = 0; c0 < CI; c0 += CH) {
+      __syncthreads();
+      for (int i = tid; i < CH * kHV; i += kT) {
+        const int c = i / kHV, h = i % kHV, hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        float v = 0.0f;
+        if (c0 + c < CI && gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
+          v = bf2f(x[(int64_t)(c0 + c) * n3 + ((int64_t)gz * n + gy) * n + gx]);
+        sx[i] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int e = tid + kT * k;
+        if (e >= E) break;
+        const int co = e / (CI * 27), ci = (e / 27) % CI, t = e % 27;
+        if (ci < c0 || ci >= c0 + CH) continue;
+        const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+        const float *xs = sx + (ci - c0) * kHV + dz * 100 + dy * 10 + dx;
+        const float *gs = sg + co * kT;
+        float s = 0.0f;
+        for (int v = 0; v < kT; ++v) s = fmaf(xs[(v >> 6) * 100 + ((v >> 3) & 7) * 10 + (v & 7)], gs[v], s);
+        acc[k] += s;
+      }
+    }
+    if (tid < CO) {
+      float s = 0.0f;
+      for (int v = 0; v < kT; ++v) s += sg[tid * kT + v];
+      bacc += s;
+    }
+  }
+  float *dst = partial + (int64_t)blockIdx.x * (E + CO);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = tid + kT * k;
+    if (e < E) dst[e] = acc[k];
+  }
+  if (tid < CO) dst[E + tid] = bacc;
+}
+
+__global__ void k_reduce_partials(int nb, int len, const float *__restrict__ partial, float *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  float s = 0.0f;
+  for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * len + e];
+  out[e] = s;
+}
+
+// 2x2x2 mean (BF16 -> BF16); n = fine extent
+__global__ void k_pool(int C, int n, const uint16_t *__restrict__ in, uint16_t *__restrict__ out) {
+  const int m = n / 2;
+  const int64_t tot = (int64_t)C * m * m * m;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const int x = (int)(i % m), y = (int)((i / m) % m), z = (int)((i / ((int64_t)m * m)) % m), c = (int)(i / ((int64_t)m * m * m));
+  const uint16_t *src = in + (int64_t)c * n * n * n;
+  float s = 0.0f;
+  for (int k = 0; k < 8; ++k)
+    s += bf2f(src[((int64_t)(2 * z + (k >> 2)) * n + 2 * y + ((k >> 1) & 1)) * n + 2 * x + (k & 1)]);
+  out[i] = f2bf(s * 0.125f);
+}
+
+// gradient through the 2x2x2 mean, plus an optional skip gradient: g_in (fine) = g_out/8 (+ skip)
+__global__ void k_pool_bw(int C, int n, const float *__restrict__ gout, const float *__restrict__ skip,
+                          float *__restrict__ gin) {
+  const int64_t tot = (int64_t)C * n * n * n;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const int m = n / 2;
+  const int x = (int)(i % n), y = (int)((i / n) % n), z = (int)((i / ((int64_t)n * n)) % n), c = (int)(i / ((int64_t)n * n * n));
+  float v = gout[(((int64_t)c * m + z / 2) * m + y / 2) * m + x / 2] * 0.125f;
+  if (skip) v += skip[i];
+  gin[i] = v;
+}
+
+// nearest-neighbour x2 upsampling (BF16); n = fine extent
+__global__ void k_up(int C, int n, const uint16_t *__restrict__ in, uint16_t *__restrict__ out) {
+  const int64_t tot = (int64_t)C * n * n * n;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const int m = n / 2;
+  const int x = (int)(i % n), y = (int)((i / n) % n), z = (int)((i / ((int64_t)n * n)) % n), c = (int)(i / ((int64_t)n * n * n));
+  out[i] = in[(((int64_t)c * m + z / 2) * m + y / 2) * m + x / 2];
+}
+
+// gradient through the upsampling: coarse = sum of its 8 fine children
+__global__ void k_up_bw(int C, int n, const float *__restrict__ gfine, float *__restrict__ gcoarse) {
+  const int m = n / 2;
+  const int64_t tot = (int64_t)C * m * m * m;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const int x = (int)(i % m), y = (int)((i / m) % m), z = (int)((i / ((int64_t)m * m)) % m), c = (int)(i / ((int64_t)m * m * m));
+  const float *src = gfine + (int64_t)c * n * n * n;
+  float s = 0.0f;
+  for (int k = 0; k < 8; ++k) s += src[((int64_t)(2 * z + (k >> 2)) * n + 2 * y + ((k >> 1) & 1)) * n + 2 * x + (k & 1)];
+  gcoarse[i] = s;
+}
+
+// g *= (y > 0): gradient through a ReLU whose output y was stored
+__global__ void k_relu_mask(int64_t tot, float *__restrict__ g, const uint16_t *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < tot && !(bf2f(y[i]) > 0.0f)) g[i] = 0.0f;
+}
+
+__global__ void k_to_bf16(int64_t tot, const float *__restrict__ in, uint16_t *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < tot) out[i] = f2bf(in[i]);
+}
+
+__global__ void k_adam(int64_t np, float *__restrict__ th, uint16_t *__restrict__ thb, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float clip) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float gi = g[i] * clip;
+  const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float t = th[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  th[i] = t;
+  thb[i] = f2bf(t);
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void layer_shapes(int (*shape)[2]) {
+  const int s[kLayers][2] = {{kC0, kCin}, {kC0, kC0}, {kC1, kC0}, {kC1, kC1}, {kC2, kC1},
+                             {kC2, kC2}, {kC1, kC2}, {kC1, kC1}, {kC0, kC1}, {1, kC0}};
+  for (int l = 0; l < kLayers; ++l) shape[l][0] = s[l][0], shape[l][1] = s[l][1];
+}
+
+int64_t num_params() {
+  int s[kLayers][2];
+  layer_shapes(s);
+  int64_t np = 0;
+  for (auto &l : s) np += (int64_t)l[0] * l[1] * 27 + l[0];
+  return np;
+}
+
+Net::~Net() {
+  for (void *q : bufs)
+    if (q) (void)hipFree(q);
+}
+
+int Net::alloc(void **p, size_t bytes) {
+  if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return GDX_ERR_ALLOC;
+  bufs.push_back(*p);
+  return GDX_OK;
+}
+
+int Net::init(int grid, uint64_t seed, hipStream_t s) {
+  n = grid;
+  st = s;
+  np = num_params();
+  int sh[kLayers][2];
+  layer_shapes(sh);
+  int64_t off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    woff[l] = off;
+    off += (int64_t)sh[l][0] * sh[l][1] * 27;
+    boff[l] = off;
+    off += sh[l][0];
+  }
+  const int64_t v0 = (int64_t)n * n * n, v1 = v0 / 8, v2 = v1 / 8;
+  int rc = GDX_OK;
+  auto F = [&](float **p, int64_t cnt) { if (!rc) rc = alloc(reinterpret_cast<void **>(p), cnt * sizeof(float)); };
+  auto B = [&](uint16_t **p, int64_t cnt) { if (!rc) rc = alloc(reinterpret_cast<void **>(p), cnt * sizeof(uint16_t)); };
+  F(&theta, np); F(&grad, np); F(&m, np); F(&v, np); B(&thb, np);
+  B(&a0, kCin * v0); B(&x1a, kC0 * v0); B(&x1, kC0 * v0); B(&p1, kC0 * v1); B(&x2a, kC1 * v1); B(&x2, kC1 * v1);
+  B(&p2, kC1 * v2); B(&x3a, kC2 * v2); B(&x3, kC2 * v2); B(&u2, kC2 * v1); B(&y2a, kC1 * v1); B(&y2, kC1 * v1);
+  B(&u1, kC1 * v0); B(&y1a, kC0 * v0);
+  F(&out, v0); F(&inp_f, kCin * v0); F(&dout, v0); F(&dinp, kCin * v0);
+  // gradient scratch: the largest tensor of each level, three per level
+  F(&gA0, kC1 * v0); F(&gB0, kC1 * v0); F(&gC0, kC1 * v0);
+  F(&gA1, kC2 * v1); F(&gB1, kC2 * v1); F(&gC1, kC2 * v1);
+  F(&gA2, kC2 * v2); F(&gB2, kC2 * v2);
+  int64_t pmax = 0;
+  for (int l = 0; l < kLayers; ++l) pmax = std::max<int64_t>(pmax, (int64_t)sh[l][0] * sh[l][1] * 27 + sh[l][0]);
+  F(&partial, pmax * kPartialBlocks);
+  F(&lossbuf, 2 * kPartialBlocks + 2);
+  if (rc) return rc;
+  // He-normal initialisation (fan-in 27 cin), last layer scaled by 0.1, zero biases
+  std::vector<float> h(np, 0.0f);
+  std::mt19937_64 gen(seed);
+  for (int l = 0; l < kLayers; ++l) {
+    std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * sh[l][1])) * (l == kLayers - 1 ? 0.1f : 1.0f));
+    for (int64_t i = 0; i < (int64_t)sh[l][0] * sh[l][1] * 27; ++i) h[woff[l] + i] = nd(gen);
+  }
+  return set_params(h.data());
+}
+
+int Net::set_params(const float *h) {
+  if (hipMemcpyAsync(theta, h, np * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(np)), dim3(kT), 0, st, np, theta, thb);
+  if (hipMemsetAsync(m, 0, np * sizeof(float), st) != hipSuccess ||
+      hipMemsetAsync(v, 0, np * sizeof(float), st) != hipSuccess)
+    return GDX_ERR_HIP;
+  adam_t = 0;
+  return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+int Net::get_params(float *h) {
+  if (hipMemcpyAsync(h, theta, np * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess) return GDX_ERR_HIP;
+  return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+template <int CI, int CO, bool RELU, bool SKIP, bool OUTF32>
+static void conv_fwd(const Net &N, int n, const uint16_t *in, int layer, const uint16_t *skip, void *out) {
+  hipLaunchKernelGGL((k_conv3<CI, CO, 0, RELU, SKIP, OUTF32, false>), dim3(tiles_of(n)), dim3(kT), 0, N.st, n, in,
+                     N.thb + N.woff[layer], N.theta + N.boff[layer], skip, out);
+}
+
+// dX of a layer (KCI = layer outputs, KCO = layer inputs)
+template <int LCO, int LCI, bool ACCUM>
+static void conv_bwd_data(const Net &N, int n, const float *g, int layer, float *dx) {
+  hipLaunchKernelGGL((k_conv3<LCO, LCI, 1, false, false, true, ACCUM>), dim3(tiles_of(n)), dim3(kT), 0, N.st, n, g,
+                     N.thb + N.woff[layer], nullptr, nullptr, dx);
+}
+
+template <int CI, int CO>
+static void conv_wgrad(const Net &N, int n, const uint16_t *x, const float *g, int layer) {
+  const int nb = std::min(tiles_of(n), kPartialBlocks);
+  hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), dim3(nb), dim3(kT), 0, N.st, n, x, g, N.partial);
+  const int len = CO * CI * 27 + CO;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(len)), dim3(kT), 0, N.st, nb, len, N.partial, N.grad + N.woff[layer]);
+}
+
+// forward from the BF16 input already in a0
+int Net::forward_dev() {
+  const int n0 = n, n1 = n / 2, n2 = n / 4;
+  const int64_t v0 = (int64_t)n0 * n0 * n0, v1 = v0 / 8;
+  conv_fwd<kCin, kC0, true, false, false>(*this, n0, a0, 0, nullptr, x1a);
+  conv_fwd<kC0, kC0, true, false, false>(*this, n0, x1a, 1, nullptr, x1);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(kC0 * v1)), dim3(kT), 0, st, kC0, n0, x1, p1);
+  conv_fwd<kC0, kC1, true, false, false>(*this, n1, p1, 2, nullptr, x2a);
+  conv_fwd<kC1, kC1, true, false, false>(*this, n1, x2a, 3, nullptr, x2);
+  hipLaunchKernelGGL(k_pool, dim3(nblk(kC1 * v1 / 8)), dim3(kT), 0, st, kC1, n1, x2, p2);
+  conv_fwd<kC1, kC2, true, false, false>(*this, n2, p2, 4, nullptr, x3a);
+  conv_fwd<kC2, kC2, true, false, false>(*this, n2, x3a, 5, nullptr, x3);
+  hipLaunchKernelGGL(k_up, dim3(nblk(kC2 * v1)), dim3(kT), 0, st, kC2, n1, x3, u2);
+  conv_fwd<kC2, kC1, true, true, false>(*this, n1, u2, 6, x2, y2a);
+  conv_fwd<kC1, kC1, true, false, false>(*this, n1, y2a, 7, nullptr, y2);
+  hipLaunchKernelGGL(k_up, dim3(nblk(kC1 * v0)), dim3(kT), 0, st, kC1, n0, y2, u1);
+  conv_fwd<kC1, kC0, true, true, false>(*this, n0, u1, 8, x1, y1a);
+  conv_fwd<kC0, 1, false, false, true>(*this, n0, y1a, 9, nullptr, out);
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+// backward of sum(dout * out) into grad (and dinp when want_dinp)
+int Net::backward_dev(bool want_dinp) {
+  const int n0 = n, n1 = n / 2, n2 = n / 4;
+  const int64_t v0 = (int64_t)n0 * n0 * n0, v1 = v0 / 8, v2 = v1 / 8;
+  // level 0, output layer
+  conv_wgrad<kC0, 1>(*this, n0, y1a, dout, 9);
+  conv_bwd_data<1, kC0, false>(*this, n0, dout, 9, gA0);  // d y1a
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC0 * v0)), dim3(kT), 0, st, kC0 * v0, gA0, y1a);  // gA0 = skip grad of x1
+  conv_wgrad<kC1, kC0>(*this, n0, u1, gA0, 8);
+  conv_bwd_data<kC0, kC1, false>(*this, n0, gA0, 8, gB0);  // d u1
+  hipLaunchKernelGGL(k_up_bw, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1, n0, gB0, gA1);  // d y2
+  // level 1, decoder
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1 * v1, gA1, y2);
+  conv_wgrad<kC1, kC1>(*this, n1, y2a, gA1, 7);
+  conv_bwd_data<kC1, kC1, false>(*this, n1, gA1, 7, gB1);  // d y2a
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1 * v1, gB1, y2a);  // gB1 = skip grad of x2
+  conv_wgrad<kC2, kC1>(*this, n1, u2, gB1, 6);
+  conv_bwd_data<kC1, kC2, false>(*this, n1, gB1, 6, gC1);  // d u2
+  hipLaunchKernelGGL(k_up_bw, dim3(nblk(kC2 * v2)), dim3(kT), 0, st, kC2, n1, gC1, gA2);  // d x3
+  // level 2
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC2 * v2)), dim3(kT), 0, st, kC2 * v2, gA2, x3);
+  conv_wgrad<kC2, kC2>(*this, n2, x3a, gA2, 5);
+  conv_bwd_data<kC2, kC2, false>(*this, n2, gA2, 5, gB2);  // d x3a
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC2 * v2)), dim3(kT), 0, st, kC2 * v2, gB2, x3a);
+  conv_wgrad<kC1, kC2>(*this, n2, p2, gB2, 4);
+  conv_bwd_data<kC2, kC1, false>(*this, n2, gB2, 4, gA2);  // d p2 (gA2 reused)
+  // level 1, encoder
+  hipLaunchKernelGGL(k_pool_bw, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1, n1, gA2, gB1, gA1);  // d x2
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1 * v1, gA1, x2);
+  conv_wgrad<kC1, kC1>(*this, n1, x2a, gA1, 3);
+  conv_bwd_data<kC1, kC1, false>(*this, n1, gA1, 3, gC1);  // d x2a
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC1 * v1)), dim3(kT), 0, st, kC1 * v1, gC1, x2a);
+  conv_wgrad<kC0, kC1>(*this, n1, p1, gC1, 2);
+  conv_bwd_data<kC1, kC0, false>(*this, n1, gC1, 2, gB1);  // d p1 (gB1 reused)
+  // level 0, encoder
+  hipLaunchKernelGGL(k_pool_bw, dim3(nblk(kC0 * v0)), dim3(kT), 0, st, kC0, n0, gB1, gA0, gC0);  // d x1
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC0 * v0)), dim3(kT), 0, st, kC0 * v0, gC0, x1);
+  conv_wgrad<kC0, kC0>(*this, n0, x1a, gC0, 1);
+  conv_bwd_data<kC0, kC0, false>(*this, n0, gC0, 1, gB0);  // d x1a
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(kC0 * v0)), dim3(kT), 0, st, kC0 * v0, gB0, x1a);
+  conv_wgrad<kCin, kC0>(*this, n0, a0, gB0, 0);
+  if (want_dinp) conv_bwd_data<kC0, kCin, false>(*this, n0, gB0, 0, dinp);
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+int Net::input_from_f32(const float *dev_in) {
+  const int64_t tot = (int64_t)kCin * n * n * n;
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(tot)), dim3(kT), 0, st, tot, dev_in, a0);
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+int Net::adam_step(float lr, float clip) {
+  ++adam_t;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(np)), dim3(kT), 0, st, np, theta, thb, grad, m, v, lr, b1, b2, 1e-8f, c1, c2,
+                     clip);
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+}  // namespace nn
+}  // namespace gdx
+
+// ---------------------------------------------------------------------------
+// C ABI (include/gdx.h)
+// ---------------------------------------------------------------------------
+namespace {
+int nn_fail(int code, const char *msg) {
+  gdx::set_error(msg);
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+int gdx_nn_create(gdx_ctx *ctx, int32_t n, uint64_t seed, gdx_nn **out) {
+  if (!ctx || !out) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (n < 4 || n % 4 != 0 || n > 512) return nn_fail(GDX_ERR_ARG, "grid extent must be a multiple of 4 in [4, 512]");
+  if (hipSetDevice(ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
+  gdx_nn *h = new (std::nothrow) gdx_nn;
+  if (!h) return nn_fail(GDX_ERR_ALLOC, "host allocation failed");
+  h->ctx = ctx;
+  int rc = h->net.init(n, seed, ctx->stream);
+  if (rc) {
+    delete h;
+    return nn_fail(rc, "network allocation failed");
+  }
+  *out = h;
+  return GDX_OK;
+}
+
+int gdx_nn_num_params(gdx_nn *nn, int64_t *np) {
+  if (!nn || !np) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  *np = nn->net.np;
+  return GDX_OK;
+}
+
+int gdx_nn_save(gdx_nn *nn, float *theta) {
+  if (!nn || !theta) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
+  return nn->net.get_params(theta);
+}
+
+int gdx_nn_load(gdx_nn *nn, const float *theta) {
+  if (!nn || !theta) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  for (int64_t i = 0; i < nn->net.np; ++i)
+    if (!(theta[i] == theta[i]) || theta[i] == INFINITY || theta[i] == -INFINITY)
+      return nn_fail(GDX_ERR_ARG, "parameters must be finite");
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
+  return nn->net.set_params(theta);
+}
+
+int gdx_nn_forward(gdx_nn *nn, const float *inp, float *out) {
+  if (!nn || !inp || !out) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  gdx::nn::Net &N = nn->net;
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
+  const int64_t v0 = (int64_t)N.n * N.n * N.n;
+  if (hipMemcpyAsync(N.inp_f, inp, gdx::nn::kCin * v0 * sizeof(float), hipMemcpyHostToDevice, N.st) != hipSuccess)
+    return nn_fail(GDX_ERR_HIP, "copy failed");
+  int rc = N.input_from_f32(N.inp_f);
+  if (!rc) rc = N.forward_dev();
+  if (rc) return nn_fail(rc, "forward launch failed");
+  if (hipMemcpyAsync(out, N.out, v0 * sizeof(float), hipMemcpyDeviceToHost, N.st) != hipSuccess ||
+      hipStreamSynchronize(N.st) != hipSuccess)
+    return nn_fail(GDX_ERR_HIP, "forward failed");
+  return GDX_OK;
+}
+
+int gdx_nn_backward(gdx_nn *nn, const float *dout, float *grad, float *dinp) {
+  if (!nn || !dout || !grad) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  gdx::nn::Net &N = nn->net;
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
+  const int64_t v0 = (int64_t)N.n * N.n * N.n;
+  if (hipMemcpyAsync(N.dout, dout, v0 * sizeof(float), hipMemcpyHostToDevice, N.st) != hipSuccess)
+    return nn_fail(GDX_ERR_HIP, "copy failed");
+  int rc = N.backward_dev(dinp != nullptr);
+  if (rc) return nn_fail(rc, "backward launch failed");
+  if (hipMemcpyAsync(grad, N.grad, N.np * sizeof(float), hipMemcpyDeviceToHost, N.st) != hipSuccess)
+    return nn_fail(GDX_ERR_HIP, "copy failed");
+  if (dinp && hipMemcpyAsync(dinp, N.dinp, gdx::nn::kCin * v0 * sizeof(float), hipMemcpyDeviceToHost, N.st) !=
+                  hipSuccess)
+    return nn_fail(GDX_ERR_HIP, "copy failed");
+  if (hipStreamSynchronize(N.st) != hipSuccess) return nn_fail(GDX_ERR_HIP, "backward failed");
+  return GDX_OK;
+}
+
+int gdx_nn_destroy(gdx_nn *nn) {
+  if (!nn) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  (void)hipSetDevice(nn->ctx->device);
+  (void)hipStreamSynchronize(nn->ctx->stream);
+  gdx::twogrid_free(nn);
+  delete nn;
+  return GDX_OK;
+}
+
+}  // extern "C"
+

@@ -248,17 +248,27 @@ int solve_spmv(gdx_problem *p, const double *xh, double *yh) {
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
-// Jacobi-PCG from x0 = 0 on the assembled system, the solution left in s->x;
-// stops when the recursive residual meets rtol, then computes the true FP64
-// residual ||b - A x|| / ||b|| explicitly.
-static int pcg_run(gdx_problem *p, SolveData *s, double rtol, int maxiter, int *iters, double *relres) {
+// Jacobi-PCG on the assembled system, the solution left in s->x; stops when
+// the recursive residual meets rtol ||b||, then computes the true FP64
+// residual ||b - A x|| / ||b|| explicitly.  rhs: device vector in global-id
+// order (nullptr = the assembled RHS); x0_given: start from s->x instead of 0.
+static int pcg_run(gdx_problem *p, SolveData *s, double rtol, int maxiter, int *iters, double *relres,
+                   const double *rhs = nullptr, bool x0_given = false) {
   hipStream_t st = p->ctx->stream;
   const int n = s->n;
   const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
-  hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+  if (rhs) {
+    if (hipMemcpyAsync(s->b, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  } else {
+    hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+  }
   hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
-  (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
-  (void)hipMemsetAsync(s->r, 0, n * sizeof(double), st);  // A x0 = 0
+  if (x0_given) {
+    spmv(p, s, s->x, s->r, st);  // r = A x0, turned into b - A x0 below
+  } else {
+    (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
+    (void)hipMemsetAsync(s->r, 0, n * sizeof(double), st);  // A x0 = 0
+  }
   hipLaunchKernelGGL(k_cg_init, g, b, 0, st, n, s->b, s->dinv, s->r, s->z, s->p);
   dot2(s, s->r, s->z, s->b, s->b, s->sc, st);  // sc[0] = rz, sc[1] = bb
   double h[2];
@@ -310,6 +320,159 @@ int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, 
   auto t1 = std::chrono::steady_clock::now();
   if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   return GDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Flexible CG (Notay's FCG with one stored direction) for a preconditioner that
+// may vary between iterations:
+//   alpha = p.r / p.q ; x += alpha p ; r -= alpha q ; z = M(r)
+//   p = z - (z.q / p.q) p
+// sc[0] = p.q, sc[1] = p.r, sc[2] = z.q, sc[3] = r.r
+// ---------------------------------------------------------------------------
+__global__ void k_fcg_update(int n, const double *__restrict__ sc, double *__restrict__ x, double *__restrict__ r,
+                             const double *__restrict__ p, const double *__restrict__ q) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) {
+    const double alpha = sc[1] / sc[0];
+    x[i] = fma(alpha, p[i], x[i]);
+    r[i] = fma(-alpha, q[i], r[i]);
+  }
+}
+
+__global__ void k_fcg_dir(int n, const double *__restrict__ sc, const double *__restrict__ z, double *__restrict__ p) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) p[i] = fma(-sc[2] / sc[0], p[i], z[i]);
+}
+
+__global__ void k_residual(int n, const double *__restrict__ b, const double *__restrict__ ax, double *__restrict__ r) {
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i < n) r[i] = b[i] - ax[i];
+}
+
+// inverse diagonal of the current values (global-id order) and the device
+// SpMV, for preconditioners built outside this file
+int solve_dinv(gdx_problem *p, const double **dinv, int *n) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  const dim3 g((s->n + kBlk - 1) / kBlk), b(kBlk);
+  hipLaunchKernelGGL(k_inv_diag, g, b, 0, p->ctx->stream, s->n, s->dpos, p->dev.values, s->dinv);
+  *dinv = s->dinv;
+  *n = s->n;
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st) {
+  spmv(p, solve_data(p), x, y, st);
+}
+
+// FCG with preconditioner prec (Jacobi-PCG when prec is null); falls back to
+// Jacobi-PCG from the current iterate when the flexible iteration stops
+// reducing the residual (or produces a non-finite value).
+int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double rtol, int maxiter, double *xh,
+               int *iters, double *relres, double *ms, int *fellback) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  const int n = s->n;
+  const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  if (fellback) *fellback = 0;
+  double *rhs_d = nullptr;
+  if (rhs_h) {
+    if (hipMalloc(reinterpret_cast<void **>(&rhs_d), std::max(n, 1) * sizeof(double)) != hipSuccess)
+      return GDX_ERR_ALLOC;
+    if (hipMemcpyAsync(rhs_d, rhs_h, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
+      (void)hipFree(rhs_d);
+      return GDX_ERR_HIP;
+    }
+  }
+  auto finish = [&](int code) {
+    if (rhs_d) (void)hipFree(rhs_d);
+    return code;
+  };
+  int it = 0;
+  if (!prec) {
+    if ((rc = pcg_run(p, s, rtol, maxiter, &it, relres, rhs_d))) return finish(rc);
+  } else {
+    if (rhs_d) {
+      if (hipMemcpyAsync(s->b, rhs_d, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return finish(GDX_ERR_HIP);
+    } else {
+      hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+    }
+    hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
+    (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
+    if (hipMemcpyAsync(s->r, s->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return finish(GDX_ERR_HIP);
+    dot2(s, s->b, s->b, nullptr, nullptr, s->sc + 6, st);
+    double bb;
+    if (hipMemcpyAsync(&bb, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return finish(GDX_ERR_HIP);
+    const double bnorm = std::sqrt(bb);
+    bool fall = false, nonfinite = false;
+    double last_check = bnorm;
+    if (bnorm > 0.0) {
+      if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return finish(rc);
+      if (hipMemcpyAsync(s->p, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return finish(GDX_ERR_HIP);
+      for (it = 1; it <= maxiter; ++it) {
+        spmv(p, s, s->p, s->q, st);
+        dot2(s, s->p, s->q, s->p, s->r, s->sc, st);  // sc[0] = pq, sc[1] = pr
+        hipLaunchKernelGGL(k_fcg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->p, s->q);
+        if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return finish(rc);
+        dot2(s, s->z, s->q, s->r, s->r, s->sc + 2, st);  // sc[2] = zq, sc[3] = rr
+        hipLaunchKernelGGL(k_fcg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+        if (it % 5 == 0 || it == maxiter) {
+          double h[2];
+          if (hipMemcpyAsync(h, s->sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess)
+            return finish(GDX_ERR_HIP);
+          const double rn = std::sqrt(h[1]);
+          if (!std::isfinite(rn) || !std::isfinite(h[0])) {
+            fall = true;
+            nonfinite = true;
+            break;
+          }
+          if (rn <= rtol * bnorm) break;
+          if (it % 50 == 0) {  // stagnation guard
+            if (rn > 0.999 * last_check) {
+              fall = true;
+              break;
+            }
+            last_check = rn;
+          }
+        }
+      }
+      if (it > maxiter) it = maxiter;
+    }
+    if (fall) {
+      // Jacobi-PCG from the current iterate (or from zero if it is not finite)
+      if (fellback) *fellback = 1;
+      int it2 = 0;
+      if ((rc = pcg_run(p, s, rtol, std::max(1, maxiter - it), &it2, relres, rhs_d, !nonfinite))) return finish(rc);
+      it += it2;
+    } else {
+      // true residual
+      spmv(p, s, s->x, s->q, st);
+      hipLaunchKernelGGL(k_residual, g, b, 0, st, n, s->b, s->q, s->r);
+      dot2(s, s->r, s->r, nullptr, nullptr, s->sc + 6, st);
+      double h;
+      if (hipMemcpyAsync(&h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return finish(GDX_ERR_HIP);
+      if (relres) *relres = bnorm > 0.0 ? std::sqrt(h) / bnorm : 0.0;
+    }
+  }
+  if (xh && (hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess))
+    return finish(GDX_ERR_HIP);
+  if (iters) *iters = it;
+  if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+  return finish(GDX_OK);
 }
 
 // x -= d

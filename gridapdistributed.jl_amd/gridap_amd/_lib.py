@@ -25,6 +25,9 @@ EXPORTED = [
     "gdx_symbolic", "gdx_numeric", "gdx_num_parts", "gdx_num_global_dofs", "gdx_part_sizes", "gdx_export_fe",
     "gdx_export_rows", "gdx_export_cols", "gdx_export_csr", "gdx_export_rhs", "gdx_device_values", "gdx_stream",
     "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg", "gdx_set_state", "gdx_newton",
+    "gdx_set_cell_coefficient", "gdx_cell_coefficient_grf", "gdx_export_cell_coefficient",
+    "gdx_nn_create", "gdx_nn_destroy", "gdx_nn_num_params", "gdx_nn_save", "gdx_nn_load", "gdx_nn_forward",
+    "gdx_nn_backward", "gdx_nn_attach", "gdx_nn_train_step", "gdx_nn_apply", "gdx_fpcg",
 ]
 
 
@@ -94,6 +97,20 @@ def lib():
         "gdx_spmv": [vp, vp, vp],
         "gdx_pcg": [vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double)],
         "gdx_set_state": [vp, vp],
+        "gdx_set_cell_coefficient": [vp, vp],
+        "gdx_cell_coefficient_grf": [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double],
+        "gdx_export_cell_coefficient": [vp, vp],
+        "gdx_nn_create": [vp, i32, ctypes.c_uint64, P(vp)],
+        "gdx_nn_destroy": [vp],
+        "gdx_nn_num_params": [vp, P(i64)],
+        "gdx_nn_save": [vp, vp],
+        "gdx_nn_load": [vp, vp],
+        "gdx_nn_forward": [vp, vp, vp],
+        "gdx_nn_backward": [vp, vp, vp, vp],
+        "gdx_nn_attach": [vp, vp],
+        "gdx_nn_train_step": [vp, ctypes.c_uint64, ctypes.c_float, i32, P(ctypes.c_float)],
+        "gdx_nn_apply": [vp, vp, vp],
+        "gdx_fpcg": [vp, vp, vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double), P(i32)],
         "gdx_newton": [vp, P(NewtonOpts), vp, P(NewtonReport)],
     }
     for name, args in sig.items():

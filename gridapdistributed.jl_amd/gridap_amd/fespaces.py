@@ -232,6 +232,23 @@ class GpuProblem:
             raise ValueError(f"state must have {self.ngdofs} free values")
         _lib.call("gdx_set_state", self.handle, x.ctypes.data)
 
+    def set_cell_coefficient(self, kappa):
+        """kappa per cell (x fastest) for the next numeric assembly; None returns to the form's kappa."""
+        if kappa is None:
+            _lib.call("gdx_set_cell_coefficient", self.handle, None)
+            return
+        kappa = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+        _lib.call("gdx_set_cell_coefficient", self.handle, kappa.ctypes.data)
+
+    def cell_coefficient_grf(self, seed, corr_len, log_scale):
+        """kappa = exp(log_scale g), g a unit-variance Gaussian random field drawn on the device."""
+        _lib.call("gdx_cell_coefficient_grf", self.handle, int(seed), float(corr_len), float(log_scale))
+
+    def export_cell_coefficient(self, ncells):
+        k = np.empty(int(ncells))
+        _lib.call("gdx_export_cell_coefficient", self.handle, k.ctypes.data)
+        return k
+
     def newton(self, x0=None, rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8, lin_maxiter=20000,
                init=_lib.NEWTON_INIT_GIVEN):
         """Newton with Jacobi-PCG corrections on the device; returns (x, report dict)."""

@@ -159,6 +159,18 @@ int gdx_spmv(gdx_problem *prob, const double *x, double *y);
 int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,
             double *ms);
 
+/* Cell-wise coefficient (the lognormal fields of the learned-preconditioner
+   workload): kappa is coef_const * kappa_c on every cell c instead of the
+   Fourier form; the source must be constant (GDX_RHS_CONST).  Replaces a
+   CellField coefficient of the bilinear form a(u,v) = int( kappa grad(v).grad(u) ).
+   gdx_set_cell_coefficient: host array of ncells values (x fastest), NULL returns
+   to the descriptor's kappa.  gdx_cell_coefficient_grf: kappa = exp(log_scale g),
+   g a unit-variance Gaussian random field with correlation exp(-r^2 / (2 corr_len^2))
+   drawn on the device from seed.  Both take effect at the next gdx_numeric. */
+int gdx_set_cell_coefficient(gdx_problem *prob, const double *kappa);
+int gdx_cell_coefficient_grf(gdx_problem *prob, uint64_t seed, double corr_len, double log_scale);
+int gdx_export_cell_coefficient(gdx_problem *prob, double *kappa);
+
 /* Nonlinear operators (GDX_OP_FLUX).  The state uh of
    residual_and_jacobian(op, uh) (test/PLaplacianTests.jl:44-48) is given by its
    free values x (host array of length ngdofs, global-id order; NULL = zero);
@@ -192,6 +204,41 @@ typedef struct gdx_newton_report {
    ngdofs, in = initial guess (NULL with GDX_NEWTON_INIT_GIVEN means zero),
    out = the solution's free values. */
 int gdx_newton(gdx_problem *prob, const gdx_newton_opts *opts, double *x, gdx_newton_report *report);
+
+/* Learned preconditioner.  A small BF16 convolutional multilevel network
+   (3 levels on n^3, n^3/8, n^3/64 grids, channels 8/16/24, 3x3x3 convolutions,
+   about 60k parameters) serves as the coarse solve of a two-grid
+   preconditioner (damped Jacobi smoothing, full weighting, trilinear
+   prolongation) for 3D Q1 problems on 2n x 2n x 2n cells.  It is trained on the
+   device by the library's own forward/backward kernels from systems assembled
+   by gdx_numeric, and used inside flexible PCG (gdx_fpcg).  These entry points
+   have no counterpart in the reference, whose solve is a direct LU
+   (README.md:76); they extend its A\b. */
+typedef struct gdx_nn gdx_nn;
+int gdx_nn_create(gdx_ctx *ctx, int32_t n, uint64_t seed, gdx_nn **out);
+int gdx_nn_destroy(gdx_nn *nn);
+int gdx_nn_num_params(gdx_nn *nn, int64_t *nparams);
+/* FP32 master parameters, layer by layer: W[out][in][27] then b[out] */
+int gdx_nn_save(gdx_nn *nn, float *theta);
+int gdx_nn_load(gdx_nn *nn, const float *theta);
+/* network alone on host arrays: inp [2][n^3], out [n^3]; backward of sum(dout * out)
+   gives grad [nparams] and, when dinp is not NULL, the input gradient [2][n^3] */
+int gdx_nn_forward(gdx_nn *nn, const float *inp, float *out);
+int gdx_nn_backward(gdx_nn *nn, const float *dout, float *grad, float *dinp);
+/* bind to a problem's current matrix (call after every gdx_numeric that changes it) */
+int gdx_nn_attach(gdx_nn *nn, gdx_problem *prob);
+/* one Adam step on a fresh sample (smooth coarse error of the given seed,
+   smoothed by smooth_passes 3x3x3 means) from the attached matrix; loss = relative squared error */
+int gdx_nn_train_step(gdx_nn *nn, uint64_t seed, float lr, int32_t smooth_passes, float *loss);
+/* z = M r on host vectors (global-id order) */
+int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
+/* Flexible PCG with the learned two-grid preconditioner (nn attached to prob),
+   or Jacobi-PCG when nn is NULL; falls back to Jacobi-PCG from the current
+   iterate when the flexible iteration stagnates (fellback = 1).  rhs: host
+   array of ngdofs (NULL = the assembled RHS); stops at ||r|| <= rtol ||b||,
+   relres = true FP64 residual recomputed at the end. */
+int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int32_t maxiter, double *xsol,
+             int32_t *iters, double *relres, double *ms, int32_t *fellback);
 
 /* device views and timing */
 int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);

@@ -69,6 +69,8 @@ typedef struct {
   int op_kind;
   double flux_p, flux_alpha, flux_beta;
   const double *state;
+  /* kappa per cell (x fastest), NULL = the Fourier form above; f = rhs_const */
+  const double *cell_kappa;
 } orc_desc;
 
 typedef struct {
@@ -103,6 +105,7 @@ typedef struct {
   orc_desc d;
   double *modes;
   double *state;
+  double *cell_kappa;
   int nparts, K, D, nq;
   i64 nn[3];       /* global node counts per dim */
   i64 ngdofs;
@@ -537,6 +540,10 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
         }
         double f;
         double kap = eval_kappa(C, xq, &f);
+        if (C->d.cell_kappa) {
+          kap = C->d.coef_const * C->d.cell_kappa[cx + C->d.cells[0] * (cy + C->d.cells[1] * cz)];
+          f = C->d.rhs_const;
+        }
         double dV = w * fabs(det);
         double phi[27], grad[27][3];
         for (int a = 0; a < ncn; a++) {
@@ -989,6 +996,13 @@ void *orc_run(const orc_desc *d, int inplace_reps) {
     memcpy(C->state, d->state, sizeof(double) * n);
   }
   C->d.state = C->state;
+  C->cell_kappa = NULL;
+  if (d->cell_kappa) {
+    i64 nc = d->cells[0] * (d->dim > 1 ? d->cells[1] : 1) * (d->dim > 2 ? d->cells[2] : 1);
+    C->cell_kappa = malloc(sizeof(double) * nc);
+    memcpy(C->cell_kappa, d->cell_kappa, sizeof(double) * nc);
+  }
+  C->d.cell_kappa = C->cell_kappa;
   double t0 = now_s();
   setup(C);
   double t1 = now_s();
@@ -1063,6 +1077,7 @@ void orc_free(void *h) {
   free(C->P);
   free(C->modes);
   free(C->state);
+  free(C->cell_kappa);
   free(C);
 }
 

@@ -27,6 +27,7 @@ class OrcDesc(ctypes.Structure):
         ("dir_kind", ctypes.c_int), ("dir_coef", ctypes.c_double * 4),
         ("op_kind", ctypes.c_int), ("flux_p", ctypes.c_double), ("flux_alpha", ctypes.c_double),
         ("flux_beta", ctypes.c_double), ("state", ctypes.POINTER(ctypes.c_double)),
+        ("cell_kappa", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -96,6 +97,13 @@ def make_desc(p):
         d.state = st.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     else:
         d.state = None
+    ck = p.get("cell_kappa")
+    if ck is not None:
+        ck = np.ascontiguousarray(ck, dtype=np.float64).ravel()
+        d._ck_keep = ck
+        d.cell_kappa = ck.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    else:
+        d.cell_kappa = None
     return d
 
 

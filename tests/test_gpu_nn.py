@@ -1,0 +1,125 @@
+"""The learned preconditioner on device 0.
+
+- the network's forward pass and its gradients (parameters and input) against
+  the CPU oracle (oracle/nn_oracle.py, which rounds to BF16 where the device
+  stores BF16 and accumulates in FP64): bounds are a few BF16 ulps;
+- training on samples from an assembled lognormal-coefficient system lowers
+  the loss;
+- flexible PCG with the trained network reaches the FP64 true residual 1e-8 in
+  fewer iterations than Jacobi-PCG and than the same preconditioner with the
+  untrained network (what the training gains);
+- a network whose outputs overflow makes FPCG fall back to Jacobi-PCG, which
+  still converges.
+"""
+import numpy as np
+import pytest
+
+import nn_oracle as NO
+from gridap_amd import LearnedCoarseSolver, fpcg, problem_from_workload
+from gridap_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+C = (8, 16, 24)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
+
+
+@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2)])
+def test_forward_backward_match_oracle(gpu_ctx, n, seed):
+    print("nn oracle case", n, seed, flush=True)
+    rng = np.random.default_rng(seed)
+    net = LearnedCoarseSolver(gpu_ctx, n, seed)
+    assert net.num_params == NO.num_params(2, C)
+    theta = NO.init_params(2, C, seed)
+    theta = theta + np.float32(0.05) * rng.normal(size=theta.shape).astype(np.float32)  # nonzero biases
+    net.set_params(theta)
+    inp = rng.normal(size=(2, n, n, n)).astype(np.float32)
+    out = net.forward(inp)
+    ref, cache = NO.forward(inp, theta, C)
+    assert rel(out, ref) < 2e-2
+    dout = rng.normal(size=(1, n, n, n)).astype(np.float32)
+    g, dinp = net.backward(dout, want_dinp=True)
+    gref, dinp_ref = NO.backward(dout, cache)
+    assert rel(g, gref) < 3e-2
+    assert rel(dinp, dinp_ref) < 3e-2
+    # layer by layer: every layer's gradient agrees (no layer hidden by the others' size)
+    k = 0
+    for co, ci in NO.layer_shapes(2, C):
+        m = co * ci * 27 + co
+        assert rel(g[k:k + m], gref[k:k + m]) < 5e-2, (co, ci)
+        k += m
+    net.close()
+
+
+def lognormal_problem(ctx, cells, seed, log_scale=2.0, corr=0.1):
+    p = W.poisson3d_q1(cells, (2, 2, 2), rhs_const=0.0)
+    assem, data = problem_from_workload(ctx, p)
+    prob = assem._problem_for(data)
+    prob.cell_coefficient_grf(seed, corr, log_scale)
+    prob.numeric()
+    prob.sync()
+    return assem, prob
+
+
+def train(net, ctx, cells, fields, steps_per_field, lr, seed0=1000):
+    losses = []
+    for f in range(fields):
+        assem, prob = lognormal_problem(ctx, cells, seed0 + f)
+        net.attach(prob)
+        for s in range(steps_per_field):
+            losses.append(net.train_step(seed0 * 7 + 31 * f + s, lr, smooth_passes=(2, 4, 8, 16)[s % 4]))
+        prob.close()
+    return losses
+
+
+def test_training_reduces_loss(gpu_ctx):
+    net = LearnedCoarseSolver(gpu_ctx, 8, 3)
+    losses = train(net, gpu_ctx, 16, fields=6, steps_per_field=20, lr=2e-3)
+    first, last = np.mean(losses[:10]), np.mean(losses[-10:])
+    print("loss", first, "->", last, flush=True)
+    assert np.all(np.isfinite(losses))
+    assert last < 0.6 * first
+    net.close()
+
+
+def test_fpcg_learned_beats_jacobi_and_untrained(gpu_ctx):
+    cells, n = 32, 16
+    net = LearnedCoarseSolver(gpu_ctx, n, 5)
+    theta0 = net.get_params()
+    train(net, gpu_ctx, cells, fields=8, steps_per_field=40, lr=2e-3)
+    assem, prob = lognormal_problem(gpu_ctx, cells, 99)  # held-out field
+    rhs = np.random.default_rng(0).normal(size=prob.ngdofs)
+    _, it_j, rr_j, _, _ = fpcg(prob, None, rhs, 1e-8)
+    net.attach(prob)
+    x, it_l, rr_l, _, fb = fpcg(prob, net, rhs, 1e-8)
+    trained = net.get_params()
+    net.set_params(theta0)
+    _, it_u, rr_u, _, fb_u = fpcg(prob, net, rhs, 1e-8)
+    print("iterations: jacobi", it_j, "untrained", it_u, fb_u, "trained", it_l, fb, flush=True)
+    assert rr_j <= 1e-8 and rr_l <= 1e-8
+    assert not fb
+    assert it_l < it_j
+    assert it_l < it_u or fb_u
+    # the solution solves the system: independent check with the device SpMV
+    r = rhs - prob.spmv(x)
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(rhs)
+    net.set_params(trained)
+    net.close()
+    prob.close()
+
+
+def test_fpcg_falls_back_on_overflowing_network(gpu_ctx):
+    cells, n = 16, 8
+    net = LearnedCoarseSolver(gpu_ctx, n, 7)
+    th = net.get_params() * np.float32(1e30)
+    net.set_params(th)  # finite weights whose products overflow
+    assem, prob = lognormal_problem(gpu_ctx, cells, 5)
+    net.attach(prob)
+    rhs = np.random.default_rng(1).normal(size=prob.ngdofs)
+    x, it, rr, _, fb = fpcg(prob, net, rhs, 1e-8)
+    assert fb and rr <= 1e-8
+    net.close()
+    prob.close()
