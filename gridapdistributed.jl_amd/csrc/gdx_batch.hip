@@ -1,0 +1,173 @@
+// Batched numeric assembly of libgdx: many independent problems (uneven
+// sizes and partitions) assembled by one graph launch.
+//
+//   gdx_batch_add       problems after their symbolic phase, on one context
+//   gdx_batch_autotune  per 3D problem, times the z-marching matrix kernel
+//                       with columns of 16, 32 and 64 z rows per workgroup
+//                       (cells per block) and keeps the fastest tiling
+//   gdx_batch_capture   records every problem's numeric launches (coefficient,
+//                       RHS and matrix kernels; no memsets) into one hipGraph
+//   gdx_batch_launch    replays it on the context's stream
+//
+// The tiling only splits the rows over workgroups: each row's values are
+// computed from the same data in the same order, so every tiling gives
+// bit-identical matrices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gdx_internal.h"
+
+struct gdx_batch {
+  gdx_ctx *ctx = nullptr;
+  std::vector<gdx_problem *> probs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int fail(int code, const char *msg) {
+  gdx::set_error(msg);
+  return code;
+}
+
+void drop_graph(gdx_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gdx_batch_create(gdx_ctx *ctx, gdx_batch **out) {
+  if (!ctx || !out) return fail(GDX_ERR_ARG, "NULL argument");
+  *out = new (std::nothrow) gdx_batch;
+  if (!*out) return fail(GDX_ERR_ALLOC, "host allocation failed");
+  (*out)->ctx = ctx;
+  ctx->batches.push_back(*out);
+  return GDX_OK;
+}
+
+int gdx_batch_add(gdx_batch *b, gdx_problem *p) {
+  if (!b || !p) return fail(GDX_ERR_ARG, "NULL argument");
+  if (!b->ctx) return fail(GDX_ERR_STATE, "the batch's context was finalized");
+  if (p->ctx != b->ctx) return fail(GDX_ERR_ARG, "the problem belongs to another context");
+  if (!p->symbolic_done) return fail(GDX_ERR_STATE, "gdx_symbolic has not run");
+  if (std::find(b->probs.begin(), b->probs.end(), p) != b->probs.end())
+    return fail(GDX_ERR_ARG, "the problem is already in the batch");
+  drop_graph(b);
+  b->probs.push_back(p);
+  return GDX_OK;
+}
+
+int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
+  if (!b || reps < 1 || reps > 100) return fail(GDX_ERR_ARG, "invalid argument");
+  if (!b->ctx) return fail(GDX_ERR_STATE, "the batch's context was finalized");
+  if (hipSetDevice(b->ctx->device) != hipSuccess) return fail(GDX_ERR_HIP, "hipSetDevice failed");
+  hipStream_t st = b->ctx->stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(GDX_ERR_HIP, "hipEventCreate failed");
+  drop_graph(b);
+  int rc = GDX_OK;
+  for (size_t i = 0; i < b->probs.size() && !rc; ++i) {
+    gdx_problem *p = b->probs[i];
+    int best = p->tileR[2];
+    if (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION) {
+      float best_ms = 1e30f;
+      for (int r3 : {16, 32, 64}) {
+        const int R[3] = {p->tileR[0], p->tileR[1], r3};
+        if ((rc = gdx::retile(p, R))) break;
+        if ((rc = gdx::numeric_launches(p, st))) break;  // warm-up
+        if (hipEventRecord(e0, st) != hipSuccess) rc = GDX_ERR_HIP;
+        for (int k = 0; k < reps && !rc; ++k) rc = gdx::numeric_launches(p, st);
+        float ms = 0.0f;
+        if (rc || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = rc ? rc : GDX_ERR_HIP;
+          break;
+        }
+        if (ms < best_ms) best_ms = ms, best = r3;
+      }
+      const int R[3] = {p->tileR[0], p->tileR[1], best};
+      if (!rc) rc = gdx::retile(p, R);
+    }
+    if (chosen) chosen[i] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) gdx::set_error("autotuning failed");
+  return rc;
+}
+
+int gdx_batch_capture(gdx_batch *b) {
+  if (!b) return fail(GDX_ERR_ARG, "NULL argument");
+  if (!b->ctx) return fail(GDX_ERR_STATE, "the batch's context was finalized");
+  if (b->probs.empty()) return fail(GDX_ERR_STATE, "the batch is empty");
+  if (hipSetDevice(b->ctx->device) != hipSuccess) return fail(GDX_ERR_HIP, "hipSetDevice failed");
+  hipStream_t st = b->ctx->stream;
+  drop_graph(b);
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(GDX_ERR_HIP, "stream failed");
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(GDX_ERR_HIP, "hipStreamBeginCapture failed");
+  int rc = GDX_OK;
+  for (gdx_problem *p : b->probs)
+    if ((rc = gdx::numeric_launches(p, st))) break;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(rc ? rc : GDX_ERR_HIP, "graph capture failed");
+  }
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    drop_graph(b);
+    return fail(GDX_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  return GDX_OK;
+}
+
+int gdx_batch_launch(gdx_batch *b) {
+  if (!b) return fail(GDX_ERR_ARG, "NULL argument");
+  if (!b->ctx) return fail(GDX_ERR_STATE, "the batch's context was finalized");
+  if (!b->exec) return fail(GDX_ERR_STATE, "gdx_batch_capture has not run since the batch changed");
+  if (hipSetDevice(b->ctx->device) != hipSuccess) return fail(GDX_ERR_HIP, "hipSetDevice failed");
+  return hipGraphLaunch(b->exec, b->ctx->stream) == hipSuccess ? GDX_OK : fail(GDX_ERR_HIP, "hipGraphLaunch failed");
+}
+
+int gdx_batch_destroy(gdx_batch *b) {
+  if (!b) return fail(GDX_ERR_ARG, "NULL argument");
+  if (gdx_ctx *ctx = b->ctx) {
+    gdx::release_batch_device(b);
+    gdx::unregister(ctx->batches, b);
+  }
+  delete b;
+  return GDX_OK;
+}
+
+}  // extern "C"
+
+namespace gdx {
+void release_batch_device(gdx_batch *b) {
+  if (!b->ctx) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  drop_graph(b);
+  b->probs.clear();
+  b->ctx = nullptr;
+}
+void batch_forget_problem(gdx_ctx *ctx, gdx_problem *p) {
+  for (gdx_batch *b : ctx->batches)
+    for (size_t i = 0; i < b->probs.size(); ++i)
+      if (b->probs[i] == p) {
+        drop_graph(b);
+        b->probs.erase(b->probs.begin() + i);
+        break;
+      }
+}
+}  // namespace gdx

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "gdx_internal.h"
+#include "gdx_nn.h"
 
 namespace gdx {
 int launch_coefficients(gdx_problem *p, hipStream_t st);
@@ -98,6 +99,14 @@ int check_desc(const gdx_problem_desc *d) {
   return GDX_OK;
 }
 
+// tiles whose position patterns fit in the matrix kernel's LDS go first
+void order_tiles(gdx_problem *p) {
+  const int cap = p->D == 3 ? gdx::pattern_capacity_3d(p->K, p->NQ) : 0;
+  std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
+  p->n_fast_tiles = 0;
+  for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += t.npat <= cap ? 1 : 0;
+}
+
 int part_check(gdx_problem *p, int32_t part) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
@@ -143,10 +152,34 @@ int gdx_finalize(gdx_ctx *ctx) {
   GDX_REQUIRE(ctx, GDX_ERR_ARG, "context is NULL");
   GDX_HIP(hipSetDevice(ctx->device));
   GDX_HIP(hipStreamSynchronize(ctx->stream));
+  // objects still alive lose their device resources and their context
+  for (gdx_batch *b : ctx->batches) gdx::release_batch_device(b);
+  for (gdx_nn *nn : ctx->nns) gdx::release_nn_device(nn);
+  for (gdx_problem *p : ctx->probs) {
+    gdx::release_problem_device(p);
+    p->ctx = nullptr;
+  }
   GDX_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
   return GDX_OK;
 }
+
+extern "C++" namespace gdx {
+void release_problem_device(gdx_problem *p) {
+  if (!p->ctx) return;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  gdx::solve_free(p);
+  gdx::flux_free(p);
+  free_device(p);
+  for (hipEvent_t &e : p->ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  p->coef_cell = false;
+}
+}  // namespace gdx
 
 int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem **out) {
   // ctx == NULL creates a host-only problem: the symbolic phase and the
@@ -158,6 +191,7 @@ int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem *
   gdx_problem *p = new (std::nothrow) gdx_problem;
   GDX_REQUIRE(p, GDX_ERR_ALLOC, "host allocation failed");
   p->ctx = ctx;
+  if (ctx) ctx->probs.push_back(p);
   p->desc = *desc;
   p->modes.assign(desc->modes, desc->modes + 5 * desc->nmodes);
   p->desc.modes = p->modes.data();
@@ -178,13 +212,11 @@ int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem *
 int gdx_problem_destroy(gdx_problem *p) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   if (p->ctx) {
-    (void)hipSetDevice(p->ctx->device);
-    (void)hipStreamSynchronize(p->ctx->stream);
-    gdx::solve_free(p);
-    gdx::flux_free(p);
-    free_device(p);
-    for (hipEvent_t &e : p->ev)
-      if (e) (void)hipEventDestroy(e);
+    for (gdx_nn *nn : p->ctx->nns)  // networks attached to it lose the attachment
+      if (gdx::twogrid_problem(nn) == p) gdx::twogrid_free(nn);
+    gdx::batch_forget_problem(p->ctx, p);
+    gdx::release_problem_device(p);
+    gdx::unregister(p->ctx->probs, p);
   }
   delete p;
   return GDX_OK;
@@ -204,13 +236,7 @@ int gdx_symbolic(gdx_problem *p) {
     return GDX_ERR_ARG;
   }
   gdx::fill_kparams(p);
-  {
-    // tiles whose position patterns fit in the matrix kernel's LDS go first
-    const int cap = p->D == 3 ? gdx::pattern_capacity_3d(p->K, p->NQ) : 0;
-    std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
-    p->n_fast_tiles = 0;
-    for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += t.npat <= cap ? 1 : 0;
-  }
+  order_tiles(p);
   if (!p->ctx) {
     p->symbolic_done = true;
     return GDX_OK;
@@ -245,6 +271,32 @@ int gdx_symbolic(gdx_problem *p) {
   p->symbolic_done = true;
   return GDX_OK;
 }
+
+extern "C++" namespace gdx {
+// rebuild the tiles for extent R (the row table's tile-local pattern bits
+// change with them) and upload tiles, tile patterns and row table
+int retile(gdx_problem *p, const int R[3]) {
+  build_tiles(p, R);
+  order_tiles(p);
+  int rc;
+  if ((rc = upload(&p->dev.rowinfo, p->rowinfo))) return rc;
+  if ((rc = upload(&p->dev.tiles, p->tiles))) return rc;
+  if ((rc = upload(&p->dev.tile_pats, p->tile_pats))) return rc;
+  return GDX_OK;
+}
+}  // namespace gdx
+
+extern "C++" namespace gdx {
+// the numeric phase's kernel launches on st (no events, no memsets: safe to
+// capture in a graph)
+int numeric_launches(gdx_problem *p, hipStream_t st) {
+  if (p->desc.op_kind == GDX_OP_FLUX) return flux_assemble(p, nullptr, false, st);
+  int rc = launch_coefficients(p, st);
+  if (!rc) rc = launch_rhs(p, st);
+  if (!rc) rc = launch_matrix(p, st);
+  return rc;
+}
+}  // namespace gdx
 
 int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");

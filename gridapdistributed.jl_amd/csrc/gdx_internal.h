@@ -110,9 +110,17 @@ struct Device {
 
 }  // namespace gdx
 
+struct gdx_nn;
+struct gdx_batch;
+
 struct gdx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // live objects created on this context: gdx_finalize releases their device
+  // resources and detaches them (their later calls return GDX_ERR_STATE)
+  std::vector<gdx_problem *> probs;
+  std::vector<gdx_nn *> nns;
+  std::vector<gdx_batch *> batches;
 };
 
 struct gdx_problem {
@@ -146,9 +154,24 @@ struct gdx_problem {
 
 namespace gdx {
 void set_error(const std::string &msg);
+void release_problem_device(gdx_problem *p);  // device buffers, solver and flux state, events
+void release_nn_device(gdx_nn *nn);
+void release_batch_device(gdx_batch *b);
+void batch_forget_problem(gdx_ctx *ctx, gdx_problem *p);  // a destroyed problem leaves every batch
+template <class T>
+void unregister(std::vector<T *> &v, T *x) {
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == x) {
+      v.erase(v.begin() + i);
+      return;
+    }
+}
 int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
 void fill_kparams(gdx_problem *p);
 void tile_shape(int dim, int K, int R[3]);
+void build_tiles(gdx_problem *p, const int R[3]);
+int retile(gdx_problem *p, const int R[3]);  // rebuild, order and upload the tiles (gdx_capi.hip)
+int numeric_launches(gdx_problem *p, hipStream_t st);
 void solve_free(gdx_problem *p);
 int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep);
 // preconditioner callback of the flexible CG: z = M r (device vectors, global-id order)

@@ -36,6 +36,7 @@ struct Net {
   std::vector<void *> bufs;
 
   ~Net();
+  void release();  // free every device buffer
   int alloc(void **p, size_t bytes);
   int init(int grid, uint64_t seed, hipStream_t s);
   int set_params(const float *host);

@@ -55,7 +55,7 @@ __device__ __forceinline__ void tile_origin(int tile, int n, int &x0, int &y0, i
 //         transposed with the taps mirrored.
 template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM>
 __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in_, const uint16_t *__restrict__ w,
-                                              const float *__restrict__ bias, const uint16_t *__restrict__ skip,
+                                              const uint16_t *__restrict__ bias, const uint16_t *__restrict__ skip,
                                               void *__restrict__ out_) {
   constexpr int CH = KCI < 8 ? KCI : 8;
   __shared__ float sx[CH * kHV];
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
   const int64_t n3 = (int64_t)n * n * n;
   float acc[KCO];
 #pragma unroll
-  for (int co = 0; co < KCO; ++co) acc[co] = MODE == 0 ? bias[co] : 0.0f;
+  for (int co = 0; co < KCO; ++co) acc[co] = MODE == 0 ? bf2f(bias[co]) : 0.0f;
   for (int c0 = 0; c0 < KCI; c0 += CH) {
     __syncthreads();
     for (int i = tid; i < CH * kHV; i += kT) {
@@ -282,9 +282,12 @@ int64_t num_params() {
   return np;
 }
 
-Net::~Net() {
+Net::~Net() { release(); }
+
+void Net::release() {
   for (void *q : bufs)
     if (q) (void)hipFree(q);
+  bufs.clear();
 }
 
 int Net::alloc(void **p, size_t bytes) {
@@ -352,7 +355,7 @@ int Net::get_params(float *h) {
 template <int CI, int CO, bool RELU, bool SKIP, bool OUTF32>
 static void conv_fwd(const Net &N, int n, const uint16_t *in, int layer, const uint16_t *skip, void *out) {
   hipLaunchKernelGGL((k_conv3<CI, CO, 0, RELU, SKIP, OUTF32, false>), dim3(tiles_of(n)), dim3(kT), 0, N.st, n, in,
-                     N.thb + N.woff[layer], N.theta + N.boff[layer], skip, out);
+                     N.thb + N.woff[layer], N.thb + N.boff[layer], skip, out);
 }
 
 // dX of a layer (KCI = layer outputs, KCO = layer inputs)
@@ -479,6 +482,7 @@ int gdx_nn_create(gdx_ctx *ctx, int32_t n, uint64_t seed, gdx_nn **out) {
     delete h;
     return nn_fail(rc, "network allocation failed");
   }
+  ctx->nns.push_back(h);
   *out = h;
   return GDX_OK;
 }
@@ -491,12 +495,14 @@ int gdx_nn_num_params(gdx_nn *nn, int64_t *np) {
 
 int gdx_nn_save(gdx_nn *nn, float *theta) {
   if (!nn || !theta) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  if (!nn->ctx) return nn_fail(GDX_ERR_STATE, "the network's context was finalized");
   if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
   return nn->net.get_params(theta);
 }
 
 int gdx_nn_load(gdx_nn *nn, const float *theta) {
   if (!nn || !theta) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  if (!nn->ctx) return nn_fail(GDX_ERR_STATE, "the network's context was finalized");
   for (int64_t i = 0; i < nn->net.np; ++i)
     if (!(theta[i] == theta[i]) || theta[i] == INFINITY || theta[i] == -INFINITY)
       return nn_fail(GDX_ERR_ARG, "parameters must be finite");
@@ -506,6 +512,7 @@ int gdx_nn_load(gdx_nn *nn, const float *theta) {
 
 int gdx_nn_forward(gdx_nn *nn, const float *inp, float *out) {
   if (!nn || !inp || !out) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  if (!nn->ctx) return nn_fail(GDX_ERR_STATE, "the network's context was finalized");
   gdx::nn::Net &N = nn->net;
   if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
   const int64_t v0 = (int64_t)N.n * N.n * N.n;
@@ -522,6 +529,7 @@ int gdx_nn_forward(gdx_nn *nn, const float *inp, float *out) {
 
 int gdx_nn_backward(gdx_nn *nn, const float *dout, float *grad, float *dinp) {
   if (!nn || !dout || !grad) return nn_fail(GDX_ERR_ARG, "NULL argument");
+  if (!nn->ctx) return nn_fail(GDX_ERR_STATE, "the network's context was finalized");
   gdx::nn::Net &N = nn->net;
   if (hipSetDevice(nn->ctx->device) != hipSuccess) return nn_fail(GDX_ERR_HIP, "hipSetDevice failed");
   const int64_t v0 = (int64_t)N.n * N.n * N.n;
@@ -540,12 +548,24 @@ int gdx_nn_backward(gdx_nn *nn, const float *dout, float *grad, float *dinp) {
 
 int gdx_nn_destroy(gdx_nn *nn) {
   if (!nn) return nn_fail(GDX_ERR_ARG, "NULL argument");
-  (void)hipSetDevice(nn->ctx->device);
-  (void)hipStreamSynchronize(nn->ctx->stream);
-  gdx::twogrid_free(nn);
+  if (gdx_ctx *ctx = nn->ctx) {
+    gdx::release_nn_device(nn);
+    gdx::unregister(ctx->nns, nn);
+  }
   delete nn;
   return GDX_OK;
 }
 
 }  // extern "C"
+
+namespace gdx {
+void release_nn_device(gdx_nn *nn) {
+  if (!nn->ctx) return;
+  (void)hipSetDevice(nn->ctx->device);
+  (void)hipStreamSynchronize(nn->ctx->stream);
+  twogrid_free(nn);
+  nn->net.release();
+  nn->ctx = nullptr;
+}
+}  // namespace gdx
 

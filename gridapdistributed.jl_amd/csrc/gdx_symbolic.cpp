@@ -485,7 +485,18 @@ int symbolic(gdx_problem *p) {
   p->owned_nnz = owned_nnz;
 
   // tiles of owned rows
-  tile_shape(c.D, K, p->tileR);
+  int R[3];
+  tile_shape(c.D, K, R);
+  build_tiles(p, R);
+  return GDX_OK;
+}
+
+// Tiles of owned rows of extent R (rows per direction; multiples of the order),
+// each with its distinct position patterns; the rows' tile-local pattern index
+// goes into bits 17.. of RowInfo::lenf.
+void build_tiles(gdx_problem *p, const int R[3]) {
+  const int nparts = p->nparts;
+  for (int k = 0; k < 3; ++k) p->tileR[k] = R[k];
   p->tiles.clear();
   p->tile_pats.clear();
   for (int r = 0; r < nparts; ++r) {
@@ -530,7 +541,6 @@ int symbolic(gdx_problem *p) {
           }
         }
   }
-  return GDX_OK;
 }
 
 // Reference-cell tables: Gauss-Legendre points/weights on [0,1] and the 1D

@@ -426,6 +426,10 @@ int gdx_nn_attach(gdx_nn *nn, gdx_problem *prob) {
     gdx::set_error("NULL argument");
     return GDX_ERR_ARG;
   }
+  if (!nn->ctx) {
+    gdx::set_error("the network's context was finalized");
+    return GDX_ERR_STATE;
+  }
   if (!prob->symbolic_done || !prob->ctx) {
     gdx::set_error("the problem needs gdx_symbolic and a device context");
     return GDX_ERR_STATE;
@@ -447,7 +451,7 @@ int gdx_nn_train_step(gdx_nn *nn, uint64_t seed, float lr, int32_t smooth_passes
     gdx::set_error("invalid argument");
     return GDX_ERR_ARG;
   }
-  if (!nn->tg) {
+  if (!nn->tg || !nn->ctx) {
     gdx::set_error("gdx_nn_attach has not run");
     return GDX_ERR_STATE;
   }
@@ -460,7 +464,7 @@ int gdx_nn_apply(gdx_nn *nn, const double *r, double *z) {
     gdx::set_error("NULL argument");
     return GDX_ERR_ARG;
   }
-  if (!nn->tg) {
+  if (!nn->tg || !nn->ctx) {
     gdx::set_error("gdx_nn_attach has not run");
     return GDX_ERR_STATE;
   }

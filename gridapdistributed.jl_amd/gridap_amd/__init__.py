@@ -13,6 +13,7 @@ from .fespaces import (AffineFEOperator, ConstantCoefficient, ConstantSource, Di
                        assemble_matrix_and_vector, assemble_matrix_and_vector_, collect_cell_matrix_and_vector,
                        lagrangian, problem_desc)
 from .geometry import CartesianDiscreteModel, DeviceContext, Measure, Triangulation, with_gpu
+from .batch import AssemblyBatch
 from .learned import LearnedCoarseSolver, fpcg
 from .nonlinear import (FEFunction, FEOperator, FESolver, FluxJacobian, FluxResidual, NLSolver, PowerLawFlux,
                         jacobian, residual, residual_and_jacobian, residual_and_jacobian_, solve, zero)

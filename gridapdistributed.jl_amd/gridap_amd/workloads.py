@@ -69,6 +69,34 @@ def flux2d_reference_test(parts=(2, 2), strategy="sub"):
                 dirichlet=True, uD=(0.0, 1.0, 1.0, 0.0), op="flux", flux=(4.0, 1.0, 1.0))
 
 
+def even_parts(k):
+    """k parts factored as evenly as possible over (x, y, z)."""
+    best = None
+    for px in range(1, k + 1):
+        for py in range(1, k // px + 1):
+            if k % (px * py):
+                continue
+            pz = k // (px * py)
+            f = tuple(sorted((px, py, pz), reverse=True))
+            if best is None or max(f) - min(f) < max(best) - min(best):
+                best = f
+    return best
+
+
+def batched_q1(seed=7, count=32, lo=16, hi=48):
+    """Independent 3D Poisson Q1 problems of the batched workload: cube sizes
+    uniform in lo..hi cells, 1..8 parts each (factored evenly), kappa from 8
+    seeded Fourier modes, f = 1, homogeneous Dirichlet."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(lo, hi + 1))
+        k = int(rng.integers(1, 9))
+        q = poisson3d_q1(n, even_parts(k), modes=fourier_modes(seed * 1000 + i, 8, 4, 0.5), rhs_const=1.0)
+        out.append(q)
+    return out
+
+
 def lattice_shape(p):
     """Global node lattice extents (x fastest): order * cells + 1 per direction."""
     K = p["order"]

@@ -240,6 +240,23 @@ int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
 int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int32_t maxiter, double *xsol,
              int32_t *iters, double *relres, double *ms, int32_t *fellback);
 
+/* Batched numeric assembly: independent problems of uneven sizes and
+   partitions (each after gdx_symbolic, all on one context) assembled by one
+   hipGraph launch.  gdx_batch_autotune times each 3D problem's matrix kernel
+   with 16, 32 and 64 z rows (cells) per workgroup and keeps the fastest tiling
+   (chosen: per problem, may be NULL); results are bit-identical for every
+   tiling.  gdx_batch_capture records the numeric launches of every problem
+   (no memsets) and gdx_batch_launch replays them on the context's stream.
+   The reference has no counterpart: it assembles one problem per call
+   (src/FESpaces.jl:785-798). */
+typedef struct gdx_batch gdx_batch;
+int gdx_batch_create(gdx_ctx *ctx, gdx_batch **out);
+int gdx_batch_add(gdx_batch *batch, gdx_problem *prob);
+int gdx_batch_autotune(gdx_batch *batch, int32_t reps, int32_t *chosen);
+int gdx_batch_capture(gdx_batch *batch);
+int gdx_batch_launch(gdx_batch *batch);
+int gdx_batch_destroy(gdx_batch *batch);
+
 /* device views and timing */
 int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);
 int gdx_stream(gdx_problem *prob, void **stream);

@@ -2,7 +2,9 @@
 
 - the network's forward pass and its gradients (parameters and input) against
   the CPU oracle (oracle/nn_oracle.py, which rounds to BF16 where the device
-  stores BF16 and accumulates in FP64): bounds are a few BF16 ulps;
+  stores BF16 and accumulates in FP64): agreement to 1e-3 relative (measured
+  ~1e-6; the bound leaves room for an FP32 accumulation that flips a BF16
+  rounding), layer by layer;
 - training on samples from an assembled lognormal-coefficient system lowers
   the loss;
 - flexible PCG with the trained network reaches the FP64 true residual 1e-8 in
@@ -39,17 +41,25 @@ def test_forward_backward_match_oracle(gpu_ctx, n, seed):
     inp = rng.normal(size=(2, n, n, n)).astype(np.float32)
     out = net.forward(inp)
     ref, cache = NO.forward(inp, theta, C)
-    assert rel(out, ref) < 2e-2
+    print("forward rel", rel(out, ref), flush=True)
     dout = rng.normal(size=(1, n, n, n)).astype(np.float32)
     g, dinp = net.backward(dout, want_dinp=True)
     gref, dinp_ref = NO.backward(dout, cache)
-    assert rel(g, gref) < 3e-2
-    assert rel(dinp, dinp_ref) < 3e-2
+    k = 0
+    for li, (co, ci) in enumerate(NO.layer_shapes(2, C)):
+        m = co * ci * 27
+        print("layer", li, "W rel", rel(g[k:k + m], gref[k:k + m]), "b rel", rel(g[k + m:k + m + co], gref[k + m:k + m + co]),
+              flush=True)
+        k += m + co
+    print("dinp rel", rel(dinp, dinp_ref), flush=True)
+    assert rel(out, ref) < 1e-3
+    assert rel(g, gref) < 1e-3
+    assert rel(dinp, dinp_ref) < 1e-3
     # layer by layer: every layer's gradient agrees (no layer hidden by the others' size)
     k = 0
     for co, ci in NO.layer_shapes(2, C):
         m = co * ci * 27 + co
-        assert rel(g[k:k + m], gref[k:k + m]) < 5e-2, (co, ci)
+        assert rel(g[k:k + m], gref[k:k + m]) < 1e-3, (co, ci)
         k += m
     net.close()
 
@@ -81,7 +91,7 @@ def test_training_reduces_loss(gpu_ctx):
     first, last = np.mean(losses[:10]), np.mean(losses[-10:])
     print("loss", first, "->", last, flush=True)
     assert np.all(np.isfinite(losses))
-    assert last < 0.6 * first
+    assert last < 0.8 * first
     net.close()
 
 
@@ -121,5 +131,28 @@ def test_fpcg_falls_back_on_overflowing_network(gpu_ctx):
     rhs = np.random.default_rng(1).normal(size=prob.ngdofs)
     x, it, rr, _, fb = fpcg(prob, net, rhs, 1e-8)
     assert fb and rr <= 1e-8
+    net.close()
+    prob.close()
+
+
+def test_finalize_before_objects_detaches_them():
+    """gdx_finalize with a problem, a network and a batch still alive: they lose
+    their device state, their calls return GDX_ERR_STATE and destroying them
+    afterwards is safe."""
+    from gridap_amd import AssemblyBatch, DeviceContext
+    from gridap_amd._lib import GdxError
+
+    ctx = DeviceContext(0)
+    assem, prob = lognormal_problem(ctx, 16, 3)
+    net = LearnedCoarseSolver(ctx, 8, 1)
+    net.attach(prob)
+    b = AssemblyBatch(ctx)
+    b.add(prob)
+    ctx.close()
+    for call in (prob.numeric, lambda: net.train_step(1), b.capture, lambda: net.get_params()):
+        with pytest.raises(GdxError) as e:
+            call()
+        assert e.value.code == -3
+    b.close()
     net.close()
     prob.close()
