@@ -173,7 +173,7 @@ def bench_learned(args):
         net.attach(prob)
         for k in range(2):
             lr = 2e-3 if f < args.fields // 2 else (1e-3 if f < 3 * args.fields // 4 else 4e-4)
-            losses.append(net.train_step(7919 * f + k, lr, smooth_passes=(4, 8, 16, 32)[(2 * f + k) % 4]))
+            losses.append(net.train_step(7919 * f + k, lr, smooth_passes=(8, 16, 32, 64)[(2 * f + k) % 4]))
             nsteps += 1
         if f % 200 == 0:
             print(f"train field {f}: loss {np.mean(losses[-20:]):.4f}", file=sys.stderr, flush=True)
@@ -187,11 +187,14 @@ def bench_learned(args):
         net.attach(prob)
         rhs = np.random.default_rng(5000 + k).normal(size=prob.ngdofs)
         _, it_j, rr_j, ms_j, _ = G.fpcg(prob, None, rhs, 1e-8, 50000)
+        net.set_options(8, use_network=True)
         _, it_l, rr_l, ms_l, fb = G.fpcg(prob, net, rhs, 1e-8, 50000)
+        net.set_options(8, use_network=False)
+        _, it_0, rr_0, ms_0, fb0 = G.fpcg(prob, net, rhs, 1e-8, 50000)
         if k >= args.warmup:
-            rows.append((it_j, rr_j, ms_j, it_l, rr_l, ms_l, fb))
-    it_j, rr_j, ms_j, it_l, rr_l, ms_l, fb = [np.array([r[i] for r in rows]) for i in range(7)]
-    s_l, s_j = ms_l.mean() / 1e3, ms_j.mean() / 1e3
+            rows.append((it_j, rr_j, ms_j, it_l, rr_l, ms_l, fb, it_0, rr_0, ms_0))
+    it_j, rr_j, ms_j, it_l, rr_l, ms_l, fb, it_0, rr_0, ms_0 = [np.array([r[i] for r in rows]) for i in range(10)]
+    s_l, s_j, s_0 = ms_l.mean() / 1e3, ms_j.mean() / 1e3, ms_0.mean() / 1e3
     out = {
         "metric": "learned-preconditioned FPCG time to 1e-8 (s)",
         "value": round(float(s_l), 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -205,8 +208,10 @@ def bench_learned(args):
                    "loss_first": round(float(np.mean(losses[:20])), 4), "loss_last": round(float(np.mean(losses[-20:])), 4),
                    "jacobi_pcg_s": round(float(s_j), 4), "jacobi_iters": int(it_j.mean()),
                    "fpcg_iters": int(it_l.mean()), "time_ratio_learned_over_jacobi": round(float(s_l / s_j), 3),
+                   "two_grid_without_network_s": round(float(s_0), 4), "two_grid_without_network_iters": int(it_0.mean()),
+                   "time_ratio_without_network_over_jacobi": round(float(s_0 / s_j), 3),
                    "relres_jacobi_max": float(rr_j.max()), "relres_fpcg_max": float(rr_l.max()),
-                   "fallbacks": int(fb.sum())},
+                   "relres_without_network_max": float(rr_0.max()), "fallbacks": int(fb.sum())},
     }
     net.close()
     prob.close()

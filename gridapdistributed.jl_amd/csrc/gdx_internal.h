@@ -182,6 +182,7 @@ struct FpcgPrec {
 int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double rtol, int maxiter, double *xh,
                int *iters, double *relres, double *ms, int *fellback);
 int solve_dinv(gdx_problem *p, const double **dinv, int *n);
+int solve_csr_view(gdx_problem *p, const long long **rstart, const int **rlen, const int **cgid);
 void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st);
 bool flux_supported(int dim, int order, int nq);
 int64_t num_cells(const gdx_problem *p);

@@ -56,6 +56,8 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p);
 int twogrid_apply(void *nn, const double *r, double *z, hipStream_t st);
 int twogrid_train_step(gdx_nn *nn, uint64_t seed, float lr, int smooth_passes, float *loss);
 gdx_problem *twogrid_problem(gdx_nn *nn);
+void twogrid_options(gdx_nn *nn, int sweeps, bool use_net);
+int twogrid_export_coarse(gdx_nn *nn, double *host);
 }  // namespace gdx
 
 struct gdx_nn {

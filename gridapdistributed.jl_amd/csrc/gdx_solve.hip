@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -156,6 +157,41 @@ struct SolveData {
 
 static SolveData *solve_data(gdx_problem *p) { return reinterpret_cast<SolveData *>(p->solve); }
 
+// One solver iteration captured as a hipGraph and replayed: the iteration's
+// kernels (SpMV, reductions, updates and the preconditioner) keep their
+// arguments from one iteration to the next.  GDX_NO_GRAPHS=1 launches them
+// eagerly instead (same kernels, same order).
+namespace {
+struct IterGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~IterGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+bool graphs_enabled() {
+  const char *e = std::getenv("GDX_NO_GRAPHS");
+  return !(e && e[0] == '1');
+}
+
+// run one iteration: capture it on first use, replay afterwards
+template <class F>
+int run_iteration(hipStream_t st, IterGraph &G, F &&body) {
+  if (!graphs_enabled()) return body();
+  if (!G.exec) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return GDX_ERR_HIP;
+    int rc = body();
+    const hipError_t e = hipStreamEndCapture(st, &G.g);
+    if (rc) return rc;
+    if (e != hipSuccess || !G.g) return GDX_ERR_HIP;
+    if (hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0) != hipSuccess) return GDX_ERR_HIP;
+  }
+  return hipGraphLaunch(G.exec, st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+}  // namespace
+
 void solve_free(gdx_problem *p) {
   SolveData *s = solve_data(p);
   if (!s) return;
@@ -279,13 +315,18 @@ static int pcg_run(gdx_problem *p, SolveData *s, double rtol, int maxiter, int *
   int it = 0;
   const int check_every = 5;
   if (bnorm > 0.0) {
+    IterGraph G;
     for (it = 1; it <= maxiter; ++it) {
-      spmv(p, s, s->p, s->q, st);
-      dot2(s, s->p, s->q, nullptr, nullptr, s->sc + 2, st);  // sc[2] = pq
-      hipLaunchKernelGGL(k_cg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->z, s->p, s->q, s->dinv);
-      dot2(s, s->r, s->z, s->r, s->r, s->sc + 4, st);  // sc[4] = rz_new, sc[5] = rr_new
-      hipLaunchKernelGGL(k_cg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
-      hipLaunchKernelGGL(k_cg_roll, dim3(1), dim3(1), 0, st, s->sc);
+      int rci = run_iteration(st, G, [&]() {
+        spmv(p, s, s->p, s->q, st);
+        dot2(s, s->p, s->q, nullptr, nullptr, s->sc + 2, st);  // sc[2] = pq
+        hipLaunchKernelGGL(k_cg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->z, s->p, s->q, s->dinv);
+        dot2(s, s->r, s->z, s->r, s->r, s->sc + 4, st);  // sc[4] = rz_new, sc[5] = rr_new
+        hipLaunchKernelGGL(k_cg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+        hipLaunchKernelGGL(k_cg_roll, dim3(1), dim3(1), 0, st, s->sc);
+        return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+      });
+      if (rci) return rci;
       if (it % check_every == 0 || it == maxiter) {
         if (hipMemcpyAsync(h, s->sc, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
@@ -362,6 +403,18 @@ int solve_dinv(gdx_problem *p, const double **dinv, int *n) {
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
+// rows of the assembled matrix in global-id order: row g holds rlen[g] values
+// at values[rstart[g]..] with column global ids cgid[rstart[g]..]
+int solve_csr_view(gdx_problem *p, const long long **rstart, const int **rlen, const int **cgid) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  *rstart = s->rstart;
+  *rlen = s->rlen;
+  *cgid = s->cgid;
+  return GDX_OK;
+}
+
 void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st) {
   spmv(p, solve_data(p), x, y, st);
 }
@@ -419,13 +472,19 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
       if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return finish(rc);
       if (hipMemcpyAsync(s->p, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
         return finish(GDX_ERR_HIP);
+      IterGraph G;
       for (it = 1; it <= maxiter; ++it) {
-        spmv(p, s, s->p, s->q, st);
-        dot2(s, s->p, s->q, s->p, s->r, s->sc, st);  // sc[0] = pq, sc[1] = pr
-        hipLaunchKernelGGL(k_fcg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->p, s->q);
-        if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return finish(rc);
-        dot2(s, s->z, s->q, s->r, s->r, s->sc + 2, st);  // sc[2] = zq, sc[3] = rr
-        hipLaunchKernelGGL(k_fcg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+        rc = run_iteration(st, G, [&]() {
+          spmv(p, s, s->p, s->q, st);
+          dot2(s, s->p, s->q, s->p, s->r, s->sc, st);  // sc[0] = pq, sc[1] = pr
+          hipLaunchKernelGGL(k_fcg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->p, s->q);
+          int r2 = prec->apply(prec->ctx, s->r, s->z, st);
+          if (r2) return r2;
+          dot2(s, s->z, s->q, s->r, s->r, s->sc + 2, st);  // sc[2] = zq, sc[3] = rr
+          hipLaunchKernelGGL(k_fcg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+          return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+        });
+        if (rc) return finish(rc);
         if (it % 5 == 0 || it == maxiter) {
           double h[2];
           if (hipMemcpyAsync(h, s->sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||

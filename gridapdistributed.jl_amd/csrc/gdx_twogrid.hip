@@ -4,15 +4,22 @@
 //   z  = w D^-1 r                          damped Jacobi pre-smoothing
 //   rc = R (r - A z)                       full weighting (R = P^T, trilinear P)
 //   ec = s N( D^-1 rc / s, log-diagonal )  network coarse solve, s = rms(D^-1 rc)
-//   z += P ec
-//   z += w D^-1 (r - A z)                  damped Jacobi post-smoothing
+//   ec += w Dc^-1 (rc - Ac ec), k times     damped-Jacobi sweeps on the Galerkin
+//                                          coarse operator Ac = P^T A P (27-point
+//                                          stencil computed from the fine matrix)
+//   z += a P ec, a = (P ec).s / (P ec).A(P ec) with s = r - A z: the
+//                                          energy-optimal step along the correction
+//   z += w D^-1 (r - A z)                  damped Jacobi post-smoothing (A z from
+//                                          the two products above: no third SpMV)
 //
 // The coarse unknowns are the fine nodes with even indices; the (n-1)^3
 // interior ones sit on the network's n^3 grid (the last plane is zero
 // padding).  The network is trained on device from the problem's own matrix:
 // a smooth random coarse error ec (white noise smoothed by repeated 3x3x3
 // means), r = A P ec, target ec for the input built from r exactly as above
-// (loss |N - ec/s|^2 / |ec/s|^2).  Every reduction has a fixed order.
+// (loss 1 - cos^2(N, ec): the preconditioner rescales the network's output to
+// the coarse energy optimum, so only its direction matters).  Every reduction
+// has a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +33,9 @@ namespace {
 
 constexpr int kB = 256;
 constexpr int kRed = 256;  // partial sums per reduction
+constexpr int kS = 3 * kRed;  // scalar slots of 'red' after three partial-sum areas
 constexpr double kOmega = 2.0 / 3.0;
+constexpr double kOmegaC = 2.0 / 3.0;  // coarse sweeps
 
 struct TwoGrid {
   gdx_problem *prob = nullptr;
@@ -35,12 +44,18 @@ struct TwoGrid {
   int ng = 0;      // free dofs
   long long *gid_node = nullptr;
   double *flat = nullptr, *dlat = nullptr;   // fine lattice vectors
-  double *t1 = nullptr, *t2 = nullptr;       // global-id temporaries
+  double *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;  // global-id temporaries
   double *rho = nullptr;                     // coarse scaled residual [n^3]
   double *red = nullptr;                     // reduction partials + scalars
   double *ec = nullptr, *ec2 = nullptr;      // training: coarse error [n^3]
   float *tgt = nullptr;
   const double *dinv = nullptr;
+  double *fst = nullptr;   // fine matrix as a 27-point stencil on the lattice [27][nf^3]
+  double *cst = nullptr;   // Galerkin coarse operator P^T A P as a stencil [27][n^3]
+  double *rc = nullptr;    // coarse residual R s [n^3]
+  double *ca = nullptr, *cb = nullptr;  // coarse correction (ping-pong) [n^3]
+  int sweeps = 8;          // coarse damped-Jacobi sweeps after the network
+  bool use_net = true;     // false: the coarse correction starts from zero
 };
 
 TwoGrid *tg_of(gdx_nn *nn) { return reinterpret_cast<TwoGrid *>(nn->tg); }
@@ -57,13 +72,14 @@ __global__ void k_gather(int ng, const long long *__restrict__ gn, const double 
 
 // rho[j] = dlat(2I) * sum_{|o|<=1} w(o) r(2I + o), I = j + 1, for interior coarse nodes; 0 on the padding plane
 __global__ void k_restrict(int n, int nf, const double *__restrict__ flat, const double *__restrict__ dlat,
-                           double *__restrict__ rho) {
+                           double *__restrict__ rho, double *__restrict__ rcoarse) {
   const int64_t tot = (int64_t)n * n * n;
   const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
   if (j >= tot) return;
   const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
   if (jx == n - 1 || jy == n - 1 || jz == n - 1) {
     rho[j] = 0.0;
+    rcoarse[j] = 0.0;
     return;
   }
   const int fx = 2 * (jx + 1), fy = 2 * (jy + 1), fz = 2 * (jz + 1);
@@ -74,7 +90,219 @@ __global__ void k_restrict(int n, int nf, const double *__restrict__ flat, const
         const double w = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
         acc = fma(w, flat[(fx + ox) + (int64_t)nf * ((fy + oy) + (int64_t)nf * (fz + oz))], acc);
       }
+  rcoarse[j] = acc;
   rho[j] = acc * dlat[fx + (int64_t)nf * (fy + (int64_t)nf * fz)];
+}
+
+// fine matrix rows -> 27-point stencil on the lattice: fst[slot][node], slot of
+// the column's offset (dx+1) + 3(dy+1) + 9(dz+1); Dirichlet columns are absent
+// from the free x free matrix and stay zero
+__global__ void k_fine_stencil(int ng, int nf, const long long *__restrict__ rstart, const int *__restrict__ rlen,
+                               const int *__restrict__ cgid, const double *__restrict__ val,
+                               const long long *__restrict__ gn, double *__restrict__ fst) {
+  const int g = blockIdx.x * kB + threadIdx.x;
+  if (g >= ng) return;
+  const long long a = gn[g];
+  const int ax = (int)(a % nf), ay = (int)((a / nf) % nf), az = (int)(a / ((int64_t)nf * nf));
+  const int64_t nfl = (int64_t)nf * nf * nf;
+  for (int k = 0; k < rlen[g]; ++k) {
+    const long long b = gn[cgid[rstart[g] + k]];
+    const int dx = (int)(b % nf) - ax, dy = (int)((b / nf) % nf) - ay, dz = (int)(b / ((int64_t)nf * nf)) - az;
+    fst[(int64_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * nfl + a] = val[rstart[g] + k];
+  }
+}
+
+// per component: the coarse offsets d (and weights) of the coarse nodes J = I + d
+// whose hat function covers fine offset t = b - 2I in [-2, 2]
+__device__ __forceinline__ int cover(int t, int *d, double *w) {
+  switch (t) {
+    case -2: d[0] = -1; w[0] = 1.0; return 1;
+    case -1: d[0] = -1; w[0] = 0.5; d[1] = 0; w[1] = 0.5; return 2;
+    case 0: d[0] = 0; w[0] = 1.0; return 1;
+    case 1: d[0] = 0; w[0] = 0.5; d[1] = 1; w[1] = 0.5; return 2;
+    default: d[0] = 1; w[0] = 1.0; return 1;
+  }
+}
+
+// Galerkin coarse operator Ac = P^T A P as a 27-point stencil on the network
+// grid: cst[slot(d)][I - 1] = sum_a sum_b P(a, I) A(a, b) P(b, I + d), with
+// entries to boundary coarse nodes (and on the padding plane) zero
+__global__ void k_galerkin(int n, int nf, const double *__restrict__ fst, double *__restrict__ cst) {
+  const int64_t tot = (int64_t)n * n * n;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
+  double acc[27];
+  for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+  const int64_t nfl = (int64_t)nf * nf * nf;
+  if (jx < n - 1 && jy < n - 1 && jz < n - 1) {
+    const int Ix = jx + 1, Iy = jy + 1, Iz = jz + 1;
+    for (int oa = 0; oa < 27; ++oa) {
+      const int ox = oa % 3 - 1, oy = (oa / 3) % 3 - 1, oz = oa / 9 - 1;
+      const double wa = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
+      const int64_t a = (2 * Ix + ox) + (int64_t)nf * ((2 * Iy + oy) + (int64_t)nf * (2 * Iz + oz));
+      for (int e = 0; e < 27; ++e) {
+        const double A = fst[e * nfl + a];
+        if (A == 0.0) continue;
+        const int tx = ox + e % 3 - 1, ty = oy + (e / 3) % 3 - 1, tz = oz + e / 9 - 1;
+        int dx[2], dy[2], dz[2];
+        double wx[2], wy[2], wz[2];
+        const int nx = cover(tx, dx, wx), ny = cover(ty, dy, wy), nz = cover(tz, dz, wz);
+        for (int i = 0; i < nx; ++i)
+          for (int k = 0; k < ny; ++k)
+            for (int l = 0; l < nz; ++l) {
+              const int Jx = Ix + dx[i], Jy = Iy + dy[k], Jz = Iz + dz[l];
+              if (Jx < 1 || Jy < 1 || Jz < 1 || Jx > n - 1 || Jy > n - 1 || Jz > n - 1) continue;
+              acc[(dx[i] + 1) + 3 * (dy[k] + 1) + 9 * (dz[l] + 1)] += wa * A * wx[i] * wy[k] * wz[l];
+            }
+      }
+    }
+  }
+  for (int k = 0; k < 27; ++k) cst[k * tot + j] = acc[k];
+}
+
+// ec = s * out (network units -> coarse error), 0 on the padding plane; or 0 everywhere
+__global__ void k_out_to_coarse(int n, const float *__restrict__ out, const double *__restrict__ s, int use,
+                                double *__restrict__ ec) {
+  const int64_t tot = (int64_t)n * n * n;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
+  ec[j] = (use && jx < n - 1 && jy < n - 1 && jz < n - 1) ? (double)out[j] * *s : 0.0;
+}
+
+// b = Ac a on the network grid (padding plane 0)
+__global__ void k_coarse_apply(int n, const double *__restrict__ cst, const double *__restrict__ a,
+                               double *__restrict__ b) {
+  const int64_t tot = (int64_t)n * n * n;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
+  double r = 0.0;
+  if (jx < n - 1 && jy < n - 1 && jz < n - 1)
+    for (int k = 0; k < 27; ++k) {
+      const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
+      if (x < 0 || y < 0 || z < 0 || x > n - 2 || y > n - 2 || z > n - 2) continue;
+      r = fma(cst[k * tot + j], a[x + (int64_t)n * (y + (int64_t)n * z)], r);
+    }
+  b[j] = r;
+}
+
+__global__ void k_scale_by(int64_t tot, const double *__restrict__ alpha, double *__restrict__ a) {
+  const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (i < tot) a[i] *= *alpha;
+}
+
+// partial sums of out.tgt, out.out, tgt.tgt (three areas of kRed)
+__global__ __launch_bounds__(kB) void k_dot3f(int64_t tot, const float *__restrict__ a, const float *__restrict__ b,
+                                             double *__restrict__ part) {
+  __shared__ double s1[kB], s2[kB], s3[kB];
+  double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kB) {
+    const double x = a[i], y = b[i];
+    p1 = fma(x, y, p1);
+    p2 = fma(x, x, p2);
+    p3 = fma(y, y, p3);
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  s3[threadIdx.x] = p3;
+  __syncthreads();
+  for (int o = kB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      s3[threadIdx.x] += s3[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[kRed + blockIdx.x] = s2[0];
+    part[2 * kRed + blockIdx.x] = s3[0];
+  }
+}
+
+// sums of the three partial areas into out[0..3), and the loss
+// 1 - (N.t)^2 / (|N|^2 |t|^2) into out[3]
+__global__ __launch_bounds__(kB) void k_finish3(const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double s1[kB], s2[kB], s3[kB];
+  double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+  for (int i = threadIdx.x; i < kRed; i += kB) {
+    p1 += part[i];
+    p2 += part[kRed + i];
+    p3 += part[2 * kRed + i];
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  s3[threadIdx.x] = p3;
+  __syncthreads();
+  for (int o = kB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      s3[threadIdx.x] += s3[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s1[0];
+    out[1] = s2[0] > 0.0 ? s2[0] : 1e-300;
+    out[2] = s3[0] > 0.0 ? s3[0] : 1e-300;
+    out[3] = 1.0 - s1[0] * s1[0] / (out[1] * out[2]);
+  }
+}
+
+// gradient of 1 - (N.t)^2 / (|N|^2 |t|^2) with respect to N:
+// -2 (N.t) / (|N|^2 |t|^2) (t - (N.t) / |N|^2 N)
+__global__ void k_cos_grad(int64_t tot, const float *__restrict__ out, const float *__restrict__ tgt,
+                           const double *__restrict__ sums, float *__restrict__ dout) {
+  const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (i >= tot) return;
+  const double nt = sums[0], nn = sums[1], tt = sums[2];
+  dout[i] = (float)(-2.0 * nt / (nn * tt) * ((double)tgt[i] - nt / nn * (double)out[i]));
+}
+
+// one damped-Jacobi sweep on the coarse system: b = a + w Dc^-1 (rc - Ac a)
+__global__ void k_coarse_sweep(int n, double w, const double *__restrict__ cst, const double *__restrict__ rc,
+                               const double *__restrict__ a, double *__restrict__ b) {
+  const int64_t tot = (int64_t)n * n * n;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
+  if (jx == n - 1 || jy == n - 1 || jz == n - 1) {
+    b[j] = 0.0;
+    return;
+  }
+  double r = rc[j];
+  for (int k = 0; k < 27; ++k) {
+    const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
+    if (x < 0 || y < 0 || z < 0 || x > n - 2 || y > n - 2 || z > n - 2) continue;
+    r -= cst[k * tot + j] * a[x + (int64_t)n * (y + (int64_t)n * z)];
+  }
+  b[j] = a[j] + w * r / cst[13 * tot + j];
+}
+
+// fine lattice value of P ec (double coarse field on the network grid)
+__global__ void k_prolong_d(int n, int nf, const double *__restrict__ ec, double *__restrict__ flat) {
+  const int64_t tot = (int64_t)nf * nf * nf;
+  const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (i >= tot) return;
+  const int x = (int)(i % nf), y = (int)((i / nf) % nf), z = (int)(i / ((int64_t)nf * nf));
+  if (x == 0 || y == 0 || z == 0 || x == nf - 1 || y == nf - 1 || z == nf - 1) {
+    flat[i] = 0.0;
+    return;
+  }
+  double acc = 0.0;
+  for (int kz = 0; kz < 1 + (z & 1); ++kz)
+    for (int ky = 0; ky < 1 + (y & 1); ++ky)
+      for (int kx = 0; kx < 1 + (x & 1); ++kx) {
+        const int Ix = (x >> 1) + kx, Iy = (y >> 1) + ky, Iz = (z >> 1) + kz;
+        if (Ix < 1 || Iy < 1 || Iz < 1 || Ix > n - 1 || Iy > n - 1 || Iz > n - 1) continue;
+        const double w = ((x & 1) ? 0.5 : 1.0) * ((y & 1) ? 0.5 : 1.0) * ((z & 1) ? 0.5 : 1.0);
+        acc = fma(w, ec[(Ix - 1) + (int64_t)n * ((Iy - 1) + (int64_t)n * (Iz - 1))], acc);
+      }
+  flat[i] = acc;
 }
 
 // partial sums of squares (and of a*b) over a grid-stride slice
@@ -229,6 +457,68 @@ __global__ void k_box(int n, const double *__restrict__ a, double *__restrict__ 
 
 __global__ void k_set(double *__restrict__ p, double v) { *p = v; }
 
+// partial sums of a.b and c.d (grid-stride slices), part[0..kRed) and part[kRed..2kRed)
+__global__ __launch_bounds__(kB) void k_dot2p(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                             const double *__restrict__ c, const double *__restrict__ d,
+                                             double *__restrict__ part) {
+  __shared__ double s1[kB], s2[kB];
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    p1 = fma(a[i], b[i], p1);
+    p2 = fma(c[i], d[i], p2);
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  __syncthreads();
+  for (int o = kB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[kRed + blockIdx.x] = s2[0];
+  }
+}
+
+// alpha = (c.s) / (c.Ac): the energy-optimal step along the coarse correction
+// (0 when the correction vanishes or is not finite)
+__global__ __launch_bounds__(kB) void k_alpha(const double *__restrict__ part, double *__restrict__ alpha) {
+  __shared__ double s1[kB], s2[kB];
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = threadIdx.x; i < kRed; i += kB) {
+    p1 += part[i];
+    p2 += part[kRed + i];
+  }
+  s1[threadIdx.x] = p1;
+  s2[threadIdx.x] = p2;
+  __syncthreads();
+  for (int o = kB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = s1[0] / s2[0];
+    *alpha = (s2[0] > 0.0 && isfinite(a)) ? a : 0.0;
+  }
+}
+
+// z += alpha c ; z += w D^-1 (s - alpha Ac)   (r - A z = s - alpha Ac after the coarse step)
+__global__ void k_coarse_post(int n, double w, const double *__restrict__ alpha, const double *__restrict__ dinv,
+                              const double *__restrict__ s, const double *__restrict__ c,
+                              const double *__restrict__ ac, double *__restrict__ z) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i < n) {
+    const double a = *alpha;
+    z[i] = fma(a, c[i], z[i]) + w * dinv[i] * fma(-a, ac[i], s[i]);
+  }
+}
+
 // ec (double) -> network-output units for the prolongation (float) and the target ec / s
 __global__ void k_ec_to_float(int64_t tot, const double *__restrict__ ec, float *__restrict__ outf) {
   const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
@@ -238,42 +528,22 @@ __global__ void k_target(int64_t tot, const double *__restrict__ ec, const doubl
   const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
   if (i < tot) tgt[i] = (float)(ec[i] / *s);
 }
-// relative squared error: dout = 2 (out - tgt) / |tgt|^2, part = (out - tgt)^2 partial sums
-__global__ __launch_bounds__(kB) void k_loss(int64_t tot, const float *__restrict__ out, const float *__restrict__ tgt,
-                                            const double *__restrict__ tt, float *__restrict__ dout,
-                                            double *__restrict__ part) {
-  __shared__ double sh[kB];
-  const double inv = 1.0 / *tt;
-  double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kB) {
-    const double d = (double)out[i] - (double)tgt[i];
-    dout[i] = (float)(2.0 * d * inv);
-    s = fma(d, d, s);
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = kB / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] * inv;
-}
-
-// rms of rho into red[kRed] (scalar slot)
+// sum of squares / rms into a scalar slot of red
 void rms(TwoGrid *T, const double *a, const float *fa, int64_t count, double *dst, int mode, hipStream_t st) {
   hipLaunchKernelGGL(k_sumsq, dim3(kRed), dim3(kB), 0, st, count, a, fa, T->red);
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(kB), 0, st, T->red, kRed, (double)count, mode, dst);
 }
 
-// network input from a fine residual in global-id order (s into red[kRed])
+// network input from a fine residual in global-id order (s into red[kS])
 void make_input(gdx_nn *nn, TwoGrid *T, const double *r, hipStream_t st) {
   nn::Net &N = nn->net;
-  const int64_t v = (int64_t)T->n * T->n * T->n, nfl = (int64_t)T->nf * T->nf * T->nf;
-  (void)hipMemsetAsync(T->flat, 0, nfl * sizeof(double), st);
+  const int64_t v = (int64_t)T->n * T->n * T->n;
+  // restriction reads only interior lattice nodes, all of which the scatter
+  // writes: no clearing (and no memset, so this can be captured in a graph)
   hipLaunchKernelGGL(k_scatter, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, T->gid_node, r, T->flat);
-  hipLaunchKernelGGL(k_restrict, dim3(nb(v)), dim3(kB), 0, st, T->n, T->nf, T->flat, T->dlat, T->rho);
-  rms(T, T->rho, nullptr, v, T->red + kRed, 0, st);
-  hipLaunchKernelGGL(k_input0, dim3(nb(v)), dim3(kB), 0, st, v, T->rho, T->red + kRed, N.inp_f);
+  hipLaunchKernelGGL(k_restrict, dim3(nb(v)), dim3(kB), 0, st, T->n, T->nf, T->flat, T->dlat, T->rho, T->rc);
+  rms(T, T->rho, nullptr, v, T->red + kS, 0, st);
+  hipLaunchKernelGGL(k_input0, dim3(nb(v)), dim3(kB), 0, st, v, T->rho, T->red + kS, N.inp_f);
   N.input_from_f32(N.inp_f);
 }
 
@@ -281,10 +551,28 @@ void make_input(gdx_nn *nn, TwoGrid *T, const double *r, hipStream_t st) {
 
 gdx_problem *twogrid_problem(gdx_nn *nn) { return tg_of(nn) ? tg_of(nn)->prob : nullptr; }
 
+int twogrid_export_coarse(gdx_nn *nn, double *h) {
+  TwoGrid *T = tg_of(nn);
+  const int64_t v = (int64_t)T->n * T->n * T->n;
+  hipStream_t st = T->prob->ctx->stream;
+  if (hipMemcpyAsync(h, T->cst, 27 * v * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GDX_ERR_HIP;
+  return GDX_OK;
+}
+
+void twogrid_options(gdx_nn *nn, int sweeps, bool use_net) {
+  if (TwoGrid *T = tg_of(nn)) {
+    T->sweeps = sweeps;
+    T->use_net = use_net;
+  }
+}
+
 void twogrid_free(gdx_nn *nn) {
   TwoGrid *T = tg_of(nn);
   if (!T) return;
-  void *ptrs[] = {T->gid_node, T->flat, T->dlat, T->t1, T->t2, T->rho, T->red, T->ec, T->ec2, T->tgt};
+  void *ptrs[] = {T->gid_node, T->flat, T->dlat, T->t1, T->t2, T->t3, T->rho, T->red, T->ec, T->ec2, T->tgt,
+                  T->fst, T->cst, T->rc, T->ca, T->cb};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete T;
@@ -330,11 +618,17 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
               hipMalloc(reinterpret_cast<void **>(&T->dlat), nfl * sizeof(double)) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->t1), gsz) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->t2), gsz) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->t3), gsz) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->rho), v * sizeof(double)) == hipSuccess &&
-              hipMalloc(reinterpret_cast<void **>(&T->red), (kRed + 8) * sizeof(double)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->red), (kS + 16) * sizeof(double)) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->ec), v * sizeof(double)) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->ec2), v * sizeof(double)) == hipSuccess &&
-              hipMalloc(reinterpret_cast<void **>(&T->tgt), v * sizeof(float)) == hipSuccess;
+              hipMalloc(reinterpret_cast<void **>(&T->tgt), v * sizeof(float)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->fst), 27 * nfl * sizeof(double)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->cst), 27 * v * sizeof(double)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->rc), v * sizeof(double)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->ca), v * sizeof(double)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&T->cb), v * sizeof(double)) == hipSuccess;
     if (!ok) {
       twogrid_free(nn);
       return GDX_ERR_ALLOC;
@@ -349,6 +643,14 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
   (void)hipMemsetAsync(T->dlat, 0, nfl * sizeof(double), st);
   hipLaunchKernelGGL(k_scatter, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, T->gid_node, T->dinv, T->dlat);
   hipLaunchKernelGGL(k_input1, dim3(nb(v)), dim3(kB), 0, st, n, nf, p->kp.h[0], T->dlat, N.inp_f + v);
+  // fine stencil and the Galerkin coarse operator of the current matrix
+  const long long *rs = nullptr;
+  const int *rl = nullptr, *cg = nullptr;
+  if ((rc = solve_csr_view(p, &rs, &rl, &cg))) return rc;
+  if (hipMemsetAsync(T->fst, 0, 27 * nfl * sizeof(double), st) != hipSuccess) return GDX_ERR_HIP;
+  hipLaunchKernelGGL(k_fine_stencil, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, nf, rs, rl, cg, p->dev.values,
+                     T->gid_node, T->fst);
+  hipLaunchKernelGGL(k_galerkin, dim3(nb(v)), dim3(kB), 0, st, n, nf, T->fst, T->cst);
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
@@ -365,13 +667,33 @@ int twogrid_apply(void *ctx, const double *r, double *z, hipStream_t st) {
   solve_spmv_dev(p, z, T->t1, st);
   hipLaunchKernelGGL(k_rminus, dim3(nb(ng)), dim3(kB), 0, st, ng, r, T->t1);
   make_input(nn, T, T->t1, st);
-  int rc = N.forward_dev();
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_prolong, dim3(nb(nfl)), dim3(kB), 0, st, T->n, T->nf, N.out, T->red + kRed, T->flat);
-  hipLaunchKernelGGL(k_gather, dim3(nb(ng)), dim3(kB), 0, st, ng, T->gid_node, T->flat, T->t2);
-  hipLaunchKernelGGL(k_add, dim3(nb(ng)), dim3(kB), 0, st, ng, z, T->t2);
-  solve_spmv_dev(p, z, T->t1, st);
-  hipLaunchKernelGGL(k_jacobi_post, dim3(nb(ng)), dim3(kB), 0, st, ng, kOmega, T->dinv, r, T->t1, z);
+  if (T->use_net) {
+    int rc = N.forward_dev();
+    if (rc) return rc;
+  }
+  const int64_t v = (int64_t)T->n * T->n * T->n;
+  hipLaunchKernelGGL(k_out_to_coarse, dim3(nb(v)), dim3(kB), 0, st, T->n, N.out, T->red + kS, T->use_net ? 1 : 0,
+                     T->ca);
+  if (T->use_net) {
+    // the network's direction, scaled to minimise the coarse energy error:
+    // a1 = (ec.rc) / (ec.Ac ec)
+    hipLaunchKernelGGL(k_coarse_apply, dim3(nb(v)), dim3(kB), 0, st, T->n, T->cst, T->ca, T->cb);
+    hipLaunchKernelGGL(k_dot2p, dim3(kRed), dim3(kB), 0, st, (int)v, T->ca, T->rc, T->ca, T->cb, T->red);
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, T->red, T->red + kS + 6);
+    hipLaunchKernelGGL(k_scale_by, dim3(nb(v)), dim3(kB), 0, st, v, T->red + kS + 6, T->ca);
+  }
+  double *ea = T->ca, *eb = T->cb;
+  for (int k = 0; k < T->sweeps; ++k) {
+    hipLaunchKernelGGL(k_coarse_sweep, dim3(nb(v)), dim3(kB), 0, st, T->n, kOmegaC, T->cst, T->rc, ea, eb);
+    std::swap(ea, eb);
+  }
+  hipLaunchKernelGGL(k_prolong_d, dim3(nb(nfl)), dim3(kB), 0, st, T->n, T->nf, ea, T->flat);
+  hipLaunchKernelGGL(k_gather, dim3(nb(ng)), dim3(kB), 0, st, ng, T->gid_node, T->flat, T->t2);  // c
+  solve_spmv_dev(p, T->t2, T->t3, st);                                                             // A c
+  hipLaunchKernelGGL(k_dot2p, dim3(kRed), dim3(kB), 0, st, ng, T->t2, T->t1, T->t2, T->t3, T->red);
+  hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, T->red, T->red + kS + 5);
+  hipLaunchKernelGGL(k_coarse_post, dim3(nb(ng)), dim3(kB), 0, st, ng, kOmega, T->red + kS + 5, T->dinv, T->t1,
+                     T->t2, T->t3, z);
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
@@ -392,22 +714,24 @@ int twogrid_train_step(gdx_nn *nn, uint64_t seed, float lr, int smooth_passes, f
   }
   // r = A P ec
   hipLaunchKernelGGL(k_ec_to_float, dim3(nb(v)), dim3(kB), 0, st, v, T->ec, N.out);
-  hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, st, T->red + kRed + 2, 1.0);  // unit scale: P ec itself
-  hipLaunchKernelGGL(k_prolong, dim3(nb(nfl)), dim3(kB), 0, st, n, T->nf, N.out, T->red + kRed + 2, T->flat);
+  hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, st, T->red + kS + 2, 1.0);  // unit scale: P ec itself
+  hipLaunchKernelGGL(k_prolong, dim3(nb(nfl)), dim3(kB), 0, st, n, T->nf, N.out, T->red + kS + 2, T->flat);
   hipLaunchKernelGGL(k_gather, dim3(nb(ng)), dim3(kB), 0, st, ng, T->gid_node, T->flat, T->t2);
   solve_spmv_dev(p, T->t2, T->t1, st);
   // input from r, target ec / s
   make_input(nn, T, T->t1, st);
-  hipLaunchKernelGGL(k_target, dim3(nb(v)), dim3(kB), 0, st, v, T->ec, T->red + kRed, T->tgt);
+  hipLaunchKernelGGL(k_target, dim3(nb(v)), dim3(kB), 0, st, v, T->ec, T->red + kS, T->tgt);
   int rc = N.forward_dev();
   if (rc) return rc;
-  rms(T, nullptr, T->tgt, v, T->red + kRed + 3, 1, st);  // |tgt|^2
-  hipLaunchKernelGGL(k_loss, dim3(kRed), dim3(kB), 0, st, v, N.out, T->tgt, T->red + kRed + 3, N.dout, T->red);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kB), 0, st, T->red, kRed, 1.0, 1, T->red + kRed + 4);
+  // scale-free loss 1 - cos^2(N, ec): the preconditioner rescales the
+  // network's output optimally, so only its direction matters
+  hipLaunchKernelGGL(k_dot3f, dim3(kRed), dim3(kB), 0, st, v, N.out, T->tgt, T->red);
+  hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kB), 0, st, T->red, T->red + kS + 8);
+  hipLaunchKernelGGL(k_cos_grad, dim3(nb(v)), dim3(kB), 0, st, v, N.out, T->tgt, T->red + kS + 8, N.dout);
   if ((rc = N.backward_dev(false))) return rc;
   if ((rc = N.adam_step(lr, 1.0f))) return rc;
   double h = 0.0;
-  if (hipMemcpyAsync(&h, T->red + kRed + 4, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(&h, T->red + kS + 11, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return GDX_ERR_HIP;
   if (loss) *loss = (float)h;
@@ -457,6 +781,32 @@ int gdx_nn_train_step(gdx_nn *nn, uint64_t seed, float lr, int32_t smooth_passes
   }
   if (hipSetDevice(nn->ctx->device) != hipSuccess) return GDX_ERR_HIP;
   return gdx::twogrid_train_step(nn, seed, lr, smooth_passes, loss);
+}
+
+int gdx_nn_set_options(gdx_nn *nn, int32_t coarse_sweeps, int32_t use_network) {
+  if (!nn || coarse_sweeps < 0 || coarse_sweeps > 64) {
+    gdx::set_error("invalid argument");
+    return GDX_ERR_ARG;
+  }
+  if (!nn->tg) {
+    gdx::set_error("gdx_nn_attach has not run");
+    return GDX_ERR_STATE;
+  }
+  gdx::twogrid_options(nn, coarse_sweeps, use_network != 0);
+  return GDX_OK;
+}
+
+int gdx_nn_coarse_operator(gdx_nn *nn, double *stencil) {
+  if (!nn || !stencil) {
+    gdx::set_error("NULL argument");
+    return GDX_ERR_ARG;
+  }
+  if (!nn->tg || !nn->ctx) {
+    gdx::set_error("gdx_nn_attach has not run");
+    return GDX_ERR_STATE;
+  }
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return GDX_ERR_HIP;
+  return gdx::twogrid_export_coarse(nn, stencil);
 }
 
 int gdx_nn_apply(gdx_nn *nn, const double *r, double *z) {

@@ -59,6 +59,17 @@ class LearnedCoarseSolver:
         _lib.call("gdx_nn_train_step", self.handle, int(seed), float(lr), int(smooth_passes), ctypes.byref(loss))
         return loss.value
 
+    def set_options(self, coarse_sweeps=8, use_network=True):
+        """Coarse damped-Jacobi sweeps after the network; use_network=False is
+        the same preconditioner with the coarse sweeps started from zero."""
+        _lib.call("gdx_nn_set_options", self.handle, int(coarse_sweeps), 1 if use_network else 0)
+
+    def coarse_operator(self):
+        """Galerkin coarse operator of the attached matrix, stencil [27][n][n][n]."""
+        st = np.empty((27, self.n, self.n, self.n))
+        _lib.call("gdx_nn_coarse_operator", self.handle, st.ctypes.data)
+        return st
+
     def apply(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.empty_like(r)

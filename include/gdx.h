@@ -230,6 +230,14 @@ int gdx_nn_attach(gdx_nn *nn, gdx_problem *prob);
 /* one Adam step on a fresh sample (smooth coarse error of the given seed,
    smoothed by smooth_passes 3x3x3 means) from the attached matrix; loss = relative squared error */
 int gdx_nn_train_step(gdx_nn *nn, uint64_t seed, float lr, int32_t smooth_passes, float *loss);
+/* coarse damped-Jacobi sweeps on the Galerkin coarse operator after the
+   network's coarse solve (default 8); use_network = 0 starts the sweeps from
+   zero instead (the same preconditioner without the network) */
+int gdx_nn_set_options(gdx_nn *nn, int32_t coarse_sweeps, int32_t use_network);
+/* the Galerkin coarse operator of the attached matrix as a 27-point stencil:
+   stencil[(dx+1) + 3(dy+1) + 9(dz+1)][j] on the network grid (j = I - 1 per
+   direction for interior coarse node I; the last plane is zero padding) */
+int gdx_nn_coarse_operator(gdx_nn *nn, double *stencil);
 /* z = M r on host vectors (global-id order) */
 int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
 /* Flexible PCG with the learned two-grid preconditioner (nn attached to prob),
