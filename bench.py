@@ -327,6 +327,13 @@ def main():
     prob = assem._problem_for(data)  # symbolic phase, once
     t_symbolic = time.perf_counter() - t0
     nnz = prob.owned_nnz()
+    # autotuner: z-marching columns of 16/32/64 rows or the per-tile kernel
+    tuner = G.AssemblyBatch(ctx)
+    tuner.add(prob)
+    chosen = int(tuner.autotune(reps=3)[0])
+    tuner.close()
+    variant, rows_z = prob.get_kernel()
+    kernel = (f"k_march3d<2,3,4,8> ({rows_z} z rows per workgroup)" if variant == 0 else "k_assemble3d<2,3,4,8,8>")
 
     for _ in range(args.warmup):
         prob.numeric()
@@ -351,7 +358,8 @@ def main():
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     # (copy-calibrated) on this workload; only valid at the default size
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_k_march3d_poisson3d_q2_64.json")
+    tname = "k_march3d" if variant == 0 else "k_assemble3d"
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{tname}_poisson3d_q2_64.json")
     if args.cells == 64 and os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh)["traffic_bytes"]
@@ -375,6 +383,7 @@ def main():
             "free_dofs": prob.ngdofs,
             "parts": 8,
             "symbolic_s": round(t_symbolic, 3),
+            "autotuned": {"variant": "z-marching" if variant == 0 else "per-tile", "z_rows_per_workgroup": rows_z},
             "kernel_ms": {"coefficients": round(coef_ms, 4), "rhs": round(rhs_ms, 4), "matrix": round(mat_ms, 4),
                           "step_events": round(tot_ms, 4)},
         },
@@ -385,7 +394,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_march3d<2,3,4,8>",
+            "kernel": kernel,
             "alg_bytes_per_launch": int(alg_bytes),
         },
     }

@@ -4,7 +4,8 @@
 //   gdx_batch_add       problems after their symbolic phase, on one context
 //   gdx_batch_autotune  per 3D problem, times the z-marching matrix kernel
 //                       with columns of 16, 32 and 64 z rows per workgroup
-//                       (cells per block) and keeps the fastest tiling
+//                       (cells per block) and the per-tile kernel, and keeps
+//                       the fastest kernel and tiling
 //   gdx_batch_capture   records every problem's numeric launches (coefficient,
 //                       RHS and matrix kernels; no memsets) into one hipGraph
 //   gdx_batch_launch    replays it on the context's stream
@@ -77,12 +78,13 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
   int rc = GDX_OK;
   for (size_t i = 0; i < b->probs.size() && !rc; ++i) {
     gdx_problem *p = b->probs[i];
-    int best = p->tileR[2];
+    int best = p->tileR[2], best_var = p->kvariant3d;
     if (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION) {
       float best_ms = 1e30f;
-      for (int r3 : {16, 32, 64}) {
-        const int R[3] = {p->tileR[0], p->tileR[1], r3};
-        if ((rc = gdx::retile(p, R))) break;
+      // candidates: z-marching columns of 16, 32, 64 z rows; the per-tile kernel (rows_z = 0)
+      for (int r3 : {16, 32, 64, 0}) {
+        const int var = r3 == 0 ? 1 : 0;
+        if ((rc = gdx::set_kernel_variant(p, var, r3))) break;
         if ((rc = gdx::numeric_launches(p, st))) break;  // warm-up
         if (hipEventRecord(e0, st) != hipSuccess) rc = GDX_ERR_HIP;
         for (int k = 0; k < reps && !rc; ++k) rc = gdx::numeric_launches(p, st);
@@ -92,12 +94,11 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
           rc = rc ? rc : GDX_ERR_HIP;
           break;
         }
-        if (ms < best_ms) best_ms = ms, best = r3;
+        if (ms < best_ms) best_ms = ms, best = p->tileR[2], best_var = var;
       }
-      const int R[3] = {p->tileR[0], p->tileR[1], best};
-      if (!rc) rc = gdx::retile(p, R);
+      if (!rc) rc = gdx::set_kernel_variant(p, best_var, best);
     }
-    if (chosen) chosen[i] = best;
+    if (chosen) chosen[i] = best_var == 1 ? -best : best;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

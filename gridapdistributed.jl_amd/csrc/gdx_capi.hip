@@ -18,7 +18,6 @@ int launch_coefficients(gdx_problem *p, hipStream_t st);
 int launch_rhs(gdx_problem *p, hipStream_t st);
 int launch_matrix(gdx_problem *p, hipStream_t st);
 bool matrix_kernel_supported(int dim, int K, int NQ);
-int pattern_capacity_3d(int K, int NQ);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -101,7 +100,9 @@ int check_desc(const gdx_problem_desc *d) {
 
 // tiles whose position patterns fit in the matrix kernel's LDS go first
 void order_tiles(gdx_problem *p) {
-  const int cap = p->D == 3 ? gdx::pattern_capacity_3d(p->K, p->NQ) : 0;
+  const int cap = p->D != 3             ? 0
+                  : p->kvariant3d == 1 ? gdx::pattern_capacity_tile3d(p->K, p->NQ)
+                                       : gdx::pattern_capacity_3d(p->K, p->NQ);
   std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
   p->n_fast_tiles = 0;
   for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += t.npat <= cap ? 1 : 0;
@@ -284,6 +285,21 @@ int retile(gdx_problem *p, const int R[3]) {
   if ((rc = upload(&p->dev.tile_pats, p->tile_pats))) return rc;
   return GDX_OK;
 }
+
+// 3D kernel variant: 0 = z-marching columns with rows_z z rows per workgroup,
+// 1 = per-tile kernel with its own tile shape (rows_z ignored)
+int set_kernel_variant(gdx_problem *p, int variant, int rows_z) {
+  if (p->D != 3) return GDX_OK;
+  int R[3];
+  if (variant == 1) {
+    tile3d_shape(p->K, R);
+  } else {
+    tile_shape(3, p->K, R);
+    R[2] = rows_z;
+  }
+  p->kvariant3d = variant;
+  return retile(p, R);
+}
 }  // namespace gdx
 
 extern "C++" namespace gdx {
@@ -334,6 +350,24 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
     return rc;
   }
   GDX_HIP(hipEventRecord(p->ev[3], st));
+  return GDX_OK;
+}
+
+int gdx_problem_set_kernel(gdx_problem *p, int32_t variant, int32_t rows_z) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done && p->ctx, GDX_ERR_STATE, "needs gdx_symbolic and a device context");
+  GDX_REQUIRE(variant == 0 || variant == 1, GDX_ERR_ARG, "variant must be 0 (z-marching) or 1 (per tile)");
+  GDX_REQUIRE(variant == 1 || (rows_z >= p->K && rows_z % p->K == 0 && rows_z <= 4096), GDX_ERR_ARG,
+              "rows_z must be a positive multiple of the order");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+  return gdx::set_kernel_variant(p, variant, rows_z);
+}
+
+int gdx_problem_get_kernel(gdx_problem *p, int32_t *variant, int32_t *rows_z) {
+  GDX_REQUIRE(p && variant && rows_z, GDX_ERR_ARG, "NULL argument");
+  *variant = p->kvariant3d;
+  *rows_z = p->tileR[2];
   return GDX_OK;
 }
 

@@ -343,7 +343,11 @@ int launch_rhs(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    return GDX_OK;  // the z-marching matrix kernel writes b itself
+    if (p->kvariant3d == 0) return GDX_OK;  // the z-marching matrix kernel writes b itself
+    if (K == 2 && NQ == 3) rhs_run<3, 2, 3, 4, 8, 8>(p, st);
+    else if (K == 2 && NQ == 2) rhs_run<3, 2, 2, 4, 8, 8>(p, st);
+    else if (K == 1 && NQ == 2) rhs_run<3, 1, 2, 8, 8, 2>(p, st);
+    else return GDX_ERR_UNSUPPORTED;
   } else {
     if (K == 2 && NQ == 3) rhs_run<2, 2, 3, 16, 16, 1>(p, st);
     else if (K == 2 && NQ == 2) rhs_run<2, 2, 2, 16, 16, 1>(p, st);
@@ -358,7 +362,7 @@ int launch_matrix(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    return launch_matrix3d(p, st);
+    return p->kvariant3d == 1 ? launch_matrix_tile3d(p, st) : launch_matrix3d(p, st);
   } else {
     if (K == 2 && NQ == 3) run2<2, 3, 16, 16>(p, st);
     else if (K == 2 && NQ == 2) run2<2, 2, 16, 16>(p, st);

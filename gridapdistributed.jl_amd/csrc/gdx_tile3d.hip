@@ -1,0 +1,296 @@
+// 3D matrix kernel of libgdx, per-tile form (the second variant next to the
+// z-marching kernel of gdx_march3d.hip; the autotuner picks per problem):
+// owner-computes-rows, sum-factorised assembly of the Q_K Poisson operator on a
+// Cartesian mesh (formulation in gdx_kernels.hip).  One workgroup = one tile of
+// R1 x R2 x R3 owned rows; the RHS comes from k_rhs over the same tiles.
+//
+// LDS timeline of a workgroup:
+//   kernel start  row table of the tile (persistent), kappa at the tile's
+//                 quadrature region
+//   stage 1       X = sum over x quadrature points, per x pair      (kappa -> X)
+//   stage 2       Y = sum over y quadrature points, per (x,y) pair  (X -> registers)
+//   stage 3       per z row: sum over z points into a slab of rows, then the
+//                 rows are stored contiguously in CSR order; two slabs
+//                 alternate and the tile's position patterns sit above them.
+#include <hip/hip_runtime.h>
+
+#include "gdx_device.h"
+
+namespace gdx {
+
+template <int K, int NQ, int R1, int R2, int R3>
+struct Tile3 {
+  static constexpr int C1 = R1 / K + 1, C2 = R2 / K + 1, C3 = R3 / K + 1;
+  static constexpr int Q1 = C1 * NQ, Q2 = C2 * NQ, Q3 = C3 * NQ;
+  static constexpr int NP1 = pair_count<K>(R1), NP2 = pair_count<K>(R2);
+  static constexpr int XS = (Q2 * Q3) | 1;  // odd stride in doubles: conflict-free lanes
+  static constexpr int NT = NP1 * NP2;
+  static constexpr int NW = NT / 64;
+  static constexpr int W = 2 * K + 1, S = W * W * W;
+  static constexpr int SLAB = R1 * R2 * S;
+  static constexpr int RPW = (R1 * R2 + NW - 1) / NW;  // slab rows per wave
+  static constexpr int ROWS = R1 * R2 * R3;
+  static constexpr int LDS_KAPPA = Q1 * Q2 * Q3;
+  static constexpr int LDS_X = 2 * NP1 * XS;
+  static constexpr int SP = pattern_stride(S);
+  static constexpr int PRES = 64 * SP / 8;  // room for at least 64 patterns
+  static constexpr int MAIN =
+      (LDS_KAPPA + LDS_X) > 2 * SLAB + PRES ? (LDS_KAPPA + LDS_X) : 2 * SLAB + PRES;
+  static constexpr int LDS_DOUBLES = MAIN + 2 * ROWS;  // + row table (16 bytes per row)
+  static constexpr int PFREE = (MAIN - 2 * SLAB) * 8 / SP;
+  static constexpr int PMAX = PFREE < 128 ? PFREE : 128;
+  static constexpr int PWORDS = (PMAX * SP / 4 + NT - 1) / NT;
+  static_assert(NT % 64 == 0, "whole waves");
+};
+
+// stage 3 for one z row u3 of the tile (compile-time, so Y stays in registers)
+template <int K, int NQ, int W, int Q3, int U3>
+__device__ __forceinline__ void z_row(double *__restrict__ slab, int rowslot, const double (&Y1)[Q3],
+                                      const double (&Y2)[Q3], const KParams &kp) {
+#pragma unroll
+  for (int s3 = -K; s3 <= K; ++s3) {
+    if (K == 2 && (U3 & 1) && (s3 == -2 || s3 == 2)) continue;
+    int lc0, nc, a0, b0;
+    pair_cells<K>(U3, s3, lc0, nc, a0, b0);
+    double v = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < 2 * NQ; ++tt) {
+      if (tt >= nc * NQ) break;
+      double G, M;
+      pair_factors<K, NQ>(kp, 2, a0, b0, tt, G, M);
+      v = fma(Y1[lc0 * NQ + tt], M, fma(Y2[lc0 * NQ + tt], G, v));
+    }
+    slab[rowslot + (s3 + K) * W * W] = v;
+  }
+}
+
+template <int K, int NQ, int W, int Q3, int R3, int U = 0>
+__device__ __forceinline__ void z_row_dispatch(int u3, double *__restrict__ slab, int rowslot, const double (&Y1)[Q3],
+                                               const double (&Y2)[Q3], const KParams &kp) {
+  if constexpr (U < R3) {
+    if (u3 == U) z_row<K, NQ, W, Q3, U>(slab, rowslot, Y1, Y2, kp);
+    else z_row_dispatch<K, NQ, W, Q3, R3, U + 1>(u3, slab, rowslot, Y1, Y2, kp);
+  }
+}
+
+// PATG: position patterns read from global memory (tiles with more distinct
+// patterns than fit in LDS); otherwise they are staged in LDS.
+template <int K, int NQ, int R1, int R2, int R3, bool PATG>
+__global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 2) void k_assemble3d(
+    const Tile *__restrict__ tiles, int tile0, int ntiles, const RowInfo *__restrict__ rowinfo,
+    const double *__restrict__ Wq, const uint8_t *__restrict__ pat_fwd, const uint8_t *__restrict__ pat_inv,
+    const int *__restrict__ tile_pats, double *__restrict__ values, double *__restrict__ b, KParams kp) {
+  using T = Tile3<K, NQ, R1, R2, R3>;
+  constexpr int NINV = (T::S + 63) / 64;
+  __shared__ double lds[T::LDS_DOUBLES];
+  double *kap = lds;                  // [Q3][Q2][Q1]
+  double *XG = lds + T::LDS_KAPPA;    // [NP1][XS]
+  double *XM = XG + T::NP1 * T::XS;   // [NP1][XS]
+  RowT *rowt = reinterpret_cast<RowT *>(lds + T::MAIN);
+  uint32_t *lpat = reinterpret_cast<uint32_t *>(lds + 2 * T::SLAB);
+  const int tb = tile_of_block(ntiles);
+  if (tb >= ntiles) return;
+  const Tile t = tiles[tile0 + tb];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+  // fetch the tile's position patterns early (stored to LDS after stage 2)
+  uint32_t pw[PATG ? 1 : T::PWORDS];
+  if (!PATG) {
+#pragma unroll
+    for (int w = 0; w < T::PWORDS; ++w) {
+      const int i = tid + w * T::NT;
+      pw[w] = 0;
+      if (i < t.npat * (T::SP / 4)) {
+        const int pid = tile_pats[t.pat_off + i / (T::SP / 4)];
+        pw[w] = reinterpret_cast<const uint32_t *>(pat_inv + (size_t)pid * T::SP)[i % (T::SP / 4)];
+      }
+    }
+  }
+  // row table of the tile
+  for (int r = tid; r < T::ROWS; r += T::NT) {
+    const int u1 = r % R1, u2 = (r / R1) % R2, u3 = r / (R1 * R2);
+    RowT e{0, 0, 0};
+    if (u1 < t.ext[0] && u2 < t.ext[1] && u3 < t.ext[2]) {
+      const RowInfo ri = rowinfo[t.rowinfo_base + row_index(t, t.org[0] + u1, t.org[1] + u2, t.org[2] + u3)];
+      if (ri.pid >= 0) e = RowT{ri.rowstart, (int)ri.bidx, ri.lenf};
+    }
+    rowt[r] = e;
+  }
+
+  // region of quadrature points: cells org/K - 1 .. org/K + R/K, zero outside the mesh
+  const int qs0 = (t.org[0] / K - 1) * NQ, qs1 = (t.org[1] / K - 1) * NQ, qs2 = (t.org[2] / K - 1) * NQ;
+  for (int i = tid; i < T::LDS_KAPPA; i += T::NT) {
+    const int a = i % T::Q1, bq = (i / T::Q1) % T::Q2, c = i / (T::Q1 * T::Q2);
+    const int q0 = qs0 + a, q1 = qs1 + bq, q2 = qs2 + c;
+    double v = 0.0;
+    if (q0 >= 0 && q0 < kp.qn[0] && q1 >= 0 && q1 < kp.qn[1] && q2 >= 0 && q2 < kp.qn[2])
+      v = Wq[q0 + (size_t)kp.qn[0] * (q1 + (size_t)kp.qn[1] * q2)];
+    kap[i] = v;
+  }
+  __syncthreads();
+
+  // stage 1: contract x quadrature points for every x pair
+  for (int p1 = wave; p1 < T::NP1; p1 += T::NW) {
+    int u, s, lc0, nc, a0, b0;
+    pair_natural<K>(p1, u, s);
+    pair_cells<K>(u, s, lc0, nc, a0, b0);
+    double G[2 * NQ], M[2 * NQ];
+#pragma unroll
+    for (int tt = 0; tt < 2 * NQ; ++tt) pair_factors<K, NQ>(kp, 0, a0, b0, tt, G[tt], M[tt]);
+    for (int it = lane; it < T::Q2 * T::Q3; it += 64) {
+      const double *kr = kap + it * T::Q1 + lc0 * NQ;  // it = q3*Q2 + q2
+      double xg = 0.0, xm = 0.0;
+      if (nc == 2) {
+#pragma unroll
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          xg = fma(kr[tt], G[tt], xg);
+          xm = fma(kr[tt], M[tt], xm);
+        }
+      } else {
+#pragma unroll
+        for (int tt = 0; tt < NQ; ++tt) {
+          xg = fma(kr[tt], G[tt], xg);
+          xm = fma(kr[tt], M[tt], xm);
+        }
+      }
+      XG[p1 * T::XS + it] = xg;
+      XM[p1 * T::XS + it] = xm;
+    }
+  }
+  __syncthreads();
+
+  // stage 2: thread (p1, p2) contracts y quadrature points for all z points
+  const int p1 = tid % T::NP1, p2 = tid / T::NP1;
+  int u1, s1, u2, s2;
+  pair_natural<K>(p1, u1, s1);
+  pair_doubles_first<K, R2>(p2, u2, s2);
+  double Y1[T::Q3], Y2[T::Q3];
+  {
+    int lc0, nc, a0, b0;
+    pair_cells<K>(u2, s2, lc0, nc, a0, b0);
+    double G[2 * NQ], M[2 * NQ];
+#pragma unroll
+    for (int tt = 0; tt < 2 * NQ; ++tt) pair_factors<K, NQ>(kp, 1, a0, b0, tt, G[tt], M[tt]);
+    const double *xg = XG + p1 * T::XS + lc0 * NQ;
+    const double *xm = XM + p1 * T::XS + lc0 * NQ;
+    if (nc == 2) {
+#pragma unroll
+      for (int q3 = 0; q3 < T::Q3; ++q3) {
+        double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < 2 * NQ; ++tt) {
+          const double g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
+          y1 = fma(g, M[tt], fma(m, G[tt], y1));
+          y2 = fma(m, M[tt], y2);
+        }
+        Y1[q3] = y1;
+        Y2[q3] = y2;
+      }
+    } else {
+#pragma unroll
+      for (int q3 = 0; q3 < T::Q3; ++q3) {
+        double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < NQ; ++tt) {
+          const double g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
+          y1 = fma(g, M[tt], fma(m, G[tt], y1));
+          y2 = fma(m, M[tt], y2);
+        }
+        Y1[q3] = y1;
+        Y2[q3] = y2;
+      }
+    }
+  }
+  __syncthreads();  // kappa and X are dead: their LDS holds two output slabs and the patterns
+
+  if (!PATG) {
+#pragma unroll
+    for (int w = 0; w < T::PWORDS; ++w) {
+      const int i = tid + w * T::NT;
+      if (i < t.npat * (T::SP / 4)) lpat[i] = pw[w];
+    }
+  }
+
+  // stage 3, one z row of the tile at a time; the two slabs alternate so one
+  // barrier per z row suffices
+  const int rowslot = (u2 * R1 + u1) * T::S + (s2 + K) * T::W + (s1 + K);
+#pragma unroll 1
+  for (int u3 = 0; u3 < R3; ++u3) {
+    double *slab = lds + (u3 & 1) * T::SLAB;
+    z_row_dispatch<K, NQ, T::W, T::Q3, R3>(u3, slab, rowslot, Y1, Y2, kp);
+    __syncthreads();
+    for (int j = 0; j < T::RPW; ++j) {
+      const int rr = wave + T::NW * j;
+      if (rr >= R1 * R2) break;
+      const RowT e = rowt[u3 * R1 * R2 + rr];
+      if (e.lenf == 0) continue;
+      const int len = e.lenf & 0xFFFF, lpid = e.lenf >> 17;
+      const double *row = slab + rr * T::S;
+      double *dst = values + e.rowstart;
+      const int gpid = (PATG || (e.lenf & 0x10000)) ? tile_pats[t.pat_off + lpid] : 0;
+#pragma unroll
+      for (int kk = 0; kk < NINV; ++kk) {
+        const int k = lane + 64 * kk;
+        if (k < len) {
+          const int sl = PATG ? (int)pat_inv[(size_t)gpid * T::SP + k]
+                              : (int)reinterpret_cast<const uint8_t *>(lpat)[lpid * T::SP + k];
+          dst[k] = row[sl];
+        }
+      }
+      if (e.lenf & 0x10000) {  // Dirichlet neighbours: b_i -= sum_j A_ij uD_j
+        const uint8_t *fwd = pat_fwd + (size_t)gpid * T::SP;
+        const int g0 = t.org[0] + rr % R1, g1 = t.org[1] + rr / R1, g2 = t.org[2] + u3;
+        double acc = 0.0;
+        for (int sl = lane; sl < T::S; sl += 64) {
+          if (fwd[sl] == kSlotDirichlet) {
+            const int d1 = sl % T::W - K, d2 = (sl / T::W) % T::W - K, d3 = sl / (T::W * T::W) - K;
+            acc += row[sl] * uD_at(kp, g0 + d1, g1 + d2, g2 + d3);
+          }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) b[e.bidx] -= acc;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launcher.  Tile shapes must match tile_shape() in gdx_symbolic.cpp.
+// ---------------------------------------------------------------------------
+template <int K, int NQ, int R1, int R2, int R3>
+static void run3(gdx_problem *p, hipStream_t st) {
+  using T = Tile3<K, NQ, R1, R2, R3>;
+  const int nt = (int)p->tiles.size(), nf = p->n_fast_tiles;
+  if (nf > 0)
+    hipLaunchKernelGGL((k_assemble3d<K, NQ, R1, R2, R3, false>), dim3(8u * ((nf + 7) / 8)), dim3(T::NT), 0, st,
+                       p->dev.tiles, 0, nf, p->dev.rowinfo, p->dev.W, p->dev.pat_fwd, p->dev.pat_inv,
+                       p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
+  if (nt > nf)
+    hipLaunchKernelGGL((k_assemble3d<K, NQ, R1, R2, R3, true>), dim3(8u * ((nt - nf + 7) / 8)), dim3(T::NT), 0, st,
+                       p->dev.tiles, nf, nt - nf, p->dev.rowinfo, p->dev.W, p->dev.pat_fwd, p->dev.pat_inv,
+                       p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
+}
+
+int pattern_capacity_tile3d(int K, int NQ) {
+  if (K == 2 && NQ == 3) return Tile3<2, 3, 4, 8, 8>::PMAX;
+  if (K == 2 && NQ == 2) return Tile3<2, 2, 4, 8, 8>::PMAX;
+  if (K == 1 && NQ == 2) return Tile3<1, 2, 8, 8, 2>::PMAX;
+  return 0;
+}
+
+void tile3d_shape(int K, int R[3]) {
+  R[0] = K == 2 ? 4 : 8;
+  R[1] = 8;
+  R[2] = K == 2 ? 8 : 2;
+}
+
+int launch_matrix_tile3d(gdx_problem *p, hipStream_t st) {
+  const int K = p->K, NQ = p->NQ;
+  if (K == 2 && NQ == 3) run3<2, 3, 4, 8, 8>(p, st);
+  else if (K == 2 && NQ == 2) run3<2, 2, 4, 8, 8>(p, st);
+  else if (K == 1 && NQ == 2) run3<1, 2, 8, 8, 2>(p, st);
+  else return GDX_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+}  // namespace gdx

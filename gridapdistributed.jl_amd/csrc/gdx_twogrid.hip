@@ -534,14 +534,16 @@ void rms(TwoGrid *T, const double *a, const float *fa, int64_t count, double *ds
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(kB), 0, st, T->red, kRed, (double)count, mode, dst);
 }
 
-// network input from a fine residual in global-id order (s into red[kS])
-void make_input(gdx_nn *nn, TwoGrid *T, const double *r, hipStream_t st) {
+// network input from a fine residual in global-id order (s into red[kS]);
+// with the network off only the coarse residual rc is formed
+void make_input(gdx_nn *nn, TwoGrid *T, const double *r, hipStream_t st, bool for_net = true) {
   nn::Net &N = nn->net;
   const int64_t v = (int64_t)T->n * T->n * T->n;
   // restriction reads only interior lattice nodes, all of which the scatter
   // writes: no clearing (and no memset, so this can be captured in a graph)
   hipLaunchKernelGGL(k_scatter, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, T->gid_node, r, T->flat);
   hipLaunchKernelGGL(k_restrict, dim3(nb(v)), dim3(kB), 0, st, T->n, T->nf, T->flat, T->dlat, T->rho, T->rc);
+  if (!for_net) return;
   rms(T, T->rho, nullptr, v, T->red + kS, 0, st);
   hipLaunchKernelGGL(k_input0, dim3(nb(v)), dim3(kB), 0, st, v, T->rho, T->red + kS, N.inp_f);
   N.input_from_f32(N.inp_f);
@@ -666,7 +668,7 @@ int twogrid_apply(void *ctx, const double *r, double *z, hipStream_t st) {
   hipLaunchKernelGGL(k_jacobi0, dim3(nb(ng)), dim3(kB), 0, st, ng, kOmega, T->dinv, r, z);
   solve_spmv_dev(p, z, T->t1, st);
   hipLaunchKernelGGL(k_rminus, dim3(nb(ng)), dim3(kB), 0, st, ng, r, T->t1);
-  make_input(nn, T, T->t1, st);
+  make_input(nn, T, T->t1, st, T->use_net);
   if (T->use_net) {
     int rc = N.forward_dev();
     if (rc) return rc;

@@ -30,7 +30,7 @@ EXPORTED = [
     "gdx_nn_backward", "gdx_nn_attach", "gdx_nn_train_step", "gdx_nn_apply", "gdx_fpcg",
     "gdx_nn_set_options", "gdx_nn_coarse_operator",
     "gdx_batch_create", "gdx_batch_add", "gdx_batch_autotune", "gdx_batch_capture", "gdx_batch_launch",
-    "gdx_batch_destroy",
+    "gdx_batch_destroy", "gdx_problem_set_kernel", "gdx_problem_get_kernel",
 ]
 
 
@@ -121,6 +121,8 @@ def lib():
         "gdx_batch_capture": [vp],
         "gdx_batch_launch": [vp],
         "gdx_batch_destroy": [vp],
+        "gdx_problem_set_kernel": [vp, i32, i32],
+        "gdx_problem_get_kernel": [vp, P(i32), P(i32)],
         "gdx_fpcg": [vp, vp, vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double), P(i32)],
         "gdx_newton": [vp, P(NewtonOpts), vp, P(NewtonReport)],
     }

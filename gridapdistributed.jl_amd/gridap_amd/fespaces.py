@@ -232,6 +232,15 @@ class GpuProblem:
             raise ValueError(f"state must have {self.ngdofs} free values")
         _lib.call("gdx_set_state", self.handle, x.ctypes.data)
 
+    def set_kernel(self, variant, rows_z=32):
+        """3D matrix kernel: 0 z-marching columns (rows_z z rows per workgroup), 1 per tile."""
+        _lib.call("gdx_problem_set_kernel", self.handle, int(variant), int(rows_z))
+
+    def get_kernel(self):
+        v, r = ctypes.c_int32(), ctypes.c_int32()
+        _lib.call("gdx_problem_get_kernel", self.handle, ctypes.byref(v), ctypes.byref(r))
+        return v.value, r.value
+
     def set_cell_coefficient(self, kappa):
         """kappa per cell (x fastest) for the next numeric assembly; None returns to the form's kappa."""
         if kappa is None:

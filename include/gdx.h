@@ -135,6 +135,12 @@ int gdx_symbolic(gdx_problem *prob);
    (src/Algebra.jl:530-535,755-758).  Ghost rows end with zero values. */
 int gdx_numeric(gdx_problem *prob, int32_t precision);
 
+/* 3D matrix kernel of a problem: variant 0 = z-marching columns with rows_z z
+   rows per workgroup (default 32), 1 = per-tile kernel (rows_z ignored).  Both
+   give the same matrix to rounding; gdx_batch_autotune picks the fastest. */
+int gdx_problem_set_kernel(gdx_problem *prob, int32_t variant, int32_t rows_z);
+int gdx_problem_get_kernel(gdx_problem *prob, int32_t *variant, int32_t *rows_z);
+
 /* queries / exports (copies device data into caller arrays) */
 int gdx_num_parts(gdx_problem *prob, int32_t *nparts);
 int gdx_num_global_dofs(gdx_problem *prob, int64_t *ngdofs);
@@ -250,10 +256,11 @@ int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int3
 
 /* Batched numeric assembly: independent problems of uneven sizes and
    partitions (each after gdx_symbolic, all on one context) assembled by one
-   hipGraph launch.  gdx_batch_autotune times each 3D problem's matrix kernel
-   with 16, 32 and 64 z rows (cells) per workgroup and keeps the fastest tiling
-   (chosen: per problem, may be NULL); results are bit-identical for every
-   tiling.  gdx_batch_capture records the numeric launches of every problem
+   hipGraph launch.  gdx_batch_autotune times each 3D problem's matrix kernels
+   (z-marching columns of 16, 32 and 64 z rows per workgroup, and the per-tile
+   kernel) and keeps the fastest (chosen, per problem, may be NULL: the z rows
+   per workgroup, negated for the per-tile kernel); results are bit-identical
+   for every choice.  gdx_batch_capture records the numeric launches of every problem
    (no memsets) and gdx_batch_launch replays them on the context's stream.
    The reference has no counterpart: it assembles one problem per call
    (src/FESpaces.jl:785-798). */

@@ -194,7 +194,7 @@ def test_galerkin_coarse_operator_matches_scipy(gpu_ctx):
     import scipy.sparse as sp
     from _helpers import owned_global, gid_to_gnode
 
-    cells, n = 12, 6
+    cells, n = 16, 8
     net = LearnedCoarseSolver(gpu_ctx, n, 1)
     assem, prob = lognormal_problem(gpu_ctx, cells, 21)
     net.attach(prob)
