@@ -79,6 +79,7 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
   for (size_t i = 0; i < b->probs.size() && !rc; ++i) {
     gdx_problem *p = b->probs[i];
     int best = p->tileR[2], best_var = p->kvariant3d;
+    p->precision = GDX_PREC_FP64;  // the choice is made for (and applies to) the FP64 form
     if (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION) {
       float best_ms = 1e30f;
       // candidates: z-marching columns of 16, 32, 64 z rows; the per-tile kernel (rows_z = 0)

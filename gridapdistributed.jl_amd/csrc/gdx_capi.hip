@@ -318,7 +318,11 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
-  GDX_REQUIRE(precision == GDX_PREC_FP64, GDX_ERR_UNSUPPORTED, "unsupported precision");
+  GDX_REQUIRE(precision == GDX_PREC_FP64 || precision == GDX_PREC_FP32, GDX_ERR_ARG, "unknown precision");
+  GDX_REQUIRE(precision == GDX_PREC_FP64 || (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION && p->kvariant3d == 0),
+              GDX_ERR_UNSUPPORTED,
+              "the FP32 form runs with the 3D z-marching kernel (gdx_problem_set_kernel(prob, 0, rows))");
+  p->precision = precision;
   GDX_HIP(hipSetDevice(p->ctx->device));
   hipStream_t st = p->ctx->stream;
   GDX_HIP(hipEventRecord(p->ev[0], st));

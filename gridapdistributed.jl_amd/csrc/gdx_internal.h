@@ -142,6 +142,7 @@ struct gdx_problem {
   std::vector<int> tile_pats;
   int n_fast_tiles = 0;  // leading tiles whose position patterns fit in LDS
   int kvariant3d = 0;    // 3D matrix kernel: 0 z-marching columns, 1 per-tile (chosen by the autotuner)
+  int precision = 0;     // contraction precision of the current numeric phase (GDX_PREC_*)
   int tileR[3] = {1, 1, 1};
   int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
   gdx::KParams kp{};

@@ -325,7 +325,7 @@ static void rhs_run(gdx_problem *p, hipStream_t st) {
                      p->dev.rowinfo, p->dev.F, p->kp, p->dev.b);
 }
 
-int launch_matrix3d(gdx_problem *p, hipStream_t st);
+int launch_matrix3d(gdx_problem *p, hipStream_t st, int precision);
 
 template <int K, int NQ, int R1, int R2>
 static void run2(gdx_problem *p, hipStream_t st) {
@@ -362,7 +362,9 @@ int launch_matrix(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    return p->kvariant3d == 1 ? launch_matrix_tile3d(p, st) : launch_matrix3d(p, st);
+    // the FP32 form exists for the z-marching kernel only (its tiles and pattern capacity)
+    if (p->precision == GDX_PREC_FP32 && p->kvariant3d == 0) return launch_matrix3d(p, st, GDX_PREC_FP32);
+    return p->kvariant3d == 1 ? launch_matrix_tile3d(p, st) : launch_matrix3d(p, st, GDX_PREC_FP64);
   } else {
     if (K == 2 && NQ == 3) run2<2, 3, 16, 16>(p, st);
     else if (K == 2 && NQ == 2) run2<2, 2, 16, 16>(p, st);

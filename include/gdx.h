@@ -44,8 +44,9 @@ extern "C" {
 #define GDX_SUB_ASSEMBLED_ROWS 0
 #define GDX_FULLY_ASSEMBLED_ROWS 1
 
-/* numeric precision of the cell contractions; the CSR accumulation is FP64 */
+/* numeric precision of the cell contractions; the CSR values are FP64 */
 #define GDX_PREC_FP64 0
+#define GDX_PREC_FP32 1 /* mixed form: sum-factorised contractions in FP32 (3D, z-marching kernel) */
 
 /* operator kinds */
 #define GDX_OP_DIFFUSION 0 /* affine: a(u,v) and l(v) below */
