@@ -123,7 +123,10 @@ def bench_plaplacian(args):
     """Newton on the p = 3 p-Laplacian, 3D Q1 96^3 cells, parts (4,2,2), f = 1,
     u = 0 on the boundary, sigma = (eps^2 + |grad u|^2)^(1/2) grad u with
     eps = 1e-2; Poisson initial guess; stops at ||r|| <= 1e-10 ||r(0)||.  One
-    step = one full Newton solve (every Jacobian re-assembled in place)."""
+    step = one full Newton solve (every Jacobian re-assembled in place), its
+    corrections solved by FPCG with the two-grid preconditioner rebuilt from
+    each Jacobian (coarse network off: it does not gain, see the learned
+    workload); the same solve with Jacobi-PCG corrections is timed beside it."""
     import numpy as np
 
     import gridap_amd as G
@@ -137,25 +140,32 @@ def bench_plaplacian(args):
     prob = op._problem()
     t_symbolic = time.perf_counter() - t0
     nnz = prob.owned_nnz()
+    net = G.LearnedCoarseSolver(ctx, n // 2, 0)
+    net.attach(prob)
+    net.set_options(8, use_network=False)
     kw = dict(rtol=1e-10, maxiter=40, lin_rtol=1e-8, lin_maxiter=20000, init=G._lib.NEWTON_INIT_POISSON)
-    for _ in range(args.warmup):
-        prob.newton(None, **kw)
-    prob.sync()
-    reps = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        x, rep = prob.newton(None, **kw)
-        reps.append(rep)
-    prob.sync()
-    s_per_solve = (time.perf_counter() - t0) / args.steps
-    # one Jacobian + residual assembly, timed with the library's hipEvents
+
+    def timed(precond):
+        for _ in range(args.warmup):
+            prob.newton(None, precond=precond, **kw)
+        prob.sync()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            x, rep = prob.newton(None, precond=precond, **kw)
+            reps.append(rep)
+        prob.sync()
+        return x, reps[-1], (time.perf_counter() - t0) / args.steps
+
+    x_j, rep_j, s_jacobi = timed(None)
+    x, rep, s_per_solve = timed(net)
+    sol_rel_diff = float(np.linalg.norm(x - x_j) / np.linalg.norm(x_j))
     prob.set_state(x)
     acc = 0.0
     for _ in range(10):
         prob.numeric()
         acc += prob.phase_times()[3]
     asm_ms = acc / 10
-    rep = reps[-1]
     out = {
         "metric": "p-Laplacian Newton solve time to 1e-10 (s)",
         "value": round(s_per_solve, 4),
@@ -164,11 +174,17 @@ def bench_plaplacian(args):
         "ms_per_step": round(s_per_solve * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"p-Laplacian p=3, eps=1e-2, 3D Q1 {n}^3 on (0,1)^3, parts (4,2,2), f=1, u=0 on the "
-                               "boundary; Newton (Poisson initial guess) with Jacobi-PCG corrections to 1e-8",
+                               "boundary; Newton (Poisson initial guess) with two-grid FPCG corrections to 1e-8",
                    "free_dofs": prob.ngdofs, "nnz_owned": nnz, "symbolic_s": round(t_symbolic, 3),
-                   "newton_iters": rep["iters"], "pcg_iters": rep["lin_iters"], "converged": rep["converged"],
+                   "newton_iters": rep["iters"], "fpcg_iters": rep["lin_iters"], "converged": rep["converged"],
+                   "fpcg_fallbacks": rep["lin_fallbacks"],
                    "res_ref": rep["res_ref"], "res_final": rep["res_final"],
                    "ms_assembly_total": round(rep["ms_assembly"], 2), "ms_solve_total": round(rep["ms_solve"], 2),
+                   "jacobi_pcg_newton": {"s": round(s_jacobi, 4), "newton_iters": rep_j["iters"],
+                                         "pcg_iters": rep_j["lin_iters"], "converged": rep_j["converged"],
+                                         "res_final": rep_j["res_final"], "ms_solve_total": round(rep_j["ms_solve"], 2)},
+                   "speedup_vs_jacobi_pcg_newton": round(s_jacobi / s_per_solve, 3),
+                   "solution_rel_diff_vs_jacobi": sol_rel_diff,
                    "jacobian_assembly_ms": round(asm_ms, 4),
                    "jacobian_assembly_gnnz_s": round(nnz / (asm_ms * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "hbm", "achieved": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
@@ -177,6 +193,7 @@ def bench_plaplacian(args):
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_flux()
+    net.close()
     prob.close()
     ctx.close()
     print(json.dumps(out), flush=True)

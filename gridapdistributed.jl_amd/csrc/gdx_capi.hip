@@ -437,6 +437,15 @@ int gdx_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_r
               GDX_ERR_ARG, "invalid Newton options");
   GDX_REQUIRE(o->init == GDX_NEWTON_INIT_GIVEN || o->init == GDX_NEWTON_INIT_POISSON, GDX_ERR_ARG,
               "unknown Newton initial guess");
+  if (gdx_nn *nn = o->precond) {
+    GDX_REQUIRE(nn->ctx, GDX_ERR_STATE, "the preconditioner's context was finalized");
+    GDX_REQUIRE(nn->ctx == p->ctx, GDX_ERR_ARG, "network and problem belong to different contexts");
+    GDX_REQUIRE(p->D == 3 && p->K == 1 && p->desc.dir_kind == 1, GDX_ERR_UNSUPPORTED,
+                "the two-grid preconditioner is built for 3D Q1 with Dirichlet data on the whole boundary");
+    for (int d = 0; d < 3; ++d)
+      GDX_REQUIRE(p->kp.ncells[d] == 2 * nn->net.n, GDX_ERR_ARG,
+                  "the problem must have 2n x 2n x 2n cells for a network grid of n^3");
+  }
   GDX_HIP(hipSetDevice(p->ctx->device));
   int rc = gdx::solve_newton(p, o, x, rep);
   if (rc) set_error("Newton iteration failed");

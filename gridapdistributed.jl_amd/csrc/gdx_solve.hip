@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gdx_internal.h"
+#include "gdx_nn.h"
 
 namespace gdx {
 
@@ -287,14 +288,16 @@ int solve_spmv(gdx_problem *p, const double *xh, double *yh) {
 // Jacobi-PCG on the assembled system, the solution left in s->x; stops when
 // the recursive residual meets rtol ||b||, then computes the true FP64
 // residual ||b - A x|| / ||b|| explicitly.  rhs: device vector in global-id
-// order (nullptr = the assembled RHS); x0_given: start from s->x instead of 0.
+// order (nullptr = the assembled RHS; may be s->b itself); x0_given: start from
+// s->x instead of 0.
 static int pcg_run(gdx_problem *p, SolveData *s, double rtol, int maxiter, int *iters, double *relres,
                    const double *rhs = nullptr, bool x0_given = false) {
   hipStream_t st = p->ctx->stream;
   const int n = s->n;
   const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
   if (rhs) {
-    if (hipMemcpyAsync(s->b, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+    if (rhs != s->b && hipMemcpyAsync(s->b, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return GDX_ERR_HIP;
   } else {
     hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
   }
@@ -419,9 +422,96 @@ void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st) 
   spmv(p, solve_data(p), x, y, st);
 }
 
-// FCG with preconditioner prec (Jacobi-PCG when prec is null); falls back to
+// FCG with preconditioner prec on the device RHS rhs (nullptr = the assembled
+// RHS), the solution left in s->x; Jacobi-PCG when prec is null.  Falls back to
 // Jacobi-PCG from the current iterate when the flexible iteration stops
-// reducing the residual (or produces a non-finite value).
+// reducing the residual, or from zero when it produces a non-finite value.
+static int fpcg_run(gdx_problem *p, SolveData *s, const FpcgPrec *prec, const double *rhs, double rtol, int maxiter,
+                    int *iters, double *relres, int *fellback) {
+  hipStream_t st = p->ctx->stream;
+  const int n = s->n;
+  const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
+  if (fellback) *fellback = 0;
+  if (!prec) return pcg_run(p, s, rtol, maxiter, iters, relres, rhs);
+  int rc, it = 0;
+  if (rhs) {
+    if (hipMemcpyAsync(s->b, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  } else {
+    hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
+  }
+  hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
+  (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
+  if (hipMemcpyAsync(s->r, s->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+  dot2(s, s->b, s->b, nullptr, nullptr, s->sc + 6, st);
+  double bb;
+  if (hipMemcpyAsync(&bb, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GDX_ERR_HIP;
+  const double bnorm = std::sqrt(bb);
+  bool fall = false, nonfinite = false;
+  double last_check = bnorm;
+  if (bnorm > 0.0) {
+    if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return rc;
+    if (hipMemcpyAsync(s->p, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return GDX_ERR_HIP;
+    IterGraph G;
+    for (it = 1; it <= maxiter; ++it) {
+      rc = run_iteration(st, G, [&]() {
+        spmv(p, s, s->p, s->q, st);
+        dot2(s, s->p, s->q, s->p, s->r, s->sc, st);  // sc[0] = pq, sc[1] = pr
+        hipLaunchKernelGGL(k_fcg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->p, s->q);
+        int r2 = prec->apply(prec->ctx, s->r, s->z, st);
+        if (r2) return r2;
+        dot2(s, s->z, s->q, s->r, s->r, s->sc + 2, st);  // sc[2] = zq, sc[3] = rr
+        hipLaunchKernelGGL(k_fcg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
+        return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+      });
+      if (rc) return rc;
+      if (it % 5 == 0 || it == maxiter) {
+        double h[2];
+        if (hipMemcpyAsync(h, s->sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return GDX_ERR_HIP;
+        const double rn = std::sqrt(h[1]);
+        if (!std::isfinite(rn) || !std::isfinite(h[0])) {
+          fall = true;
+          nonfinite = true;
+          break;
+        }
+        if (rn <= rtol * bnorm) break;
+        if (it % 50 == 0) {  // stagnation guard
+          if (rn > 0.999 * last_check) {
+            fall = true;
+            break;
+          }
+          last_check = rn;
+        }
+      }
+    }
+    if (it > maxiter) it = maxiter;
+  }
+  if (fall) {
+    // Jacobi-PCG from the current iterate (or from zero if it is not finite);
+    // s->b still holds the right-hand side
+    if (fellback) *fellback = 1;
+    int it2 = 0;
+    if ((rc = pcg_run(p, s, rtol, std::max(1, maxiter - it), &it2, relres, s->b, !nonfinite))) return rc;
+    it += it2;
+  } else {
+    // true residual
+    spmv(p, s, s->x, s->q, st);
+    hipLaunchKernelGGL(k_residual, g, b, 0, st, n, s->b, s->q, s->r);
+    dot2(s, s->r, s->r, nullptr, nullptr, s->sc + 6, st);
+    double h;
+    if (hipMemcpyAsync(&h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return GDX_ERR_HIP;
+    if (relres) *relres = bnorm > 0.0 ? std::sqrt(h) / bnorm : 0.0;
+  }
+  if (iters) *iters = it;
+  return GDX_OK;
+}
+
+// fpcg_run on a host RHS (nullptr = the assembled RHS), the solution copied to xh
 int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double rtol, int maxiter, double *xh,
                int *iters, double *relres, double *ms, int *fellback) {
   int rc = solve_setup(p);
@@ -429,10 +519,8 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
   SolveData *s = solve_data(p);
   hipStream_t st = p->ctx->stream;
   const int n = s->n;
-  const dim3 g((n + kBlk - 1) / kBlk), b(kBlk);
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
-  if (fellback) *fellback = 0;
   double *rhs_d = nullptr;
   if (rhs_h) {
     if (hipMalloc(reinterpret_cast<void **>(&rhs_d), std::max(n, 1) * sizeof(double)) != hipSuccess)
@@ -442,96 +530,17 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
       return GDX_ERR_HIP;
     }
   }
-  auto finish = [&](int code) {
-    if (rhs_d) (void)hipFree(rhs_d);
-    return code;
-  };
-  int it = 0;
-  if (!prec) {
-    if ((rc = pcg_run(p, s, rtol, maxiter, &it, relres, rhs_d))) return finish(rc);
-  } else {
-    if (rhs_d) {
-      if (hipMemcpyAsync(s->b, rhs_d, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
-        return finish(GDX_ERR_HIP);
-    } else {
-      hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
-    }
-    hipLaunchKernelGGL(k_inv_diag, g, b, 0, st, n, s->dpos, p->dev.values, s->dinv);
-    (void)hipMemsetAsync(s->x, 0, n * sizeof(double), st);
-    if (hipMemcpyAsync(s->r, s->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
-      return finish(GDX_ERR_HIP);
-    dot2(s, s->b, s->b, nullptr, nullptr, s->sc + 6, st);
-    double bb;
-    if (hipMemcpyAsync(&bb, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return finish(GDX_ERR_HIP);
-    const double bnorm = std::sqrt(bb);
-    bool fall = false, nonfinite = false;
-    double last_check = bnorm;
-    if (bnorm > 0.0) {
-      if ((rc = prec->apply(prec->ctx, s->r, s->z, st))) return finish(rc);
-      if (hipMemcpyAsync(s->p, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
-        return finish(GDX_ERR_HIP);
-      IterGraph G;
-      for (it = 1; it <= maxiter; ++it) {
-        rc = run_iteration(st, G, [&]() {
-          spmv(p, s, s->p, s->q, st);
-          dot2(s, s->p, s->q, s->p, s->r, s->sc, st);  // sc[0] = pq, sc[1] = pr
-          hipLaunchKernelGGL(k_fcg_update, g, b, 0, st, n, s->sc, s->x, s->r, s->p, s->q);
-          int r2 = prec->apply(prec->ctx, s->r, s->z, st);
-          if (r2) return r2;
-          dot2(s, s->z, s->q, s->r, s->r, s->sc + 2, st);  // sc[2] = zq, sc[3] = rr
-          hipLaunchKernelGGL(k_fcg_dir, g, b, 0, st, n, s->sc, s->z, s->p);
-          return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
-        });
-        if (rc) return finish(rc);
-        if (it % 5 == 0 || it == maxiter) {
-          double h[2];
-          if (hipMemcpyAsync(h, s->sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-              hipStreamSynchronize(st) != hipSuccess)
-            return finish(GDX_ERR_HIP);
-          const double rn = std::sqrt(h[1]);
-          if (!std::isfinite(rn) || !std::isfinite(h[0])) {
-            fall = true;
-            nonfinite = true;
-            break;
-          }
-          if (rn <= rtol * bnorm) break;
-          if (it % 50 == 0) {  // stagnation guard
-            if (rn > 0.999 * last_check) {
-              fall = true;
-              break;
-            }
-            last_check = rn;
-          }
-        }
-      }
-      if (it > maxiter) it = maxiter;
-    }
-    if (fall) {
-      // Jacobi-PCG from the current iterate (or from zero if it is not finite)
-      if (fellback) *fellback = 1;
-      int it2 = 0;
-      if ((rc = pcg_run(p, s, rtol, std::max(1, maxiter - it), &it2, relres, rhs_d, !nonfinite))) return finish(rc);
-      it += it2;
-    } else {
-      // true residual
-      spmv(p, s, s->x, s->q, st);
-      hipLaunchKernelGGL(k_residual, g, b, 0, st, n, s->b, s->q, s->r);
-      dot2(s, s->r, s->r, nullptr, nullptr, s->sc + 6, st);
-      double h;
-      if (hipMemcpyAsync(&h, s->sc + 6, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return finish(GDX_ERR_HIP);
-      if (relres) *relres = bnorm > 0.0 ? std::sqrt(h) / bnorm : 0.0;
-    }
+  rc = fpcg_run(p, s, prec, rhs_d, rtol, maxiter, iters, relres, fellback);
+  if (!rc && xh &&
+      (hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+       hipStreamSynchronize(st) != hipSuccess))
+    rc = GDX_ERR_HIP;
+  if (rhs_d) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(rhs_d);
   }
-  if (xh && (hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-             hipStreamSynchronize(st) != hipSuccess))
-    return finish(GDX_ERR_HIP);
-  if (iters) *iters = it;
   if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
-  return finish(GDX_OK);
+  return rc;
 }
 
 // x -= d
@@ -541,7 +550,7 @@ __global__ void k_sub(int n, double *__restrict__ x, const double *__restrict__ 
 }
 
 // Newton on the flux operator: r = residual at x (into the RHS), J = Jacobian
-// (into the CSR values), J dx = r by Jacobi-PCG, x -= dx.  All vectors stay on
+// (into the CSR values), J dx = r by Jacobi-PCG or two-grid FPCG, x -= dx.  All vectors stay on
 // the device; the host reads one norm per iteration.
 int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newton_report *rep) {
   int rc = solve_setup(p);
@@ -576,10 +585,21 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
     ms_asm += ms_since(ta);
     return GDX_OK;
   };
+  // FPCG with the two-grid preconditioner, rebuilt from the current Jacobian
+  // (fine diagonal and Galerkin coarse operator), or Jacobi-PCG
+  const FpcgPrec prec{twogrid_apply, o->precond};
+  int fallbacks = 0;
   auto correct = [&](int *li) -> int {
     const auto ts = clk::now();
     double rr;
-    int r = pcg_run(p, s, o->lin_rtol, o->lin_maxiter, li, &rr);
+    int r, fb = 0;
+    if (o->precond) {
+      if ((r = twogrid_attach(o->precond, p))) return r;
+      r = fpcg_run(p, s, &prec, nullptr, o->lin_rtol, o->lin_maxiter, li, &rr, &fb);
+      fallbacks += fb;
+    } else {
+      r = pcg_run(p, s, o->lin_rtol, o->lin_maxiter, li, &rr);
+    }
     if (r) return r;
     hipLaunchKernelGGL(k_sub, g, b, 0, st, n, xs, s->x);
     if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
@@ -623,6 +643,7 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
     rep->ms_total = ms_since(t0);
     rep->ms_assembly = ms_asm;
     rep->ms_solve = ms_sol;
+    rep->lin_fallbacks = fallbacks;
   }
   return GDX_OK;
 }

@@ -760,8 +760,8 @@ int gdx_nn_attach(gdx_nn *nn, gdx_problem *prob) {
     gdx::set_error("the problem needs gdx_symbolic and a device context");
     return GDX_ERR_STATE;
   }
-  if (prob->ctx->device != nn->ctx->device) {
-    gdx::set_error("network and problem live on different devices");
+  if (prob->ctx != nn->ctx) {  // the network's kernels run on its context's stream
+    gdx::set_error("network and problem belong to different contexts");
     return GDX_ERR_ARG;
   }
   if (hipSetDevice(prob->ctx->device) != hipSuccess) return GDX_ERR_HIP;

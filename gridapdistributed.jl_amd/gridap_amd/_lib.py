@@ -53,13 +53,14 @@ class ProblemDesc(ctypes.Structure):
 
 class NewtonOpts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiter", ctypes.c_int32),
-                ("lin_rtol", ctypes.c_double), ("lin_maxiter", ctypes.c_int32), ("init", ctypes.c_int32)]
+                ("lin_rtol", ctypes.c_double), ("lin_maxiter", ctypes.c_int32), ("init", ctypes.c_int32),
+                ("precond", ctypes.c_void_p)]
 
 
 class NewtonReport(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32), ("lin_iters", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("res_ref", ctypes.c_double), ("res_final", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("ms_assembly", ctypes.c_double), ("ms_solve", ctypes.c_double)]
+                ("ms_assembly", ctypes.c_double), ("ms_solve", ctypes.c_double), ("lin_fallbacks", ctypes.c_int32)]
 
 
 _LIB = None

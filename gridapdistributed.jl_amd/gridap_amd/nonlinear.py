@@ -144,11 +144,12 @@ def jacobian(op, uh):
 
 
 class NLSolver:
-    """NLSolver(method=:newton) with device Jacobi-PCG corrections.
+    """NLSolver(method=:newton) with device corrections: Jacobi-PCG, or flexible
+    PCG with a LearnedCoarseSolver's two-grid preconditioner (preconditioner=).
     ftol: relative residual tolerance ||r|| <= ftol ||r(u0)||."""
 
     def __init__(self, method="newton", ftol=1e-10, atol=0.0, iterations=50, linear_rtol=1e-8,
-                 linear_iterations=20000, initial_guess="given", show_trace=False):
+                 linear_iterations=20000, initial_guess="given", show_trace=False, preconditioner=None):
         if method != "newton":
             raise NotImplementedError("only method='newton' runs on the GPU path")
         if initial_guess not in ("given", "poisson"):
@@ -156,6 +157,7 @@ class NLSolver:
         self.ftol, self.atol, self.iterations = ftol, atol, iterations
         self.linear_rtol, self.linear_iterations = linear_rtol, linear_iterations
         self.initial_guess, self.show_trace = initial_guess, show_trace
+        self.preconditioner = preconditioner
         self.report = None
 
 
@@ -172,9 +174,9 @@ def solve(solver, op, uh0=None):
     x0 = None if uh0 is None else uh0.free_values
     init = _lib.NEWTON_INIT_POISSON if nls.initial_guess == "poisson" else _lib.NEWTON_INIT_GIVEN
     x, rep = prob.newton(x0, rtol=nls.ftol, atol=nls.atol, maxiter=nls.iterations, lin_rtol=nls.linear_rtol,
-                         lin_maxiter=nls.linear_iterations, init=init)
+                         lin_maxiter=nls.linear_iterations, init=init, precond=nls.preconditioner)
     nls.report = rep
     if nls.show_trace:
         print(f"Newton: {rep['iters']} iterations, ||r|| {rep['res_ref']:.3e} -> {rep['res_final']:.3e}, "
-              f"{rep['lin_iters']} PCG iterations, {rep['ms_total']:.1f} ms")
+              f"{rep['lin_iters']} PCG iterations ({rep['lin_fallbacks']} fallbacks), {rep['ms_total']:.1f} ms")
     return FEFunction(op.trial, x)

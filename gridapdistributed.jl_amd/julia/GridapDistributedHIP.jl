@@ -35,10 +35,11 @@ end
 # gdx_newton_opts / gdx_newton_report
 struct GdxNewtonOpts
   rtol::Float64; atol::Float64; maxiter::Int32; lin_rtol::Float64; lin_maxiter::Int32; init::Int32
+  precond::Ptr{Cvoid}   # gdx_nn* or C_NULL (Jacobi-PCG corrections)
 end
 struct GdxNewtonReport
   iters::Int32; lin_iters::Int32; converged::Int32; res_ref::Float64; res_final::Float64
-  ms_total::Float64; ms_assembly::Float64; ms_solve::Float64
+  ms_total::Float64; ms_assembly::Float64; ms_solve::Float64; lin_fallbacks::Int32
 end
 
 check(rc, f) = rc == 0 || error("$f failed ($rc): ", unsafe_string(ccall((:gdx_last_error, libgdx), Cstring, ())))
@@ -129,15 +130,16 @@ function residual_and_jacobian_hip(p::GdxProblem, ranks, ngdofs, x::Vector{Float
 end
 
 """
-    newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false) -> (x, report)
+    newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false, precond=C_NULL) -> (x, report)
 
 solve(FESolver(NLSolver(method=:newton)), op) of test/PLaplacianTests.jl:54-57
-with device Jacobi-PCG corrections; x is the solution's free values by gid.
+with device corrections (Jacobi-PCG, or two-grid FPCG when precond is a gdx_nn
+handle on the same context); x is the solution's free values by gid.
 """
 function newton_hip(p::GdxProblem, x0::Vector{Float64}; rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8,
-                    lin_maxiter=20000, poisson_init=false)
+                    lin_maxiter=20000, poisson_init=false, precond::Ptr{Cvoid}=C_NULL)
   x = copy(x0)
-  o = GdxNewtonOpts(rtol, atol, maxiter, lin_rtol, lin_maxiter, poisson_init ? 1 : 0)
+  o = GdxNewtonOpts(rtol, atol, maxiter, lin_rtol, lin_maxiter, poisson_init ? 1 : 0, precond)
   rep = Ref{GdxNewtonReport}()
   GC.@preserve x check(ccall((:gdx_newton, libgdx), Cint,
                              (Ptr{Cvoid}, Ref{GdxNewtonOpts}, Ptr{Float64}, Ref{GdxNewtonReport}), p.h, o, x, rep),

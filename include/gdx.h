@@ -58,6 +58,7 @@ extern "C" {
 
 typedef struct gdx_ctx gdx_ctx;
 typedef struct gdx_problem gdx_problem;
+typedef struct gdx_nn gdx_nn; /* learned preconditioner, declared below */
 
 /*
  * Problem descriptor: the restricted weak-form vocabulary the GPU path lowers.
@@ -192,9 +193,13 @@ typedef struct gdx_newton_opts {
   double rtol;         /* stop when ||r(x)|| <= max(atol, rtol ||r(x_given)||) */
   double atol;
   int32_t maxiter;     /* Newton iterations */
-  double lin_rtol;     /* Jacobi-PCG tolerance of each Newton correction */
+  double lin_rtol;     /* relative tolerance of each Newton correction's linear solve */
   int32_t lin_maxiter;
   int32_t init;        /* GDX_NEWTON_INIT_* */
+  gdx_nn *precond;     /* NULL: Jacobi-PCG corrections.  Else flexible PCG with this
+                          network's two-grid preconditioner (gdx_nn_set_options
+                          applies), re-attached to every new Jacobian; the problem
+                          must fit the network (2n x 2n x 2n Q1 cells). */
 } gdx_newton_opts;
 typedef struct gdx_newton_report {
   int32_t iters;       /* Newton corrections applied */
@@ -203,11 +208,12 @@ typedef struct gdx_newton_report {
   double res_ref;      /* ||r(x_given)|| (2-norm over free dofs) */
   double res_final;    /* ||r(x)|| at the returned x */
   double ms_total, ms_assembly, ms_solve;
+  int32_t lin_fallbacks; /* corrections in which FPCG fell back to Jacobi-PCG */
 } gdx_newton_report;
 /* Newton's method on the flux operator: NLSolver(method=:newton) +
    FESolver + solve(solver, op) (test/PLaplacianTests.jl:54-57), with the
    Jacobian and residual re-assembled in place every iteration and each
-   correction solved by Jacobi-PCG on the device.  x: host array of length
+   correction solved on the device (Jacobi-PCG, or FPCG with opts->precond).  x: host array of length
    ngdofs, in = initial guess (NULL with GDX_NEWTON_INIT_GIVEN means zero),
    out = the solution's free values. */
 int gdx_newton(gdx_problem *prob, const gdx_newton_opts *opts, double *x, gdx_newton_report *report);
@@ -221,7 +227,6 @@ int gdx_newton(gdx_problem *prob, const gdx_newton_opts *opts, double *x, gdx_ne
    by gdx_numeric, and used inside flexible PCG (gdx_fpcg).  These entry points
    have no counterpart in the reference, whose solve is a direct LU
    (README.md:76); they extend its A\b. */
-typedef struct gdx_nn gdx_nn;
 int gdx_nn_create(gdx_ctx *ctx, int32_t n, uint64_t seed, gdx_nn **out);
 int gdx_nn_destroy(gdx_nn *nn);
 int gdx_nn_num_params(gdx_nn *nn, int64_t *nparams);

@@ -16,6 +16,7 @@ from gridap_amd import (FESolver, NLSolver, flux_operator_from_workload, residua
                         residual_and_jacobian_, solve, zero)
 from gridap_amd.nonlinear import FEFunction
 from gridap_amd import workloads as W
+from gridap_amd._lib import NEWTON_INIT_POISSON
 
 pytestmark = pytest.mark.gpu
 
@@ -151,3 +152,42 @@ def test_newton_plaplacian_residual_checked_by_oracle(gpu_ctx):
     assert abs(np.linalg.norm(r0) - rep["res_ref"]) <= 1e-12 * np.linalg.norm(r0)
     assert np.linalg.norm(r1) <= 1e-10 * np.linalg.norm(r0)
     assert np.all(uh.free_values > 0.0)  # f = 1 > 0 with u = 0 on the boundary: maximum principle
+
+
+@pytest.mark.parametrize("use_network", [False, True], ids=["two-grid", "two-grid+network"])
+def test_newton_with_two_grid_fpcg_corrections(gpu_ctx, use_network):
+    """Newton whose corrections are solved by FPCG with the two-grid
+    preconditioner (rebuilt from every Jacobian) reaches the same solution as
+    Newton with Jacobi-PCG corrections, with fewer linear iterations when the
+    coarse network is off; the returned state's residual is checked by the
+    oracle."""
+    from gridap_amd import LearnedCoarseSolver
+
+    cells = 32
+    q = W.plaplacian3d_q1(cells, (2, 2, 2), p=3.0, eps=1e-2)
+    op = flux_operator_from_workload(gpu_ctx, q)
+    prob = op._problem()
+    kw = dict(rtol=1e-10, maxiter=30, lin_rtol=1e-10, init=NEWTON_INIT_POISSON)
+    x_j, rep_j = prob.newton(None, **kw)
+    net = LearnedCoarseSolver(gpu_ctx, cells // 2, 3)
+    net.attach(prob)
+    net.set_options(8, use_network=use_network)
+    x_t, rep_t = prob.newton(None, precond=net, **kw)
+    print("jacobi", rep_j, "\ntwo-grid", rep_t, flush=True)
+    assert rep_j["converged"] and rep_t["converged"]
+    assert rep_t["lin_fallbacks"] == 0 or use_network
+    assert np.linalg.norm(x_t - x_j) <= 1e-8 * np.linalg.norm(x_j)
+    if not use_network:
+        assert rep_t["lin_iters"] < rep_j["lin_iters"]
+    g2n = np.full(prob.ngdofs, -1, np.int64)
+    for k in range(prob.nparts):
+        P = prob.export_part(k)
+        g2n[P["ldof_gid"]] = P["ldof_gnode"]
+    u = np.zeros(int(np.prod(W.lattice_shape(q))))
+    u[g2n] = x_t
+    q0, q1 = dict(q), dict(q)
+    q0["state"], q1["state"] = np.zeros_like(u), u
+    _, r0 = owned_global(oracle.run(q0).parts, prob.ngdofs)
+    _, r1 = owned_global(oracle.run(q1).parts, prob.ngdofs)
+    assert np.linalg.norm(r1) <= 1e-10 * np.linalg.norm(r0)
+    net.close()
