@@ -82,8 +82,24 @@ int check_desc(const gdx_problem_desc *d) {
   GDX_REQUIRE(d->nmodes >= 0 && d->nmodes <= 1024, GDX_ERR_ARG, "nmodes must be in [0,1024]");
   GDX_REQUIRE(d->nmodes == 0 || d->modes, GDX_ERR_ARG, "modes is NULL with nmodes > 0");
   GDX_REQUIRE(d->rhs_kind == GDX_RHS_CONST || d->rhs_kind == GDX_RHS_MANUFACTURED, GDX_ERR_ARG, "unknown rhs kind");
-  GDX_REQUIRE(d->dir_kind == 0 || d->dir_kind == 1, GDX_ERR_ARG, "unknown Dirichlet kind");
-  GDX_REQUIRE(d->op_kind == GDX_OP_DIFFUSION || d->op_kind == GDX_OP_FLUX, GDX_ERR_ARG, "unknown operator kind");
+  GDX_REQUIRE(d->dir_kind == 0 || d->dir_kind == 1 || d->dir_kind == GDX_DIR_CLAMPED_X0, GDX_ERR_ARG,
+              "unknown Dirichlet kind");
+  GDX_REQUIRE(d->op_kind == GDX_OP_DIFFUSION || d->op_kind == GDX_OP_FLUX || d->op_kind == GDX_OP_ELASTICITY,
+              GDX_ERR_ARG, "unknown operator kind");
+  if (d->op_kind == GDX_OP_ELASTICITY) {
+    GDX_REQUIRE(d->dim == 3, GDX_ERR_UNSUPPORTED, "linear elasticity is built for 3D meshes");
+    GDX_REQUIRE(d->nmodes == 0 && d->rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
+                "linear elasticity takes constant Lame parameters and a constant body force");
+    GDX_REQUIRE(d->dir_kind == 0 || d->dir_kind == GDX_DIR_CLAMPED_X0, GDX_ERR_UNSUPPORTED,
+                "linear elasticity supports no Dirichlet boundary or the clamped face x = x0");
+    GDX_REQUIRE(d->dir_coef[0] == 0.0 && d->dir_coef[1] == 0.0 && d->dir_coef[2] == 0.0 && d->dir_coef[3] == 0.0,
+                GDX_ERR_UNSUPPORTED, "the clamped boundary takes u = 0 (dir_coef must be zero)");
+    GDX_REQUIRE(std::isfinite(d->lame_lambda) && std::isfinite(d->lame_mu) && d->lame_mu > 0.0 &&
+                    d->lame_lambda > -2.0 / 3.0 * d->lame_mu,
+                GDX_ERR_ARG, "Lame parameters need mu > 0 and 3 lambda + 2 mu > 0");
+    return GDX_OK;
+  }
+  GDX_REQUIRE(d->dir_kind != GDX_DIR_CLAMPED_X0, GDX_ERR_UNSUPPORTED, "the clamped boundary is an elasticity option");
   if (d->op_kind == GDX_OP_FLUX) {
     GDX_REQUIRE(gdx::flux_supported(d->dim, d->order, d->qdeg / 2 + 1), GDX_ERR_UNSUPPORTED,
                 "the flux operator is built for Q1 with at most 3 Gauss points per direction");
@@ -172,6 +188,7 @@ void release_problem_device(gdx_problem *p) {
   (void)hipStreamSynchronize(p->ctx->stream);
   gdx::solve_free(p);
   gdx::flux_free(p);
+  gdx::elast_free(p);
   free_device(p);
   for (hipEvent_t &e : p->ev)
     if (e) {
@@ -205,6 +222,7 @@ int gdx_problem_create(gdx_ctx *ctx, const gdx_problem_desc *desc, gdx_problem *
   p->D = desc->dim;
   p->K = desc->order;
   p->NQ = desc->qdeg / 2 + 1;
+  p->ncomp = desc->op_kind == GDX_OP_ELASTICITY ? 3 : 1;
   p->nparts = p->desc.parts[0] * p->desc.parts[1] * p->desc.parts[2];
   *out = p;
   return GDX_OK;
@@ -243,8 +261,11 @@ int gdx_symbolic(gdx_problem *p) {
     return GDX_OK;
   }
   free_device(p);
+  gdx::elast_free(p);
   gdx::Device &d = p->dev;
-  const size_t nq = (size_t)p->kp.qn[0] * p->kp.qn[1] * p->kp.qn[2];
+  const bool elast = p->desc.op_kind == GDX_OP_ELASTICITY;
+  // kappa and f at the quadrature points (constant-coefficient elasticity has none)
+  const size_t nq = elast ? 1 : (size_t)p->kp.qn[0] * p->kp.qn[1] * p->kp.qn[2];
   int qnmax = std::max(p->kp.qn[0], std::max(p->kp.qn[1], p->kp.qn[2]));
   GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.W), nq * sizeof(double)));
   GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.F), nq * sizeof(double)));
@@ -267,6 +288,10 @@ int gdx_symbolic(gdx_problem *p) {
   if ((rc = upload(&d.pat_dir, p->pat_dir))) return rc;
   if ((rc = upload(&d.tiles, p->tiles))) return rc;
   if ((rc = upload(&d.tile_pats, p->tile_pats))) return rc;
+  if (elast && (rc = gdx::elast_setup(p))) {
+    set_error("uploading the elasticity tables failed");
+    return rc;
+  }
   for (hipEvent_t &e : p->ev)
     if (!e) GDX_HIP(hipEventCreate(&e));
   p->symbolic_done = true;
@@ -307,6 +332,7 @@ extern "C++" namespace gdx {
 // capture in a graph)
 int numeric_launches(gdx_problem *p, hipStream_t st) {
   if (p->desc.op_kind == GDX_OP_FLUX) return flux_assemble(p, nullptr, false, st);
+  if (p->desc.op_kind == GDX_OP_ELASTICITY) return elast_assemble(p, st);
   int rc = launch_coefficients(p, st);
   if (!rc) rc = launch_rhs(p, st);
   if (!rc) rc = launch_matrix(p, st);
@@ -338,6 +364,18 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
     GDX_HIP(hipEventRecord(p->ev[3], st));
     return GDX_OK;
   }
+  if (p->desc.op_kind == GDX_OP_ELASTICITY) {
+    // matrix and RHS in one launch; no coefficient phase
+    GDX_HIP(hipEventRecord(p->ev[1], st));
+    GDX_HIP(hipEventRecord(p->ev[2], st));
+    int rce = gdx::elast_assemble(p, st);
+    if (rce) {
+      set_error("elasticity kernel launch failed");
+      return rce;
+    }
+    GDX_HIP(hipEventRecord(p->ev[3], st));
+    return GDX_OK;
+  }
   int rc = gdx::launch_coefficients(p, st);
   if (rc) {
     set_error("coefficient kernel launch failed");
@@ -361,6 +399,7 @@ int gdx_problem_set_kernel(gdx_problem *p, int32_t variant, int32_t rows_z) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done && p->ctx, GDX_ERR_STATE, "needs gdx_symbolic and a device context");
   GDX_REQUIRE(variant == 0 || variant == 1, GDX_ERR_ARG, "variant must be 0 (z-marching) or 1 (per tile)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION, GDX_ERR_UNSUPPORTED, "kernel variants are for the diffusion operator");
   GDX_REQUIRE(variant == 1 || (rows_z >= p->K && rows_z % p->K == 0 && rows_z <= 4096), GDX_ERR_ARG,
               "rows_z must be a positive multiple of the order");
   GDX_HIP(hipSetDevice(p->ctx->device));

@@ -129,6 +129,7 @@ struct gdx_problem {
   std::vector<double> modes;
   int nparts = 0;
   int K = 1, D = 2, NQ = 2;
+  int ncomp = 1;  // dofs per node: 3 for GDX_OP_ELASTICITY
   int64_t nn[3] = {1, 1, 1};
   int64_t ngdofs = 0;
   std::vector<gdx::Part> P;
@@ -137,6 +138,9 @@ struct gdx_problem {
   std::vector<gdx::RowInfo> rowinfo;
   std::vector<uint8_t> pat_fwd, pat_inv, pat_invr;
   std::vector<int> pat_len, pat_dir;
+  // vector-valued rows (ncomp > 1): position -> stencil slot * ncomp + column component
+  std::vector<uint16_t> pat16_inv;
+  int pat16_stride = 0;
   int npat = 0;
   std::vector<gdx::Tile> tiles;
   std::vector<int> tile_pats;
@@ -151,6 +155,7 @@ struct gdx_problem {
   double phase_ms[4] = {0, 0, 0, 0};
   void *solve = nullptr;  // solver state (gdx_solve.hip), built on the first solve
   void *flux = nullptr;   // nonlinear operator state (gdx_flux.hip)
+  void *elast = nullptr;  // elasticity tables (gdx_elast.hip)
   bool coef_cell = false; // kappa from dev.cellk instead of the Fourier modes
 };
 
@@ -196,6 +201,9 @@ int64_t num_cells(const gdx_problem *p);
 int cell_coef_alloc(gdx_problem *p);
 int launch_cell_coefficients(gdx_problem *p, hipStream_t st);
 int grf_cell_coefficient(gdx_problem *p, uint64_t seed, double corr_len, double log_scale);
+int elast_setup(gdx_problem *p);  // factor tables and patterns to HBM
+void elast_free(gdx_problem *p);
+int elast_assemble(gdx_problem *p, hipStream_t st);
 int flux_setup(gdx_problem *p);
 void flux_free(gdx_problem *p);
 double *flux_state(gdx_problem *p);

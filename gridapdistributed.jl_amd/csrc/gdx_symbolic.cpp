@@ -68,6 +68,7 @@ inline int64_t lin(const int64_t e[3], int64_t x, int64_t y, int64_t z) { return
 
 bool is_dirichlet(const Ctx &c, const int64_t g[3]) {
   if (c.p->desc.dir_kind == 0) return false;
+  if (c.p->desc.dir_kind == GDX_DIR_CLAMPED_X0) return g[0] == 0;
   for (int d = 0; d < c.D; ++d)
     if (g[d] == 0 || g[d] == c.nn[d] - 1) return true;
   return false;
@@ -151,6 +152,141 @@ inline void shared_range(int K, int64_t i, int64_t clo, int64_t chi, int64_t &jl
   jlo = K * cmin;
   jhi = K * cmax + K;
   if (cmin > cmax) jlo = 1, jhi = 0;
+}
+
+// Local CSR of a vector-valued space (nc dofs per node) and the position
+// patterns of its owned rows.  The nc rows of a node share one column set: for
+// every stencil node j, its nc component columns.  Own columns (local id
+// nc * own index + component) sort like their nodes; ghost columns follow the
+// col PRange's first-encounter order, which interleaves components by cell.
+// A pattern maps row position -> stencil slot * nc + column component (16 bit:
+// a Q2 vertex row holds 125 * 3 entries).
+void vector_csr(gdx_problem *p, const Ctx &c) {
+  const int K = c.K, nc = p->ncomp, W = 2 * K + 1;
+  const int S = c.D == 3 ? W * W * W : W * W;
+  const int SN = S * nc;
+  const int SP = (SN + 8) & ~7;  // padded stride, at least one spare entry
+  std::unordered_map<std::string, int> pat_ids;
+  p->pat_fwd.clear();
+  p->pat_inv.clear();
+  p->pat_invr.clear();
+  p->pat_len.clear();
+  p->pat_dir.clear();
+  p->pat16_inv.clear();
+  p->pat16_stride = SP;
+  p->rowinfo.clear();
+  int64_t val_base = 0, row_base = 0, owned_nnz = 0;
+  constexpr uint16_t kAbsent16 = 0xFFFF, kDir16 = 0xFFFE;
+  std::vector<uint16_t> key(SN);
+  std::vector<std::pair<int64_t, int>> nodes;  // (comp-0 col, node slot) of own stencil nodes
+  std::vector<std::pair<int64_t, int>> ents;   // (col, slot * nc + comp)
+  for (int r = 0; r < p->nparts; ++r) {
+    Part &P = p->P[r];
+    P.val_base = val_base;
+    P.row_base = row_base;
+    P.rowptr.assign(P.nrows + 1, 0);
+    P.colidx.clear();
+    std::vector<int> own_pid(P.n_own, -1);
+    const int64_t own_cols = P.n_own;  // local col ids below this are own
+    for (int64_t row = 0; row < P.nrows; row += nc) {
+      if (P.row_ldof[row] % nc != 0) throw std::runtime_error("internal: rows of a node are not grouped");
+      int64_t g[3];
+      gnode_of(c, P, P.ldof_lnode[P.row_ldof[row]], g);
+      const bool own = row < P.n_own;
+      int64_t jlo[3] = {0, 0, 0}, jhi[3] = {0, 0, 0};
+      for (int k = 0; k < c.D; ++k) {
+        if (own) shared_range(K, g[k], 0, c.n[k], jlo[k], jhi[k]);
+        else shared_range(K, g[k], P.own_lo[k], P.own_hi[k], jlo[k], jhi[k]);
+      }
+      nodes.clear();
+      ents.clear();
+      std::fill(key.begin(), key.end(), kAbsent16);
+      bool has_dir = false;
+      for (int64_t jz = jlo[2]; jz <= jhi[2]; ++jz)
+        for (int64_t jy = jlo[1]; jy <= jhi[1]; ++jy)
+          for (int64_t jx = jlo[0]; jx <= jhi[0]; ++jx) {
+            int64_t j[3] = {jx, jy, jz};
+            int slot = (int)(jx - g[0] + K) + W * (int)(jy - g[1] + K);
+            if (c.D == 3) slot += W * W * (int)(jz - g[2] + K);
+            if (is_dirichlet(c, j)) {
+              for (int k = 0; k < nc; ++k) key[slot * nc + k] = kDir16;
+              has_dir = true;
+              continue;
+            }
+            int64_t lj = lnode_in(c, P, j);
+            int64_t j0 = lj >= 0 ? P.lnode_dof[lj] : -1;
+            int64_t col0 = j0 >= 0 ? P.ldof_col[j0] : -1;
+            if (col0 < 0) throw std::runtime_error("internal: stencil column missing from the col PRange");
+            if (col0 < own_cols) {
+              nodes.emplace_back(col0, slot);
+            } else {
+              for (int k = 0; k < nc; ++k) ents.emplace_back(P.ldof_col[j0 + k], slot * nc + k);
+            }
+          }
+      std::sort(nodes.begin(), nodes.end());
+      std::sort(ents.begin(), ents.end());
+      const int64_t base = (int64_t)P.colidx.size();
+      int e = 0;
+      for (const auto &nd : nodes)
+        for (int k = 0; k < nc; ++k, ++e) {
+          P.colidx.push_back((int32_t)(nd.first + k));
+          key[nd.second * nc + k] = (uint16_t)e;
+        }
+      for (const auto &en : ents) {
+        P.colidx.push_back((int32_t)en.first);
+        key[en.second] = (uint16_t)e++;
+      }
+      // the node's other rows repeat the column set
+      for (int k = 1; k < nc; ++k) P.colidx.insert(P.colidx.end(), P.colidx.begin() + base, P.colidx.begin() + base + e);
+      for (int k = 0; k < nc; ++k) P.rowptr[row + k + 1] = base + (int64_t)(k + 1) * e;
+      if (own) {
+        std::string skey(reinterpret_cast<const char *>(key.data()), key.size() * sizeof(uint16_t));
+        auto it = pat_ids.find(skey);
+        int pid;
+        if (it == pat_ids.end()) {
+          pid = (int)p->pat_len.size();
+          pat_ids.emplace(std::move(skey), pid);
+          std::vector<uint16_t> inv(SP, 0);
+          for (int s3 = 0; s3 < SN; ++s3)
+            if (key[s3] < kDir16) inv[key[s3]] = (uint16_t)s3;
+          p->pat16_inv.insert(p->pat16_inv.end(), inv.begin(), inv.end());
+          p->pat_len.push_back(e);
+          p->pat_dir.push_back(has_dir ? 1 : 0);
+        } else {
+          pid = it->second;
+        }
+        for (int k = 0; k < nc; ++k) own_pid[row + k] = pid;
+      }
+    }
+    owned_nnz += P.rowptr[P.n_own];
+    val_base += P.rowptr[P.nrows];
+    row_base += P.nrows;
+    // row table over the owned node box: the node's first row
+    P.rowinfo_base = (int64_t)p->rowinfo.size();
+    std::vector<int64_t> ldof_own(P.n_ldofs, -1);
+    for (int64_t o = 0; o < P.n_own; ++o) ldof_own[P.own_ldofs[o]] = o;
+    for (int64_t z = 0; z < P.onb_n[2]; ++z)
+      for (int64_t y = 0; y < P.onb_n[1]; ++y)
+        for (int64_t x = 0; x < P.onb_n[0]; ++x) {
+          int64_t g[3] = {P.onb_lo[0] + x, P.onb_lo[1] + y, P.onb_lo[2] + z};
+          RowInfo ri{0, 0, -1, 0};
+          int64_t l = lnode_in(c, P, g);
+          int64_t ld = l >= 0 ? P.lnode_dof[l] : -1;
+          if (ld >= 0) {
+            int64_t o = ldof_own[ld];
+            if (o < 0) throw std::runtime_error("internal: owned node box holds a non-owned dof");
+            ri.rowstart = P.val_base + P.rowptr[o];
+            ri.bidx = P.row_base + o;
+            ri.pid = own_pid[o];
+            ri.lenf = p->pat_len[ri.pid] | (p->pat_dir[ri.pid] << 16);
+          }
+          p->rowinfo.push_back(ri);
+        }
+  }
+  p->npat = (int)p->pat_len.size();
+  p->nnz_total = val_base;
+  p->rows_total = row_base;
+  p->owned_nnz = owned_nnz;
 }
 
 }  // namespace
@@ -248,6 +384,34 @@ int symbolic(gdx_problem *p) {
       P.ldof_gid[i] = Q.ldof_gid[Q.lnode_dof[lq]];
     }
   }
+  // vector-valued space: each free node carries ncomp consecutive dofs (Gridap
+  // numbers a face's own dofs together; Q1/Q2 faces own one node), so local
+  // and global ids become ncomp * (node id) + component, owners per node
+  const int nc = p->ncomp;
+  if (nc > 1) {
+    for (Part &P : p->P) {
+      std::vector<int64_t> lnode(P.n_ldofs * nc), gid(P.n_ldofs * nc), own;
+      std::vector<int32_t> owner(P.n_ldofs * nc);
+      for (int64_t i = 0; i < P.n_ldofs; ++i)
+        for (int k = 0; k < nc; ++k) {
+          lnode[nc * i + k] = P.ldof_lnode[i];
+          gid[nc * i + k] = nc * P.ldof_gid[i] + k;
+          owner[nc * i + k] = P.ldof_owner[i];
+        }
+      own.reserve(P.own_ldofs.size() * nc);
+      for (int64_t o : P.own_ldofs)
+        for (int k = 0; k < nc; ++k) own.push_back(nc * o + k);
+      for (int64_t &l : P.lnode_dof)
+        if (l >= 0) l *= nc;
+      P.ldof_lnode.swap(lnode);
+      P.ldof_gid.swap(gid);
+      P.ldof_owner.swap(owner);
+      P.own_ldofs.swap(own);
+      P.n_ldofs *= nc;
+      P.n_own *= nc;
+    }
+    p->ngdofs *= nc;
+  }
 
   int off[27][3];
   const int ncd = cell_dof_offsets(c.D, c.K, off);
@@ -296,14 +460,18 @@ int symbolic(gdx_problem *p) {
       P.col_owner.push_back(r);
     }
     std::vector<int64_t> ghosts;  // ldofs of ghost cols in PRange order
+    // a cell's dofs in the reference FE's order: component-major over its nodes
     auto visit_cell_cols = [&](int64_t cx, int64_t cy, int64_t cz) {
-      for (int b = 0; b < ncd; ++b) {
-        int64_t j = cell_ldof(P, cx, cy, cz, b);
-        if (j >= 0 && P.ldof_owner[j] != r && P.ldof_col[j] == -1) {
-          P.ldof_col[j] = -2;  // seen
-          ghosts.push_back(j);
+      for (int k = 0; k < nc; ++k)
+        for (int b = 0; b < ncd; ++b) {
+          int64_t j = cell_ldof(P, cx, cy, cz, b);
+          if (j < 0) continue;
+          j += k;
+          if (P.ldof_owner[j] != r && P.ldof_col[j] == -1) {
+            P.ldof_col[j] = -2;  // seen
+            ghosts.push_back(j);
+          }
         }
-      }
     };
     auto cell_has_row = [&](const Part &Q, int64_t cx, int64_t cy, int64_t cz, int want_owner) {
       for (int a = 0; a < ncd; ++a) {
@@ -354,6 +522,14 @@ int symbolic(gdx_problem *p) {
     }
     P.ncols = (int64_t)P.col_gid.size();
     P.n_ghost_cols = P.ncols - P.n_own;
+  }
+
+  if (nc > 1) {
+    vector_csr(p, c);
+    int R[3];
+    tile_shape(c.D, K, R);
+    build_tiles(p, R);
+    return GDX_OK;
   }
 
   // local CSR patterns and the position patterns of owned rows

@@ -7,7 +7,8 @@ AffineFEOperator) over the C ABI of libgdx.so (include/gdx.h).
 from . import workloads
 from ._lib import GdxError
 from .algebra import FullyAssembledRows, LocalIndices, PRange, PSparseMatrix, PVector, SubAssembledRows
-from .fespaces import (AffineFEOperator, ConstantCoefficient, ConstantSource, Diffusion, FESpace, Float64,
+from .fespaces import (CLAMPED_X0_TAGS, AffineFEOperator, BodyForce, ConstantCoefficient, ConstantSource, Diffusion,
+                       Elasticity, FESpace, Float64, VectorValue,
                        FourierExpCoefficient, GpuProblem, Integral, LinearFunction, ManufacturedSource, ReferenceFE,
                        SparseMatrixAssembler, Source, TestFESpace, TrialFESpace, allocate_matrix_and_vector,
                        assemble_matrix_and_vector, assemble_matrix_and_vector_, collect_cell_matrix_and_vector,
@@ -23,6 +24,17 @@ def problem_from_workload(ctx, p, strategy=None):
     """Build model/spaces/forms for a workload dict (gridap_amd.workloads) and
     return (assembler, data) ready for assemble_matrix_and_vector."""
     model = CartesianDiscreteModel(ctx, p["parts"], p["domain"], p["cells"])
+    if strategy is None:
+        strategy = FullyAssembledRows() if p.get("strategy") == "full" else SubAssembledRows()
+    if p.get("op") == "elasticity":
+        reffe = ReferenceFE(lagrangian, VectorValue(3, Float64), p["order"])
+        V = TestFESpace(model, reffe, dirichlet_tags="clamped_x0" if p.get("dirichlet") == "clamped_x0" else None)
+        U = TrialFESpace(V, LinearFunction())
+        dO = Measure(Triangulation(model), p["qdeg"])
+        a = Integral(Elasticity(*p["lame"]), dO)
+        l = Integral(BodyForce(p["body_force"]), dO)
+        assem = SparseMatrixAssembler(U, V, strategy)
+        return assem, collect_cell_matrix_and_vector(U, V, a, l, None)
     reffe = ReferenceFE(lagrangian, Float64, p["order"])
     V = TestFESpace(model, reffe, dirichlet_tags="boundary" if p.get("dirichlet", True) else None)
     uD = p.get("uD", (0.0, 0.0, 0.0, 0.0))

@@ -15,7 +15,8 @@ ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ER
 SUB_ASSEMBLED_ROWS, FULLY_ASSEMBLED_ROWS = 0, 1
 PREC_FP64 = 0
 RHS_CONST, RHS_MANUFACTURED = 0, 1
-OP_DIFFUSION, OP_FLUX = 0, 1
+OP_DIFFUSION, OP_FLUX, OP_ELASTICITY = 0, 1, 2
+DIR_CLAMPED_X0 = 2
 NEWTON_INIT_GIVEN, NEWTON_INIT_POISSON = 0, 1
 SZ_NLDOFS, SZ_NOWN, SZ_NGHOST_ROWS, SZ_NGHOST_COLS, SZ_NNZ, SZ_COUNT = range(6)
 
@@ -48,6 +49,7 @@ class ProblemDesc(ctypes.Structure):
         ("rhs_kind", ctypes.c_int32), ("rhs_const", ctypes.c_double), ("dir_kind", ctypes.c_int32),
         ("dir_coef", ctypes.c_double * 4), ("op_kind", ctypes.c_int32), ("flux_p", ctypes.c_double),
         ("flux_alpha", ctypes.c_double), ("flux_beta", ctypes.c_double),
+        ("lame_lambda", ctypes.c_double), ("lame_mu", ctypes.c_double), ("body_force", ctypes.c_double * 3),
     ]
 
 

@@ -5,6 +5,11 @@ The GPU path lowers a restricted weak-form vocabulary:
     a(u,v) = int( kappa * grad(v) . grad(u) ) dOmega      -> Diffusion(kappa)
     l(v)   = int( f * v ) dOmega                          -> Source(f)
 
+and, on ReferenceFE(lagrangian, VectorValue(3, Float64), order), linear elasticity
+
+    a(u,v) = int( eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)) ) dOmega -> Elasticity(lambda, mu)
+    l(v)   = int( v . f ) dOmega                                      -> BodyForce(f)
+
 with kappa a ``ConstantCoefficient`` or ``FourierExpCoefficient`` and f a
 ``ConstantSource`` or ``ManufacturedSource`` (f = -div(kappa grad uD)).
 ``SparseMatrixAssembler(U, V, strategy)`` owns one device problem (gdx_problem):
@@ -28,12 +33,29 @@ lagrangian = _Lagrangian()
 Float64 = np.float64
 
 
+class VectorValue:
+    """The value type VectorValue{n,Float64} of a vector-valued reference FE."""
+
+    def __init__(self, n, T=np.float64):
+        if T is not np.float64:
+            raise NotImplementedError("only Float64 components are lowered to the GPU")
+        self.n = int(n)
+
+
 class ReferenceFE:
-    """ReferenceFE(lagrangian, Float64, order)."""
+    """ReferenceFE(lagrangian, Float64, order) or ReferenceFE(lagrangian, VectorValue(3, Float64), order)."""
 
     def __init__(self, kind, T, order):
-        if not isinstance(kind, _Lagrangian) or T is not np.float64:
-            raise NotImplementedError("only scalar Lagrangian Float64 reference FEs are lowered to the GPU")
+        if not isinstance(kind, _Lagrangian):
+            raise NotImplementedError("only Lagrangian reference FEs are lowered to the GPU")
+        if isinstance(T, VectorValue):
+            if T.n != 3:
+                raise NotImplementedError("vector-valued reference FEs are lowered for 3 components")
+            self.ncomp = 3
+        elif T is np.float64:
+            self.ncomp = 1
+        else:
+            raise NotImplementedError("value type outside the GPU vocabulary")
         if order not in (1, 2):
             raise NotImplementedError("only order 1 and 2 are supported")
         self.order = int(order)
@@ -51,14 +73,27 @@ class LinearFunction:
         return self.coef[0] + sum(self.coef[1 + d] * x[:, d] for d in range(x.shape[1]))
 
 
+# the entities of CartesianDiscreteModel's face labeling on the face x = x0 of
+# a 3D box: face 25, its edges 13, 15, 17, 19 and vertices 1, 3, 5, 7
+CLAMPED_X0_TAGS = frozenset(f"tag_{k:02d}" for k in (1, 3, 5, 7, 13, 15, 17, 19, 25))
+
+
 class FESpace:
-    """TestFESpace(model, reffe; dirichlet_tags="boundary") (src/FESpaces.jl:570-614)."""
+    """TestFESpace(model, reffe; dirichlet_tags="boundary") (src/FESpaces.jl:570-614).
+    Vector-valued spaces also take the face x = x0 with its closure:
+    dirichlet_tags="clamped_x0" or the tag list CLAMPED_X0_TAGS."""
 
     def __init__(self, model, reffe, dirichlet_tags=None):
-        if dirichlet_tags not in (None, "boundary"):
-            raise NotImplementedError("only dirichlet_tags='boundary' (or none) is lowered to the GPU")
+        tags = dirichlet_tags
+        if isinstance(tags, (list, tuple, set, frozenset)) and frozenset(tags) == CLAMPED_X0_TAGS:
+            tags = "clamped_x0"
+        if tags not in (None, "boundary", "clamped_x0"):
+            raise NotImplementedError("dirichlet_tags outside the GPU vocabulary ('boundary', 'clamped_x0' or none)")
+        if tags == "clamped_x0" and (reffe.ncomp == 1 or model.dim != 3):
+            raise NotImplementedError("the clamped face is lowered for 3D vector-valued spaces")
         self.model, self.reffe = model, reffe
-        self.dirichlet = dirichlet_tags == "boundary"
+        self.dirichlet = tags == "boundary"
+        self.clamped = tags == "clamped_x0"
 
 
 TestFESpace = FESpace
@@ -112,6 +147,22 @@ class Source:
         self.f = f if f is not None else ConstantSource(0.0)
 
 
+class Elasticity:
+    """eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)), constant Lame parameters."""
+
+    def __init__(self, lame_lambda, lame_mu):
+        self.lame_lambda, self.lame_mu = float(lame_lambda), float(lame_mu)
+
+
+class BodyForce:
+    """v . f with a constant body force f."""
+
+    def __init__(self, f=(0.0, 0.0, 0.0)):
+        self.f = tuple(float(v) for v in f)
+        if len(self.f) != 3:
+            raise ValueError("the body force has 3 components")
+
+
 class Integral:
     """int(integrand) dOmega -- a DistributedDomainContribution in the reference."""
 
@@ -123,10 +174,13 @@ def collect_cell_matrix_and_vector(U, V, a, l, uhd=None):
     """Collect the (lazy) cell data of the bilinear and linear forms
     (src/FESpaces.jl:703-715).  Here the cell data stays symbolic; it is lowered
     to a problem descriptor and integrated on the GPU."""
-    if not (isinstance(a, Integral) and isinstance(a.integrand, Diffusion)):
+    vector = V.reffe.ncomp > 1
+    if not (isinstance(a, Integral) and isinstance(a.integrand, Elasticity if vector else Diffusion)):
         raise NotImplementedError("bilinear form outside the GPU vocabulary")
-    if not (isinstance(l, Integral) and isinstance(l.integrand, Source)):
+    if not (isinstance(l, Integral) and isinstance(l.integrand, BodyForce if vector else Source)):
         raise NotImplementedError("linear form outside the GPU vocabulary")
+    if vector and any(U.uD.coef):
+        raise NotImplementedError("vector-valued spaces take zero Dirichlet data on the GPU path")
     if a.measure.degree != l.measure.degree:
         raise NotImplementedError("both forms must use the same Measure")
     return dict(U=U, V=V, a=a, l=l)
@@ -149,6 +203,15 @@ def problem_desc(data, strategy):
     dom = list(m.domain) + [0.0, 1.0] * (3 - m.dim)
     for k in range(6):
         d.domain[k] = dom[k]
+    if isinstance(a.integrand, Elasticity):
+        d.op_kind = _lib.OP_ELASTICITY
+        d.lame_lambda, d.lame_mu = a.integrand.lame_lambda, a.integrand.lame_mu
+        for k in range(3):
+            d.body_force[k] = l.integrand.f[k]
+        d.coef_const, d.nmodes, d.modes = 1.0, 0, None
+        d.rhs_kind, d.rhs_const = _lib.RHS_CONST, 0.0
+        d.dir_kind = _lib.DIR_CLAMPED_X0 if V.clamped else 0
+        return d, None
     kap = a.integrand.kappa
     keep = None
     if isinstance(kap, ConstantCoefficient):

@@ -14,6 +14,11 @@ Cartesian mesh, in the vocabulary of the reference's drivers
   dirichlet, uD    Dirichlet on the whole boundary with uD = c0 + c1 x + c2 y + c3 z
   op, flux         "flux": the nonlinear operator of test/PLaplacianTests.jl:24-31 with
                    sigma(g) = (alpha + beta |g|^2)^((p-2)/2) g, flux = (p, alpha, beta)
+  op, lame,        "elasticity": ReferenceFE(lagrangian, VectorValue{3,Float64}, order),
+  body_force       a(u,v) = int( eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)) ),
+                   l(v) = int( v . body_force ), lame = (lambda, mu); dirichlet
+                   "clamped_x0" (u = 0 on the face x = x0 with its edges and
+                   vertices) or False
 """
 import numpy as np
 
@@ -50,6 +55,21 @@ def poisson3d_q1(n, parts=(1, 1, 1), strategy="sub", modes=None, rhs_const=1.0):
                 domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy=strategy, coef_const=1.0,
                 modes=np.zeros((0, 5)) if modes is None else modes, rhs="const", rhs_const=rhs_const,
                 dirichlet=True, uD=(0.0, 0.0, 0.0, 0.0))
+
+
+def lame_parameters(E, nu):
+    """(lambda, mu) of Young's modulus E and Poisson's ratio nu."""
+    return E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+
+
+def elasticity3d_q2(cells=(96, 80, 72), parts=(3, 2, 2), E=1.0, nu=0.3, body_force=(0.0, 0.0, -1.0),
+                    clamped=True, order=2, domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), strategy="sub"):
+    """3D linear elasticity, vector-valued Lagrangian Q2 (81 x 81 cell matrices),
+    quadrature degree 4, E = 1, nu = 0.3, clamped at x = 0, body force (0,0,-1)."""
+    return dict(dim=3, order=order, qdeg=2 * order, cells=tuple(cells), parts=tuple(parts), domain=tuple(domain),
+                strategy=strategy, coef_const=1.0, modes=np.zeros((0, 5)), rhs="const", rhs_const=0.0,
+                dirichlet="clamped_x0" if clamped else False, uD=(0.0, 0.0, 0.0, 0.0), op="elasticity",
+                lame=lame_parameters(E, nu), body_force=tuple(float(v) for v in body_force))
 
 
 def plaplacian3d_q1(n=96, parts=(4, 2, 2), p=3.0, eps=1e-2, strategy="sub"):

@@ -51,6 +51,8 @@ extern "C" {
 /* operator kinds */
 #define GDX_OP_DIFFUSION 0 /* affine: a(u,v) and l(v) below */
 #define GDX_OP_FLUX 1      /* nonlinear flux operator at a state, see gdx_set_state */
+#define GDX_OP_ELASTICITY 2 /* affine, vector-valued: linear elasticity, see below */
+#define GDX_DIR_CLAMPED_X0 2 /* dir_kind: u = 0 on the face x = x0 (closure) */
 
 /* right-hand side kinds */
 #define GDX_RHS_CONST 0        /* f = rhs_const */
@@ -79,6 +81,15 @@ typedef struct gdx_nn gdx_nn; /* learned preconditioner, declared below */
  *   sigma(g)  = (flux_alpha + flux_beta |g|^2)^((flux_p - 2)/2) g
  * (the reference test is flux_p = 4, flux_alpha = flux_beta = 1; the
  * p-Laplacian is flux_p = p with flux_alpha = eps^2 > 0, flux_beta = 1).
+ *
+ * op_kind GDX_OP_ELASTICITY (dim 3, order 1 or 2, no kappa modes) is linear
+ * elasticity on ReferenceFE(lagrangian, VectorValue{3,Float64}, order):
+ *   a(u,v) = int( eps(v) : (lame_lambda tr(eps(u)) I + 2 lame_mu eps(u)) ) dOmega
+ *   l(v)   = int( v . body_force ) dOmega
+ * with dir_kind 0 or GDX_DIR_CLAMPED_X0 (u = 0 at every node of the face
+ * x = x0, its edges and vertices included; dir_coef must be zero).  Each node's
+ * three components are consecutive dofs (node-major); cell dofs are
+ * component-major, which fixes the first-encounter order of ghost columns.
  */
 typedef struct gdx_problem_desc {
   int32_t dim;      /* 2 or 3 */
@@ -93,10 +104,12 @@ typedef struct gdx_problem_desc {
   const double *modes;
   int32_t rhs_kind;
   double rhs_const;
-  int32_t dir_kind; /* 0: no Dirichlet boundary, 1: Dirichlet on the whole boundary */
+  int32_t dir_kind; /* 0: none, 1: whole boundary, GDX_DIR_CLAMPED_X0 (elasticity) */
   double dir_coef[4];
-  int32_t op_kind;  /* GDX_OP_DIFFUSION or GDX_OP_FLUX */
+  int32_t op_kind;  /* GDX_OP_DIFFUSION, GDX_OP_FLUX or GDX_OP_ELASTICITY */
   double flux_p, flux_alpha, flux_beta;
+  double lame_lambda, lame_mu; /* GDX_OP_ELASTICITY */
+  double body_force[3];
 } gdx_problem_desc;
 
 /* sizes reported per part by gdx_part_sizes (index into the int64 array) */

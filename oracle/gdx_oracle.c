@@ -71,6 +71,15 @@ typedef struct {
   const double *state;
   /* kappa per cell (x fastest), NULL = the Fourier form above; f = rhs_const */
   const double *cell_kappa;
+  /* op_kind 2: linear elasticity, a vector-valued Lagrangian space with dim
+     components per node (cell dofs component-major, each node's components
+     numbered consecutively):
+       a(u,v) = int( eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)) ) dOmega
+       l(v)   = int( v . body_force ) dOmega
+     with dir_kind 0 (no Dirichlet) or 2 (clamped: u = 0 at every node of the
+     face x = x0, its edges and vertices included). */
+  double lame_lambda, lame_mu;
+  double body_force[3];
 } orc_desc;
 
 typedef struct {
@@ -107,6 +116,7 @@ typedef struct {
   double *state;
   double *cell_kappa;
   int nparts, K, D, nq;
+  int nc;          /* components per node: dim for elasticity, else 1 */
   i64 nn[3];       /* global node counts per dim */
   i64 ngdofs;
   part_t *P;
@@ -227,6 +237,7 @@ static inline i64 lidx(const i64 lnn[3], i64 x, i64 y, i64 z) { return x + lnn[0
 
 static int is_dirichlet_gnode(const orc_ctx *C, const i64 g[3]) {
   if (C->d.dir_kind == 0) return 0;
+  if (C->d.dir_kind == 2) return g[0] == 0;
   for (int d = 0; d < C->D; d++)
     if (g[d] == 0 || g[d] == C->nn[d] - 1) return 1;
   return 0;
@@ -287,24 +298,35 @@ static void number_local_dofs(orc_ctx *C, part_t *P) {
           mark[l] = 1;
         }
   }
+  /* a node's own dofs are its components, consecutive (Gridap: each face's own
+     dofs in order; Q1/Q2 faces own one node) */
+  const int nc = C->nc;
   P->lnode_dof = malloc(sizeof(i64) * n);
-  P->ldof_lnode = malloc(sizeof(i64) * n);
+  P->ldof_lnode = malloc(sizeof(i64) * n * nc);
   i64 nf = 0, nd = 0;
   for (i64 k = 0; k < cnt; k++) {
     i64 l = list[k];
     i64 g[3] = {P->loc_lo[0] * K + l % lnn[0], P->loc_lo[1] * K + (l / lnn[0]) % lnn[1],
                 D == 3 ? P->loc_lo[2] * K + l / (lnn[0] * lnn[1]) : 0};
     if (is_dirichlet_gnode(C, g)) {
-      P->lnode_dof[l] = -1 - nd++;
+      P->lnode_dof[l] = -1 - nd;
+      nd += nc;
     } else {
-      P->ldof_lnode[nf] = l;
-      P->lnode_dof[l] = nf++;
+      for (int c = 0; c < nc; c++) P->ldof_lnode[nf + c] = l;
+      P->lnode_dof[l] = nf;
+      nf += nc;
     }
   }
   P->n_ldofs = nf;
   P->n_ddofs = nd;
   free(list);
   free(mark);
+}
+
+/* local dof of component comp at local node l (negative: Dirichlet) */
+static inline i64 node_dof(const part_t *P, i64 l, int comp) {
+  i64 d0 = P->lnode_dof[l];
+  return d0 >= 0 ? d0 + comp : d0 - comp;
 }
 
 /* local node of a global node in part P, or -1 */
@@ -363,9 +385,10 @@ static void setup(orc_ctx *C) {
     FOR_BOX(P->loc_lo, P->loc_hi, D, cx, cy, cz) {
       int co = cell_owner_rank(C, cx, cy, cz);
       i64 c[3] = {cx - P->loc_lo[0], cy - P->loc_lo[1], cz - P->loc_lo[2]};
-      for (int a = 0; a < ncn; a++) {
+      for (int k = 0; k < C->nc * ncn; k++) {
+        int a = k % ncn;
         i64 l = lidx(P->lnn, K * c[0] + coff[a][0], K * c[1] + coff[a][1], K * c[2] + coff[a][2]);
-        i64 dof = P->lnode_dof[l];
+        i64 dof = node_dof(P, l, k / ncn);
         if (dof >= 0 && co > own1[r][dof]) own1[r][dof] = co;
       }
     }
@@ -380,13 +403,14 @@ static void setup(orc_ctx *C) {
       if (co == r) continue;
       part_t *Q = &C->P[co];
       i64 c[3] = {cx - P->loc_lo[0], cy - P->loc_lo[1], cz - P->loc_lo[2]};
-      for (int a = 0; a < ncn; a++) {
+      for (int k = 0; k < C->nc * ncn; k++) {
+        int a = k % ncn;
         i64 l = lidx(P->lnn, K * c[0] + coff[a][0], K * c[1] + coff[a][1], K * c[2] + coff[a][2]);
-        i64 dof = P->lnode_dof[l];
+        i64 dof = node_dof(P, l, k / ncn);
         if (dof < 0) continue;
         i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
         i64 lq = lnode_of_gnode(C, Q, g);
-        P->ldof_owner[dof] = own1[co][Q->lnode_dof[lq]];
+        P->ldof_owner[dof] = own1[co][node_dof(Q, lq, k / ncn)];
       }
     }
   }
@@ -427,14 +451,15 @@ static void setup(orc_ctx *C) {
         if (co == r) continue;
         part_t *Q = &C->P[co];
         i64 c[3] = {cx - P->loc_lo[0], cy - P->loc_lo[1], cz - P->loc_lo[2]};
-        for (int a = 0; a < ncn; a++) {
+        for (int k = 0; k < C->nc * ncn; k++) {
+          int a = k % ncn;
           i64 l = lidx(P->lnn, K * c[0] + coff[a][0], K * c[1] + coff[a][1], K * c[2] + coff[a][2]);
-          i64 dof = P->lnode_dof[l];
+          i64 dof = node_dof(P, l, k / ncn);
           if (dof < 0) continue;
           if (round == 0 && P->ldof_owner[dof] != co) continue;
           i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
           i64 lq = lnode_of_gnode(C, Q, g);
-          P->ldof_gid[dof] = snap[co][Q->lnode_dof[lq]];
+          P->ldof_gid[dof] = snap[co][node_dof(Q, lq, k / ncn)];
         }
       }
     }
@@ -497,8 +522,9 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
     i64 g[3] = {K * (cx + o[0] / 2), K * (cy + o[1] / 2), K * (cz + o[2] / 2)};
     for (int d = 0; d < 3; d++) vx[v][d] = d < D ? gnode_coord(C, d, g[d]) : 0.0;
   }
-  for (int a = 0; a < ncn * ncn; a++) Ac[a] = 0.0;
-  for (int a = 0; a < ncn; a++) bc[a] = 0.0;
+  const int nc = C->nc, ncd = nc * ncn; /* cell dofs: component-major */
+  for (int a = 0; a < ncd * ncd; a++) Ac[a] = 0.0;
+  for (int a = 0; a < ncd; a++) bc[a] = 0.0;
   int nqz = D == 3 ? nq : 1;
   for (int q3 = 0; q3 < nqz; q3++)
     for (int q2 = 0; q2 < nq; q2++)
@@ -560,6 +586,25 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
             for (int d = 0; d < D; d++) grad[a][e] += iJ[d][e] * ref[d];
           }
         }
+        if (C->d.op_kind == 2) {
+          /* eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)) with v = phi_a e_ca,
+             u = phi_b e_cb:  lambda d_ca(phi_a) d_cb(phi_b)
+                            + mu d_cb(phi_a) d_ca(phi_b) + mu [ca == cb] grad(phi_a).grad(phi_b) */
+          const double lam = C->d.lame_lambda, mu = C->d.lame_mu;
+          for (int ca = 0; ca < nc; ca++)
+            for (int a = 0; a < ncn; a++) {
+              const int ia = ca * ncn + a;
+              bc[ia] += C->d.body_force[ca] * phi[a] * dV;
+              for (int cb = 0; cb < nc; cb++)
+                for (int b = 0; b < ncn; b++) {
+                  double v = lam * grad[a][ca] * grad[b][cb] + mu * grad[a][cb] * grad[b][ca];
+                  if (ca == cb)
+                    for (int e = 0; e < D; e++) v += mu * grad[a][e] * grad[b][e];
+                  Ac[ia * ncd + cb * ncn + b] += v * dV;
+                }
+            }
+          continue;
+        }
         if (C->d.op_kind == 1) {
           double gu[3] = {0.0, 0.0, 0.0}, uq = 0.0;
           for (int b = 0; b < ncn; b++) {
@@ -600,9 +645,9 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
   /* Dirichlet lifting: b_c -= A_c u_D (the flux operator's residual already
      sees the Dirichlet values through u) */
   if (C->d.op_kind == 1) return;
-  for (int a = 0; a < ncn; a++)
-    for (int b = 0; b < ncn; b++)
-      if (isdir[b]) bc[a] -= Ac[a * ncn + b] * uD[b];
+  for (int a = 0; a < ncd; a++)
+    for (int b = 0; b < ncd; b++)
+      if (isdir[b]) bc[a] -= Ac[a * ncd + b] * uD[b];
 }
 
 /* u at global lattice node g: the state, or uD on Dirichlet nodes */
@@ -631,35 +676,37 @@ static void numeric_loop(orc_ctx *C, part_t *P, int mode, const i64 *fe_to_row, 
   int D = C->D, K = C->K;
   int coff[27][3];
   int ncn = cell_nodes(D, K, coff);
-  double Ac[27 * 27], bc[27], uD[27], ucell[27];
-  int isdir[27];
-  i64 row[27], col[27];
+  const int ncd = C->nc * ncn;
+  double Ac[81 * 81], bc[81], uD[81], ucell[81];
+  int isdir[81];
+  i64 row[81], col[81];
   int FA = C->d.strategy == 1;
   const i64 *lo = FA ? P->loc_lo : P->own_lo;
   const i64 *hi = FA ? P->loc_hi : P->own_hi;
   FOR_BOX(lo, hi, D, cx, cy, cz) {
     i64 c[3] = {cx - P->loc_lo[0], cy - P->loc_lo[1], cz - P->loc_lo[2]};
-    for (int a = 0; a < ncn; a++) {
+    for (int k = 0; k < ncd; k++) {
+      const int a = k % ncn;
       i64 l = lidx(P->lnn, K * c[0] + coff[a][0], K * c[1] + coff[a][1], K * c[2] + coff[a][2]);
-      i64 dof = P->lnode_dof[l];
-      col[a] = dof;
-      row[a] = dof;
-      if (FA && dof >= 0 && P->ldof_owner[dof] != P->rank) row[a] = -1; /* ghost rows dropped */
-      isdir[a] = dof < 0;
+      i64 dof = node_dof(P, l, k / ncn);
+      col[k] = dof;
+      row[k] = dof;
+      if (FA && dof >= 0 && P->ldof_owner[dof] != P->rank) row[k] = -1; /* ghost rows dropped */
+      isdir[k] = dof < 0;
       i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
       double x[3] = {0, 0, 0};
       for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
-      uD[a] = isdir[a] ? eval_uD(C, x) : 0.0;
-      ucell[a] = state_at(C, g, isdir[a], uD[a]);
+      uD[k] = isdir[k] && C->d.op_kind != 2 ? eval_uD(C, x) : 0.0; /* elasticity: clamped, u = 0 */
+      ucell[k] = state_at(C, g, isdir[k], uD[k]);
     }
     integrate_cell(C, cx, cy, cz, ncn, coff, Ac, bc, uD, isdir, ucell, NULL);
     /* add_entries!: columns outer, rows inner, skipping non-positive ids */
-    for (int b = 0; b < ncn; b++) {
+    for (int b = 0; b < ncd; b++) {
       if (col[b] < 0) continue;
-      for (int a = 0; a < ncn; a++) {
+      for (int a = 0; a < ncd; a++) {
         if (row[a] < 0) continue;
         if (mode == 0) {
-          coo_push(P, row[a], col[b], Ac[a * ncn + b]);
+          coo_push(P, row[a], col[b], Ac[a * ncd + b]);
         } else {
           i64 r = fe_to_row[row[a]], cc = fe_to_col[col[b]];
           i64 lo2 = P->rowptr[r], hi2 = P->rowptr[r + 1] - 1;
@@ -667,11 +714,11 @@ static void numeric_loop(orc_ctx *C, part_t *P, int mode, const i64 *fe_to_row, 
             i64 mid = (lo2 + hi2) / 2;
             if (P->colidx[mid] < cc) lo2 = mid + 1; else hi2 = mid;
           }
-          P->val[lo2] += Ac[a * ncn + b];
+          P->val[lo2] += Ac[a * ncd + b];
         }
       }
     }
-    for (int a = 0; a < ncn; a++) {
+    for (int a = 0; a < ncd; a++) {
       if (row[a] < 0) continue;
       if (mode == 0) {
         P->touched[row[a]] = 1;
@@ -982,6 +1029,7 @@ void *orc_run(const orc_desc *d, int inplace_reps) {
   C->D = d->dim;
   C->K = d->order;
   C->nq = d->qdeg / 2 + 1;
+  C->nc = d->op_kind == 2 ? d->dim : 1;
   if (d->nmodes > 0) {
     C->modes = malloc(sizeof(double) * 5 * d->nmodes);
     memcpy(C->modes, d->modes, sizeof(double) * 5 * d->nmodes);

@@ -173,3 +173,80 @@ def assemble_global(p):
                 bI.append(gn[free])
                 bV.append(bv[free])
     return np.concatenate(I), np.concatenate(J), np.concatenate(V), np.concatenate(bI), np.concatenate(bV)
+
+
+def assembled_factors_1d(p, d):
+    """1D matrices assembled over the mesh along direction d (dense, nn_d x nn_d):
+    K = int phi_i' phi_j', M = int phi_i phi_j, P = int phi_i' phi_j,
+    Q = int phi_i phi_j', and m = int phi_i, with the problem's Gauss rule."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    nq = p["qdeg"] // 2 + 1
+    xq, wq = gauss01(nq)
+    B = lagrange01(K)
+    phi = np.array([nppoly.polyval(xq, c) for c in B])
+    dphi = np.array([nppoly.polyval(xq, nppoly.polyder(c)) for c in B]) / h[d]
+    n = nn[d]
+    Km, Mm, Pm, Qm, m = (np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n)), np.zeros(n))
+    w = wq * h[d]
+    for e in range(cells[d]):
+        s = slice(K * e, K * e + K + 1)
+        Km[s, s] += np.einsum("aq,bq,q->ab", dphi, dphi, w)
+        Mm[s, s] += np.einsum("aq,bq,q->ab", phi, phi, w)
+        Pm[s, s] += np.einsum("aq,bq,q->ab", dphi, phi, w)
+        Qm[s, s] += np.einsum("aq,bq,q->ab", phi, dphi, w)
+        m[s] += phi @ w
+    return Km, Mm, Pm, Qm, m
+
+
+def elasticity_global(p):
+    """Linear elasticity on the whole lattice, dof = 3 * lattice node + component
+    (x-fastest nodes, no Dirichlet elimination): the matrix as a scipy sparse
+    matrix and the body-force vector.  Built from 1D assembled factors as the sum
+    of Kronecker products of the bilinear form's separable terms,
+      (a, b) block = lambda d_a(v) d_b(u) + mu d_b(v) d_a(u) + mu [a == b] grad(v).grad(u),
+    an independent route from the C oracle's per-cell 3D quadrature."""
+    import scipy.sparse as sp
+
+    lam, mu = p["lame"]
+    F = [assembled_factors_1d(p, d) for d in range(3)]
+
+    def kron3(f0, f1, f2):  # x fastest: node = x + nx (y + ny z)
+        return sp.kron(sp.csr_matrix(f2), sp.kron(sp.csr_matrix(f1), sp.csr_matrix(f0)))
+
+    KMPQ = [dict(K=F[d][0], M=F[d][1], P=F[d][2], Q=F[d][3]) for d in range(3)]
+
+    def term(kinds):  # kinds[d] in K, M, P, Q
+        return kron3(KMPQ[0][kinds[0]], KMPQ[1][kinds[1]], KMPQ[2][kinds[2]])
+
+    A = None
+    for a in range(3):
+        for b in range(3):
+            if a == b:
+                blk = None
+                for e in range(3):
+                    kinds = ["M", "M", "M"]
+                    kinds[e] = "K"
+                    t = term(kinds) * (mu + (lam + mu if e == a else 0.0))
+                    blk = t if blk is None else blk + t
+            else:
+                k1 = ["M", "M", "M"]
+                k1[a], k1[b] = "P", "Q"  # lambda d_a(phi_i) d_b(phi_j)
+                k2 = ["M", "M", "M"]
+                k2[a], k2[b] = "Q", "P"  # mu d_b(phi_i) d_a(phi_j)
+                blk = lam * term(k1) + mu * term(k2)
+            E = sp.csr_matrix(([1.0], ([a], [b])), shape=(3, 3))
+            t = sp.kron(blk, E)
+            A = t if A is None else A + t
+    m = np.kron(F[2][4], np.kron(F[1][4], F[0][4]))
+    bf = np.asarray(p["body_force"], np.float64)
+    bvec = (m[:, None] * bf[None, :]).ravel()
+    return A.tocsr(), bvec
+
+
+def elasticity_free_dofs(p):
+    """Lattice dofs (3 * node + component) that are free: all, or those off the
+    clamped face x = x0."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    nodes = np.arange(int(np.prod(nn)))
+    free = np.ones(len(nodes), bool) if p.get("dirichlet") != "clamped_x0" else (nodes % nn[0]) != 0
+    return np.repeat(free, 3)

@@ -28,6 +28,7 @@ class OrcDesc(ctypes.Structure):
         ("op_kind", ctypes.c_int), ("flux_p", ctypes.c_double), ("flux_alpha", ctypes.c_double),
         ("flux_beta", ctypes.c_double), ("state", ctypes.POINTER(ctypes.c_double)),
         ("cell_kappa", ctypes.POINTER(ctypes.c_double)),
+        ("lame_lambda", ctypes.c_double), ("lame_mu", ctypes.c_double), ("body_force", ctypes.c_double * 3),
     ]
 
 
@@ -82,12 +83,19 @@ def make_desc(p):
     d.coef_const = p.get("coef_const", 1.0)
     d.rhs_kind = {"const": 0, "manufactured": 1}[p.get("rhs", "const")]
     d.rhs_const = p.get("rhs_const", 0.0)
-    d.dir_kind = 1 if p.get("dirichlet", True) else 0
+    dk = p.get("dirichlet", True)
+    d.dir_kind = 2 if dk == "clamped_x0" else (1 if dk else 0)
     dc = list(p.get("uD", (0.0, 0.0, 0.0, 0.0)))
     for i in range(4):
         d.dir_coef[i] = dc[i]
     # nonlinear flux operator sigma(g) = (alpha + beta|g|^2)^((p-2)/2) g at a lattice state
-    d.op_kind = 1 if p.get("op", "diffusion") == "flux" else 0
+    d.op_kind = {"diffusion": 0, "flux": 1, "elasticity": 2}[p.get("op", "diffusion")]
+    # linear elasticity: Lame parameters and a constant body force
+    lam, mu = p.get("lame", (0.0, 0.0))
+    d.lame_lambda, d.lame_mu = lam, mu
+    bf = list(p.get("body_force", (0.0, 0.0, 0.0)))
+    for i in range(3):
+        d.body_force[i] = bf[i]
     fp, fa, fb = p.get("flux", (2.0, 1.0, 0.0))
     d.flux_p, d.flux_alpha, d.flux_beta = fp, fa, fb
     st = p.get("state")
