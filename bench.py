@@ -274,12 +274,82 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def elasticity_problem(ctx, G, W):
+    """The elasticity problem of the batched workload: Q2 VectorValue{3}, 96x80x72
+    cells on (0,1)^3, parts (3,2,2), E = 1, nu = 0.3, clamped at x = 0, body
+    force (0,0,-1)."""
+    p = W.elasticity3d_q2(cells=(96, 80, 72), parts=(3, 2, 2))
+    t0 = time.perf_counter()
+    assem, data = G.problem_from_workload(ctx, p)
+    prob = assem._problem_for(data)
+    return assem, prob, time.perf_counter() - t0
+
+
+def elasticity_kernel_ms(prob, reps=5):
+    """Average k_elast duration from the library's hipEvents around its launch."""
+    acc = 0.0
+    for _ in range(reps):
+        prob.numeric()
+        acc += prob.phase_times()[3]
+    return acc / reps
+
+
+def elasticity_roofline(prob, ms):
+    nnz = prob.owned_nnz()
+    alg = 8 * nnz  # FP64 value written per stored owned entry
+    traffic = None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_elast_elasticity_q2_96x80x72.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            traffic = json.load(fh).get("traffic_bytes")
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_elast<2>",
+            "alg_bytes_per_launch": int(alg)}
+
+
+def bench_elasticity(args):
+    """Linear elasticity Q2 (81 x 81 cell matrices) on 96x80x72 cells, parts
+    (3,2,2), clamped at x = 0: the numeric assembly alone (one step = one
+    assembly of the 2.5 G-entry matrix and the body-force vector)."""
+    import gridap_amd as G
+    from gridap_amd import workloads as W
+
+    ctx = G.DeviceContext(0)
+    assem, prob, t_sym = elasticity_problem(ctx, G, W)
+    nnz = prob.owned_nnz()
+    for _ in range(args.warmup):
+        prob.numeric()
+    prob.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prob.numeric()
+    prob.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kms = elasticity_kernel_ms(prob)
+    out = {
+        "metric": "3D Q2 linear elasticity assembly Gnnz/s (FP64)",
+        "value": round(nnz / (ms * 1e-3) / 1e9, 2), "unit": "Gnnz/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "linear elasticity Q2 VectorValue{3}, 96x80x72 cells on (0,1)^3, parts (3,2,2), E=1, "
+                               "nu=0.3, clamped at x=0, body force (0,0,-1), SubAssembledRows",
+                   "free_dofs": prob.ngdofs, "nnz_owned": nnz, "symbolic_s": round(t_sym, 2),
+                   "kernel_ms": round(kms, 4)},
+        "roofline": elasticity_roofline(prob, kms),
+    }
+    prob.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_batched(args):
-    """32 independent 3D Poisson Q1 problems (cubes of 16^3..48^3 cells drawn from
-    a seed, 1..8 parts each, 8-mode Fourier kappa, f = 1): every numeric
-    assembly in one hipGraph launch after per-problem autotuning of the cells
-    per workgroup, against the same launches issued one by one.  One step = one
-    batch (all 32 problems)."""
+    """The batched workload: 32 independent 3D Poisson Q1 problems (cubes of
+    16^3..48^3 cells drawn from a seed, 1..8 parts each, 8-mode Fourier kappa,
+    f = 1) plus the linear-elasticity Q2 problem (96x80x72 cells, parts (3,2,2)),
+    every numeric assembly in one hipGraph launch after per-problem autotuning,
+    against the same launches issued one by one.  One step = one batch (all 33
+    problems).  --no-elasticity leaves the elasticity problem out."""
     import gridap_amd as G
     from gridap_amd import workloads as W
 
@@ -291,6 +361,12 @@ def bench_batched(args):
         assem, data = G.problem_from_workload(ctx, q)
         probs.append((assem, assem._problem_for(data)))
     t_sym = time.perf_counter() - t0
+    nnz_q1 = sum(p.owned_nnz() for _, p in probs)
+    el = None
+    if not args.no_elasticity:
+        ea, ep, t_sym_el = elasticity_problem(ctx, G, W)
+        el = (ea, ep)
+        probs.append(el)
     nnz = sum(p.owned_nnz() for _, p in probs)
     # eager: one problem after the other
     for _ in range(args.warmup):
@@ -316,19 +392,29 @@ def bench_batched(args):
         b.launch()
     b.sync()
     ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+    chosen = chosen[:len(ws)]
+    cfg = {"workload": "32 independent 3D Poisson Q1 problems, cubes 16^3..48^3 cells (seed 7), 1..8 parts each, "
+                       "8-mode Fourier kappa, f=1" +
+                       ("" if el is None else " + linear elasticity Q2 96x80x72, parts (3,2,2), E=1, nu=0.3, clamped "
+                                              "at x=0, body force (0,0,-1)") +
+                       "; numeric assembly of all in one hipGraph launch",
+           "nnz_owned_total": nnz, "nnz_owned_q1": nnz_q1, "symbolic_s_q1_total": round(t_sym, 3),
+           "ms_eager_one_by_one": round(ms_eager, 4), "speedup_graph_vs_eager": round(ms_eager / ms_graph, 3),
+           "kernel_chosen_q1": {"z-marching 16": int((chosen == 16).sum()), "z-marching 32": int((chosen == 32).sum()),
+                                "z-marching 64": int((chosen == 64).sum()), "per-tile": int((chosen < 0).sum())}}
     out = {
-        "metric": "batched 3D Q1 assembly Gnnz/s (FP64), 32 problems per hipGraph launch",
+        "metric": "batched assembly Gnnz/s (FP64), 32 Q1 problems" + ("" if el is None else " + elasticity Q2") +
+                  " per hipGraph launch",
         "value": round(nnz / (ms_graph * 1e-3) / 1e9, 3), "unit": "Gnnz/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_graph, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "32 independent 3D Poisson Q1 problems, cubes 16^3..48^3 cells (seed 7), 1..8 parts "
-                               "each, 8-mode Fourier kappa, f=1; numeric assembly of all in one hipGraph launch",
-                   "nnz_owned_total": nnz, "symbolic_s_total": round(t_sym, 3),
-                   "ms_eager_one_by_one": round(ms_eager, 4), "speedup_graph_vs_eager": round(ms_eager / ms_graph, 3),
-                   "kernel_chosen": {"z-marching 16": int((chosen == 16).sum()), "z-marching 32": int((chosen == 32).sum()),
-                                     "z-marching 64": int((chosen == 64).sum()), "per-tile": int((chosen < 0).sum())},
-                   "elasticity_q2": "not built"},
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfg,
     }
+    if el is not None:
+        kms = elasticity_kernel_ms(el[1])
+        cfg["elasticity"] = {"nnz_owned": el[1].owned_nnz(), "free_dofs": el[1].ngdofs,
+                             "symbolic_s": round(t_sym_el, 2), "kernel_ms": round(kms, 4),
+                             "gnnz_s": round(el[1].owned_nnz() / (kms * 1e-3) / 1e9, 2)}
+        out["roofline"] = elasticity_roofline(el[1], kms)
     b.close()
     for _, p in probs:
         p.close()
@@ -343,9 +429,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["poisson-q2", "plaplacian", "learned", "batched"], default="poisson-q2",
+    ap.add_argument("--workload", choices=["poisson-q2", "plaplacian", "learned", "batched", "elasticity"],
+                    default="poisson-q2",
                     help="poisson-q2: 3D Q2 assembly throughput (default); plaplacian: p=3 Newton on 3D Q1; "
-                         "learned: FPCG with the learned preconditioner vs Jacobi-PCG on 3D Q1 128^3")
+                         "learned: FPCG with the learned preconditioner vs Jacobi-PCG on 3D Q1 128^3; batched: 32 Q1 "
+                         "problems + elasticity Q2 in one hipGraph; elasticity: the elasticity Q2 assembly alone")
+    ap.add_argument("--no-elasticity", action="store_true", help="batched workload without the elasticity problem")
     ap.add_argument("--fields", type=int, default=2000, help="training fields (learned workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (scale reference for HBM byte counters)")
@@ -372,6 +461,8 @@ def main():
         return bench_learned(args)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "elasticity":
+        return bench_elasticity(args)
 
     import gridap_amd as G
     from gridap_amd import workloads as W

@@ -18,13 +18,15 @@
 // and b(i, a) = f_a m0(i0) m1(i1) m2(i2) with m_d(i) = int phi_i.  Clamped
 // (zero) Dirichlet data needs no lifting.
 //
-// Kernel: one wave per owned row node (its 3 rows share one column set and one
-// position pattern).  The node's 1D factors (3 directions x 2K+1 offsets x
-// {K,M,P,Q}) go to LDS; each lane then walks row positions e = lane, lane+64,
-// ..., decodes e -> (stencil slot, column component) through the pattern and
-// writes the 3 rows' values at e: every store instruction writes 64
-// consecutive doubles.  HBM traffic is the 8 B value writes (plus 24 B of row
-// table per node); the arithmetic is ~10 FLOP per stored value.
+// Kernel: each wave takes two owned row nodes (a node's 3 rows share one column
+// set and one position pattern), nodes in the order of their rows in memory.
+// All loads of both nodes are issued before the first store: the node's 1D
+// factors (3 directions x 2K+1 offsets x {K,M,P,Q}, to LDS) and each lane's
+// slice of the pattern (row positions e = lane, lane+64, ...), so no global
+// load sits between the stores.  A lane decodes e -> (stencil slot, column
+// component) and writes the 3 rows' values at e: every store instruction
+// writes 64 consecutive doubles.  HBM traffic is the 8 B value writes (plus
+// 32 B of row table per node); the arithmetic is ~10 FLOP per stored value.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,19 +38,28 @@ namespace gdx {
 
 namespace {
 
-constexpr int kWaves = 4;  // waves (row nodes) per workgroup
+constexpr int kWaves = 4;         // waves per workgroup
+constexpr int kNodesPerWave = 2;  // row nodes per wave, their loads issued together
+                                  // (measured on 96x80x72 Q2: 1 node 5.28 ms, 2 nodes 5.14 ms,
+                                  // 4 nodes spill registers, 6.56 ms)
 
-struct ElastPart {
-  long long rowinfo_base;
-  int onb_lo[3], onb_n[3];
+// One owned row node, in the order of its rows in the values buffer.
+struct ElastRow {
+  long long rowstart;  // first value of the node's first row
+  long long bidx;      // RHS index of the node's first row
+  int pid;             // position pattern
+  int len;             // row length (the node's 3 rows have the same)
+  unsigned short g[3]; // global lattice coordinates
+  unsigned short pad;
 };
+static_assert(sizeof(ElastRow) == 32, "ElastRow is 32 bytes");
 
 struct ElastData {
   double *fac = nullptr;    // [3][nnmax][W][4]: K, M, P, Q at (node, offset + K)
   double *mass = nullptr;   // [3][nnmax]
   uint16_t *pinv = nullptr; // [npat][stride]: position -> slot * 3 + component
-  ElastPart *parts = nullptr;
-  int nparts = 0, nnmax = 0, stride = 0;
+  ElastRow *rows = nullptr;
+  int nnmax = 0, stride = 0;
   long long nrownodes = 0;
 };
 
@@ -56,73 +67,88 @@ ElastData *ed(gdx_problem *p) { return reinterpret_cast<ElastData *>(p->elast); 
 
 struct ElastArgs {
   double lam, mu, f[3];
-  int nnmax, stride, nparts;
+  int nnmax, stride;
   long long nrownodes;
 };
 
-template <int K>
-__global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastPart *__restrict__ parts,
-                                                       const RowInfo *__restrict__ rowinfo,
+template <int K, int NPW>
+__global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastRow *__restrict__ rows,
                                                        const double *__restrict__ fac, const double *__restrict__ mass,
                                                        const uint16_t *__restrict__ pinv, double *__restrict__ values,
                                                        double *__restrict__ b) {
   constexpr int W = 2 * K + 1;
-  constexpr int NF = 3 * W * 4;  // factors of one row node
-  __shared__ __align__(16) double lds[kWaves][NF];
+  constexpr int NF = 3 * W * 4;                  // factors of one row node
+  constexpr int MAXE = (3 * W * W * W + 63) / 64;  // row positions per lane
+  __shared__ __align__(16) double lds[kWaves][NPW][NF];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long long node = (long long)blockIdx.x * kWaves + wave;
-  if (node >= A.nrownodes) return;  // whole waves exit together; no barrier below
-  const RowInfo ri = rowinfo[node];
-  if (ri.pid < 0) return;
-  // the node's part (parts sorted by rowinfo_base) and its global coordinates
-  int lo = 0, hi = A.nparts - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (parts[mid].rowinfo_base <= node) lo = mid;
-    else hi = mid - 1;
+  const long long node0 = ((long long)blockIdx.x * kWaves + wave) * NPW;
+  if (node0 >= A.nrownodes) return;  // whole waves exit together; no barrier below
+  const int nn = (int)min((long long)NPW, A.nrownodes - node0);
+  // all loads of the wave's NPW nodes are issued before any store
+  ElastRow ri[NPW];
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) ri[j] = rows[node0 + (j < nn ? j : 0)];
+  double fv[NPW];
+  int s3v[NPW][MAXE];
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) {
+    fv[j] = 0.0;
+    if (lane < NF) {
+      const int d = lane / (W * 4), r = lane % (W * 4);
+      fv[j] = fac[((size_t)d * A.nnmax + ri[j].g[d]) * (W * 4) + r];
+    }
+    const uint16_t *pat = pinv + (size_t)ri[j].pid * A.stride;
+#pragma unroll
+    for (int k = 0; k < MAXE; ++k) {
+      const int e = lane + 64 * k;
+      s3v[j][k] = e < ri[j].len ? pat[e] : 0;
+    }
   }
-  const ElastPart &P = parts[lo];
-  const long long li = node - P.rowinfo_base;
-  const int g[3] = {P.onb_lo[0] + (int)(li % P.onb_n[0]), P.onb_lo[1] + (int)((li / P.onb_n[0]) % P.onb_n[1]),
-                    P.onb_lo[2] + (int)(li / ((long long)P.onb_n[0] * P.onb_n[1]))};
-  double *F = lds[wave];
-  if (lane < NF) {
-    const int d = lane / (W * 4), r = lane % (W * 4);
-    F[lane] = fac[((size_t)d * A.nnmax + g[d]) * (W * 4) + r];
-  }
-  const int len = ri.lenf & 0xFFFF;
-  if (lane < 3) {
-    const double m = mass[g[0]] * mass[A.nnmax + g[1]] * mass[2 * A.nnmax + g[2]];
-    b[ri.bidx + lane] = A.f[lane] * m;
-  }
+#pragma unroll
+  for (int j = 0; j < NPW; ++j)
+    if (j < nn && lane < 3) {
+      const double m = mass[ri[j].g[0]] * mass[A.nnmax + ri[j].g[1]] * mass[2 * A.nnmax + ri[j].g[2]];
+      b[ri[j].bidx + lane] = A.f[lane] * m;
+    }
+#pragma unroll
+  for (int j = 0; j < NPW; ++j)
+    if (lane < NF) lds[wave][j][lane] = fv[j];
   __builtin_amdgcn_wave_barrier();  // LDS written and read by the same wave
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const uint16_t *pat = pinv + (size_t)ri.pid * A.stride;
-  double *row0 = values + ri.rowstart;
   const double lam = A.lam, mu = A.mu, lpm = A.lam + A.mu;
-  for (int e = lane; e < len; e += 64) {
-    const int s3 = pat[e];
-    const int s = s3 / 3, bc = s3 - 3 * s;
-    const int o0 = s % W, o1 = (s / W) % W, o2 = s / (W * W);
-    const double2 *f0 = reinterpret_cast<const double2 *>(F + (0 * W + o0) * 4);
-    const double2 *f1 = reinterpret_cast<const double2 *>(F + (1 * W + o1) * 4);
-    const double2 *f2 = reinterpret_cast<const double2 *>(F + (2 * W + o2) * 4);
-    const double2 km0 = f0[0], pq0 = f0[1], km1 = f1[0], pq1 = f1[1], km2 = f2[0], pq2 = f2[1];
-    const double Kd[3] = {km0.x, km1.x, km2.x}, Md[3] = {km0.y, km1.y, km2.y};
-    const double Pd[3] = {pq0.x, pq1.x, pq2.x}, Qd[3] = {pq0.y, pq1.y, pq2.y};
-    const double lap = mu * (Kd[0] * Md[1] * Md[2] + Md[0] * Kd[1] * Md[2] + Md[0] * Md[1] * Kd[2]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double v;
-      if (a == bc) {
-        const int c1 = (a + 1) % 3, c2 = (a + 2) % 3;
-        v = lap + lpm * Kd[a] * Md[c1] * Md[c2];
-      } else {
-        const int c = 3 - a - bc;
-        v = (lam * Pd[a] * Qd[bc] + mu * Qd[a] * Pd[bc]) * Md[c];
+  for (int j = 0; j < NPW; ++j) {
+    if (j >= nn) break;
+    const double *F = lds[wave][j];
+    const int len = ri[j].len;
+    double *row0 = values + ri[j].rowstart;
+#pragma unroll
+    for (int k = 0; k < MAXE; ++k) {
+      const int e = lane + 64 * k;
+      if (e >= len) break;
+      const int s3 = s3v[j][k];
+      const int s = s3 / 3, bc = s3 - 3 * s;
+      const int o0 = s % W, o1 = (s / W) % W, o2 = s / (W * W);
+      const double2 *f0 = reinterpret_cast<const double2 *>(F + (0 * W + o0) * 4);
+      const double2 *f1 = reinterpret_cast<const double2 *>(F + (1 * W + o1) * 4);
+      const double2 *f2 = reinterpret_cast<const double2 *>(F + (2 * W + o2) * 4);
+      const double2 km0 = f0[0], pq0 = f0[1], km1 = f1[0], pq1 = f1[1], km2 = f2[0], pq2 = f2[1];
+      const double Kd[3] = {km0.x, km1.x, km2.x}, Md[3] = {km0.y, km1.y, km2.y};
+      const double Pd[3] = {pq0.x, pq1.x, pq2.x}, Qd[3] = {pq0.y, pq1.y, pq2.y};
+      const double lap = mu * (Kd[0] * Md[1] * Md[2] + Md[0] * Kd[1] * Md[2] + Md[0] * Md[1] * Kd[2]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double v;
+        if (a == bc) {
+          const int c1 = (a + 1) % 3, c2 = (a + 2) % 3;
+          v = lap + lpm * Kd[a] * Md[c1] * Md[c2];
+        } else {
+          const int c = 3 - a - bc;
+          v = (lam * Pd[a] * Qd[bc] + mu * Qd[a] * Pd[bc]) * Md[c];
+        }
+        row0[(size_t)a * len + e] = v;
       }
-      row0[(size_t)a * len + e] = v;
     }
   }
 }
@@ -132,7 +158,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastP
 void elast_free(gdx_problem *p) {
   ElastData *E = ed(p);
   if (!E) return;
-  void *ptrs[] = {E->fac, E->mass, E->pinv, E->parts};
+  void *ptrs[] = {E->fac, E->mass, E->pinv, E->rows};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete E;
@@ -147,6 +173,7 @@ int elast_setup(gdx_problem *p) {
   const int K = p->K, W = 2 * K + 1, NQ = p->NQ;
   int nnmax = 0;
   for (int d = 0; d < 3; ++d) nnmax = std::max(nnmax, kp.nn[d]);
+  if (nnmax > 65535) return GDX_ERR_UNSUPPORTED;  // 16-bit node coordinates in ElastRow
   std::vector<double> fac((size_t)3 * nnmax * W * 4, 0.0), mass((size_t)3 * nnmax, 0.0);
   for (int d = 0; d < 3; ++d) {
     const double h = kp.h[d];
@@ -172,24 +199,34 @@ int elast_setup(gdx_problem *p) {
         }
       }
   }
-  std::vector<ElastPart> parts(p->nparts);
-  long long nrow = 0;
+  // owned row nodes in the order of their rows in the values buffer, so that
+  // consecutive waves write consecutive memory
+  std::vector<ElastRow> rows;
+  rows.reserve(p->rowinfo.size());
   for (int r = 0; r < p->nparts; ++r) {
     const Part &P = p->P[r];
-    parts[r].rowinfo_base = P.rowinfo_base;
-    for (int k = 0; k < 3; ++k) {
-      parts[r].onb_lo[k] = (int)P.onb_lo[k];
-      parts[r].onb_n[k] = (int)P.onb_n[k];
+    const int64_t n = P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+    for (int64_t li = 0; li < n; ++li) {
+      const RowInfo &ri = p->rowinfo[P.rowinfo_base + li];
+      if (ri.pid < 0) continue;
+      ElastRow er{};
+      er.rowstart = ri.rowstart;
+      er.bidx = ri.bidx;
+      er.pid = ri.pid;
+      er.len = ri.lenf & 0xFFFF;
+      er.g[0] = (unsigned short)(P.onb_lo[0] + li % P.onb_n[0]);
+      er.g[1] = (unsigned short)(P.onb_lo[1] + (li / P.onb_n[0]) % P.onb_n[1]);
+      er.g[2] = (unsigned short)(P.onb_lo[2] + li / (P.onb_n[0] * P.onb_n[1]));
+      rows.push_back(er);
     }
-    nrow += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
   }
-  if (nrow != (long long)p->rowinfo.size()) return GDX_ERR_HIP;
+  std::sort(rows.begin(), rows.end(),
+            [](const ElastRow &x, const ElastRow &y) { return x.rowstart < y.rowstart; });
   ElastData *E = new ElastData;
   p->elast = E;
-  E->nparts = p->nparts;
   E->nnmax = nnmax;
   E->stride = p->pat16_stride;
-  E->nrownodes = nrow;
+  E->nrownodes = (long long)rows.size();
   auto up = [](void **dst, const void *src, size_t bytes) {
     if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
     return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -197,7 +234,7 @@ int elast_setup(gdx_problem *p) {
   bool ok = up(reinterpret_cast<void **>(&E->fac), fac.data(), fac.size() * sizeof(double)) &&
             up(reinterpret_cast<void **>(&E->mass), mass.data(), mass.size() * sizeof(double)) &&
             up(reinterpret_cast<void **>(&E->pinv), p->pat16_inv.data(), p->pat16_inv.size() * sizeof(uint16_t)) &&
-            up(reinterpret_cast<void **>(&E->parts), parts.data(), parts.size() * sizeof(ElastPart));
+            up(reinterpret_cast<void **>(&E->rows), rows.data(), rows.size() * sizeof(ElastRow));
   if (!ok) {
     elast_free(p);
     return GDX_ERR_ALLOC;
@@ -215,16 +252,16 @@ int elast_assemble(gdx_problem *p, hipStream_t st) {
   for (int k = 0; k < 3; ++k) A.f[k] = p->desc.body_force[k];
   A.nnmax = E->nnmax;
   A.stride = E->stride;
-  A.nparts = E->nparts;
   A.nrownodes = E->nrownodes;
-  const long long blocks = (E->nrownodes + kWaves - 1) / kWaves;
+  const long long per = (long long)kWaves * kNodesPerWave;
+  const long long blocks = (E->nrownodes + per - 1) / per;
   if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
   const dim3 grid((unsigned)blocks), block(64 * kWaves);
   if (p->K == 2)
-    hipLaunchKernelGGL(k_elast<2>, grid, block, 0, st, A, E->parts, p->dev.rowinfo, E->fac, E->mass, E->pinv,
+    hipLaunchKernelGGL((k_elast<2, kNodesPerWave>), grid, block, 0, st, A, E->rows, E->fac, E->mass, E->pinv,
                        p->dev.values, p->dev.b);
   else
-    hipLaunchKernelGGL(k_elast<1>, grid, block, 0, st, A, E->parts, p->dev.rowinfo, E->fac, E->mass, E->pinv,
+    hipLaunchKernelGGL((k_elast<1, kNodesPerWave>), grid, block, 0, st, A, E->rows, E->fac, E->mass, E->pinv,
                        p->dev.values, p->dev.b);
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }

@@ -30,6 +30,7 @@ struct GdxProblemDesc
   coef_const::Float64; nmodes::Int32; modes::Ptr{Float64}
   rhs_kind::Int32; rhs_const::Float64; dir_kind::Int32; dir_coef::NTuple{4,Float64}
   op_kind::Int32; flux_p::Float64; flux_alpha::Float64; flux_beta::Float64
+  lame_lambda::Float64; lame_mu::Float64; body_force::NTuple{3,Float64}   # op_kind 2: linear elasticity
 end
 
 # gdx_newton_opts / gdx_newton_report
