@@ -401,7 +401,9 @@ def bench_batched(args):
            "nnz_owned_total": nnz, "nnz_owned_q1": nnz_q1, "symbolic_s_q1_total": round(t_sym, 3),
            "ms_eager_one_by_one": round(ms_eager, 4), "speedup_graph_vs_eager": round(ms_eager / ms_graph, 3),
            "kernel_chosen_q1": {"z-marching 16": int((chosen == 16).sum()), "z-marching 32": int((chosen == 32).sum()),
-                                "z-marching 64": int((chosen == 64).sum()), "per-tile": int((chosen < 0).sum())}}
+                                "z-marching 64": int((chosen == 64).sum()), "per-tile": int((chosen == -1).sum()),
+                                "per-tile direct-store": int((chosen == -2).sum()),
+                                "per-tile wave-slab": int((chosen == -3).sum())}}
     out = {
         "metric": "batched assembly Gnnz/s (FP64), 32 Q1 problems" + ("" if el is None else " + elasticity Q2") +
                   " per hipGraph launch",
@@ -474,13 +476,14 @@ def main():
     prob = assem._problem_for(data)  # symbolic phase, once
     t_symbolic = time.perf_counter() - t0
     nnz = prob.owned_nnz()
-    # autotuner: z-marching columns of 16/32/64 rows or the per-tile kernel
+    # autotuner: z-marching columns of 16/32/64 rows, the per-tile kernel or its direct-store / wave-slab forms
     tuner = G.AssemblyBatch(ctx)
     tuner.add(prob)
     chosen = int(tuner.autotune(reps=3)[0])
     tuner.close()
     variant, rows_z = prob.get_kernel()
-    kernel = (f"k_march3d<2,3,4,8> ({rows_z} z rows per workgroup)" if variant == 0 else "k_assemble3d<2,3,4,8,8>")
+    kernel = {0: f"k_march3d<2,3,4,8> ({rows_z} z rows per workgroup)", 1: "k_assemble3d<2,3,4,8,8>",
+              2: "k_assemble3d_direct<2,3,4,8,8>", 3: "k_assemble3d_wslab<3,8>"}[variant]
 
     for _ in range(args.warmup):
         prob.numeric()
@@ -506,7 +509,7 @@ def main():
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     # (copy-calibrated) on this workload; only valid at the default size
     traffic = None
-    tname = "k_march3d" if variant == 0 else "k_assemble3d"
+    tname = {0: "k_march3d", 1: "k_assemble3d", 2: "k_assemble3d_direct", 3: "k_assemble3d_wslab"}[variant]
     tpath = os.path.join(ROOT, "profiles", f"traffic_{tname}_poisson3d_q2_64.json")
     if args.cells == 64 and os.path.exists(tpath):
         with open(tpath) as fh:
@@ -531,7 +534,9 @@ def main():
             "free_dofs": prob.ngdofs,
             "parts": 8,
             "symbolic_s": round(t_symbolic, 3),
-            "autotuned": {"variant": "z-marching" if variant == 0 else "per-tile", "z_rows_per_workgroup": rows_z},
+            "autotuned": {"variant": {0: "z-marching", 1: "per-tile", 2: "per-tile direct-store",
+                                      3: "per-tile wave-slab"}[variant],
+                          "z_rows_per_workgroup": rows_z},
             "mixed_fp32": mixed,
             "kernel_ms": {"coefficients": round(coef_ms, 4), "rhs": round(rhs_ms, 4), "matrix": round(mat_ms, 4),
                           "step_events": round(tot_ms, 4)},

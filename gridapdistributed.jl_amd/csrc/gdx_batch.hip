@@ -4,15 +4,16 @@
 //   gdx_batch_add       problems after their symbolic phase, on one context
 //   gdx_batch_autotune  per 3D problem, times the z-marching matrix kernel
 //                       with columns of 16, 32 and 64 z rows per workgroup
-//                       (cells per block) and the per-tile kernel, and keeps
-//                       the fastest kernel and tiling
+//                       (cells per block), the per-tile kernel and its
+//                       direct-store and wave-slab forms, and keeps the fastest
 //   gdx_batch_capture   records every problem's numeric launches (coefficient,
 //                       RHS and matrix kernels; no memsets) into one hipGraph
 //   gdx_batch_launch    replays it on the context's stream
 //
-// The tiling only splits the rows over workgroups: each row's values are
-// computed from the same data in the same order, so every tiling gives
-// bit-identical matrices.
+// The z-marching column length only splits the rows over workgroups: each
+// row's values are computed from the same data in the same order, so those
+// tilings give bit-identical matrices; the per-tile kernels contract in
+// another order and agree to rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -82,9 +83,11 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
     p->precision = GDX_PREC_FP64;  // the choice is made for (and applies to) the FP64 form
     if (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION) {
       float best_ms = 1e30f;
-      // candidates: z-marching columns of 16, 32, 64 z rows; the per-tile kernel (rows_z = 0)
-      for (int r3 : {16, 32, 64, 0}) {
-        const int var = r3 == 0 ? 1 : 0;
+      // candidates: z-marching columns of 16, 32, 64 z rows; the per-tile kernel
+      // (-1), its direct-store form (-2) and its wave-slab form (-3, Q2)
+      for (int r3 : {16, 32, 64, -1, -2, -3}) {
+        const int var = r3 < 0 ? -r3 : 0;
+        if (var == 3 && !gdx::wslab_supported(p->K, p->NQ)) continue;
         if ((rc = gdx::set_kernel_variant(p, var, r3))) break;
         if ((rc = gdx::numeric_launches(p, st))) break;  // warm-up
         if (hipEventRecord(e0, st) != hipSuccess) rc = GDX_ERR_HIP;
@@ -99,7 +102,7 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
       }
       if (!rc) rc = gdx::set_kernel_variant(p, best_var, best);
     }
-    if (chosen) chosen[i] = best_var == 1 ? -best : best;
+    if (chosen) chosen[i] = best_var == 0 ? best : -best_var;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -118,6 +118,8 @@ int check_desc(const gdx_problem_desc *d) {
 void order_tiles(gdx_problem *p) {
   const int cap = p->D != 3             ? 0
                   : p->kvariant3d == 1 ? gdx::pattern_capacity_tile3d(p->K, p->NQ)
+                  : p->kvariant3d == 2 ? gdx::pattern_capacity_tile3d_direct(p->K, p->NQ)
+                  : p->kvariant3d == 3 ? gdx::pattern_capacity_tile3d_wslab(p->K, p->NQ)
                                        : gdx::pattern_capacity_3d(p->K, p->NQ);
   std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
   p->n_fast_tiles = 0;
@@ -312,11 +314,13 @@ int retile(gdx_problem *p, const int R[3]) {
 }
 
 // 3D kernel variant: 0 = z-marching columns with rows_z z rows per workgroup,
-// 1 = per-tile kernel with its own tile shape (rows_z ignored)
+// 1 = per-tile kernel, 2 = per-tile direct-store kernel, 3 = per-tile
+// wave-slab kernel (Q2 only); 1-3 use their own tile shape (rows_z ignored)
 int set_kernel_variant(gdx_problem *p, int variant, int rows_z) {
   if (p->D != 3) return GDX_OK;
+  if (variant == 3 && !wslab_supported(p->K, p->NQ)) return GDX_ERR_UNSUPPORTED;
   int R[3];
-  if (variant == 1) {
+  if (variant >= 1) {
     tile3d_shape(p->K, R);
   } else {
     tile_shape(3, p->K, R);
@@ -398,9 +402,12 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
 int gdx_problem_set_kernel(gdx_problem *p, int32_t variant, int32_t rows_z) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done && p->ctx, GDX_ERR_STATE, "needs gdx_symbolic and a device context");
-  GDX_REQUIRE(variant == 0 || variant == 1, GDX_ERR_ARG, "variant must be 0 (z-marching) or 1 (per tile)");
+  GDX_REQUIRE(variant >= 0 && variant <= 3, GDX_ERR_ARG,
+              "variant must be 0 (z-marching), 1 (per tile), 2 (per tile, direct stores) or 3 (per tile, wave slabs)");
+  GDX_REQUIRE(variant != 3 || gdx::wslab_supported(p->K, p->NQ), GDX_ERR_UNSUPPORTED,
+              "the wave-slab kernel is built for Q2");
   GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION, GDX_ERR_UNSUPPORTED, "kernel variants are for the diffusion operator");
-  GDX_REQUIRE(variant == 1 || (rows_z >= p->K && rows_z % p->K == 0 && rows_z <= 4096), GDX_ERR_ARG,
+  GDX_REQUIRE(variant != 0 || (rows_z >= p->K && rows_z % p->K == 0 && rows_z <= 4096), GDX_ERR_ARG,
               "rows_z must be a positive multiple of the order");
   GDX_HIP(hipSetDevice(p->ctx->device));
   GDX_HIP(hipStreamSynchronize(p->ctx->stream));

@@ -95,6 +95,16 @@ __device__ __forceinline__ void pair_factors(const KParams &kp, int d, int a0, i
   M = kp.fM[d][a][b][q];
 }
 
+// weight of region quadrature point lcq (cell lc = lcq/NQ, point t) in the
+// source integral of tile node u along direction d: w_t h_d phi_a(t)
+template <int K, int NQ>
+__device__ __forceinline__ double node_factor(const KParams &kp, int d, int u, int lcq) {
+  const int v = u + K, lc = lcq / NQ, t = lcq % NQ;
+  const int a = v - K * lc;
+  if (a < 0 || a > K) return 0.0;
+  return kp.qw[t] * kp.h[d] * kp.phi[a][t];
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;

@@ -145,7 +145,8 @@ struct gdx_problem {
   std::vector<gdx::Tile> tiles;
   std::vector<int> tile_pats;
   int n_fast_tiles = 0;  // leading tiles whose position patterns fit in LDS
-  int kvariant3d = 0;    // 3D matrix kernel: 0 z-marching columns, 1 per-tile (chosen by the autotuner)
+  int kvariant3d = 0;    // 3D matrix kernel: 0 z-marching columns, 1 per-tile, 2 per-tile direct-store,
+                         // 3 per-tile wave-slab (Q2); chosen by the autotuner
   int precision = 0;     // contraction precision of the current numeric phase (GDX_PREC_*)
   int tileR[3] = {1, 1, 1};
   int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
@@ -183,6 +184,11 @@ int pattern_capacity_3d(int K, int NQ);
 int pattern_capacity_tile3d(int K, int NQ);
 void tile3d_shape(int K, int R[3]);
 int launch_matrix_tile3d(gdx_problem *p, hipStream_t st);
+int pattern_capacity_tile3d_direct(int K, int NQ);
+int launch_matrix_tile3d_direct(gdx_problem *p, hipStream_t st);
+bool wslab_supported(int K, int NQ);
+int pattern_capacity_tile3d_wslab(int K, int NQ);
+int launch_matrix_tile3d_wslab(gdx_problem *p, hipStream_t st);
 int set_kernel_variant(gdx_problem *p, int variant, int rows_z);  // retiles for the variant
 void solve_free(gdx_problem *p);
 int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_report *rep);

@@ -61,7 +61,7 @@ __global__ void k_fourier_tables(const double *__restrict__ modes, KParams kp, i
 // in registers across the lines; the per-line (y,z) products b = e2 e3 and
 // (k.c) b are tabulated in LDS, so each (point, mode) costs four FMAs:
 //   g += Re(e1 b),  s += Im(e1 (k.c) b).
-constexpr int kCoefLines = 16, kCoefModes = 16;
+constexpr int kCoefLines = 8, kCoefModes = 16;
 
 __global__ __launch_bounds__(256) void k_coefficients(const double *__restrict__ modes,
                                                       const double *__restrict__ E, KParams kp, int qnmax,
@@ -136,14 +136,6 @@ struct RhsTile {
   static constexpr int NT = 256;
 };
 
-template <int K, int NQ>
-__device__ __forceinline__ double node_factor(const KParams &kp, int d, int u, int lcq) {
-  // contribution of region qp index lcq (cell lc = lcq/NQ, point t) to node u
-  const int v = u + K, lc = lcq / NQ, t = lcq % NQ;
-  const int a = v - K * lc;
-  if (a < 0 || a > K) return 0.0;
-  return kp.qw[t] * kp.h[d] * kp.phi[a][t];
-}
 
 template <int DIM, int K, int NQ, int R1, int R2, int R3>
 __global__ __launch_bounds__(256) void k_rhs(const Tile *__restrict__ tiles, int ntiles,
@@ -343,7 +335,7 @@ int launch_rhs(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    if (p->kvariant3d == 0) return GDX_OK;  // the z-marching matrix kernel writes b itself
+    if (p->kvariant3d != 1) return GDX_OK;  // the z-marching and direct-store kernels write b themselves
     if (K == 2 && NQ == 3) rhs_run<3, 2, 3, 4, 8, 8>(p, st);
     else if (K == 2 && NQ == 2) rhs_run<3, 2, 2, 4, 8, 8>(p, st);
     else if (K == 1 && NQ == 2) rhs_run<3, 1, 2, 8, 8, 2>(p, st);
@@ -364,6 +356,8 @@ int launch_matrix(gdx_problem *p, hipStream_t st) {
   if (p->D == 3) {
     // the FP32 form exists for the z-marching kernel only (its tiles and pattern capacity)
     if (p->precision == GDX_PREC_FP32 && p->kvariant3d == 0) return launch_matrix3d(p, st, GDX_PREC_FP32);
+    if (p->kvariant3d == 2) return launch_matrix_tile3d_direct(p, st);
+    if (p->kvariant3d == 3) return launch_matrix_tile3d_wslab(p, st);
     return p->kvariant3d == 1 ? launch_matrix_tile3d(p, st) : launch_matrix3d(p, st, GDX_PREC_FP64);
   } else {
     if (K == 2 && NQ == 3) run2<2, 3, 16, 16>(p, st);

@@ -296,7 +296,8 @@ class GpuProblem:
         _lib.call("gdx_set_state", self.handle, x.ctypes.data)
 
     def set_kernel(self, variant, rows_z=32):
-        """3D matrix kernel: 0 z-marching columns (rows_z z rows per workgroup), 1 per tile."""
+        """3D matrix kernel: 0 z-marching columns (rows_z z rows per workgroup), 1 per tile,
+        2 per tile with direct stores, 3 per tile with wave-owned rows (Q2)."""
         _lib.call("gdx_problem_set_kernel", self.handle, int(variant), int(rows_z))
 
     def get_kernel(self):

@@ -151,8 +151,10 @@ int gdx_symbolic(gdx_problem *prob);
 int gdx_numeric(gdx_problem *prob, int32_t precision);
 
 /* 3D matrix kernel of a problem: variant 0 = z-marching columns with rows_z z
-   rows per workgroup (default 32), 1 = per-tile kernel (rows_z ignored).  Both
-   give the same matrix to rounding; gdx_batch_autotune picks the fastest. */
+   rows per workgroup (default 32), 1 = per-tile kernel, 2 = per-tile kernel
+   storing straight from registers, 3 = per-tile kernel whose waves own whole
+   rows (Q2 only; rows_z ignored for 1-3).  All give the same matrix to
+   rounding; gdx_batch_autotune picks the fastest. */
 int gdx_problem_set_kernel(gdx_problem *prob, int32_t variant, int32_t rows_z);
 int gdx_problem_get_kernel(gdx_problem *prob, int32_t *variant, int32_t *rows_z);
 
@@ -276,10 +278,11 @@ int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int3
 /* Batched numeric assembly: independent problems of uneven sizes and
    partitions (each after gdx_symbolic, all on one context) assembled by one
    hipGraph launch.  gdx_batch_autotune times each 3D problem's matrix kernels
-   (z-marching columns of 16, 32 and 64 z rows per workgroup, and the per-tile
-   kernel) and keeps the fastest (chosen, per problem, may be NULL: the z rows
-   per workgroup, negated for the per-tile kernel); results are bit-identical
-   for every choice.  gdx_batch_capture records the numeric launches of every problem
+   (z-marching columns of 16, 32 and 64 z rows per workgroup, the per-tile
+   kernel and its direct-store form) and keeps the fastest (chosen, per
+   problem, may be NULL: the z rows per workgroup, -1 for the per-tile kernel,
+   -2 for the direct-store form, -3 for the wave-slab form); the matrices
+   agree to rounding for every choice.  gdx_batch_capture records the numeric launches of every problem
    (no memsets) and gdx_batch_launch replays them on the context's stream.
    The reference has no counterpart: it assembles one problem per call
    (src/FESpaces.jl:785-798). */
