@@ -57,6 +57,9 @@ int twogrid_apply(void *nn, const double *r, double *z, hipStream_t st);
 int twogrid_train_step(gdx_nn *nn, uint64_t seed, float lr, int smooth_passes, float *loss);
 gdx_problem *twogrid_problem(gdx_nn *nn);
 void twogrid_options(gdx_nn *nn, int sweeps, bool use_net);
+void twogrid_multigrid(gdx_nn *nn, bool on, int nu);
+int twogrid_levels(gdx_nn *nn);
+int twogrid_export_level(gdx_nn *nn, int l, int *m, double *h);
 int twogrid_export_coarse(gdx_nn *nn, double *host);
 }  // namespace gdx
 

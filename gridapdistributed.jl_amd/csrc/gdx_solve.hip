@@ -466,7 +466,9 @@ static int fpcg_run(gdx_problem *p, SolveData *s, const FpcgPrec *prec, const do
         return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
       });
       if (rc) return rc;
-      if (it % 5 == 0 || it == maxiter) {
+      // the residual is read back every iteration: a preconditioned iteration
+      // costs far more than the read, and solves take few iterations
+      {
         double h[2];
         if (hipMemcpyAsync(h, s->sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "gdx_nn.h"
 
@@ -36,6 +37,15 @@ constexpr int kRed = 256;  // partial sums per reduction
 constexpr int kS = 3 * kRed;  // scalar slots of 'red' after three partial-sum areas
 constexpr double kOmega = 2.0 / 3.0;
 constexpr double kOmegaC = 2.0 / 3.0;  // coarse sweeps
+
+// A structured coarse level: m cells per direction, unknowns at the interior
+// nodes 1..m-1 stored at [0, m-1) of an m^3 array (the last plane is padding).
+struct Level {
+  int m = 0;
+  double *cst = nullptr;  // Galerkin operator as a 27-point stencil [27][m^3]
+  bool own_cst = true;    // false for the network grid (its stencil is TwoGrid::cst)
+  double *rhs = nullptr, *e = nullptr, *e2 = nullptr, *res = nullptr;  // [m^3]
+};
 
 struct TwoGrid {
   gdx_problem *prob = nullptr;
@@ -54,8 +64,14 @@ struct TwoGrid {
   double *cst = nullptr;   // Galerkin coarse operator P^T A P as a stencil [27][n^3]
   double *rc = nullptr;    // coarse residual R s [n^3]
   double *ca = nullptr, *cb = nullptr;  // coarse correction (ping-pong) [n^3]
-  int sweeps = 8;          // coarse damped-Jacobi sweeps after the network
+  int sweeps = 8;          // coarse damped-Jacobi sweeps after the network (two-grid form)
   bool use_net = true;     // false: the coarse correction starts from zero
+  // multigrid form: a V-cycle from the network grid down to a grid of <= 8^3
+  // cells replaces the coarse sweeps; lv[0] is the network grid
+  bool mg = true;
+  int nu = 1;              // damped-Jacobi sweeps before and after each coarse-grid correction
+  int coarsest_sweeps = 24;
+  std::vector<Level> lv;
 };
 
 TwoGrid *tg_of(gdx_nn *nn) { return reinterpret_cast<TwoGrid *>(nn->tg); }
@@ -112,53 +128,52 @@ __global__ void k_fine_stencil(int ng, int nf, const long long *__restrict__ rst
   }
 }
 
-// per component: the coarse offsets d (and weights) of the coarse nodes J = I + d
-// whose hat function covers fine offset t = b - 2I in [-2, 2]
-__device__ __forceinline__ int cover(int t, int *d, double *w) {
-  switch (t) {
-    case -2: d[0] = -1; w[0] = 1.0; return 1;
-    case -1: d[0] = -1; w[0] = 0.5; d[1] = 0; w[1] = 0.5; return 2;
-    case 0: d[0] = 0; w[0] = 1.0; return 1;
-    case 1: d[0] = 0; w[0] = 0.5; d[1] = 1; w[1] = 0.5; return 2;
-    default: d[0] = 1; w[0] = 1.0; return 1;
-  }
-}
-
-// Galerkin coarse operator Ac = P^T A P as a 27-point stencil on the network
-// grid: cst[slot(d)][I - 1] = sum_a sum_b P(a, I) A(a, b) P(b, I + d), with
-// entries to boundary coarse nodes (and on the padding plane) zero
-__global__ void k_galerkin(int n, int nf, const double *__restrict__ fst, double *__restrict__ cst) {
-  const int64_t tot = (int64_t)n * n * n;
-  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
-  if (j >= tot) return;
-  const int jx = (int)(j % n), jy = (int)((j / n) % n), jz = (int)(j / ((int64_t)n * n));
-  double acc[27];
-  for (int k = 0; k < 27; ++k) acc[k] = 0.0;
-  const int64_t nfl = (int64_t)nf * nf * nf;
-  if (jx < n - 1 && jy < n - 1 && jz < n - 1) {
-    const int Ix = jx + 1, Iy = jy + 1, Iz = jz + 1;
-    for (int oa = 0; oa < 27; ++oa) {
-      const int ox = oa % 3 - 1, oy = (oa / 3) % 3 - 1, oz = oa / 9 - 1;
-      const double wa = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
-      const int64_t a = (2 * Ix + ox) + (int64_t)nf * ((2 * Iy + oy) + (int64_t)nf * (2 * Iz + oz));
-      for (int e = 0; e < 27; ++e) {
-        const double A = fst[e * nfl + a];
-        if (A == 0.0) continue;
-        const int tx = ox + e % 3 - 1, ty = oy + (e / 3) % 3 - 1, tz = oz + e / 9 - 1;
-        int dx[2], dy[2], dz[2];
-        double wx[2], wy[2], wz[2];
-        const int nx = cover(tx, dx, wx), ny = cover(ty, dy, wy), nz = cover(tz, dz, wz);
-        for (int i = 0; i < nx; ++i)
-          for (int k = 0; k < ny; ++k)
-            for (int l = 0; l < nz; ++l) {
-              const int Jx = Ix + dx[i], Jy = Iy + dy[k], Jz = Iz + dz[l];
-              if (Jx < 1 || Jy < 1 || Jz < 1 || Jx > n - 1 || Jy > n - 1 || Jz > n - 1) continue;
-              acc[(dx[i] + 1) + 3 * (dy[k] + 1) + 9 * (dz[l] + 1)] += wa * A * wx[i] * wy[k] * wz[l];
-            }
+// Galerkin product, one thread per output entry (coarse node J, offset d):
+//   Ac(J, J+d) = sum_{a = 2J+oa, b = 2(J+d)+ob} w(oa) A(a, b) w(ob)
+// Per direction only the (oa, ob) with |2d + ob - oa| <= 1 meet the 27-point
+// stencil of A: 7 pairs for d = 0, 3 for d = +-1, summed in a fixed order.
+// The fine operator is a stencil [27][.] on a grid of stride S whose node I
+// sits at index I - off (off = 0: the fine lattice, 1: a padded coarse level).
+__constant__ signed char kGalPairs[3][7][2] = {
+    {{-1, 0}, {-1, 1}, {0, 1}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},                  // d = -1
+    {{-1, -1}, {-1, 0}, {0, -1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}},                // d = 0
+    {{0, -1}, {1, -1}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}};                 // d = +1
+__global__ void k_galerkin_e(int mc, int S, int off, int64_t fine_tot, const double *__restrict__ af,
+                             double *__restrict__ cst) {
+  const int64_t tot = (int64_t)mc * mc * mc;
+  const int64_t t = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (t >= 27 * tot) return;
+  const int k = (int)(t / tot);  // output slot: consecutive threads share it (coalesced stores)
+  const int64_t j = t - k * tot;
+  const int jx = (int)(j % mc), jy = (int)((j / mc) % mc), jz = (int)(j / ((int64_t)mc * mc));
+  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+  const int Jx = jx + 1, Jy = jy + 1, Jz = jz + 1;
+  double acc = 0.0;
+  const bool live = jx < mc - 1 && jy < mc - 1 && jz < mc - 1 && Jx + dx >= 1 && Jy + dy >= 1 && Jz + dz >= 1 &&
+                    Jx + dx <= mc - 1 && Jy + dy <= mc - 1 && Jz + dz <= mc - 1;
+  if (live) {
+    const int nx = dx ? 3 : 7, ny = dy ? 3 : 7, nz = dz ? 3 : 7;
+    for (int pz = 0; pz < nz; ++pz) {
+      const int oaz = kGalPairs[dz + 1][pz][0], obz = kGalPairs[dz + 1][pz][1];
+      const double wz = (oaz ? 0.5 : 1.0) * (obz ? 0.5 : 1.0);
+      const int ez = 2 * dz + obz - oaz + 1;
+      const int64_t az = 2 * Jz + oaz - off;
+      for (int py = 0; py < ny; ++py) {
+        const int oay = kGalPairs[dy + 1][py][0], oby = kGalPairs[dy + 1][py][1];
+        const double wyz = wz * (oay ? 0.5 : 1.0) * (oby ? 0.5 : 1.0);
+        const int ey = 2 * dy + oby - oay + 1;
+        const int64_t ayz = (2 * Jy + oay - off) + (int64_t)S * az;
+        for (int px = 0; px < nx; ++px) {
+          const int oax = kGalPairs[dx + 1][px][0], obx = kGalPairs[dx + 1][px][1];
+          const double w = wyz * (oax ? 0.5 : 1.0) * (obx ? 0.5 : 1.0);
+          const int ex = 2 * dx + obx - oax + 1;
+          const int64_t a = (2 * Jx + oax - off) + (int64_t)S * ayz;
+          acc = fma(w, af[(int64_t)(ex + 3 * ey + 9 * ez) * fine_tot + a], acc);
+        }
       }
     }
   }
-  for (int k = 0; k < 27; ++k) cst[k * tot + j] = acc[k];
+  cst[k * tot + j] = acc;
 }
 
 // ec = s * out (network units -> coarse error), 0 on the padding plane; or 0 everywhere
@@ -303,6 +318,129 @@ __global__ void k_prolong_d(int n, int nf, const double *__restrict__ ec, double
         acc = fma(w, ec[(Ix - 1) + (int64_t)n * ((Iy - 1) + (int64_t)n * (Iz - 1))], acc);
       }
   flat[i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid on the structured coarse levels (padded interior storage)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool padding(int64_t j, int m, int &x, int &y, int &z) {
+  x = (int)(j % m);
+  y = (int)((j / m) % m);
+  z = (int)(j / ((int64_t)m * m));
+  return x == m - 1 || y == m - 1 || z == m - 1;
+}
+
+// res = rhs - A e
+__global__ void k_resid(int m, const double *__restrict__ cst, const double *__restrict__ rhs,
+                        const double *__restrict__ e, double *__restrict__ res) {
+  const int64_t tot = (int64_t)m * m * m;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  int jx, jy, jz;
+  if (padding(j, m, jx, jy, jz)) {
+    res[j] = 0.0;
+    return;
+  }
+  double r = rhs[j];
+  for (int k = 0; k < 27; ++k) {
+    const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
+    if (x < 0 || y < 0 || z < 0 || x > m - 2 || y > m - 2 || z > m - 2) continue;
+    r -= cst[k * tot + j] * e[x + (int64_t)m * (y + (int64_t)m * z)];
+  }
+  res[j] = r;
+}
+
+// first damped-Jacobi sweep from zero: e = w Dc^-1 rhs
+__global__ void k_sweep0(int m, double w, const double *__restrict__ cst, const double *__restrict__ rhs,
+                         double *__restrict__ e) {
+  const int64_t tot = (int64_t)m * m * m;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  int jx, jy, jz;
+  e[j] = padding(j, m, jx, jy, jz) ? 0.0 : w * rhs[j] / cst[13 * tot + j];
+}
+
+// full weighting from level mf to mf/2: rc[J] = sum_o w(o) r[2J + o]
+__global__ void k_restrict_c(int mf, const double *__restrict__ rf, double *__restrict__ rc) {
+  const int mc = mf / 2;
+  const int64_t tot = (int64_t)mc * mc * mc;
+  const int64_t j = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (j >= tot) return;
+  int jx, jy, jz;
+  if (padding(j, mc, jx, jy, jz)) {
+    rc[j] = 0.0;
+    return;
+  }
+  const int fx = 2 * (jx + 1) - 1, fy = 2 * (jy + 1) - 1, fz = 2 * (jz + 1) - 1;  // storage of fine node 2J
+  double acc = 0.0;
+  for (int oz = -1; oz <= 1; ++oz)
+    for (int oy = -1; oy <= 1; ++oy)
+      for (int ox = -1; ox <= 1; ++ox) {
+        const double w = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
+        acc = fma(w, rf[(fx + ox) + (int64_t)mf * ((fy + oy) + (int64_t)mf * (fz + oz))], acc);
+      }
+  rc[j] = acc;
+}
+
+// ef += P ec (trilinear, from level mf/2 to mf)
+__global__ void k_prolong_add_c(int mf, const double *__restrict__ ec, double *__restrict__ ef) {
+  const int mc = mf / 2;
+  const int64_t tot = (int64_t)mf * mf * mf;
+  const int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (i >= tot) return;
+  int ix, iy, iz;
+  if (padding(i, mf, ix, iy, iz)) return;
+  const int x = ix + 1, y = iy + 1, z = iz + 1;  // fine node
+  double acc = 0.0;
+  for (int kz = 0; kz < 1 + (z & 1); ++kz)
+    for (int ky = 0; ky < 1 + (y & 1); ++ky)
+      for (int kx = 0; kx < 1 + (x & 1); ++kx) {
+        const int Jx = (x >> 1) + kx, Jy = (y >> 1) + ky, Jz = (z >> 1) + kz;
+        if (Jx < 1 || Jy < 1 || Jz < 1 || Jx > mc - 1 || Jy > mc - 1 || Jz > mc - 1) continue;
+        const double w = ((x & 1) ? 0.5 : 1.0) * ((y & 1) ? 0.5 : 1.0) * ((z & 1) ? 0.5 : 1.0);
+        acc = fma(w, ec[(Jx - 1) + (int64_t)mc * ((Jy - 1) + (int64_t)mc * (Jz - 1))], acc);
+      }
+  ef[i] += acc;
+}
+
+// coarsest level (m <= 8): damped-Jacobi sweeps from zero in one workgroup;
+// the stencil is staged in LDS when it fits (m <= 6)
+constexpr int kCoarsestMax = 8;
+constexpr int kCoarsestLds = 6 * 6 * 6;
+__global__ __launch_bounds__(512) void k_coarsest(int m, double w, int sweeps, const double *__restrict__ cstg,
+                                                  const double *__restrict__ rhs, double *__restrict__ e) {
+  __shared__ double ea[512], eb[512], cl[27 * kCoarsestLds];
+  const int tot = m * m * m;
+  const int j = threadIdx.x;
+  int jx = 0, jy = 0, jz = 0;
+  const bool live = j < tot && !padding(j, m, jx, jy, jz);
+  const bool staged = tot <= kCoarsestLds;
+  if (staged)
+    for (int i = j; i < 27 * tot; i += blockDim.x) cl[i] = cstg[i];
+  const double *cst = staged ? cl : cstg;
+  if (j < tot) ea[j] = 0.0;
+  __syncthreads();
+  double *a = ea, *b = eb;
+  for (int s = 0; s < sweeps; ++s) {
+    if (j < tot) {
+      double v = 0.0;
+      if (live) {
+        double r = rhs[j];
+        for (int k = 0; k < 27; ++k) {
+          const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
+          if (x < 0 || y < 0 || z < 0 || x > m - 2 || y > m - 2 || z > m - 2) continue;
+          r -= cst[k * tot + j] * a[x + m * (y + m * z)];
+        }
+        v = a[j] + w * r / cst[13 * tot + j];
+      }
+      b[j] = v;
+    }
+    __syncthreads();
+    double *t = a;
+    a = b;
+    b = t;
+  }
+  if (j < tot) e[j] = a[j];
 }
 
 // partial sums of squares (and of a*b) over a grid-stride slice
@@ -549,6 +687,83 @@ void make_input(gdx_nn *nn, TwoGrid *T, const double *r, hipStream_t st, bool fo
   N.input_from_f32(N.inp_f);
 }
 
+void mg_free(TwoGrid *T) {
+  for (Level &L : T->lv) {
+    void *ptrs[] = {L.own_cst ? L.cst : nullptr, L.rhs, L.e, L.e2, L.res};
+    for (void *q : ptrs)
+      if (q) (void)hipFree(q);
+  }
+  T->lv.clear();
+}
+
+// levels below the network grid: halve while the cell count is even and > 4
+// (the coarsest grid has at most 8 cells per direction); buffers are kept
+// across attachments to the same problem, the operators are recomputed
+int mg_setup(TwoGrid *T, hipStream_t st) {
+  if (T->lv.empty()) {
+    std::vector<int> ms{T->n};
+    while (ms.back() % 2 == 0 && ms.back() > 4) ms.push_back(ms.back() / 2);
+    if (ms.back() > kCoarsestMax) ms.clear();  // no coarsest grid small enough: two-grid form only
+    for (size_t l = 0; l < ms.size(); ++l) {
+      Level L;
+      L.m = ms[l];
+      const size_t v = (size_t)L.m * L.m * L.m;
+      T->lv.push_back(L);
+      Level &M = T->lv.back();
+      bool ok = hipMalloc(reinterpret_cast<void **>(&M.rhs), v * sizeof(double)) == hipSuccess &&
+                hipMalloc(reinterpret_cast<void **>(&M.e), v * sizeof(double)) == hipSuccess &&
+                hipMalloc(reinterpret_cast<void **>(&M.e2), v * sizeof(double)) == hipSuccess &&
+                hipMalloc(reinterpret_cast<void **>(&M.res), v * sizeof(double)) == hipSuccess;
+      if (l == 0) {
+        M.cst = T->cst;
+        M.own_cst = false;
+      } else {
+        ok = ok && hipMalloc(reinterpret_cast<void **>(&M.cst), 27 * v * sizeof(double)) == hipSuccess;
+      }
+      if (!ok) {
+        mg_free(T);
+        return GDX_ERR_ALLOC;
+      }
+    }
+  }
+  for (size_t l = 1; l < T->lv.size(); ++l) {
+    const Level &F = T->lv[l - 1];
+    const int64_t vc = (int64_t)T->lv[l].m * T->lv[l].m * T->lv[l].m;
+    hipLaunchKernelGGL(k_galerkin_e, dim3(nb(27 * vc)), dim3(kB), 0, st, T->lv[l].m, F.m, 1,
+                       (int64_t)F.m * F.m * F.m, F.cst, T->lv[l].cst);
+  }
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+// V-cycle on level l for the right-hand side lv[l].rhs, from a zero guess:
+// nu damped-Jacobi sweeps, coarse-grid correction, nu sweeps (the coarsest
+// level: sweeps in one workgroup).  Returns the buffer holding the correction.
+double *vcycle(TwoGrid *T, size_t l, hipStream_t st) {
+  Level &L = T->lv[l];
+  const int64_t v = (int64_t)L.m * L.m * L.m;
+  if (l + 1 == T->lv.size()) {
+    hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(512), 0, st, L.m, kOmegaC, T->coarsest_sweeps, L.cst, L.rhs, L.e);
+    return L.e;
+  }
+  double *a = L.e, *b = L.e2;
+  hipLaunchKernelGGL(k_sweep0, dim3(nb(v)), dim3(kB), 0, st, L.m, kOmegaC, L.cst, L.rhs, a);
+  for (int k = 1; k < T->nu; ++k) {
+    hipLaunchKernelGGL(k_coarse_sweep, dim3(nb(v)), dim3(kB), 0, st, L.m, kOmegaC, L.cst, L.rhs, a, b);
+    std::swap(a, b);
+  }
+  hipLaunchKernelGGL(k_resid, dim3(nb(v)), dim3(kB), 0, st, L.m, L.cst, L.rhs, a, L.res);
+  Level &C = T->lv[l + 1];
+  const int64_t vc = (int64_t)C.m * C.m * C.m;
+  hipLaunchKernelGGL(k_restrict_c, dim3(nb(vc)), dim3(kB), 0, st, L.m, L.res, C.rhs);
+  const double *c = vcycle(T, l + 1, st);
+  hipLaunchKernelGGL(k_prolong_add_c, dim3(nb(v)), dim3(kB), 0, st, L.m, c, a);
+  for (int k = 0; k < T->nu; ++k) {
+    hipLaunchKernelGGL(k_coarse_sweep, dim3(nb(v)), dim3(kB), 0, st, L.m, kOmegaC, L.cst, L.rhs, a, b);
+    std::swap(a, b);
+  }
+  return a;
+}
+
 }  // namespace
 
 gdx_problem *twogrid_problem(gdx_nn *nn) { return tg_of(nn) ? tg_of(nn)->prob : nullptr; }
@@ -563,6 +778,21 @@ int twogrid_export_coarse(gdx_nn *nn, double *h) {
   return GDX_OK;
 }
 
+// level l of the V-cycle: its cell count m, and (h != NULL) its stencil [27][m^3]
+int twogrid_export_level(gdx_nn *nn, int l, int *m, double *h) {
+  TwoGrid *T = tg_of(nn);
+  if (l < 0 || l >= (int)T->lv.size()) return GDX_ERR_ARG;
+  const Level &L = T->lv[l];
+  *m = L.m;
+  if (!h) return GDX_OK;
+  const int64_t v = (int64_t)L.m * L.m * L.m;
+  hipStream_t st = T->prob->ctx->stream;
+  if (hipMemcpyAsync(h, L.cst, 27 * v * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GDX_ERR_HIP;
+  return GDX_OK;
+}
+
 void twogrid_options(gdx_nn *nn, int sweeps, bool use_net) {
   if (TwoGrid *T = tg_of(nn)) {
     T->sweeps = sweeps;
@@ -570,9 +800,22 @@ void twogrid_options(gdx_nn *nn, int sweeps, bool use_net) {
   }
 }
 
+void twogrid_multigrid(gdx_nn *nn, bool on, int nu) {
+  if (TwoGrid *T = tg_of(nn)) {
+    T->mg = on;
+    T->nu = nu;
+  }
+}
+
+int twogrid_levels(gdx_nn *nn) {
+  TwoGrid *T = tg_of(nn);
+  return T ? (int)T->lv.size() : 0;
+}
+
 void twogrid_free(gdx_nn *nn) {
   TwoGrid *T = tg_of(nn);
   if (!T) return;
+  mg_free(T);
   void *ptrs[] = {T->gid_node, T->flat, T->dlat, T->t1, T->t2, T->t3, T->rho, T->red, T->ec, T->ec2, T->tgt,
                   T->fst, T->cst, T->rc, T->ca, T->cb};
   for (void *q : ptrs)
@@ -652,7 +895,8 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
   if (hipMemsetAsync(T->fst, 0, 27 * nfl * sizeof(double), st) != hipSuccess) return GDX_ERR_HIP;
   hipLaunchKernelGGL(k_fine_stencil, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, nf, rs, rl, cg, p->dev.values,
                      T->gid_node, T->fst);
-  hipLaunchKernelGGL(k_galerkin, dim3(nb(v)), dim3(kB), 0, st, n, nf, T->fst, T->cst);
+  hipLaunchKernelGGL(k_galerkin_e, dim3(nb(27 * v)), dim3(kB), 0, st, n, nf, 0, nfl, T->fst, T->cst);
+  if ((rc = mg_setup(T, st))) return rc;
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
@@ -685,9 +929,16 @@ int twogrid_apply(void *ctx, const double *r, double *z, hipStream_t st) {
     hipLaunchKernelGGL(k_scale_by, dim3(nb(v)), dim3(kB), 0, st, v, T->red + kS + 6, T->ca);
   }
   double *ea = T->ca, *eb = T->cb;
-  for (int k = 0; k < T->sweeps; ++k) {
-    hipLaunchKernelGGL(k_coarse_sweep, dim3(nb(v)), dim3(kB), 0, st, T->n, kOmegaC, T->cst, T->rc, ea, eb);
-    std::swap(ea, eb);
+  if (T->mg && T->lv.size() > 1) {
+    // V-cycle on the coarse residual of the (network's or zero) guess
+    hipLaunchKernelGGL(k_resid, dim3(nb(v)), dim3(kB), 0, st, T->n, T->cst, T->rc, ea, T->lv[0].rhs);
+    const double *d = vcycle(T, 0, st);
+    hipLaunchKernelGGL(k_add, dim3(nb(v)), dim3(kB), 0, st, (int)v, ea, d);
+  } else {
+    for (int k = 0; k < T->sweeps; ++k) {
+      hipLaunchKernelGGL(k_coarse_sweep, dim3(nb(v)), dim3(kB), 0, st, T->n, kOmegaC, T->cst, T->rc, ea, eb);
+      std::swap(ea, eb);
+    }
   }
   hipLaunchKernelGGL(k_prolong_d, dim3(nb(nfl)), dim3(kB), 0, st, T->n, T->nf, ea, T->flat);
   hipLaunchKernelGGL(k_gather, dim3(nb(ng)), dim3(kB), 0, st, ng, T->gid_node, T->flat, T->t2);  // c
@@ -809,6 +1060,37 @@ int gdx_nn_coarse_operator(gdx_nn *nn, double *stencil) {
   }
   if (hipSetDevice(nn->ctx->device) != hipSuccess) return GDX_ERR_HIP;
   return gdx::twogrid_export_coarse(nn, stencil);
+}
+
+int gdx_nn_set_multigrid(gdx_nn *nn, int32_t enable, int32_t nu, int32_t *levels) {
+  if (!nn || nu < 1 || nu > 8) {
+    gdx::set_error("invalid argument");
+    return GDX_ERR_ARG;
+  }
+  if (!nn->tg) {
+    gdx::set_error("gdx_nn_attach has not run");
+    return GDX_ERR_STATE;
+  }
+  gdx::twogrid_multigrid(nn, enable != 0, nu);
+  if (levels) *levels = gdx::twogrid_levels(nn);
+  return GDX_OK;
+}
+
+int gdx_nn_level_operator(gdx_nn *nn, int32_t level, int32_t *m, double *stencil) {
+  if (!nn || !m) {
+    gdx::set_error("NULL argument");
+    return GDX_ERR_ARG;
+  }
+  if (!nn->tg || !nn->ctx) {
+    gdx::set_error("gdx_nn_attach has not run");
+    return GDX_ERR_STATE;
+  }
+  if (hipSetDevice(nn->ctx->device) != hipSuccess) return GDX_ERR_HIP;
+  int mm = 0;
+  const int rc = gdx::twogrid_export_level(nn, level, &mm, stencil);
+  if (rc == GDX_ERR_ARG) gdx::set_error("no such level");
+  *m = mm;
+  return rc;
 }
 
 int gdx_nn_apply(gdx_nn *nn, const double *r, double *z) {

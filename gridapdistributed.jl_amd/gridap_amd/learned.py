@@ -64,6 +64,21 @@ class LearnedCoarseSolver:
         the same preconditioner with the coarse sweeps started from zero."""
         _lib.call("gdx_nn_set_options", self.handle, int(coarse_sweeps), 1 if use_network else 0)
 
+    def set_multigrid(self, enable=True, nu=1):
+        """V-cycle coarse correction (default) or the two-grid coarse sweeps;
+        returns the number of grids of the V-cycle."""
+        lv = ctypes.c_int32()
+        _lib.call("gdx_nn_set_multigrid", self.handle, 1 if enable else 0, int(nu), ctypes.byref(lv))
+        return lv.value
+
+    def level_operator(self, level):
+        """Galerkin operator of V-cycle level `level` (0 = network grid): stencil [27][m][m][m]."""
+        m = ctypes.c_int32()
+        _lib.call("gdx_nn_level_operator", self.handle, int(level), ctypes.byref(m), None)
+        st = np.empty((27, m.value, m.value, m.value))
+        _lib.call("gdx_nn_level_operator", self.handle, int(level), ctypes.byref(m), st.ctypes.data)
+        return st
+
     def coarse_operator(self):
         """Galerkin coarse operator of the attached matrix, stencil [27][n][n][n]."""
         st = np.empty((27, self.n, self.n, self.n))

@@ -258,9 +258,22 @@ int gdx_nn_attach(gdx_nn *nn, gdx_problem *prob);
    smoothed by smooth_passes 3x3x3 means) from the attached matrix; loss = relative squared error */
 int gdx_nn_train_step(gdx_nn *nn, uint64_t seed, float lr, int32_t smooth_passes, float *loss);
 /* coarse damped-Jacobi sweeps on the Galerkin coarse operator after the
-   network's coarse solve (default 8); use_network = 0 starts the sweeps from
+   network's coarse solve in the two-grid form (default 8; see
+   gdx_nn_set_multigrid); use_network = 0 starts the coarse correction from
    zero instead (the same preconditioner without the network) */
 int gdx_nn_set_options(gdx_nn *nn, int32_t coarse_sweeps, int32_t use_network);
+/* coarse correction form: enable = 1 (the default) runs a V-cycle from the
+   network's guess: nu damped-Jacobi sweeps before and after each coarse-grid
+   correction, Galerkin operators P^T A P computed on the device level by
+   level, halving the grid while its cell count is even and above 4, damped
+   Jacobi sweeps in one workgroup on the coarsest grid (<= 8^3 cells).
+   enable = 0 uses the coarse sweeps of gdx_nn_set_options (two-grid form).
+   levels (may be NULL): grids of the V-cycle, the network grid included. */
+int gdx_nn_set_multigrid(gdx_nn *nn, int32_t enable, int32_t nu, int32_t *levels);
+/* level l of the V-cycle (0 = the network grid): its cells per direction m
+   and, when stencil is not NULL, its Galerkin operator as [27][m^3] in the
+   layout of gdx_nn_coarse_operator */
+int gdx_nn_level_operator(gdx_nn *nn, int32_t level, int32_t *m, double *stencil);
 /* the Galerkin coarse operator of the attached matrix as a 27-point stencil:
    stencil[(dx+1) + 3(dy+1) + 9(dz+1)][j] on the network grid (j = I - 1 per
    direction for interior coarse node I; the last plane is zero padding) */
