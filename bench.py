@@ -124,7 +124,7 @@ def bench_plaplacian(args):
     u = 0 on the boundary, sigma = (eps^2 + |grad u|^2)^(1/2) grad u with
     eps = 1e-2; Poisson initial guess; stops at ||r|| <= 1e-10 ||r(0)||.  One
     step = one full Newton solve (every Jacobian re-assembled in place), its
-    corrections solved by FPCG with the two-grid preconditioner rebuilt from
+    corrections solved by FPCG with the multilevel preconditioner rebuilt from
     each Jacobian (coarse network off: it does not gain, see the learned
     workload); the same solve with Jacobi-PCG corrections is timed beside it."""
     import numpy as np
@@ -174,7 +174,7 @@ def bench_plaplacian(args):
         "ms_per_step": round(s_per_solve * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"p-Laplacian p=3, eps=1e-2, 3D Q1 {n}^3 on (0,1)^3, parts (4,2,2), f=1, u=0 on the "
-                               "boundary; Newton (Poisson initial guess) with two-grid FPCG corrections to 1e-8",
+                               "boundary; Newton (Poisson initial guess) with multilevel FPCG corrections to 1e-8",
                    "free_dofs": prob.ngdofs, "nnz_owned": nnz, "symbolic_s": round(t_symbolic, 3),
                    "newton_iters": rep["iters"], "fpcg_iters": rep["lin_iters"], "converged": rep["converged"],
                    "fpcg_fallbacks": rep["lin_fallbacks"],

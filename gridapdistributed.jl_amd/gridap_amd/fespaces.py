@@ -325,7 +325,7 @@ class GpuProblem:
     def newton(self, x0=None, rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8, lin_maxiter=20000,
                init=_lib.NEWTON_INIT_GIVEN, precond=None):
         """Newton on the device; returns (x, report dict).  Corrections by
-        Jacobi-PCG, or by FPCG with precond's two-grid preconditioner (a
+        Jacobi-PCG, or by FPCG with precond's multilevel preconditioner (a
         LearnedCoarseSolver on the same context)."""
         x = np.zeros(self.ngdofs) if x0 is None else np.array(x0, dtype=np.float64)
         if x.shape != (self.ngdofs,):

@@ -1,4 +1,4 @@
-"""Learned two-grid preconditioner and flexible PCG (device), over the C ABI.
+"""Learned multilevel preconditioner and flexible PCG (device), over the C ABI.
 
     net = LearnedCoarseSolver(ctx, n, seed)     # n^3 network grid: problems of (2n)^3 Q1 cells
     net.attach(prob)                            # after every numeric assembly of prob
@@ -14,7 +14,8 @@ from . import _lib
 
 class LearnedCoarseSolver:
     """BF16 convolutional multilevel network (channels 8/16/24, ~60k parameters)
-    used as the coarse solve of a two-grid preconditioner; trained on device."""
+    used as the coarse-grid guess of a multilevel preconditioner (Galerkin
+    V-cycle below it); trained on device."""
 
     def __init__(self, ctx, n, seed=0):
         h = ctypes.c_void_p()

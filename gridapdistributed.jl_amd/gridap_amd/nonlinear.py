@@ -145,7 +145,7 @@ def jacobian(op, uh):
 
 class NLSolver:
     """NLSolver(method=:newton) with device corrections: Jacobi-PCG, or flexible
-    PCG with a LearnedCoarseSolver's two-grid preconditioner (preconditioner=).
+    PCG with a LearnedCoarseSolver's multilevel preconditioner (preconditioner=).
     ftol: relative residual tolerance ||r|| <= ftol ||r(u0)||."""
 
     def __init__(self, method="newton", ftol=1e-10, atol=0.0, iterations=50, linear_rtol=1e-8,

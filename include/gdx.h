@@ -212,9 +212,10 @@ typedef struct gdx_newton_opts {
   int32_t lin_maxiter;
   int32_t init;        /* GDX_NEWTON_INIT_* */
   gdx_nn *precond;     /* NULL: Jacobi-PCG corrections.  Else flexible PCG with this
-                          network's two-grid preconditioner (gdx_nn_set_options
-                          applies), re-attached to every new Jacobian; the problem
-                          must fit the network (2n x 2n x 2n Q1 cells). */
+                          network's multilevel preconditioner (gdx_nn_set_options and
+                          gdx_nn_set_multigrid apply), re-attached to every new
+                          Jacobian; the problem must fit the network (2n x 2n x 2n
+                          Q1 cells). */
 } gdx_newton_opts;
 typedef struct gdx_newton_report {
   int32_t iters;       /* Newton corrections applied */
@@ -235,9 +236,10 @@ int gdx_newton(gdx_problem *prob, const gdx_newton_opts *opts, double *x, gdx_ne
 
 /* Learned preconditioner.  A small BF16 convolutional multilevel network
    (3 levels on n^3, n^3/8, n^3/64 grids, channels 8/16/24, 3x3x3 convolutions,
-   about 60k parameters) serves as the coarse solve of a two-grid
+   about 60k parameters) gives the coarse-grid guess of a multilevel
    preconditioner (damped Jacobi smoothing, full weighting, trilinear
-   prolongation) for 3D Q1 problems on 2n x 2n x 2n cells.  It is trained on the
+   prolongation, Galerkin coarse operators with a V-cycle below the network's
+   grid) for 3D Q1 problems on 2n x 2n x 2n cells.  It is trained on the
    device by the library's own forward/backward kernels from systems assembled
    by gdx_numeric, and used inside flexible PCG (gdx_fpcg).  These entry points
    have no counterpart in the reference, whose solve is a direct LU
@@ -280,7 +282,7 @@ int gdx_nn_level_operator(gdx_nn *nn, int32_t level, int32_t *m, double *stencil
 int gdx_nn_coarse_operator(gdx_nn *nn, double *stencil);
 /* z = M r on host vectors (global-id order) */
 int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
-/* Flexible PCG with the learned two-grid preconditioner (nn attached to prob),
+/* Flexible PCG with the learned multilevel preconditioner (nn attached to prob),
    or Jacobi-PCG when nn is NULL; falls back to Jacobi-PCG from the current
    iterate when the flexible iteration stagnates (fellback = 1).  rhs: host
    array of ngdofs (NULL = the assembled RHS); stops at ||r|| <= rtol ||b||,
