@@ -8,13 +8,14 @@
 // (test/PLaplacianTests.jl:24-31 is p = 4, alpha = beta = 1; the p-Laplacian
 // is p = 3 with alpha = eps^2 regularising grad(u) = 0).
 //
-// Owner computes rows: one thread per owned free row visits the row's 2^D
-// cells, evaluates grad(u) at their Gauss points from the state on the global
+// Owner computes rows: 2^D lanes per owned free row, one per adjacent cell,
+// evaluate grad(u) at the cell's Gauss points from the state on the global
 // node lattice (free values scattered there by global id, Dirichlet nodes hold
-// uD), and accumulates the row's 3^D Jacobian entries and its residual in
-// registers, in a fixed order.  Each Gauss point is evaluated by the 2^D rows
-// of its cell: recomputation instead of a scatter, so there are no atomics and
-// two assemblies are bit-identical.  Rows are written through their position
+// uD) and accumulate the cell's contributions to the row; the row's 3^D
+// Jacobian entries and its residual are then summed over the cells in a fixed
+// order.  Each Gauss point is evaluated by the 2^D rows of its cell:
+// recomputation instead of a scatter, so there are no atomics and two
+// assemblies are bit-identical.  Rows are written through their position
 // patterns (Dirichlet columns dropped, as in the affine path); ghost rows are
 // never written and stay zero.
 #include <hip/hip_runtime.h>
@@ -54,13 +55,15 @@ FluxParams flux_params(const gdx_problem_desc &d, bool linear) {
   return fp;
 }
 
+// kappa(s) = s^((p-2)/2) by the exponent's form (MODE as FluxParams::mode):
+// the common exponents compile without pow, whose register needs would
+// otherwise hold down the kernel's occupancy
+template <int MODE>
 __device__ __forceinline__ double flux_kappa(double s, const FluxParams &fp) {
-  switch (fp.mode) {
-    case 1: return 1.0;
-    case 2: return sqrt(s);
-    case 3: return s;
-    default: return pow(s, 0.5 * (fp.p - 2.0));
-  }
+  if constexpr (MODE == 1) return 1.0;
+  else if constexpr (MODE == 2) return sqrt(s);
+  else if constexpr (MODE == 3) return s;
+  else return pow(s, 0.5 * (fp.p - 2.0));
 }
 
 // Dirichlet data on boundary nodes, zero elsewhere (free nodes are overwritten
@@ -82,13 +85,34 @@ __global__ void k_state_scatter(int64_t n, const long long *__restrict__ gid_nod
   if (g < n) ulat[gid_node[g]] = x ? x[g] : 0.0;
 }
 
-template <int D, int NQ>
+// 2^D lanes per row, one per adjacent cell: a lane evaluates grad(u) at its
+// cell's Gauss points and accumulates the cell's contributions to the row's
+// 2^D columns and to the residual; the row's lanes then sum the 3^D slots from
+// LDS in a fixed cell order.  Each Gauss point is still evaluated once per row
+// of its cell (no scatter, no atomics), but a lane holds only 2^D accumulators.
+template <int D, int NQ, int MODE>
 __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles, int ntiles,
                                                  const RowInfo *__restrict__ rowinfo,
                                                  const uint8_t *__restrict__ pat_fwd, int pstride,
                                                  const double *__restrict__ ulat, double *__restrict__ values,
                                                  double *__restrict__ b, KParams kp, FluxParams fp) {
   constexpr int S = D == 3 ? 27 : 9, NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ;
+  constexpr int RPB = 256 / NC;  // rows per pass of the workgroup
+  __shared__ double part[256][NC + 1];
+  // reference-cell tables (the kernel argument is read at static offsets only:
+  // a runtime index into it would copy the whole struct to scratch)
+  __shared__ double tphi[2][NQ], tdphi[2][NQ], tqw[NQ];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      tphi[0][q] = kp.phi[0][q];
+      tphi[1][q] = kp.phi[1][q];
+      tdphi[0][q] = kp.dphi[0][q];
+      tdphi[1][q] = kp.dphi[1][q];
+      tqw[q] = kp.qw[q];
+    }
+  }
+  __syncthreads();
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
   const Tile t = tiles[tb];
@@ -96,93 +120,123 @@ __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles,
   const double ih0 = 1.0 / kp.h[0], ih1 = 1.0 / kp.h[1], ih2 = D == 3 ? 1.0 / kp.h[2] : 0.0;
   const double vol = kp.h[0] * kp.h[1] * (D == 3 ? kp.h[2] : 1.0);
   const int64_t sy = kp.nn[0], sz = (int64_t)kp.nn[0] * kp.nn[1];
-  for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-    const int u1 = r % t.ext[0], u2 = (r / t.ext[0]) % t.ext[1], u3 = r / (t.ext[0] * t.ext[1]);
-    const int g0 = t.org[0] + u1, g1 = t.org[1] + u2, g2 = t.org[2] + u3;
-    const RowInfo ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
-    if (ri.pid < 0) continue;
-    double acc[S];
+  const int tid = threadIdx.x, e = tid % NC, rl = tid / NC;
+  const int e0 = e & 1, e1 = (e >> 1) & 1, e2 = D == 3 ? (e >> 2) & 1 : 0;
+  const int a0 = 1 - e0, a1 = 1 - e1, a2 = D == 3 ? 1 - e2 : 0;  // the row node in the cell
+  for (int r0 = 0; r0 < nrows; r0 += RPB) {  // uniform over the workgroup
+    const int r = r0 + rl;
+    int g0 = 0, g1 = 0, g2 = 0;
+    RowInfo ri{0, 0, -1, 0};
+    if (r < nrows) {
+      const int u1 = r % t.ext[0], u2 = (r / t.ext[0]) % t.ext[1], u3 = r / (t.ext[0] * t.ext[1]);
+      g0 = t.org[0] + u1, g1 = t.org[1] + u2, g2 = t.org[2] + u3;
+      ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
+    }
+    const bool live = ri.pid >= 0;
+    double acc8[NC];
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = 0.0;
+    for (int bb = 0; bb < NC; ++bb) acc8[bb] = 0.0;
     double res = 0.0;
-#pragma unroll
-    for (int e = 0; e < NC; ++e) {
-      const int e0 = e & 1, e1 = (e >> 1) & 1, e2 = D == 3 ? (e >> 2) & 1 : 0;
-      const int c0 = g0 - 1 + e0, c1 = g1 - 1 + e1, c2 = D == 3 ? g2 - 1 + e2 : 0;
-      if (c0 < 0 || c0 >= kp.ncells[0] || c1 < 0 || c1 >= kp.ncells[1]) continue;
-      if (D == 3 && (c2 < 0 || c2 >= kp.ncells[2])) continue;
-      // the row node is local node (1-e0, 1-e1, 1-e2) of the cell
+    const int c0 = g0 - 1 + e0, c1 = g1 - 1 + e1, c2 = D == 3 ? g2 - 1 + e2 : 0;
+    const bool incell = live && c0 >= 0 && c0 < kp.ncells[0] && c1 >= 0 && c1 < kp.ncells[1] &&
+                        (D == 2 || (c2 >= 0 && c2 < kp.ncells[2]));
+    if (incell) {
       const double *ub = ulat + c0 + sy * c1 + sz * c2;
       double uc[NC];
 #pragma unroll
       for (int bb = 0; bb < NC; ++bb)
         uc[bb] = ub[(bb & 1) + sy * ((bb >> 1) & 1) + (D == 3 ? sz * ((bb >> 2) & 1) : 0)];
-#pragma unroll
+#pragma unroll 1
       for (int q = 0; q < NQD; ++q) {
         const int q0 = q % NQ, q1 = (q / NQ) % NQ, q2 = D == 3 ? q / (NQ * NQ) : 0;
         // 1D values and derivatives of the two local nodes per direction
-        double p0[2], p1[2], p2[2], d0[2], d1[2], d2[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          p0[a] = kp.phi[a][q0];
-          p1[a] = kp.phi[a][q1];
-          p2[a] = D == 3 ? kp.phi[a][q2] : 1.0;
-          d0[a] = kp.dphi[a][q0] * ih0;
-          d1[a] = kp.dphi[a][q1] * ih1;
-          d2[a] = D == 3 ? kp.dphi[a][q2] * ih2 : 0.0;
-        }
-        double gb[NC][3];
-#pragma unroll
-        for (int bb = 0; bb < NC; ++bb) {
-          const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
-          gb[bb][0] = d0[b0] * p1[b1] * p2[b2];
-          gb[bb][1] = p0[b0] * d1[b1] * p2[b2];
-          gb[bb][2] = p0[b0] * p1[b1] * d2[b2];
-        }
+        const double p00 = tphi[0][q0], p01 = tphi[1][q0], p10 = tphi[0][q1], p11 = tphi[1][q1];
+        const double p20 = D == 3 ? tphi[0][q2] : 1.0, p21 = D == 3 ? tphi[1][q2] : 1.0;
+        const double d00 = tdphi[0][q0] * ih0, d01 = tdphi[1][q0] * ih0;
+        const double d10 = tdphi[0][q1] * ih1, d11 = tdphi[1][q1] * ih1;
+        const double d20 = D == 3 ? tdphi[0][q2] * ih2 : 0.0, d21 = D == 3 ? tdphi[1][q2] * ih2 : 0.0;
+        auto P0 = [&](int i) { return i ? p01 : p00; };
+        auto P1 = [&](int i) { return i ? p11 : p10; };
+        auto P2 = [&](int i) { return i ? p21 : p20; };
+        auto G0 = [&](int i) { return i ? d01 : d00; };
+        auto G1 = [&](int i) { return i ? d11 : d10; };
+        auto G2 = [&](int i) { return i ? d21 : d20; };
         double gu0 = 0.0, gu1 = 0.0, gu2 = 0.0;
 #pragma unroll
         for (int bb = 0; bb < NC; ++bb) {
-          gu0 = fma(uc[bb], gb[bb][0], gu0);
-          gu1 = fma(uc[bb], gb[bb][1], gu1);
-          gu2 = fma(uc[bb], gb[bb][2], gu2);
+          const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
+          gu0 = fma(uc[bb], G0(b0) * P1(b1) * P2(b2), gu0);
+          gu1 = fma(uc[bb], P0(b0) * G1(b1) * P2(b2), gu1);
+          gu2 = fma(uc[bb], P0(b0) * P1(b1) * G2(b2), gu2);
         }
-        const double s = fp.alpha + fp.beta * (gu0 * gu0 + gu1 * gu1 + gu2 * gu2);
-        const double kap = flux_kappa(s, fp);
-        const double gam = (fp.mode == 1 || s <= 0.0) ? 0.0 : (fp.p - 2.0) * fp.beta * kap / s;
-        const double wd = kp.qw[q0] * kp.qw[q1] * (D == 3 ? kp.qw[q2] : 1.0) * vol;
-        const int a =(1 - e0) | ((1 - e1) << 1) | (D == 3 ? (1 - e2) << 2 : 0);
-        const double ga0 = gb[a][0], ga1 = gb[a][1], ga2 = gb[a][2];
+        const double s2 = fp.alpha + fp.beta * (gu0 * gu0 + gu1 * gu1 + gu2 * gu2);
+        const double kap = flux_kappa<MODE>(s2, fp);
+        const double gam = (MODE == 1 || s2 <= 0.0) ? 0.0 : (fp.p - 2.0) * fp.beta * kap / s2;
+        const double wd = tqw[q0] * tqw[q1] * (D == 3 ? tqw[q2] : 1.0) * vol;
+        const double ga0 = G0(a0) * P1(a1) * P2(a2), ga1 = P0(a0) * G1(a1) * P2(a2), ga2 = P0(a0) * P1(a1) * G2(a2);
         const double gua = gu0 * ga0 + gu1 * ga1 + gu2 * ga2;
-        const double phia = p0[1 - e0] * p1[1 - e1] * p2[D == 3 ? 1 - e2 : 0];
+        const double phia = P0(a0) * P1(a1) * P2(a2);
         res = fma(wd, kap * gua - fp.f * phia, res);
         const double wk = wd * kap, wg = wd * gam * gua;
 #pragma unroll
         for (int bb = 0; bb < NC; ++bb) {
           const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
-          const int slot = (e0 + b0) + 3 * (e1 + b1) + (D == 3 ? 9 * (e2 + b2) : 0);
-          const double gab = ga0 * gb[bb][0] + ga1 * gb[bb][1] + ga2 * gb[bb][2];
-          const double gub = gu0 * gb[bb][0] + gu1 * gb[bb][1] + gu2 * gb[bb][2];
-          acc[slot] = fma(wk, gab, fma(wg, gub, acc[slot]));
+          const double gb0 = G0(b0) * P1(b1) * P2(b2), gb1 = P0(b0) * G1(b1) * P2(b2), gb2 = P0(b0) * P1(b1) * G2(b2);
+          const double gab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
+          const double gub = gu0 * gb0 + gu1 * gb1 + gu2 * gb2;
+          acc8[bb] = fma(wk, gab, fma(wg, gub, acc8[bb]));
         }
       }
     }
-    const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
-    double *dst = values + ri.rowstart;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int pos = fwd[s];
-      if (pos < kSlotDirichlet) dst[pos] = acc[s];
+    for (int bb = 0; bb < NC; ++bb) part[tid][bb] = acc8[bb];
+    part[tid][NC] = res;
+    __syncthreads();
+    if (live) {
+      // slot (s0, s1, s2) of the row = sum over the cells c holding both nodes of
+      // c's contribution to column (s - c), cells in a fixed order
+      const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
+      double *dst = values + ri.rowstart;
+      const int base = rl * NC;
+      for (int sl = e; sl < S; sl += NC) {
+        const int s0 = sl % 3, s1 = (sl / 3) % 3, s2 = D == 3 ? sl / 9 : 0;
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const int b0 = s0 - (c & 1), b1 = s1 - ((c >> 1) & 1), b2 = D == 3 ? s2 - ((c >> 2) & 1) : 0;
+          if (b0 < 0 || b0 > 1 || b1 < 0 || b1 > 1 || b2 < 0 || b2 > 1) continue;
+          v += part[base + c][b0 | (b1 << 1) | (b2 << 2)];
+        }
+        const int pos = fwd[sl];
+        if (pos < kSlotDirichlet) dst[pos] = v;
+      }
+      if (e == 0) {
+        double rr = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rr += part[base + c][NC];
+        b[ri.bidx] = rr;
+      }
     }
-    b[ri.bidx] = res;
+    __syncthreads();  // part is rewritten by the next pass
   }
+}
+
+template <int D, int NQ, int MODE>
+void launch_flux_m(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
+  const int nt = (int)p->tiles.size();
+  hipLaunchKernelGGL((k_flux_q1<D, NQ, MODE>), dim3(8u * ((nt + 7) / 8)), dim3(256), 0, st, p->dev.tiles, nt,
+                     p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->ulat, p->dev.values, p->dev.b,
+                     p->kp, fp);
 }
 
 template <int D, int NQ>
 void launch_flux_t(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
-  const int nt = (int)p->tiles.size();
-  hipLaunchKernelGGL((k_flux_q1<D, NQ>), dim3(8u * ((nt + 7) / 8)), dim3(256), 0, st, p->dev.tiles, nt,
-                     p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->ulat, p->dev.values, p->dev.b,
-                     p->kp, fp);
+  switch (fp.mode) {
+    case 1: launch_flux_m<D, NQ, 1>(p, fd, fp, st); break;
+    case 2: launch_flux_m<D, NQ, 2>(p, fd, fp, st); break;
+    case 3: launch_flux_m<D, NQ, 3>(p, fd, fp, st); break;
+    default: launch_flux_m<D, NQ, 0>(p, fd, fp, st); break;
+  }
 }
 
 }  // namespace
