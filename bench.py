@@ -166,6 +166,11 @@ def bench_plaplacian(args):
         prob.numeric()
         acc += prob.phase_times()[3]
     asm_ms = acc / 10
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_flux_q1_plaplacian_{n}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("traffic_bytes")
     out = {
         "metric": "p-Laplacian Newton solve time to 1e-10 (s)",
         "value": round(s_per_solve, 4),
@@ -189,7 +194,7 @@ def bench_plaplacian(args):
                    "jacobian_assembly_gnnz_s": round(nnz / (asm_ms * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "hbm", "achieved": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_flux_q1<3,2>", "alg_bytes_per_launch": int(8 * nnz)},
+                     "traffic": traffic, "kernel": "k_flux_q1<3,2>", "alg_bytes_per_launch": int(8 * nnz)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_flux()
