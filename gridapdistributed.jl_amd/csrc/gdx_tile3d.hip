@@ -275,13 +275,20 @@ struct Tile3D : Tile3<K, NQ, R1, R2, R3> {
   static constexpr int LF = B::LDS_KAPPA, LZ1 = R1 * B::Q3 * B::Q2, LZ2 = R1 * R2 * B::Q3;
   static constexpr int OFF_Z1 = LF, OFF_Z2 = OFF_Z1 + LZ1, OFF_BP = OFF_Z2 + LZ2, OFF_LIFT = OFF_BP + B::ROWS;
   static constexpr int LIFT = R3 * B::NT;
-  static constexpr int OFF_PAT = OFF_LIFT + LIFT;
+  // source-integral node factors w_t h_d phi_a(t) per (direction, tile node, region point)
+  static constexpr int NF0 = R1 * B::Q1, NF1 = R2 * B::Q2, NF2 = R3 * B::Q3, NFN = NF0 + NF1 + NF2;
+  static constexpr int OFF_NF = OFF_LIFT + LIFT;
+  static constexpr int OFF_PAT = OFF_NF + NFN;
+  // y pair factors (G then M, [a][b][q]) for stage 2, kept beyond the row table
+  static constexpr int PFY = (K + 1) * (K + 1) * NQ;
+  static constexpr int LDS_ALL = B::LDS_DOUBLES + 2 * PFY;
   static constexpr int PFREE = (B::MAIN - OFF_PAT) * 8 / B::SP;
   static constexpr int PMAX = PFREE < 128 ? PFREE : 128;
   static constexpr int PWORDS = (PMAX * B::SP / 4 + B::NT - 1) / B::NT;
   static constexpr int FPT = (LF + B::NT - 1) / B::NT;  // f values prefetched per thread
   static constexpr bool WROWS = K == 2 && R1 == 4 && R2 == 8;  // wave-owned rows (see wave_row_lane)
   static_assert(PMAX >= 16, "room for the position patterns");
+  static_assert(NFN <= B::NT, "one node factor per thread");
 };
 
 // Q2 4 x 8 tiles: wave lane -> (row rank in the wave's 2x2 block of rows, s1, s2).
@@ -364,7 +371,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const double *__restrict__ Wq, const double *__restrict__ Fq, const uint8_t *__restrict__ pat_fwd,
     const int *__restrict__ tile_pats, double *__restrict__ values, double *__restrict__ b, KParams kp) {
   using T = Tile3D<K, NQ, R1, R2, R3>;
-  __shared__ double lds[T::LDS_DOUBLES];
+  __shared__ double lds[T::LDS_ALL];
   double *kap = lds;                  // [Q3][Q2][Q1]
   double *XG = lds + T::LDS_KAPPA;    // [NP1][XS]
   double *XM = XG + T::NP1 * T::XS;   // [NP1][XS]
@@ -373,8 +380,10 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
   double *Z2 = lds + T::OFF_Z2;       // [u1][u2][q3]
   double *bpart = lds + T::OFF_BP;    // source integral per tile row
   double *liftp = lds + T::OFF_LIFT;  // [R3][NT]
+  double *nfac = lds + T::OFF_NF;     // after stage 2: node factors [NF0 | NF1 | NF2]
   uint32_t *lpat = reinterpret_cast<uint32_t *>(lds + T::OFF_PAT);
   RowT *rowt = reinterpret_cast<RowT *>(lds + T::MAIN);
+  double *pfy = lds + T::LDS_DOUBLES;  // y pair factors: G [a][b][q], then M
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
   const Tile t = tiles[tile0 + tb];
@@ -401,6 +410,20 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
       if (ri.pid >= 0) e = RowT{ri.rowstart, (int)ri.bidx, ri.lenf};
     }
     rowt[r] = e;
+  }
+  // the small factor tables, read from the kernel argument while the region
+  // loads are in flight (a per-lane index into the argument is a memory load
+  // each time; from LDS it is not): y pair factors now, one node factor per
+  // thread kept in a register until stage 2 has freed its LDS
+  for (int i = tid; i < 2 * T::PFY; i += T::NT) {
+    const int j = i % T::PFY, q = j % NQ, bb = (j / NQ) % (K + 1), a = j / (NQ * (K + 1));
+    pfy[i] = i < T::PFY ? kp.fG[1][a][bb][q] : kp.fM[1][a][bb][q];
+  }
+  double nfv = 0.0;
+  if (tid < T::NFN) {
+    if (tid < T::NF0) nfv = node_factor<K, NQ>(kp, 0, tid / T::Q1, tid % T::Q1);
+    else if (tid < T::NF0 + T::NF1) nfv = node_factor<K, NQ>(kp, 1, (tid - T::NF0) / T::Q2, (tid - T::NF0) % T::Q2);
+    else nfv = node_factor<K, NQ>(kp, 2, (tid - T::NF0 - T::NF1) / T::Q3, (tid - T::NF0 - T::NF1) % T::Q3);
   }
   const int qs0 = (t.org[0] / K - 1) * NQ, qs1 = (t.org[1] / K - 1) * NQ, qs2 = (t.org[2] / K - 1) * NQ;
   double fpre[T::FPT];
@@ -470,7 +493,12 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     pair_cells<K>(u2, s2, lc0, nc, a0, b0);
     double G[2 * NQ], M[2 * NQ];
 #pragma unroll
-    for (int tt = 0; tt < 2 * NQ; ++tt) pair_factors<K, NQ>(kp, 1, a0, b0, tt, G[tt], M[tt]);
+    for (int tt = 0; tt < 2 * NQ; ++tt) {
+      const int ci = tt / NQ, q = tt % NQ;
+      const int j = ((a0 - K * ci) * (K + 1) + (b0 - K * ci)) * NQ + q;
+      G[tt] = pfy[j];
+      M[tt] = pfy[T::PFY + j];
+    }
     const double *xg = XG + p1 * T::XS + lc0 * NQ;
     const double *xm = XM + p1 * T::XS + lc0 * NQ;
     if (nc == 2) {
@@ -514,6 +542,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const int i = tid + j * T::NT;
     if (i < T::LF) Fl[i] = fpre[j];
   }
+  if (tid < T::NFN) nfac[tid] = nfv;
   __syncthreads();
   // source integral of every tile row, sum-factorised (x, y, z) as k_rhs
   for (int i = tid; i < T::LZ1; i += T::NT) {
@@ -522,7 +551,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const int v = uu + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
     double acc = 0.0;
     for (int lc = cmin; lc <= cmax; ++lc)
-      for (int tt = 0; tt < NQ; ++tt) acc = fma(fr[lc * NQ + tt], node_factor<K, NQ>(kp, 0, uu, lc * NQ + tt), acc);
+      for (int tt = 0; tt < NQ; ++tt) acc = fma(fr[lc * NQ + tt], nfac[uu * T::Q1 + lc * NQ + tt], acc);
     Z1[i] = acc;
   }
   __syncthreads();
@@ -532,7 +561,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const int v = uu2 + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
     double acc = 0.0;
     for (int lc = cmin; lc <= cmax; ++lc)
-      for (int tt = 0; tt < NQ; ++tt) acc = fma(zr[lc * NQ + tt], node_factor<K, NQ>(kp, 1, uu2, lc * NQ + tt), acc);
+      for (int tt = 0; tt < NQ; ++tt) acc = fma(zr[lc * NQ + tt], nfac[T::NF0 + uu2 * T::Q2 + lc * NQ + tt], acc);
     Z2[i] = acc;
   }
   __syncthreads();
@@ -542,7 +571,8 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const int v = uu3 + K, cmin = (v + K - 1) / K - 1, cmax = v / K;
     double acc = 0.0;
     for (int lc = cmin; lc <= cmax; ++lc)
-      for (int tt = 0; tt < NQ; ++tt) acc = fma(zr[lc * NQ + tt], node_factor<K, NQ>(kp, 2, uu3, lc * NQ + tt), acc);
+      for (int tt = 0; tt < NQ; ++tt)
+        acc = fma(zr[lc * NQ + tt], nfac[T::NF0 + T::NF1 + uu3 * T::Q3 + lc * NQ + tt], acc);
     bpart[r] = acc;
   }
 
