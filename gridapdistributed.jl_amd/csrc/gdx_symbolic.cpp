@@ -3,10 +3,15 @@
 // numeric kernels scatter through.  Runs once per problem; its outputs are
 // uploaded to HBM by gdx_capi.hip.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "gdx_internal.h"
@@ -14,6 +19,45 @@
 namespace gdx {
 
 namespace {
+
+// Host threads for the per-part loops: GDX_HOST_THREADS, else the hardware
+// threads capped at 16 (parts are independent; every id and order below is
+// the sequential one, whatever the thread count).
+int host_threads() {
+  const char *e = std::getenv("GDX_HOST_THREADS");
+  if (e && std::atoi(e) > 0) return std::atoi(e);
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw, 16u));
+}
+
+// f(r) for every part r in [0, n), parts spread over the host threads; the
+// first exception thrown by any part is rethrown here
+template <class F>
+void for_parts(int n, F &&f) {
+  const int nt = std::min(n, host_threads());
+  if (nt <= 1) {
+    for (int r = 0; r < n; ++r) f(r);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::exception_ptr err;
+  std::mutex m;
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&]() {
+      for (int r; (r = next.fetch_add(1)) < n;) {
+        try {
+          f(r);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(m);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (std::thread &t : th) t.join();
+  if (err) std::rethrow_exception(err);
+}
 
 // src/PArraysExtras.jl:72-84 (_local_range), 0-based and half-open: the last
 // `rem` parts get one extra cell, `ghost` adds one layer on interior sides.
@@ -166,27 +210,27 @@ void vector_csr(gdx_problem *p, const Ctx &c) {
   const int S = c.D == 3 ? W * W * W : W * W;
   const int SN = S * nc;
   const int SP = (SN + 8) & ~7;  // padded stride, at least one spare entry
-  std::unordered_map<std::string, int> pat_ids;
-  p->pat_fwd.clear();
-  p->pat_inv.clear();
-  p->pat_invr.clear();
-  p->pat_len.clear();
-  p->pat_dir.clear();
-  p->pat16_inv.clear();
-  p->pat16_stride = SP;
-  p->rowinfo.clear();
-  int64_t val_base = 0, row_base = 0, owned_nnz = 0;
   constexpr uint16_t kAbsent16 = 0xFFFF, kDir16 = 0xFFFE;
-  std::vector<uint16_t> key(SN);
-  std::vector<std::pair<int64_t, int>> nodes;  // (comp-0 col, node slot) of own stencil nodes
-  std::vector<std::pair<int64_t, int>> ents;   // (col, slot * nc + comp)
-  for (int r = 0; r < p->nparts; ++r) {
+  const int nparts = p->nparts;
+  // per part: its CSR and its distinct patterns in first-occurrence order,
+  // merged below in part order (ids as one sequential pass would give them)
+  struct LocalPats {
+    std::vector<std::string> keys;
+    std::vector<uint16_t> inv;  // [npat][SP]
+    std::vector<int> len, dir;
+    std::vector<int> own_pid;  // per owned row: local id, then global id
+  };
+  std::vector<LocalPats> lp(nparts);
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
-    P.val_base = val_base;
-    P.row_base = row_base;
+    LocalPats &L = lp[r];
+    std::unordered_map<std::string, int> ids;
+    std::vector<uint16_t> key(SN);
+    std::vector<std::pair<int64_t, int>> nodes;  // (comp-0 col, node slot) of own stencil nodes
+    std::vector<std::pair<int64_t, int>> ents;   // (col, slot * nc + comp)
     P.rowptr.assign(P.nrows + 1, 0);
     P.colidx.clear();
-    std::vector<int> own_pid(P.n_own, -1);
+    L.own_pid.assign(P.n_own, -1);
     const int64_t own_cols = P.n_own;  // local col ids below this are own
     for (int64_t row = 0; row < P.nrows; row += nc) {
       if (P.row_ldof[row] % nc != 0) throw std::runtime_error("internal: rows of a node are not grouped");
@@ -241,48 +285,84 @@ void vector_csr(gdx_problem *p, const Ctx &c) {
       for (int k = 0; k < nc; ++k) P.rowptr[row + k + 1] = base + (int64_t)(k + 1) * e;
       if (own) {
         std::string skey(reinterpret_cast<const char *>(key.data()), key.size() * sizeof(uint16_t));
-        auto it = pat_ids.find(skey);
+        auto it = ids.find(skey);
         int pid;
-        if (it == pat_ids.end()) {
-          pid = (int)p->pat_len.size();
-          pat_ids.emplace(std::move(skey), pid);
+        if (it == ids.end()) {
+          pid = (int)L.len.size();
+          ids.emplace(skey, pid);
+          L.keys.push_back(std::move(skey));
           std::vector<uint16_t> inv(SP, 0);
           for (int s3 = 0; s3 < SN; ++s3)
             if (key[s3] < kDir16) inv[key[s3]] = (uint16_t)s3;
-          p->pat16_inv.insert(p->pat16_inv.end(), inv.begin(), inv.end());
-          p->pat_len.push_back(e);
-          p->pat_dir.push_back(has_dir ? 1 : 0);
+          L.inv.insert(L.inv.end(), inv.begin(), inv.end());
+          L.len.push_back(e);
+          L.dir.push_back(has_dir ? 1 : 0);
         } else {
           pid = it->second;
         }
-        for (int k = 0; k < nc; ++k) own_pid[row + k] = pid;
+        for (int k = 0; k < nc; ++k) L.own_pid[row + k] = pid;
       }
     }
+  });
+  std::unordered_map<std::string, int> pat_ids;
+  p->pat_fwd.clear();
+  p->pat_inv.clear();
+  p->pat_invr.clear();
+  p->pat_len.clear();
+  p->pat_dir.clear();
+  p->pat16_inv.clear();
+  p->pat16_stride = SP;
+  int64_t val_base = 0, row_base = 0, owned_nnz = 0, rinfo_n = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    LocalPats &L = lp[r];
+    std::vector<int> to_global(L.len.size());
+    for (size_t l = 0; l < L.len.size(); ++l) {
+      auto it = pat_ids.find(L.keys[l]);
+      if (it != pat_ids.end()) {
+        to_global[l] = it->second;
+        continue;
+      }
+      const int pid = (int)p->pat_len.size();
+      pat_ids.emplace(L.keys[l], pid);
+      to_global[l] = pid;
+      p->pat16_inv.insert(p->pat16_inv.end(), L.inv.begin() + l * SP, L.inv.begin() + (l + 1) * SP);
+      p->pat_len.push_back(L.len[l]);
+      p->pat_dir.push_back(L.dir[l]);
+    }
+    for (int &pid : L.own_pid) pid = to_global[pid];
+    P.val_base = val_base;
+    P.row_base = row_base;
     owned_nnz += P.rowptr[P.n_own];
     val_base += P.rowptr[P.nrows];
     row_base += P.nrows;
-    // row table over the owned node box: the node's first row
-    P.rowinfo_base = (int64_t)p->rowinfo.size();
+    P.rowinfo_base = rinfo_n;
+    rinfo_n += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+  }
+  // row table over the owned node box: the node's first row
+  p->rowinfo.assign(rinfo_n, RowInfo{0, 0, -1, 0});
+  for_parts(nparts, [&](int r) {
+    const Part &P = p->P[r];
+    const std::vector<int> &own_pid = lp[r].own_pid;
     std::vector<int64_t> ldof_own(P.n_ldofs, -1);
     for (int64_t o = 0; o < P.n_own; ++o) ldof_own[P.own_ldofs[o]] = o;
+    RowInfo *out = p->rowinfo.data() + P.rowinfo_base;
     for (int64_t z = 0; z < P.onb_n[2]; ++z)
       for (int64_t y = 0; y < P.onb_n[1]; ++y)
-        for (int64_t x = 0; x < P.onb_n[0]; ++x) {
+        for (int64_t x = 0; x < P.onb_n[0]; ++x, ++out) {
           int64_t g[3] = {P.onb_lo[0] + x, P.onb_lo[1] + y, P.onb_lo[2] + z};
-          RowInfo ri{0, 0, -1, 0};
           int64_t l = lnode_in(c, P, g);
           int64_t ld = l >= 0 ? P.lnode_dof[l] : -1;
           if (ld >= 0) {
             int64_t o = ldof_own[ld];
             if (o < 0) throw std::runtime_error("internal: owned node box holds a non-owned dof");
-            ri.rowstart = P.val_base + P.rowptr[o];
-            ri.bidx = P.row_base + o;
-            ri.pid = own_pid[o];
-            ri.lenf = p->pat_len[ri.pid] | (p->pat_dir[ri.pid] << 16);
+            out->rowstart = P.val_base + P.rowptr[o];
+            out->bidx = P.row_base + o;
+            out->pid = own_pid[o];
+            out->lenf = p->pat_len[out->pid] | (p->pat_dir[out->pid] << 16);
           }
-          p->rowinfo.push_back(ri);
         }
-  }
+  });
   p->npat = (int)p->pat_len.size();
   p->nnz_total = val_base;
   p->rows_total = row_base;
@@ -329,7 +409,7 @@ int symbolic(gdx_problem *p) {
   const int nparts = p->nparts;
   p->P.assign(nparts, Part{});
   const bool FA = d.strategy == GDX_FULLY_ASSEMBLED_ROWS;
-  for (int r = 0; r < nparts; ++r) {
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
     P.rank = r;
     P.pc[0] = r % d.parts[0];
@@ -351,10 +431,10 @@ int symbolic(gdx_problem *p) {
       }
     }
     number_part(c, P);
-  }
-  // owners and rank-blocked global ids (src/FESpaces.jl:139-261)
-  int64_t first = 0;
-  for (int r = 0; r < nparts; ++r) {
+  });
+  // owners and rank-blocked global ids (src/FESpaces.jl:139-261): owned dofs of
+  // part r get consecutive ids after those of parts < r
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
     P.ldof_owner.resize(P.n_ldofs);
     P.ldof_gid.assign(P.n_ldofs, -1);
@@ -363,16 +443,18 @@ int symbolic(gdx_problem *p) {
       int64_t g[3];
       gnode_of(c, P, P.ldof_lnode[i], g);
       P.ldof_owner[i] = node_owner(c, g);
-      if (P.ldof_owner[i] == r) {
-        P.ldof_gid[i] = first + (int64_t)P.own_ldofs.size();
-        P.own_ldofs.push_back(i);
-      }
+      if (P.ldof_owner[i] == r) P.own_ldofs.push_back(i);
     }
     P.n_own = (int64_t)P.own_ldofs.size();
-    first += P.n_own;
-  }
-  p->ngdofs = first;
-  for (int r = 0; r < nparts; ++r) {
+  });
+  std::vector<int64_t> gid_first(nparts + 1, 0);
+  for (int r = 0; r < nparts; ++r) gid_first[r + 1] = gid_first[r] + p->P[r].n_own;
+  for_parts(nparts, [&](int r) {
+    Part &P = p->P[r];
+    for (int64_t o = 0; o < P.n_own; ++o) P.ldof_gid[P.own_ldofs[o]] = gid_first[r] + o;
+  });
+  p->ngdofs = gid_first[nparts];
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
     for (int64_t i = 0; i < P.n_ldofs; ++i) {
       if (P.ldof_gid[i] >= 0) continue;
@@ -383,13 +465,14 @@ int symbolic(gdx_problem *p) {
       if (lq < 0 || Q.lnode_dof[lq] < 0) throw std::runtime_error("internal: ghost dof missing in its owner");
       P.ldof_gid[i] = Q.ldof_gid[Q.lnode_dof[lq]];
     }
-  }
+  });
   // vector-valued space: each free node carries ncomp consecutive dofs (Gridap
   // numbers a face's own dofs together; Q1/Q2 faces own one node), so local
   // and global ids become ncomp * (node id) + component, owners per node
   const int nc = p->ncomp;
   if (nc > 1) {
-    for (Part &P : p->P) {
+    for_parts(nparts, [&](int r) {
+      Part &P = p->P[r];
       std::vector<int64_t> lnode(P.n_ldofs * nc), gid(P.n_ldofs * nc), own;
       std::vector<int32_t> owner(P.n_ldofs * nc);
       for (int64_t i = 0; i < P.n_ldofs; ++i)
@@ -409,7 +492,7 @@ int symbolic(gdx_problem *p) {
       P.own_ldofs.swap(own);
       P.n_ldofs *= nc;
       P.n_own *= nc;
-    }
+    });
     p->ngdofs *= nc;
   }
 
@@ -423,7 +506,7 @@ int symbolic(gdx_problem *p) {
   };
 
   // row and col PRanges (src/Algebra.jl:593-646, 1136-1257)
-  for (int r = 0; r < nparts; ++r) {
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
     P.row_gid.clear();
     P.row_owner.clear();
@@ -522,7 +605,7 @@ int symbolic(gdx_problem *p) {
     }
     P.ncols = (int64_t)P.col_gid.size();
     P.n_ghost_cols = P.ncols - P.n_own;
-  }
+  });
 
   if (nc > 1) {
     vector_csr(p, c);
@@ -532,24 +615,27 @@ int symbolic(gdx_problem *p) {
     return GDX_OK;
   }
 
-  // local CSR patterns and the position patterns of owned rows
+  // local CSR patterns and the position patterns of owned rows.  Each part
+  // builds its CSR and a table of its distinct patterns in first-occurrence
+  // order; merging the tables in part order gives every pattern the id of its
+  // first occurrence over (part, row), as one sequential pass would.
   const int S = (c.D == 3) ? (2 * K + 1) * (2 * K + 1) * (2 * K + 1) : (2 * K + 1) * (2 * K + 1);
-  std::unordered_map<std::string, int> pat_ids;
-  p->pat_fwd.clear();
-  p->pat_inv.clear();
-  p->pat_invr.clear();
-  p->pat_len.clear();
-  p->pat_dir.clear();
-  p->rowinfo.clear();
-  int64_t val_base = 0, row_base = 0, owned_nnz = 0;
-  for (int r = 0; r < nparts; ++r) {
+  const int SP = pattern_stride(S);
+  struct LocalPats {
+    std::vector<std::string> keys;
+    std::vector<uint8_t> fwd, inv, invr;  // [npat][SP]
+    std::vector<int> len, dir;
+    std::vector<int> own_pid;  // per owned row: local id, then global id
+  };
+  std::vector<LocalPats> lp(nparts);
+  for_parts(nparts, [&](int r) {
     Part &P = p->P[r];
-    P.val_base = val_base;
-    P.row_base = row_base;
+    LocalPats &L = lp[r];
+    std::unordered_map<std::string, int> ids;
     P.rowptr.assign(P.nrows + 1, 0);
     P.colidx.clear();
     std::vector<std::pair<int32_t, int>> ents;
-    std::vector<int> own_pid(P.n_own, -1);
+    L.own_pid.assign(P.n_own, -1);
     std::string key(S, (char)kSlotAbsent);
     for (int64_t row = 0; row < P.nrows; ++row) {
       int64_t g[3];
@@ -585,12 +671,12 @@ int symbolic(gdx_problem *p) {
       }
       P.rowptr[row + 1] = (int64_t)P.colidx.size();
       if (own) {
-        auto it = pat_ids.find(key);
+        auto it = ids.find(key);
         int pid;
-        if (it == pat_ids.end()) {
-          pid = (int)p->pat_len.size();
-          pat_ids.emplace(key, pid);
-          const int SP = pattern_stride(S);
+        if (it == ids.end()) {
+          pid = (int)L.len.size();
+          ids.emplace(key, pid);
+          L.keys.push_back(key);
           std::vector<uint8_t> inv(SP, 0), fwd(SP, kSlotAbsent);
           bool has_dir = false;
           for (int s = 0; s < S; ++s) {
@@ -599,8 +685,8 @@ int symbolic(gdx_problem *p) {
             else if (v != kSlotAbsent) inv[v] = (uint8_t)s;
             fwd[s] = v;
           }
-          p->pat_fwd.insert(p->pat_fwd.end(), fwd.begin(), fwd.end());
-          p->pat_inv.insert(p->pat_inv.end(), inv.begin(), inv.end());
+          L.fwd.insert(L.fwd.end(), fwd.begin(), fwd.end());
+          L.inv.insert(L.inv.end(), inv.begin(), inv.end());
           // reduced numbering (the pattern fixes the node parity of its rows)
           std::vector<uint8_t> invr(SP, 0);
           int ndir = 0;
@@ -621,40 +707,75 @@ int symbolic(gdx_problem *p) {
           for (int s = 0; s < S; ++s)
             if (fwd[s] == kSlotDirichlet) invr[ents.size() + ndir++] = (uint8_t)reduced(s);
           invr[SP - 1] = (uint8_t)ndir;
-          p->pat_invr.insert(p->pat_invr.end(), invr.begin(), invr.end());
-          p->pat_len.push_back((int)ents.size());
-          p->pat_dir.push_back(has_dir ? 1 : 0);
+          L.invr.insert(L.invr.end(), invr.begin(), invr.end());
+          L.len.push_back((int)ents.size());
+          L.dir.push_back(has_dir ? 1 : 0);
         } else {
           pid = it->second;
         }
-        own_pid[row] = pid;
+        L.own_pid[row] = pid;
       }
     }
+  });
+  std::unordered_map<std::string, int> pat_ids;
+  p->pat_fwd.clear();
+  p->pat_inv.clear();
+  p->pat_invr.clear();
+  p->pat_len.clear();
+  p->pat_dir.clear();
+  int64_t val_base = 0, row_base = 0, owned_nnz = 0, rinfo_n = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    LocalPats &L = lp[r];
+    std::vector<int> to_global(L.len.size());
+    for (size_t l = 0; l < L.len.size(); ++l) {
+      auto it = pat_ids.find(L.keys[l]);
+      if (it != pat_ids.end()) {
+        to_global[l] = it->second;
+        continue;
+      }
+      const int pid = (int)p->pat_len.size();
+      pat_ids.emplace(L.keys[l], pid);
+      to_global[l] = pid;
+      p->pat_fwd.insert(p->pat_fwd.end(), L.fwd.begin() + l * SP, L.fwd.begin() + (l + 1) * SP);
+      p->pat_inv.insert(p->pat_inv.end(), L.inv.begin() + l * SP, L.inv.begin() + (l + 1) * SP);
+      p->pat_invr.insert(p->pat_invr.end(), L.invr.begin() + l * SP, L.invr.begin() + (l + 1) * SP);
+      p->pat_len.push_back(L.len[l]);
+      p->pat_dir.push_back(L.dir[l]);
+    }
+    for (int &pid : L.own_pid) pid = to_global[pid];
+    P.val_base = val_base;
+    P.row_base = row_base;
     owned_nnz += P.rowptr[P.n_own];
     val_base += P.rowptr[P.nrows];
     row_base += P.nrows;
-    // row table over the owned node box
-    P.rowinfo_base = (int64_t)p->rowinfo.size();
+    P.rowinfo_base = rinfo_n;
+    rinfo_n += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+  }
+  // row table over each part's owned node box
+  p->rowinfo.assign(rinfo_n, RowInfo{0, 0, -1, 0});
+  for_parts(nparts, [&](int r) {
+    const Part &P = p->P[r];
+    const std::vector<int> &own_pid = lp[r].own_pid;
     std::vector<int64_t> ldof_own(P.n_ldofs, -1);
     for (int64_t o = 0; o < P.n_own; ++o) ldof_own[P.own_ldofs[o]] = o;
+    RowInfo *out = p->rowinfo.data() + P.rowinfo_base;
     for (int64_t z = 0; z < P.onb_n[2]; ++z)
       for (int64_t y = 0; y < P.onb_n[1]; ++y)
-        for (int64_t x = 0; x < P.onb_n[0]; ++x) {
+        for (int64_t x = 0; x < P.onb_n[0]; ++x, ++out) {
           int64_t g[3] = {P.onb_lo[0] + x, P.onb_lo[1] + y, P.onb_lo[2] + z};
-          RowInfo ri{0, 0, -1, 0};
           int64_t l = lnode_in(c, P, g);
           int64_t ld = l >= 0 ? P.lnode_dof[l] : -1;
           if (ld >= 0) {
             int64_t o = ldof_own[ld];
             if (o < 0) throw std::runtime_error("internal: owned node box holds a non-owned dof");
-            ri.rowstart = P.val_base + P.rowptr[o];
-            ri.bidx = P.row_base + o;
-            ri.pid = own_pid[o];
-            ri.lenf = p->pat_len[ri.pid] | (p->pat_dir[ri.pid] << 16);
+            out->rowstart = P.val_base + P.rowptr[o];
+            out->bidx = P.row_base + o;
+            out->pid = own_pid[o];
+            out->lenf = p->pat_len[out->pid] | (p->pat_dir[out->pid] << 16);
           }
-          p->rowinfo.push_back(ri);
         }
-  }
+  });
   p->npat = (int)p->pat_len.size();
   p->nnz_total = val_base;
   p->rows_total = row_base;
@@ -673,12 +794,16 @@ int symbolic(gdx_problem *p) {
 void build_tiles(gdx_problem *p, const int R[3]) {
   const int nparts = p->nparts;
   for (int k = 0; k < 3; ++k) p->tileR[k] = R[k];
-  p->tiles.clear();
-  p->tile_pats.clear();
-  for (int r = 0; r < nparts; ++r) {
+  // per part (its rows' RowInfo entries are its own), then concatenated in part order
+  std::vector<std::vector<Tile>> ptiles(nparts);
+  std::vector<std::vector<int>> ppats(nparts);
+  for_parts(nparts, [&](int r) {
     const Part &P = p->P[r];
+    std::vector<Tile> &tiles = ptiles[r];
+    std::vector<int> &tpats = ppats[r];
     int64_t nt[3];
     for (int k = 0; k < 3; ++k) nt[k] = (P.onb_n[k] + p->tileR[k] - 1) / p->tileR[k];
+    std::vector<int> pids;
     for (int64_t tz = 0; tz < nt[2]; ++tz)
       for (int64_t ty = 0; ty < nt[1]; ++ty)
         for (int64_t tx = 0; tx < nt[0]; ++tx) {
@@ -694,8 +819,8 @@ void build_tiles(gdx_problem *p, const int R[3]) {
           }
           t.rowinfo_base = P.rowinfo_base;
           // distinct patterns of the tile; rows get their tile-local index
-          std::vector<int> pids;
-          t.pat_off = (long long)p->tile_pats.size();
+          pids.clear();
+          t.pat_off = (long long)tpats.size();  // part-relative until concatenated
           for (int z = 0; z < t.ext[2]; ++z)
             for (int y = 0; y < t.ext[1]; ++y)
               for (int x = 0; x < t.ext[0]; ++x) {
@@ -712,10 +837,20 @@ void build_tiles(gdx_problem *p, const int R[3]) {
               }
           t.npat = (int)pids.size();
           if (any) {
-            p->tile_pats.insert(p->tile_pats.end(), pids.begin(), pids.end());
-            p->tiles.push_back(t);
+            tpats.insert(tpats.end(), pids.begin(), pids.end());
+            tiles.push_back(t);
           }
         }
+  });
+  p->tiles.clear();
+  p->tile_pats.clear();
+  for (int r = 0; r < nparts; ++r) {
+    const long long off = (long long)p->tile_pats.size();
+    for (Tile &t : ptiles[r]) {
+      t.pat_off += off;
+      p->tiles.push_back(t);
+    }
+    p->tile_pats.insert(p->tile_pats.end(), ppats[r].begin(), ppats[r].end());
   }
 }
 

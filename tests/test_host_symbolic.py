@@ -164,3 +164,22 @@ def test_flux_descriptor_errors(host_ctx):
     rep = _lib.NewtonReport()
     assert L.gdx_newton(gp.handle, ctypes.byref(o), None, ctypes.byref(rep)) == -3
     gp.close()
+
+
+@pytest.mark.parametrize("p", [W.poisson3d_q2_fourier(6, (2, 2, 2)), W.poisson3d_q1(7, (3, 2, 2), strategy="full"),
+                               W.poisson2d_q1(9, (3, 2)),
+                               W.elasticity3d_q2(cells=(5, 4, 3), parts=(3, 2, 1), clamped=True)],
+                         ids=["q2-sub", "q1-full", "2d", "elasticity"])
+def test_structure_independent_of_host_threads(host_ctx, p, monkeypatch):
+    """The per-part loops of the symbolic phase run on several host threads;
+    one thread and several give the same exports (pattern ids are merged in
+    part order, offsets come from prefix sums)."""
+    out = []
+    for threads in ("1", "3"):
+        monkeypatch.setenv("GDX_HOST_THREADS", threads)
+        gp = host_problem(host_ctx, p)
+        out.append([gp.export_part(k) for k in range(gp.nparts)])
+        gp.close()
+    for A, B in zip(*out):
+        for key in STRUCT_KEYS:
+            assert np.array_equal(A[key], B[key]), key
