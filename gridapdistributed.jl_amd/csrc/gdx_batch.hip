@@ -8,6 +8,11 @@
 //                       direct-store and wave-slab forms, and keeps the fastest
 //   gdx_batch_capture   records every problem's numeric launches (coefficient,
 //                       RHS and matrix kernels; no memsets) into one hipGraph
+//                       of independent branches: the problems are spread over
+//                       kBranches streams (largest first, each to the branch
+//                       with the least work), forked from and joined back into
+//                       the context's stream, so small problems that fill few
+//                       CUs run beside each other and beside the large ones
 //   gdx_batch_launch    replays it on the context's stream
 //
 // The z-marching column length only splits the rows over workgroups: each
@@ -26,9 +31,37 @@ struct gdx_batch {
   std::vector<gdx_problem *> probs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // capture-time fork/join streams and events (default priority)
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;  // [0] fork, [1..] joins
 };
 
 namespace {
+
+// branches of the batch graph: the context's stream plus kBranches - 1 side
+// streams (the device runs at most 4 hardware queues per process by default)
+constexpr int kBranches = 4;
+
+int ensure_branches(gdx_batch *b) {
+  while ((int)b->side.size() < kBranches - 1) {
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return GDX_ERR_HIP;
+    b->side.push_back(s);
+  }
+  while ((int)b->ev.size() < kBranches) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return GDX_ERR_HIP;
+    b->ev.push_back(e);
+  }
+  return GDX_OK;
+}
+
+void free_branches(gdx_batch *b) {
+  for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  b->side.clear();
+  b->ev.clear();
+}
 
 int fail(int code, const char *msg) {
   gdx::set_error(msg);
@@ -118,11 +151,33 @@ int gdx_batch_capture(gdx_batch *b) {
   hipStream_t st = b->ctx->stream;
   drop_graph(b);
   if (hipStreamSynchronize(st) != hipSuccess) return fail(GDX_ERR_HIP, "stream failed");
+  if (ensure_branches(b)) return fail(GDX_ERR_HIP, "stream or event creation failed");
+  // largest problems first, each to the least loaded branch (work = stored values);
+  // within a branch the problems keep the batch order
+  const int nb = std::min<int>(kBranches, (int)b->probs.size());
+  std::vector<size_t> order(b->probs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return b->probs[x]->nnz_total > b->probs[y]->nnz_total; });
+  std::vector<int> branch(b->probs.size(), 0);
+  std::vector<double> load(nb, 0.0);
+  for (size_t i : order) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    branch[i] = k;
+    load[k] += (double)b->probs[i]->nnz_total + 1.0;
+  }
+  auto stream_of = [&](int k) { return k == 0 ? st : b->side[k - 1]; };
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(GDX_ERR_HIP, "hipStreamBeginCapture failed");
   int rc = GDX_OK;
-  for (gdx_problem *p : b->probs)
-    if ((rc = gdx::numeric_launches(p, st))) break;
+  if (hipEventRecord(b->ev[0], st) != hipSuccess) rc = GDX_ERR_HIP;
+  for (int k = 1; k < nb && !rc; ++k)
+    if (hipStreamWaitEvent(stream_of(k), b->ev[0], 0) != hipSuccess) rc = GDX_ERR_HIP;
+  for (size_t i = 0; i < b->probs.size() && !rc; ++i) rc = gdx::numeric_launches(b->probs[i], stream_of(branch[i]));
+  // join every side branch that was forked, also after a failed launch, so the capture can end
+  for (int k = 1; k < nb; ++k)
+    if (hipEventRecord(b->ev[k], stream_of(k)) != hipSuccess || hipStreamWaitEvent(st, b->ev[k], 0) != hipSuccess)
+      rc = rc ? rc : GDX_ERR_HIP;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(st, &g);
   if (rc || e != hipSuccess) {
@@ -163,6 +218,7 @@ void release_batch_device(gdx_batch *b) {
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
   drop_graph(b);
+  free_branches(b);
   b->probs.clear();
   b->ctx = nullptr;
 }
