@@ -419,19 +419,29 @@ __global__ __launch_bounds__(512) void k_coarsest(int m, double w, int sweeps, c
     for (int i = j; i < 27 * tot; i += blockDim.x) cl[i] = cstg[i];
   const double *cst = staged ? cl : cstg;
   if (j < tot) ea[j] = 0.0;
+  // the right-hand side and the diagonal are read once, not per sweep
+  const double rj = live ? rhs[j] : 0.0;
   __syncthreads();
+  const double dj = live ? cst[13 * tot + j] : 1.0;
+  // neighbours inside the interior, as a bit mask over the 27 stencil slots
+  uint32_t nbr = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
+    if (live && x >= 0 && y >= 0 && z >= 0 && x <= m - 2 && y <= m - 2 && z <= m - 2) nbr |= 1u << k;
+  }
   double *a = ea, *b = eb;
   for (int s = 0; s < sweeps; ++s) {
     if (j < tot) {
       double v = 0.0;
       if (live) {
-        double r = rhs[j];
+        double r = rj;
+#pragma unroll
         for (int k = 0; k < 27; ++k) {
-          const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
-          if (x < 0 || y < 0 || z < 0 || x > m - 2 || y > m - 2 || z > m - 2) continue;
-          r -= cst[k * tot + j] * a[x + m * (y + m * z)];
+          if (!(nbr >> k & 1u)) continue;
+          const int off = (k % 3 - 1) + m * ((k / 3) % 3 - 1) + m * m * (k / 9 - 1);
+          r -= cst[k * tot + j] * a[j + off];
         }
-        v = a[j] + w * r / cst[13 * tot + j];
+        v = a[j] + w * r / dj;
       }
       b[j] = v;
     }
