@@ -44,6 +44,66 @@ __global__ __launch_bounds__(kBlk) void k_spmv(int n, const long long *__restric
   if (row < n && lane == 0) y[row] = acc;
 }
 
+// Block form for short rows (Q1: at most 32 entries): RB consecutive rows per
+// NT-thread block.  When the rows' values are stored back to back (own rows of
+// one part), the block streams their values and column ids coalesced, each
+// thread staging its U entries in registers before the gathers of x, and puts
+// the products in LDS; LPR = NT / RB lanes per row then sum their row in a
+// fixed order.  Other blocks (a part boundary or a ghost-row gap inside) read
+// their rows directly with LPR lanes each.
+template <int RB, int NT>
+__global__ __launch_bounds__(NT) void k_spmv_blk(int n, const long long *__restrict__ rstart,
+                                                 const int *__restrict__ rlen, const int *__restrict__ cgid,
+                                                 const double *__restrict__ val, const double *__restrict__ x,
+                                                 double *__restrict__ y) {
+  constexpr int U = 8, CAP = U * NT, LPR = NT / RB;
+  static_assert(NT % RB == 0 && RB <= NT, "whole rows per block");
+  __shared__ double buf[CAP];
+  __shared__ long long rs[RB];
+  __shared__ int rl[RB];
+  const int t = threadIdx.x, r0 = blockIdx.x * RB;
+  const int nr = min(RB, n - r0);
+  if (t < nr) {
+    rs[t] = rstart[r0 + t];
+    rl[t] = rlen[r0 + t];
+  }
+  __syncthreads();
+  int ok = 1;
+  if (t < nr - 1) ok = rs[t] + rl[t] == rstart[r0 + t + 1];
+  const long long A = rs[0], B = rs[nr - 1] + rl[nr - 1];
+  ok = __syncthreads_and(ok && (B - A) <= CAP);
+  const int row = t / LPR, q = t % LPR;
+  double acc = 0.0;
+  if (ok) {
+    const int m = (int)(B - A);
+    double vv[U];
+    int cc[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const int k = t + NT * i;
+      vv[i] = k < m ? val[A + k] : 0.0;
+      cc[i] = k < m ? cgid[A + k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const int k = t + NT * i;
+      if (k < m) buf[k] = vv[i] * x[cc[i]];
+    }
+    __syncthreads();
+    if (row < nr) {
+      const int s = (int)(rs[row] - A), len = rl[row];
+      for (int k = q; k < len; k += LPR) acc += buf[s + k];
+    }
+  } else if (row < nr) {
+    const long long s = rs[row];
+    const int len = rl[row];
+    for (int k = q; k < len; k += LPR) acc = fma(val[s + k], x[cgid[s + k]], acc);
+  }
+#pragma unroll
+  for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+  if (row < nr && q == 0) y[r0 + row] = acc;
+}
+
 // partial sums of up to two dot products: block i sums a grid-stride slice
 __global__ __launch_bounds__(kBlk) void k_dot2(int n, const double *__restrict__ a, const double *__restrict__ b,
                                                const double *__restrict__ c, const double *__restrict__ d,
@@ -259,8 +319,9 @@ static int solve_setup(gdx_problem *p) {
 static void spmv(gdx_problem *p, SolveData *s, const double *x, double *y, hipStream_t st) {
   const int rows_per_block = kBlk / s->lpr;
   const dim3 g((s->n + rows_per_block - 1) / rows_per_block);
-  if (s->lpr == 8)
-    hipLaunchKernelGGL(k_spmv<8>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
+  if (s->lpr == 8)  // rows of at most 32 entries
+    hipLaunchKernelGGL((k_spmv_blk<128, 512>), dim3((s->n + 127) / 128), dim3(512), 0, st, s->n, s->rstart, s->rlen,
+                       s->cgid, p->dev.values, x, y);
   else if (s->lpr == 16)
     hipLaunchKernelGGL(k_spmv<16>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
   else
