@@ -23,8 +23,9 @@
 // All loads of both nodes are issued before the first store: the node's 1D
 // factors (3 directions x 2K+1 offsets x {K,M,P,Q}, to LDS) and each lane's
 // slice of the pattern (row positions e = lane, lane+64, ...), so no global
-// load sits between the stores.  A lane decodes e -> (stencil slot, column
-// component) and writes the 3 rows' values at e: every store instruction
+// load sits between the stores.  The pattern holds, per row position, the
+// column's stencil offsets and component packed in bit fields (decoded once at
+// setup), and a lane writes the 3 rows' values at e: every store instruction
 // writes 64 consecutive doubles.  HBM traffic is the 8 B value writes (plus
 // 32 B of row table per node); the arithmetic is ~10 FLOP per stored value.
 #include <hip/hip_runtime.h>
@@ -57,7 +58,7 @@ static_assert(sizeof(ElastRow) == 32, "ElastRow is 32 bytes");
 struct ElastData {
   double *fac = nullptr;    // [3][nnmax][W][4]: K, M, P, Q at (node, offset + K)
   double *mass = nullptr;   // [3][nnmax]
-  uint16_t *pinv = nullptr; // [npat][stride]: position -> slot * 3 + component
+  uint16_t *pinv = nullptr; // [npat][stride]: position -> o0 | o1 << 3 | o2 << 6 | component << 9
   ElastRow *rows = nullptr;
   int nnmax = 0, stride = 0;
   long long nrownodes = 0;
@@ -127,28 +128,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastR
     for (int k = 0; k < MAXE; ++k) {
       const int e = lane + 64 * k;
       if (e >= len) break;
-      const int s3 = s3v[j][k];
-      const int s = s3 / 3, bc = s3 - 3 * s;
-      const int o0 = s % W, o1 = (s / W) % W, o2 = s / (W * W);
+      const int code = s3v[j][k];
+      const int o0 = code & 7, o1 = (code >> 3) & 7, o2 = (code >> 6) & 7, bc = code >> 9;
       const double2 *f0 = reinterpret_cast<const double2 *>(F + (0 * W + o0) * 4);
       const double2 *f1 = reinterpret_cast<const double2 *>(F + (1 * W + o1) * 4);
       const double2 *f2 = reinterpret_cast<const double2 *>(F + (2 * W + o2) * 4);
       const double2 km0 = f0[0], pq0 = f0[1], km1 = f1[0], pq1 = f1[1], km2 = f2[0], pq2 = f2[1];
-      const double Kd[3] = {km0.x, km1.x, km2.x}, Md[3] = {km0.y, km1.y, km2.y};
-      const double Pd[3] = {pq0.x, pq1.x, pq2.x}, Qd[3] = {pq0.y, pq1.y, pq2.y};
-      const double lap = mu * (Kd[0] * Md[1] * Md[2] + Md[0] * Kd[1] * Md[2] + Md[0] * Md[1] * Kd[2]);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        double v;
-        if (a == bc) {
-          const int c1 = (a + 1) % 3, c2 = (a + 2) % 3;
-          v = lap + lpm * Kd[a] * Md[c1] * Md[c2];
-        } else {
-          const int c = 3 - a - bc;
-          v = (lam * Pd[a] * Qd[bc] + mu * Qd[a] * Pd[bc]) * Md[c];
-        }
-        row0[(size_t)a * len + e] = v;
-      }
+      // t_a = K_a times the M of the other two directions
+      const double t0 = km0.x * (km1.y * km2.y), t1 = km1.x * (km0.y * km2.y), t2 = km2.x * (km0.y * km1.y);
+      const double lap = mu * (t0 + t1 + t2);
+      const bool b0 = bc == 0, b1 = bc == 1, b2 = bc == 2;
+      const double Qb = b0 ? pq0.y : (b1 ? pq1.y : pq2.y), Pb = b0 ? pq0.x : (b1 ? pq1.x : pq2.x);
+      // row component a: diagonal block when a == bc, else M of the third direction
+      const double v0 = b0 ? fma(lpm, t0, lap) : (lam * pq0.x * Qb + mu * pq0.y * Pb) * (b1 ? km2.y : km1.y);
+      const double v1 = b1 ? fma(lpm, t1, lap) : (lam * pq1.x * Qb + mu * pq1.y * Pb) * (b0 ? km2.y : km0.y);
+      const double v2 = b2 ? fma(lpm, t2, lap) : (lam * pq2.x * Qb + mu * pq2.y * Pb) * (b0 ? km1.y : km0.y);
+      row0[e] = v0;
+      row0[(size_t)len + e] = v1;
+      row0[(size_t)2 * len + e] = v2;
     }
   }
 }
@@ -222,6 +219,14 @@ int elast_setup(gdx_problem *p) {
   }
   std::sort(rows.begin(), rows.end(),
             [](const ElastRow &x, const ElastRow &y) { return x.rowstart < y.rowstart; });
+  // position -> packed (stencil offsets, column component): slot * 3 + component
+  // with slot = o0 + W (o1 + W o2); padding entries stay as they are
+  std::vector<uint16_t> code(p->pat16_inv.size());
+  for (size_t i = 0; i < code.size(); ++i) {
+    const int s3 = p->pat16_inv[i], sl = s3 / 3, bc = s3 % 3;
+    const int o0 = sl % W, o1 = (sl / W) % W, o2 = sl / (W * W);
+    code[i] = (uint16_t)(o0 | o1 << 3 | o2 << 6 | bc << 9);
+  }
   ElastData *E = new ElastData;
   p->elast = E;
   E->nnmax = nnmax;
@@ -233,7 +238,7 @@ int elast_setup(gdx_problem *p) {
   };
   bool ok = up(reinterpret_cast<void **>(&E->fac), fac.data(), fac.size() * sizeof(double)) &&
             up(reinterpret_cast<void **>(&E->mass), mass.data(), mass.size() * sizeof(double)) &&
-            up(reinterpret_cast<void **>(&E->pinv), p->pat16_inv.data(), p->pat16_inv.size() * sizeof(uint16_t)) &&
+            up(reinterpret_cast<void **>(&E->pinv), code.data(), code.size() * sizeof(uint16_t)) &&
             up(reinterpret_cast<void **>(&E->rows), rows.data(), rows.size() * sizeof(ElastRow));
   if (!ok) {
     elast_free(p);
