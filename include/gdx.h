@@ -298,7 +298,10 @@ int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int3
    problem, may be NULL: the z rows per workgroup, -1 for the per-tile kernel,
    -2 for the direct-store form, -3 for the wave-slab form); the matrices
    agree to rounding for every choice.  gdx_batch_capture records the numeric launches of every problem
-   (no memsets) and gdx_batch_launch replays them on the context's stream.
+   (no memsets) as independent branches of one graph: the problems are spread over four streams
+   (largest first, to the least loaded), forked from and joined back into the context's stream, so
+   small problems run beside each other; gdx_batch_launch replays the graph on the context's stream
+   and the results equal separate gdx_numeric calls bit for bit.
    The reference has no counterpart: it assembles one problem per call
    (src/FESpaces.jl:785-798). */
 typedef struct gdx_batch gdx_batch;
