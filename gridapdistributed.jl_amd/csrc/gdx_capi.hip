@@ -191,6 +191,7 @@ void release_problem_device(gdx_problem *p) {
   gdx::solve_free(p);
   gdx::flux_free(p);
   gdx::elast_free(p);
+  gdx::pvector_release(p);
   free_device(p);
   for (hipEvent_t &e : p->ev)
     if (e) {

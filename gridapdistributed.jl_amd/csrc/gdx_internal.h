@@ -112,6 +112,7 @@ struct Device {
 
 struct gdx_nn;
 struct gdx_batch;
+struct gdx_pvector;
 
 struct gdx_ctx {
   int device = 0;
@@ -158,6 +159,8 @@ struct gdx_problem {
   void *flux = nullptr;   // nonlinear operator state (gdx_flux.hip)
   void *elast = nullptr;  // elasticity tables (gdx_elast.hip)
   bool coef_cell = false; // kappa from dev.cellk instead of the Fourier modes
+  void *pv = nullptr;     // PRange maps of the device PVectors (gdx_pvector.hip)
+  std::vector<gdx_pvector *> pvecs;  // live PVectors on this problem
 };
 
 namespace gdx {
@@ -165,6 +168,7 @@ void set_error(const std::string &msg);
 void release_problem_device(gdx_problem *p);  // device buffers, solver and flux state, events
 void release_nn_device(gdx_nn *nn);
 void release_batch_device(gdx_batch *b);
+void pvector_release(gdx_problem *p);  // PRange maps; the problem's PVectors lose their device memory
 void batch_forget_problem(gdx_ctx *ctx, gdx_problem *p);  // a destroyed problem leaves every batch
 template <class T>
 void unregister(std::vector<T *> &v, T *x) {

@@ -6,7 +6,7 @@ AffineFEOperator) over the C ABI of libgdx.so (include/gdx.h).
 """
 from . import workloads
 from ._lib import GdxError
-from .algebra import FullyAssembledRows, LocalIndices, PRange, PSparseMatrix, PVector, SubAssembledRows
+from .algebra import DevicePVector, FullyAssembledRows, LocalIndices, PRange, PSparseMatrix, PVector, SubAssembledRows
 from .fespaces import (CLAMPED_X0_TAGS, AffineFEOperator, BodyForce, ConstantCoefficient, ConstantSource, Diffusion,
                        Elasticity, FESpace, Float64, VectorValue,
                        FourierExpCoefficient, GpuProblem, Integral, LinearFunction, ManufacturedSource, ReferenceFE,

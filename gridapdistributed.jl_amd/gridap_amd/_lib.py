@@ -32,7 +32,11 @@ EXPORTED = [
     "gdx_nn_set_options", "gdx_nn_coarse_operator", "gdx_nn_set_multigrid", "gdx_nn_level_operator",
     "gdx_batch_create", "gdx_batch_add", "gdx_batch_autotune", "gdx_batch_capture", "gdx_batch_launch",
     "gdx_batch_destroy", "gdx_problem_set_kernel", "gdx_problem_get_kernel",
+    "gdx_pvector_create", "gdx_pvector_destroy", "gdx_pvector_length", "gdx_pvector_device", "gdx_pvector_set",
+    "gdx_pvector_get", "gdx_pvector_fill", "gdx_pvector_copy_rhs", "gdx_pvector_consistent", "gdx_pvector_assemble",
+    "gdx_pvector_axpy", "gdx_pvector_change_ghost", "gdx_psparse_mul",
 ]
+PRANGE_ROWS, PRANGE_COLS = 0, 1
 
 
 class GdxError(RuntimeError):
@@ -98,6 +102,19 @@ def lib():
         "gdx_device_values": [vp, P(vp), P(i64)],
         "gdx_stream": [vp, P(vp)],
         "gdx_sync": [vp],
+        "gdx_pvector_create": [vp, i32, P(vp)],
+        "gdx_pvector_destroy": [vp],
+        "gdx_pvector_length": [vp, P(i64)],
+        "gdx_pvector_device": [vp, P(vp)],
+        "gdx_pvector_set": [vp, vp],
+        "gdx_pvector_get": [vp, vp],
+        "gdx_pvector_fill": [vp, ctypes.c_double],
+        "gdx_pvector_copy_rhs": [vp],
+        "gdx_pvector_consistent": [vp],
+        "gdx_pvector_assemble": [vp],
+        "gdx_pvector_axpy": [ctypes.c_double, vp, vp],
+        "gdx_pvector_change_ghost": [vp, vp, i32],
+        "gdx_psparse_mul": [vp, vp, vp],
         "gdx_phase_times": [vp, P(ctypes.c_double), i32],
         "gdx_owned_nnz": [vp, P(i64)],
         "gdx_spmv": [vp, vp, vp],

@@ -5,13 +5,21 @@ PartitionedArrays types the reference returns, src/Algebra.jl:1013-1015).
 * ``LocalIndices`` / ``PRange``   -- OwnAndGhostIndices built at src/Algebra.jl:1240-1252
 * ``PSparseMatrix``              -- fields matrix_partition, row_partition, col_partition
                                     (test/FESpacesTests.jl:29-31)
-* ``PVector``                    -- vector on the row partition of A (src/Algebra.jl:720-753)
+* ``PVector``                    -- vector on the row partition of A (src/Algebra.jl:720-753),
+                                    with PArrays' consistent! and assemble!
+* ``DevicePVector``              -- the same layout in device memory, with consistent!,
+                                    assemble!, axpy! (src/Algebra.jl:42-49), change_ghost
+                                    (src/Algebra.jl:270-290) and mul!(y, A, x) on the device
 
 Local matrices are 0-based CSR (the SparseMatrixCSR{0,Float64,Int} layout of
 test/PoissonTests.jl:40); ``to_csc()`` gives the default SparseMatrixCSC view.
 """
+import ctypes
+
 import numpy as np
 import scipy.sparse as sp
+
+from . import _lib
 
 
 class SubAssembledRows:
@@ -88,6 +96,98 @@ class PVector:
 
     def copy(self):
         return PVector([v.copy() for v in self.partition], self.index_partition)
+
+    def _own_global(self):
+        g = np.zeros(self.index_partition[0].ngids)
+        for v, ip in zip(self.partition, self.index_partition):
+            g[ip.own_to_global] = v[: ip.own_length()]
+        return g
+
+    def consistent(self):
+        """consistent!: every ghost entry takes its owner's value."""
+        g = self._own_global()
+        for v, ip in zip(self.partition, self.index_partition):
+            v[ip.own_length():] = g[ip.ghost_to_global]
+        return self
+
+    def assemble(self):
+        """assemble!: ghost entries are added into their owner's entry (the owner's
+        value first, then the copies in ascending part and local index), then zeroed."""
+        g = self._own_global()
+        for v, ip in zip(self.partition, self.index_partition):
+            np.add.at(g, ip.ghost_to_global, v[ip.own_length():])  # sequential, in index order
+        for v, ip in zip(self.partition, self.index_partition):
+            v[: ip.own_length()] = g[ip.own_to_global]
+            v[ip.own_length():] = 0.0
+        return self
+
+    def concatenated(self):
+        """Every part's local vector, concatenated in part order (the DevicePVector layout)."""
+        return np.concatenate(self.partition) if self.partition else np.zeros(0)
+
+
+class DevicePVector:
+    """A PVector in device memory on a problem's row (default) or column PRange:
+    every part's local vector (own entries, then ghost entries) concatenated in
+    part order, owned by the library and tied to its problem."""
+
+    def __init__(self, problem, which=_lib.PRANGE_ROWS):
+        h = ctypes.c_void_p()
+        _lib.call("gdx_pvector_create", problem.handle, int(which), ctypes.byref(h))
+        self.handle, self.problem, self.which = h, problem, int(which)
+        n = ctypes.c_int64()
+        _lib.call("gdx_pvector_length", h, ctypes.byref(n))
+        self.n = n.value
+
+    def set(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (self.n,):
+            raise ValueError(f"a PVector on this PRange has {self.n} local entries")
+        _lib.call("gdx_pvector_set", self.handle, v.ctypes.data)
+        return self
+
+    def get(self):
+        out = np.empty(self.n)
+        _lib.call("gdx_pvector_get", self.handle, out.ctypes.data)
+        return out
+
+    def fill(self, value):
+        _lib.call("gdx_pvector_fill", self.handle, float(value))
+        return self
+
+    def copy_rhs(self):
+        """The assembled b (row PRange)."""
+        _lib.call("gdx_pvector_copy_rhs", self.handle)
+        return self
+
+    def consistent(self):
+        _lib.call("gdx_pvector_consistent", self.handle)
+        return self
+
+    def assemble(self):
+        _lib.call("gdx_pvector_assemble", self.handle)
+        return self
+
+    def axpy(self, alpha, x):
+        """axpy!(alpha, x, self): self = alpha x + self, then consistent!."""
+        _lib.call("gdx_pvector_axpy", float(alpha), x.handle, self.handle)
+        return self
+
+    def change_ghost_from(self, src, make_consistent=False):
+        """change_ghost: src's own values on this PVector's PRange, ghosts zero."""
+        _lib.call("gdx_pvector_change_ghost", src.handle, self.handle, int(bool(make_consistent)))
+        return self
+
+    def close(self):
+        if self.handle:
+            _lib.call("gdx_pvector_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class PSparseMatrix:

@@ -22,7 +22,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from .algebra import FullyAssembledRows, LocalIndices, PSparseMatrix, PVector, SubAssembledRows
+from .algebra import DevicePVector, FullyAssembledRows, LocalIndices, PSparseMatrix, PVector, SubAssembledRows
 
 
 class _Lagrangian:
@@ -275,6 +275,16 @@ class GpuProblem:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty(self.ngdofs)
         _lib.call("gdx_spmv", self.handle, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def pvector(self, which=_lib.PRANGE_ROWS):
+        """A device PVector on A's row (default) or column PRange."""
+        return DevicePVector(self, which)
+
+    def mul(self, y, x):
+        """mul!(y, A, x) on the device: x (column PRange) is made consistent first,
+        then every part's local CSR times its local x; y is on the row PRange."""
+        _lib.call("gdx_psparse_mul", self.handle, y.handle, x.handle)
         return y
 
     def pcg(self, rtol=1e-8, maxiter=10000, want_x=True):

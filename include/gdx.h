@@ -312,6 +312,42 @@ int gdx_batch_capture(gdx_batch *batch);
 int gdx_batch_launch(gdx_batch *batch);
 int gdx_batch_destroy(gdx_batch *batch);
 
+/* Device PVectors on a problem's row or column PRange and the PArrays operations
+   of the solve side.  A PVector holds every part's local vector (own entries,
+   then ghost entries, in the order of gdx_export_rows / gdx_export_cols)
+   concatenated in part order, in device memory the library owns; on the row
+   PRange that is the layout of the assembled b.  PVectors belong to their
+   problem: gdx_problem_destroy (or gdx_finalize) releases their memory and
+   their later calls return GDX_ERR_STATE; gdx_pvector_destroy frees the handle.
+     consistent!    ghost entries take their owner's value
+     assemble!      ghost entries are added into their owner's entry (owner
+                    first, then the copies in ascending part and local index),
+                    then zeroed
+     axpy!          y = alpha x + y, then consistent!(y)      src/Algebra.jl:42-49
+     change_ghost   own values of src onto dst's PRange, dst's ghosts zero,
+                    then consistent!(dst) if make_consistent  src/Algebra.jl:270-290
+     mul!(y, A, x)  consistent!(x), then every part's local CSR times its local
+                    x; y on A's row PRange, x on its column PRange (the
+                    PSparseMatrix product the reference's tests use,
+                    test/PLaplacianTests.jl:50-52)
+   Calls queue work on the context's stream; set and get synchronize. */
+#define GDX_PRANGE_ROWS 0
+#define GDX_PRANGE_COLS 1
+typedef struct gdx_pvector gdx_pvector;
+int gdx_pvector_create(gdx_problem *prob, int32_t which, gdx_pvector **out);
+int gdx_pvector_destroy(gdx_pvector *v);
+int gdx_pvector_length(gdx_pvector *v, int64_t *n);
+int gdx_pvector_device(gdx_pvector *v, double **data);
+int gdx_pvector_set(gdx_pvector *v, const double *host);
+int gdx_pvector_get(gdx_pvector *v, double *host);
+int gdx_pvector_fill(gdx_pvector *v, double value);
+int gdx_pvector_copy_rhs(gdx_pvector *v);
+int gdx_pvector_consistent(gdx_pvector *v);
+int gdx_pvector_assemble(gdx_pvector *v);
+int gdx_pvector_axpy(double alpha, gdx_pvector *x, gdx_pvector *y);
+int gdx_pvector_change_ghost(gdx_pvector *src, gdx_pvector *dst, int32_t make_consistent);
+int gdx_psparse_mul(gdx_problem *prob, gdx_pvector *y, gdx_pvector *x);
+
 /* device views and timing */
 int gdx_device_values(gdx_problem *prob, double **values, int64_t *nnz_total);
 int gdx_stream(gdx_problem *prob, void **stream);
