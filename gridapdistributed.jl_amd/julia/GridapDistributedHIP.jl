@@ -21,6 +21,7 @@ using Gridap.FESpaces
 using GridapDistributed
 using PartitionedArrays
 using SparseMatricesCSR
+using LinearAlgebra
 
 const libgdx = joinpath(@__DIR__, "..", "libgdx.so")
 
@@ -146,6 +147,68 @@ function newton_hip(p::GdxProblem, x0::Vector{Float64}; rtol=1e-10, atol=0.0, ma
                              (Ptr{Cvoid}, Ref{GdxNewtonOpts}, Ptr{Float64}, Ref{GdxNewtonReport}), p.h, o, x, rep),
                        :gdx_newton)
   x, rep[]
+end
+
+"""
+    GdxPVector(p, which)   # which = 0: A's row PRange, 1: its column PRange
+
+A PVector in device memory: every part's local vector (own, then ghost
+entries, in the order of the exported PRanges) concatenated in part order.
+The PArrays operations below run on the device; like PArrays' own
+`consistent!(v) |> wait`, they return something to `wait` on (work is queued
+on the problem's stream; `Vector(v)` synchronizes).
+"""
+mutable struct GdxPVector
+  h::Ptr{Cvoid}
+  p::GdxProblem
+  n::Int64
+  function GdxPVector(p::GdxProblem, which::Integer)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gdx_pvector_create, libgdx), Cint, (Ptr{Cvoid}, Int32, Ref{Ptr{Cvoid}}), p.h, which, h),
+          :gdx_pvector_create)
+    n = Ref{Int64}(0)
+    check(ccall((:gdx_pvector_length, libgdx), Cint, (Ptr{Cvoid}, Ref{Int64}), h[], n), :gdx_pvector_length)
+    v = new(h[], p, n[])
+    finalizer(x -> ccall((:gdx_pvector_destroy, libgdx), Cint, (Ptr{Cvoid},), x.h), v)
+  end
+end
+
+struct GdxQueued end
+Base.wait(::GdxQueued) = nothing
+
+function Base.copyto!(v::GdxPVector, x::Vector{Float64})
+  length(x) == v.n || throw(DimensionMismatch("a GdxPVector on this PRange has $(v.n) local entries"))
+  GC.@preserve x check(ccall((:gdx_pvector_set, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), v.h, x), :gdx_pvector_set)
+  v
+end
+function Base.Vector(v::GdxPVector)
+  x = Vector{Float64}(undef, v.n)
+  GC.@preserve x check(ccall((:gdx_pvector_get, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), v.h, x), :gdx_pvector_get)
+  x
+end
+function PartitionedArrays.consistent!(v::GdxPVector)
+  check(ccall((:gdx_pvector_consistent, libgdx), Cint, (Ptr{Cvoid},), v.h), :gdx_pvector_consistent)
+  GdxQueued()
+end
+function PartitionedArrays.assemble!(v::GdxPVector)
+  check(ccall((:gdx_pvector_assemble, libgdx), Cint, (Ptr{Cvoid},), v.h), :gdx_pvector_assemble)
+  GdxQueued()
+end
+# src/Algebra.jl:42-49
+function LinearAlgebra.axpy!(α::Real, x::GdxPVector, y::GdxPVector)
+  check(ccall((:gdx_pvector_axpy, libgdx), Cint, (Float64, Ptr{Cvoid}, Ptr{Cvoid}), α, x.h, y.h), :gdx_pvector_axpy)
+  y
+end
+# src/Algebra.jl:270-290: own values of a onto b's PRange
+function change_ghost!(b::GdxPVector, a::GdxPVector; make_consistent=false)
+  check(ccall((:gdx_pvector_change_ghost, libgdx), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int32), a.h, b.h,
+              make_consistent ? 1 : 0), :gdx_pvector_change_ghost)
+  b
+end
+# mul!(y, A, x) of PSparseMatrix (test/PLaplacianTests.jl:50-52): y on A's rows, x on its cols
+function LinearAlgebra.mul!(y::GdxPVector, A::GdxProblem, x::GdxPVector)
+  check(ccall((:gdx_psparse_mul, libgdx), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), A.h, y.h, x.h), :gdx_psparse_mul)
+  y
 end
 
 end # module
