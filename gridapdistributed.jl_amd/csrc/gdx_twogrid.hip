@@ -404,53 +404,55 @@ __global__ void k_prolong_add_c(int mf, const double *__restrict__ ec, double *_
 }
 
 // coarsest level (m <= 8): damped-Jacobi sweeps from zero in one workgroup;
-// the stencil is staged in LDS when it fits (m <= 6)
+// the stencil is staged in LDS when it fits (m <= 6).  L lanes per grid point
+// (the largest power of two with L * m^3 <= 1024) split the point's 27
+// stencil terms and sum them with a fixed shuffle tree, so a sweep is not one
+// lane's chain of 27 dependent LDS reads.
 constexpr int kCoarsestMax = 8;
 constexpr int kCoarsestLds = 6 * 6 * 6;
-__global__ __launch_bounds__(512) void k_coarsest(int m, double w, int sweeps, const double *__restrict__ cstg,
-                                                  const double *__restrict__ rhs, double *__restrict__ e) {
+constexpr int kCoarsestThreads = 1024;
+__global__ __launch_bounds__(kCoarsestThreads) void k_coarsest(int m, double w, int sweeps,
+                                                               const double *__restrict__ cstg,
+                                                               const double *__restrict__ rhs,
+                                                               double *__restrict__ e) {
   __shared__ double ea[512], eb[512], cl[27 * kCoarsestLds];
   const int tot = m * m * m;
-  const int j = threadIdx.x;
+  int L = 32;
+  while (L > 1 && L * tot > kCoarsestThreads) L >>= 1;
+  const int t = threadIdx.x, j = t / L, sub = t % L;  // the L lanes of a point share one wave
   int jx = 0, jy = 0, jz = 0;
   const bool live = j < tot && !padding(j, m, jx, jy, jz);
   const bool staged = tot <= kCoarsestLds;
   if (staged)
-    for (int i = j; i < 27 * tot; i += blockDim.x) cl[i] = cstg[i];
+    for (int i = t; i < 27 * tot; i += blockDim.x) cl[i] = cstg[i];
   const double *cst = staged ? cl : cstg;
-  if (j < tot) ea[j] = 0.0;
+  if (sub == 0 && j < tot) ea[j] = 0.0;
   // the right-hand side and the diagonal are read once, not per sweep
   const double rj = live ? rhs[j] : 0.0;
   __syncthreads();
   const double dj = live ? cst[13 * tot + j] : 1.0;
-  // neighbours inside the interior, as a bit mask over the 27 stencil slots
+  // this lane's stencil slots inside the interior, as a bit mask
   uint32_t nbr = 0;
-  for (int k = 0; k < 27; ++k) {
+  for (int k = sub; k < 27; k += L) {
     const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
     if (live && x >= 0 && y >= 0 && z >= 0 && x <= m - 2 && y <= m - 2 && z <= m - 2) nbr |= 1u << k;
   }
   double *a = ea, *b = eb;
   for (int s = 0; s < sweeps; ++s) {
-    if (j < tot) {
-      double v = 0.0;
-      if (live) {
-        double r = rj;
-#pragma unroll
-        for (int k = 0; k < 27; ++k) {
-          if (!(nbr >> k & 1u)) continue;
-          const int off = (k % 3 - 1) + m * ((k / 3) % 3 - 1) + m * m * (k / 9 - 1);
-          r -= cst[k * tot + j] * a[j + off];
-        }
-        v = a[j] + w * r / dj;
-      }
-      b[j] = v;
+    double part = 0.0;
+    for (int k = sub; k < 27; k += L) {
+      if (!(nbr >> k & 1u)) continue;
+      const int off = (k % 3 - 1) + m * ((k / 3) % 3 - 1) + m * m * (k / 9 - 1);
+      part = fma(cst[k * tot + j], a[j + off], part);
     }
+    for (int o = L / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, L);
+    if (sub == 0 && j < tot) b[j] = live ? a[j] + w * (rj - part) / dj : 0.0;
     __syncthreads();
-    double *t = a;
+    double *tmp = a;
     a = b;
-    b = t;
+    b = tmp;
   }
-  if (j < tot) e[j] = a[j];
+  if (sub == 0 && j < tot) e[j] = a[j];
 }
 
 // partial sums of squares (and of a*b) over a grid-stride slice
@@ -752,7 +754,8 @@ double *vcycle(TwoGrid *T, size_t l, hipStream_t st) {
   Level &L = T->lv[l];
   const int64_t v = (int64_t)L.m * L.m * L.m;
   if (l + 1 == T->lv.size()) {
-    hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(512), 0, st, L.m, kOmegaC, T->coarsest_sweeps, L.cst, L.rhs, L.e);
+    hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(kCoarsestThreads), 0, st, L.m, kOmegaC, T->coarsest_sweeps, L.cst, L.rhs,
+                       L.e);
     return L.e;
   }
   double *a = L.e, *b = L.e2;
