@@ -112,19 +112,78 @@ __global__ void k_restrict(int n, int nf, const double *__restrict__ flat, const
 
 // fine matrix rows -> 27-point stencil on the lattice: fst[slot][node], slot of
 // the column's offset (dx+1) + 3(dy+1) + 9(dz+1); Dirichlet columns are absent
-// from the free x free matrix and stay zero
-__global__ void k_fine_stencil(int ng, int nf, const long long *__restrict__ rstart, const int *__restrict__ rlen,
-                               const int *__restrict__ cgid, const double *__restrict__ val,
-                               const long long *__restrict__ gn, double *__restrict__ fst) {
-  const int g = blockIdx.x * kB + threadIdx.x;
-  if (g >= ng) return;
-  const long long a = gn[g];
-  const int ax = (int)(a % nf), ay = (int)((a / nf) % nf), az = (int)(a / ((int64_t)nf * nf));
+// from the free x free matrix and stay zero (fst is cleared before).
+//
+// kFsRows consecutive rows per block.  When their entries are stored back to
+// back, the block reads them coalesced (one entry per thread, its row found by
+// bisection over the block's row starts), places each value in an LDS tile
+// [slot][row] and then writes the tile plane by plane, so consecutive lanes
+// store consecutive lattice nodes.  Other blocks walk their rows one thread
+// each.  The slot comes from the node difference: b - a + (1 + nf + nf^2) has
+// base-nf digits (dx+1, dy+1, dz+1) in 0..2, found by comparisons.
+constexpr int kFsRows = 64;
+__device__ __forceinline__ int stencil_slot(long long d, int nf) {
+  const long long nf2 = (long long)nf * nf;
+  long long t = d + 1 + nf + nf2;
+  if (t < 0 || t >= 3 * nf2) return -1;
+  const int dz1 = t >= 2 * nf2 ? 2 : (t >= nf2 ? 1 : 0);
+  t -= dz1 * nf2;
+  const int dy1 = t >= 2 * (long long)nf ? 2 : (t >= nf ? 1 : 0);
+  t -= (long long)dy1 * nf;
+  return t > 2 ? -1 : (int)t + 3 * dy1 + 9 * dz1;
+}
+__global__ __launch_bounds__(kB) void k_fine_stencil(int ng, int nf, const long long *__restrict__ rstart,
+                                                     const int *__restrict__ rlen, const int *__restrict__ cgid,
+                                                     const double *__restrict__ val,
+                                                     const long long *__restrict__ gn, double *__restrict__ fst) {
+  constexpr int CAP = 27 * kFsRows;
+  __shared__ double tile[27][kFsRows];
+  __shared__ long long rs[kFsRows], an[kFsRows];
+  __shared__ int rl[kFsRows];
+  __shared__ unsigned int present[kFsRows];
+  const int t = threadIdx.x, g0 = blockIdx.x * kFsRows;
+  const int nr = min(kFsRows, ng - g0);
+  if (nr <= 0) return;  // uniform over the block
   const int64_t nfl = (int64_t)nf * nf * nf;
-  for (int k = 0; k < rlen[g]; ++k) {
-    const long long b = gn[cgid[rstart[g] + k]];
-    const int dx = (int)(b % nf) - ax, dy = (int)((b / nf) % nf) - ay, dz = (int)(b / ((int64_t)nf * nf)) - az;
-    fst[(int64_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * nfl + a] = val[rstart[g] + k];
+  if (t < nr) {
+    rs[t] = rstart[g0 + t];
+    rl[t] = rlen[g0 + t];
+    an[t] = gn[g0 + t];
+    present[t] = 0u;
+  }
+  __syncthreads();
+  int ok = 1;
+  if (t < nr - 1) ok = rs[t] + rl[t] == rstart[g0 + t + 1];
+  const long long A = rs[0], B = rs[nr - 1] + rl[nr - 1];
+  ok = __syncthreads_and(ok && (B - A) <= CAP);
+  if (!ok) {
+    if (t < nr) {
+      const long long a = an[t], s0 = rs[t];
+      for (int k = 0; k < rl[t]; ++k) {
+        const int sl = stencil_slot(gn[cgid[s0 + k]] - a, nf);
+        if (sl >= 0) fst[(int64_t)sl * nfl + a] = val[s0 + k];
+      }
+    }
+    return;
+  }
+  const int m = (int)(B - A);
+  for (int k = t; k < m; k += kB) {
+    int lo = 0, hi = nr - 1;  // the row holding entry k: the last r with rs[r] - A <= k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rs[mid] - A <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int sl = stencil_slot(gn[cgid[A + k]] - an[lo], nf);
+    if (sl >= 0) {
+      tile[sl][lo] = val[A + k];
+      atomicOr(&present[lo], 1u << sl);
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < 27 * nr; i += kB) {
+    const int sl = i / nr, r = i - sl * nr;
+    if (present[r] >> sl & 1u) fst[(int64_t)sl * nfl + an[r]] = tile[sl][r];
   }
 }
 
@@ -906,8 +965,8 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
   const int *rl = nullptr, *cg = nullptr;
   if ((rc = solve_csr_view(p, &rs, &rl, &cg))) return rc;
   if (hipMemsetAsync(T->fst, 0, 27 * nfl * sizeof(double), st) != hipSuccess) return GDX_ERR_HIP;
-  hipLaunchKernelGGL(k_fine_stencil, dim3(nb(T->ng)), dim3(kB), 0, st, T->ng, nf, rs, rl, cg, p->dev.values,
-                     T->gid_node, T->fst);
+  hipLaunchKernelGGL(k_fine_stencil, dim3((unsigned)std::max(1, (T->ng + kFsRows - 1) / kFsRows)), dim3(kB), 0, st,
+                     T->ng, nf, rs, rl, cg, p->dev.values, T->gid_node, T->fst);
   hipLaunchKernelGGL(k_galerkin_e, dim3(nb(27 * v)), dim3(kB), 0, st, n, nf, 0, nfl, T->fst, T->cst);
   if ((rc = mg_setup(T, st))) return rc;
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
