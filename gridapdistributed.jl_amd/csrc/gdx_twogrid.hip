@@ -197,12 +197,19 @@ __constant__ signed char kGalPairs[3][7][2] = {
     {{-1, 0}, {-1, 1}, {0, 1}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},                  // d = -1
     {{-1, -1}, {-1, 0}, {0, -1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}},                // d = 0
     {{0, -1}, {1, -1}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}}};                 // d = +1
-__global__ void k_galerkin_e(int mc, int S, int off, int64_t fine_tot, const double *__restrict__ af,
-                             double *__restrict__ cst) {
+// L lanes per output entry (a power of two <= 64, so a group sits in one wave)
+// split its (oa, ob) pairs, flattened z-major, and sum them with a fixed
+// shuffle tree; with L = 1 the order is the nested loops' order.  Small levels
+// use many lanes: one lane's chain of up to 7^3 dependent loads would bound them.
+template <int L>
+__global__ __launch_bounds__(kB) void k_galerkin_e(int mc, int S, int off, int64_t fine_tot,
+                                                   const double *__restrict__ af, double *__restrict__ cst) {
   const int64_t tot = (int64_t)mc * mc * mc;
-  const int64_t t = blockIdx.x * (int64_t)kB + threadIdx.x;
-  if (t >= 27 * tot) return;
-  const int k = (int)(t / tot);  // output slot: consecutive threads share it (coalesced stores)
+  const int64_t tq = blockIdx.x * (int64_t)kB + threadIdx.x;
+  const int64_t t = tq / L;
+  const int sub = (int)(tq % L);
+  if (t >= 27 * tot) return;  // whole lane groups: kB is a multiple of L
+  const int k = (int)(t / tot);  // output slot: consecutive outputs share it (coalesced stores)
   const int64_t j = t - k * tot;
   const int jx = (int)(j % mc), jy = (int)((j / mc) % mc), jz = (int)(j / ((int64_t)mc * mc));
   const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
@@ -210,7 +217,7 @@ __global__ void k_galerkin_e(int mc, int S, int off, int64_t fine_tot, const dou
   double acc = 0.0;
   const bool live = jx < mc - 1 && jy < mc - 1 && jz < mc - 1 && Jx + dx >= 1 && Jy + dy >= 1 && Jz + dz >= 1 &&
                     Jx + dx <= mc - 1 && Jy + dy <= mc - 1 && Jz + dz <= mc - 1;
-  if (live) {
+  if (live && L == 1) {
     const int nx = dx ? 3 : 7, ny = dy ? 3 : 7, nz = dz ? 3 : 7;
     for (int pz = 0; pz < nz; ++pz) {
       const int oaz = kGalPairs[dz + 1][pz][0], obz = kGalPairs[dz + 1][pz][1];
@@ -231,8 +238,44 @@ __global__ void k_galerkin_e(int mc, int S, int off, int64_t fine_tot, const dou
         }
       }
     }
+  } else if (live) {
+    const int nx = dx ? 3 : 7, ny = dy ? 3 : 7, nz = dz ? 3 : 7;
+    const int n = nx * ny * nz;
+    for (int q = sub; q < n; q += L) {
+      const int px = q % nx, py = (q / nx) % ny, pz = q / (nx * ny);
+      const int oaz = kGalPairs[dz + 1][pz][0], obz = kGalPairs[dz + 1][pz][1];
+      const int oay = kGalPairs[dy + 1][py][0], oby = kGalPairs[dy + 1][py][1];
+      const int oax = kGalPairs[dx + 1][px][0], obx = kGalPairs[dx + 1][px][1];
+      const double wz = (oaz ? 0.5 : 1.0) * (obz ? 0.5 : 1.0);
+      const double wyz = wz * (oay ? 0.5 : 1.0) * (oby ? 0.5 : 1.0);
+      const double w = wyz * (oax ? 0.5 : 1.0) * (obx ? 0.5 : 1.0);
+      const int ex = 2 * dx + obx - oax + 1, ey = 2 * dy + oby - oay + 1, ez = 2 * dz + obz - oaz + 1;
+      const int64_t az = 2 * Jz + oaz - off;
+      const int64_t ayz = (2 * Jy + oay - off) + (int64_t)S * az;
+      const int64_t a = (2 * Jx + oax - off) + (int64_t)S * ayz;
+      acc = fma(w, af[(int64_t)(ex + 3 * ey + 9 * ez) * fine_tot + a], acc);
+    }
   }
-  cst[k * tot + j] = acc;
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+  if (sub == 0) cst[k * tot + j] = acc;
+}
+
+// Galerkin product launch: lanes per output so that a level has about 2^20 threads
+void galerkin(int mc, int S, int off, int64_t fine_tot, const double *af, double *cst, hipStream_t st) {
+  const int64_t outs = 27 * (int64_t)mc * mc * mc;
+  int L = 1;
+  while (L < 64 && outs * L < (1 << 20)) L *= 2;
+  const dim3 g((unsigned)((outs * L + kB - 1) / kB)), b(kB);
+  switch (L) {
+    case 1: hipLaunchKernelGGL(k_galerkin_e<1>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    case 2: hipLaunchKernelGGL(k_galerkin_e<2>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    case 4: hipLaunchKernelGGL(k_galerkin_e<4>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    case 8: hipLaunchKernelGGL(k_galerkin_e<8>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    case 16: hipLaunchKernelGGL(k_galerkin_e<16>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    case 32: hipLaunchKernelGGL(k_galerkin_e<32>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+    default: hipLaunchKernelGGL(k_galerkin_e<64>, g, b, 0, st, mc, S, off, fine_tot, af, cst); break;
+  }
 }
 
 // ec = s * out (network units -> coarse error), 0 on the padding plane; or 0 everywhere
@@ -799,9 +842,7 @@ int mg_setup(TwoGrid *T, hipStream_t st) {
   }
   for (size_t l = 1; l < T->lv.size(); ++l) {
     const Level &F = T->lv[l - 1];
-    const int64_t vc = (int64_t)T->lv[l].m * T->lv[l].m * T->lv[l].m;
-    hipLaunchKernelGGL(k_galerkin_e, dim3(nb(27 * vc)), dim3(kB), 0, st, T->lv[l].m, F.m, 1,
-                       (int64_t)F.m * F.m * F.m, F.cst, T->lv[l].cst);
+    galerkin(T->lv[l].m, F.m, 1, (int64_t)F.m * F.m * F.m, F.cst, T->lv[l].cst, st);
   }
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
@@ -967,7 +1008,7 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
   if (hipMemsetAsync(T->fst, 0, 27 * nfl * sizeof(double), st) != hipSuccess) return GDX_ERR_HIP;
   hipLaunchKernelGGL(k_fine_stencil, dim3((unsigned)std::max(1, (T->ng + kFsRows - 1) / kFsRows)), dim3(kB), 0, st,
                      T->ng, nf, rs, rl, cg, p->dev.values, T->gid_node, T->fst);
-  hipLaunchKernelGGL(k_galerkin_e, dim3(nb(27 * v)), dim3(kB), 0, st, n, nf, 0, nfl, T->fst, T->cst);
+  galerkin(n, nf, 0, nfl, T->fst, T->cst, st);
   if ((rc = mg_setup(T, st))) return rc;
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
