@@ -381,6 +381,28 @@ __global__ void k_cos_grad(int64_t tot, const float *__restrict__ out, const flo
 }
 
 // one damped-Jacobi sweep on the coarse system: b = a + w Dc^-1 (rc - Ac a)
+// r0 - sum over the interior neighbours of cst[k][j] x[neighbour k], in stencil
+// order.  The 54 loads are predicated (zero outside the interior) and issued
+// before the subtractions, so a small grid is not one chain of dependent
+// loads; a skipped term subtracts 0 * 0, which leaves r unchanged exactly.
+__device__ __forceinline__ double stencil_residual(double r0, int n, int64_t tot, int64_t j, int jx, int jy, int jz,
+                                                   const double *__restrict__ cst,
+                                                   const double *__restrict__ x) {
+  double cv[27], xv[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
+    const int X = jx + ox, Y = jy + oy, Z = jz + oz;
+    const bool in = X >= 0 && Y >= 0 && Z >= 0 && X <= n - 2 && Y <= n - 2 && Z <= n - 2;
+    cv[k] = in ? cst[k * tot + j] : 0.0;
+    xv[k] = in ? x[j + ox + (int64_t)n * (oy + (int64_t)n * oz)] : 0.0;
+  }
+  double r = r0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) r -= cv[k] * xv[k];
+  return r;
+}
+
 __global__ void k_coarse_sweep(int n, double w, const double *__restrict__ cst, const double *__restrict__ rc,
                                const double *__restrict__ a, double *__restrict__ b) {
   const int64_t tot = (int64_t)n * n * n;
@@ -391,12 +413,7 @@ __global__ void k_coarse_sweep(int n, double w, const double *__restrict__ cst, 
     b[j] = 0.0;
     return;
   }
-  double r = rc[j];
-  for (int k = 0; k < 27; ++k) {
-    const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
-    if (x < 0 || y < 0 || z < 0 || x > n - 2 || y > n - 2 || z > n - 2) continue;
-    r -= cst[k * tot + j] * a[x + (int64_t)n * (y + (int64_t)n * z)];
-  }
+  const double r = stencil_residual(rc[j], n, tot, j, jx, jy, jz, cst, a);
   b[j] = a[j] + w * r / cst[13 * tot + j];
 }
 
@@ -443,13 +460,7 @@ __global__ void k_resid(int m, const double *__restrict__ cst, const double *__r
     res[j] = 0.0;
     return;
   }
-  double r = rhs[j];
-  for (int k = 0; k < 27; ++k) {
-    const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
-    if (x < 0 || y < 0 || z < 0 || x > m - 2 || y > m - 2 || z > m - 2) continue;
-    r -= cst[k * tot + j] * e[x + (int64_t)m * (y + (int64_t)m * z)];
-  }
-  res[j] = r;
+  res[j] = stencil_residual(rhs[j], m, tot, j, jx, jy, jz, cst, e);
 }
 
 // first damped-Jacobi sweep from zero: e = w Dc^-1 rhs
