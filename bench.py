@@ -81,6 +81,32 @@ def cpu_baseline_flux(n=48):
                       f"in-place t={r.times[2]:.3f}s; host {cpu_model()}; wall {wall:.1f}s"}
 
 
+def cpu_baseline_solve(n=28):
+    """The reference's A\\b (README.md:76: the system gathered and solved by a
+    sparse LU) on the host, one thread: scipy's SuperLU on the oracle's matrix
+    of a bounded Q1 sample (Fourier kappa; the factorisation time does not
+    depend on the coefficient's values)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import scipy.sparse.linalg as spla
+    import oracle
+    from _helpers import owned_global
+    from gridap_amd import workloads as W
+
+    p = W.poisson3d_q1(n, (2, 2, 2), modes=W.fourier_modes(7, 8, 4, 0.5), rhs_const=1.0)
+    r = oracle.run(p)
+    A, b = owned_global(r.parts, r.ngdofs)
+    A = A.tocsc()
+    t0 = time.perf_counter()
+    x = spla.spsolve(A, b)
+    t = time.perf_counter() - t0
+    rr = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+    return {"value": round(t, 3), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"sparse LU solve (scipy SuperLU) of the oracle's 3D Q1 {n}^3 matrix, {r.ngdofs} free dofs, "
+                      f"relres {rr:.1e}; host {cpu_model()}"}
+
+
 def bench_mixed(prob, args, nnz, ms_fp64, variant, rows_z):
     """The mixed form (FP32 sum-factorised contractions, FP64 CSR values) on
     the same problem: best z-marching column length, its Gnnz/s over the FP64
@@ -259,6 +285,15 @@ def bench_learned(args):
             for name, r in rows.items():
                 res[name].append(r[1:])  # iterations, true relres, ms, fell back
     levels = net.set_multigrid(True, 1)
+    # roofline of the solves' dominant kernel, the CSR SpMV, timed live with hipEvents
+    nnz, nd = prob.owned_nnz(), prob.ngdofs
+    ms_spmv = prob.spmv_device_ms(50)
+    spmv_bytes = 12 * nnz + 28 * nd  # value + column id per entry; row start, length, x and y per row
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_spmv_blk_learned_q1_{cells}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("traffic_bytes")
     summ = {}
     for name, rs in res.items():
         it, rr, ms, fb = (np.array([r[i] for r in rs], dtype=float) for i in range(4))
@@ -281,8 +316,13 @@ def bench_learned(args):
                    "training_steps": nsteps, "training_s": round(t_train, 1), "vcycle_grids": levels,
                    "loss_first": round(float(np.mean(losses[:20])), 4), "loss_last": round(float(np.mean(losses[-20:])), 4),
                    "time_ratio_learned_over_jacobi": round(s_l / s_j, 3),
-                   "solvers": summ},
+                   "solvers": summ, "spmv_ms": round(ms_spmv, 4), "nnz_owned": nnz},
+        "roofline": {"bound": "hbm", "achieved": round(spmv_bytes / (ms_spmv * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(spmv_bytes / (ms_spmv * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_spmv_blk<128,512>", "alg_bytes_per_launch": int(spmv_bytes)},
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_solve()
     net.close()
     prob.close()
     ctx.close()

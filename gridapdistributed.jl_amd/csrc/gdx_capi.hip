@@ -640,6 +640,17 @@ int gdx_spmv(gdx_problem *p, const double *x, double *y) {
   return rc;
 }
 
+int gdx_spmv_timed(gdx_problem *p, int32_t reps, double *ms) {
+  GDX_REQUIRE(p && ms, GDX_ERR_ARG, "NULL argument");
+  GDX_REQUIRE(reps >= 1 && reps <= 100000, GDX_ERR_ARG, "reps must be in 1..100000");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  int rc = gdx::solve_spmv_timed(p, reps, ms);
+  if (rc) set_error("gdx_spmv_timed failed");
+  return rc;
+}
+
 int gdx_pcg(gdx_problem *p, double rtol, int32_t maxiter, double *x, int32_t *iters, double *relres, double *ms) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(rtol > 0.0 && maxiter >= 0, GDX_ERR_ARG, "rtol must be > 0 and maxiter >= 0");

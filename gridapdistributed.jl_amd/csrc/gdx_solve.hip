@@ -335,6 +335,32 @@ static void dot2(SolveData *s, const double *a, const double *b, const double *c
   hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlk), 0, st, s->part, nb, out);
 }
 
+// device time of one SpMV of the assembled matrix (the solver's kernel on
+// its own vectors), averaged over reps launches between two hipEvents
+int solve_spmv_timed(gdx_problem *p, int reps, double *ms) {
+  int rc = solve_setup(p);
+  if (rc) return rc;
+  SolveData *s = solve_data(p);
+  hipStream_t st = p->ctx->stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return GDX_ERR_HIP;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return GDX_ERR_HIP;
+  }
+  spmv(p, s, s->x, s->q, st);  // warm-up
+  float t = 0.0f;
+  rc = hipEventRecord(e0, st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+  for (int k = 0; k < reps && !rc; ++k) spmv(p, s, s->x, s->q, st);
+  if (!rc && (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    rc = GDX_ERR_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) *ms = (double)t / reps;
+  return rc;
+}
+
 int solve_spmv(gdx_problem *p, const double *xh, double *yh) {
   int rc = solve_setup(p);
   if (rc) return rc;

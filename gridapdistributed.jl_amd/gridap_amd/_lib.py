@@ -34,7 +34,7 @@ EXPORTED = [
     "gdx_batch_destroy", "gdx_problem_set_kernel", "gdx_problem_get_kernel",
     "gdx_pvector_create", "gdx_pvector_destroy", "gdx_pvector_length", "gdx_pvector_device", "gdx_pvector_set",
     "gdx_pvector_get", "gdx_pvector_fill", "gdx_pvector_copy_rhs", "gdx_pvector_consistent", "gdx_pvector_assemble",
-    "gdx_pvector_axpy", "gdx_pvector_change_ghost", "gdx_psparse_mul",
+    "gdx_pvector_axpy", "gdx_pvector_change_ghost", "gdx_psparse_mul", "gdx_spmv_timed",
 ]
 PRANGE_ROWS, PRANGE_COLS = 0, 1
 
@@ -118,6 +118,7 @@ def lib():
         "gdx_phase_times": [vp, P(ctypes.c_double), i32],
         "gdx_owned_nnz": [vp, P(i64)],
         "gdx_spmv": [vp, vp, vp],
+        "gdx_spmv_timed": [vp, i32, P(ctypes.c_double)],
         "gdx_pcg": [vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double)],
         "gdx_set_state": [vp, vp],
         "gdx_set_cell_coefficient": [vp, vp],

@@ -277,6 +277,12 @@ class GpuProblem:
         _lib.call("gdx_spmv", self.handle, x.ctypes.data, y.ctypes.data)
         return y
 
+    def spmv_device_ms(self, reps=20):
+        """Device time (ms) of one SpMV of the assembled matrix (hipEvents, averaged over reps)."""
+        ms = ctypes.c_double()
+        _lib.call("gdx_spmv_timed", self.handle, int(reps), ctypes.byref(ms))
+        return ms.value
+
     def pvector(self, which=_lib.PRANGE_ROWS):
         """A device PVector on A's row (default) or column PRange."""
         return DevicePVector(self, which)

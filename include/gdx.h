@@ -177,6 +177,10 @@ int gdx_export_rhs(gdx_problem *prob, int32_t part, double *b);
    reference solves with A\b, README.md:76, and applies with A*x / mul!,
    test/FESpacesTests.jl:24).  x, y, xsol are host arrays of length ngdofs. */
 int gdx_spmv(gdx_problem *prob, const double *x, double *y);
+/* Device time (ms) of one SpMV of the assembled matrix, averaged over reps
+   launches between two hipEvents on the library's stream (the solver's kernel
+   on its own vectors; a measurement helper with no counterpart in the reference) */
+int gdx_spmv_timed(gdx_problem *prob, int32_t reps, double *ms);
 /* Jacobi-preconditioned CG from x0 = 0 until ||r|| <= rtol ||b||; relres is the
    true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time */
 int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,
