@@ -81,6 +81,39 @@ def cpu_baseline_flux(n=48):
                       f"in-place t={r.times[2]:.3f}s; host {cpu_model()}; wall {wall:.1f}s"}
 
 
+def cpu_baseline_elasticity(cells=(12, 10, 9)):
+    """The oracle's in-place numeric re-assembly of the elasticity problem (81 x 81
+    Q2 cell matrices, LocalView + assemble!) on a bounded sample, one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from gridap_amd import workloads as W
+
+    p = W.elasticity3d_q2(cells=cells, parts=(3, 2, 2))
+    t0 = time.perf_counter()
+    r = oracle.run(p, inplace_reps=1)
+    wall = time.perf_counter() - t0
+    nnz = sum(int(P["rowptr"][P["n_own"]]) for P in r.parts)
+    return {"value": nnz / r.times[2] / 1e9, "unit": "Gnnz/s", "cores": 1, "kind": "port",
+            "sample": f"linear elasticity Q2 {cells[0]}x{cells[1]}x{cells[2]} cells, parts (3,2,2), E=1, nu=0.3, "
+                      f"clamped at x=0; numeric re-assembly into the allocated PSparseMatrix, nnz={nnz}, "
+                      f"t={r.times[2]:.2f}s; host {cpu_model()}; wall {wall:.1f}s"}
+
+
+def cpu_baseline_q1(n=32):
+    """The oracle's in-place numeric re-assembly of one Q1 cube of the batch's
+    kind (8-mode Fourier kappa, f = 1) on a bounded sample, one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from gridap_amd import workloads as W
+
+    p = W.poisson3d_q1(n, (2, 2, 2), modes=W.fourier_modes(7, 8, 4, 0.5), rhs_const=1.0)
+    r = oracle.run(p, inplace_reps=1)
+    nnz = sum(int(P["rowptr"][P["n_own"]]) for P in r.parts)
+    return {"value": nnz / r.times[2] / 1e9, "unit": "Gnnz/s", "cores": 1, "kind": "port",
+            "sample": f"3D Poisson Q1 {n}^3, parts (2,2,2); numeric re-assembly into the allocated PSparseMatrix, "
+                      f"nnz={nnz}, t={r.times[2]:.2f}s; host {cpu_model()}"}
+
+
 def cpu_baseline_solve(n=28):
     """The reference's A\\b (README.md:76: the system gathered and solved by a
     sparse LU) on the host, one thread: scipy's SuperLU on the oracle's matrix
@@ -395,6 +428,8 @@ def bench_elasticity(args):
     }
     prob.close()
     ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_elasticity()
     print(json.dumps(out), flush=True)
 
 
@@ -476,6 +511,11 @@ def bench_batched(args):
     for _, p in probs:
         p.close()
     ctx.close()
+    if not args.no_cpu_baseline:
+        if el is not None:  # the elasticity problem holds 98.6% of the batch's entries
+            out["cpu_baseline"] = cpu_baseline_elasticity()
+        else:
+            out["cpu_baseline"] = cpu_baseline_q1()
     print(json.dumps(out), flush=True)
 
 
