@@ -58,3 +58,19 @@ def test_pcg_matches_direct_solve_and_repeats(gpu_ctx):
     x2, it2, rr2, _ = prob.pcg(rtol=1e-10, maxiter=5000)
     assert it2 == it and np.array_equal(x.view(np.int64), x2.view(np.int64))
     prob.close()
+
+
+def test_spmv_timed_runs_the_solver_kernel(gpu_ctx):
+    """gdx_spmv_timed: a positive device time per SpMV, and the SpMV it times
+    still gives the oracle's product afterwards."""
+    p = W.poisson3d_q1(20, (2, 2, 1))
+    prob = _prob(gpu_ctx, p)
+    ms = prob.spmv_device_ms(10)
+    assert 0.0 < ms < 100.0
+    r = oracle.run(p)
+    A, _ = owned_global(r.parts, r.ngdofs)
+    x = np.random.default_rng(2).standard_normal(prob.ngdofs)
+    assert rel_err(prob.spmv(x), A @ x) <= 1e-13
+    with pytest.raises(Exception):
+        prob.spmv_device_ms(0)
+    prob.close()
