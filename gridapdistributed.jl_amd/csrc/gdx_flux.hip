@@ -99,30 +99,45 @@ __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles,
   constexpr int S = D == 3 ? 27 : 9, NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ;
   constexpr int RPB = 256 / NC;  // rows per pass of the workgroup
   __shared__ double part[256][NC + 1];
-  // reference-cell tables (the kernel argument is read at static offsets only:
-  // a runtime index into it would copy the whole struct to scratch)
-  __shared__ double tphi[2][NQ], tdphi[2][NQ], tqw[NQ];
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      tphi[0][q] = kp.phi[0][q];
-      tphi[1][q] = kp.phi[1][q];
-      tdphi[0][q] = kp.dphi[0][q];
-      tdphi[1][q] = kp.dphi[1][q];
-      tqw[q] = kp.qw[q];
+  // reference-cell tables at every Gauss point, identical for every cell of the
+  // uniform mesh: gradients of the 2^D local basis functions (1/h applied),
+  // values, weight x cell volume, and grad(phi_a).grad(phi_b) (rows padded to
+  // NC + 1 doubles against bank conflicts).  Filled once from the 1D tables of
+  // the kernel argument, so the Gauss-point loop only reads LDS.
+  __shared__ double tg[NQD][NC][D], tp[NQD][NC], tw[NQD], tgab[NQD][NC][NC + 1];
+  {
+    const double ih[3] = {1.0 / kp.h[0], 1.0 / kp.h[1], D == 3 ? 1.0 / kp.h[2] : 0.0};
+    const double vol = kp.h[0] * kp.h[1] * (D == 3 ? kp.h[2] : 1.0);
+    for (int i = threadIdx.x; i < NQD * NC; i += blockDim.x) {
+      const int q = i / NC, bb = i % NC;
+      const int q0 = q % NQ, q1 = (q / NQ) % NQ, q2 = D == 3 ? q / (NQ * NQ) : 0;
+      const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
+      const double p0 = kp.phi[b0][q0], p1 = kp.phi[b1][q1], p2 = D == 3 ? kp.phi[b2][q2] : 1.0;
+      const double d0 = kp.dphi[b0][q0] * ih[0], d1 = kp.dphi[b1][q1] * ih[1];
+      const double d2 = D == 3 ? kp.dphi[b2][q2] * ih[2] : 0.0;
+      tg[q][bb][0] = d0 * p1 * p2;
+      tg[q][bb][1] = p0 * d1 * p2;
+      if constexpr (D == 3) tg[q][bb][2] = p0 * p1 * d2;
+      tp[q][bb] = p0 * p1 * p2;
+      if (bb == 0) tw[q] = kp.qw[q0] * kp.qw[q1] * (D == 3 ? kp.qw[q2] : 1.0) * vol;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NQD * NC * NC; i += blockDim.x) {
+      const int q = i / (NC * NC), aa = (i / NC) % NC, bb = i % NC;
+      double v = tg[q][aa][0] * tg[q][bb][0] + tg[q][aa][1] * tg[q][bb][1];
+      if constexpr (D == 3) v += tg[q][aa][2] * tg[q][bb][2];
+      tgab[q][aa][bb] = v;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
   const Tile t = tiles[tb];
   const int nrows = t.ext[0] * t.ext[1] * t.ext[2];
-  const double ih0 = 1.0 / kp.h[0], ih1 = 1.0 / kp.h[1], ih2 = D == 3 ? 1.0 / kp.h[2] : 0.0;
-  const double vol = kp.h[0] * kp.h[1] * (D == 3 ? kp.h[2] : 1.0);
   const int64_t sy = kp.nn[0], sz = (int64_t)kp.nn[0] * kp.nn[1];
   const int tid = threadIdx.x, e = tid % NC, rl = tid / NC;
   const int e0 = e & 1, e1 = (e >> 1) & 1, e2 = D == 3 ? (e >> 2) & 1 : 0;
-  const int a0 = 1 - e0, a1 = 1 - e1, a2 = D == 3 ? 1 - e2 : 0;  // the row node in the cell
+  const int ai = (NC - 1) - e;  // the row node's index in the cell: (1 - e0, 1 - e1, 1 - e2)
   for (int r0 = 0; r0 < nrows; r0 += RPB) {  // uniform over the workgroup
     const int r = r0 + rl;
     int g0 = 0, g1 = 0, g2 = 0;
@@ -148,43 +163,26 @@ __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles,
         uc[bb] = ub[(bb & 1) + sy * ((bb >> 1) & 1) + (D == 3 ? sz * ((bb >> 2) & 1) : 0)];
 #pragma unroll 1
       for (int q = 0; q < NQD; ++q) {
-        const int q0 = q % NQ, q1 = (q / NQ) % NQ, q2 = D == 3 ? q / (NQ * NQ) : 0;
-        // 1D values and derivatives of the two local nodes per direction
-        const double p00 = tphi[0][q0], p01 = tphi[1][q0], p10 = tphi[0][q1], p11 = tphi[1][q1];
-        const double p20 = D == 3 ? tphi[0][q2] : 1.0, p21 = D == 3 ? tphi[1][q2] : 1.0;
-        const double d00 = tdphi[0][q0] * ih0, d01 = tdphi[1][q0] * ih0;
-        const double d10 = tdphi[0][q1] * ih1, d11 = tdphi[1][q1] * ih1;
-        const double d20 = D == 3 ? tdphi[0][q2] * ih2 : 0.0, d21 = D == 3 ? tdphi[1][q2] * ih2 : 0.0;
-        auto P0 = [&](int i) { return i ? p01 : p00; };
-        auto P1 = [&](int i) { return i ? p11 : p10; };
-        auto P2 = [&](int i) { return i ? p21 : p20; };
-        auto G0 = [&](int i) { return i ? d01 : d00; };
-        auto G1 = [&](int i) { return i ? d11 : d10; };
-        auto G2 = [&](int i) { return i ? d21 : d20; };
         double gu0 = 0.0, gu1 = 0.0, gu2 = 0.0;
 #pragma unroll
         for (int bb = 0; bb < NC; ++bb) {
-          const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
-          gu0 = fma(uc[bb], G0(b0) * P1(b1) * P2(b2), gu0);
-          gu1 = fma(uc[bb], P0(b0) * G1(b1) * P2(b2), gu1);
-          gu2 = fma(uc[bb], P0(b0) * P1(b1) * G2(b2), gu2);
+          gu0 = fma(uc[bb], tg[q][bb][0], gu0);
+          gu1 = fma(uc[bb], tg[q][bb][1], gu1);
+          if constexpr (D == 3) gu2 = fma(uc[bb], tg[q][bb][2], gu2);
         }
         const double s2 = fp.alpha + fp.beta * (gu0 * gu0 + gu1 * gu1 + gu2 * gu2);
         const double kap = flux_kappa<MODE>(s2, fp);
         const double gam = (MODE == 1 || s2 <= 0.0) ? 0.0 : (fp.p - 2.0) * fp.beta * kap / s2;
-        const double wd = tqw[q0] * tqw[q1] * (D == 3 ? tqw[q2] : 1.0) * vol;
-        const double ga0 = G0(a0) * P1(a1) * P2(a2), ga1 = P0(a0) * G1(a1) * P2(a2), ga2 = P0(a0) * P1(a1) * G2(a2);
-        const double gua = gu0 * ga0 + gu1 * ga1 + gu2 * ga2;
-        const double phia = P0(a0) * P1(a1) * P2(a2);
-        res = fma(wd, kap * gua - fp.f * phia, res);
+        const double wd = tw[q];
+        double gua = gu0 * tg[q][ai][0] + gu1 * tg[q][ai][1];
+        if constexpr (D == 3) gua += gu2 * tg[q][ai][2];
+        res = fma(wd, kap * gua - fp.f * tp[q][ai], res);
         const double wk = wd * kap, wg = wd * gam * gua;
 #pragma unroll
         for (int bb = 0; bb < NC; ++bb) {
-          const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
-          const double gb0 = G0(b0) * P1(b1) * P2(b2), gb1 = P0(b0) * G1(b1) * P2(b2), gb2 = P0(b0) * P1(b1) * G2(b2);
-          const double gab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
-          const double gub = gu0 * gb0 + gu1 * gb1 + gu2 * gb2;
-          acc8[bb] = fma(wk, gab, fma(wg, gub, acc8[bb]));
+          double gub = gu0 * tg[q][bb][0] + gu1 * tg[q][bb][1];
+          if constexpr (D == 3) gub += gu2 * tg[q][bb][2];
+          acc8[bb] = fma(wk, tgab[q][ai][bb], fma(wg, gub, acc8[bb]));
         }
       }
     }
