@@ -53,20 +53,26 @@ __device__ __forceinline__ void tile_origin(int tile, int n, int &x0, int &y0, i
 // MODE 1: backward data, in = FP32 gradient with KCI (= the layer's outputs)
 //         channels, out = FP32 gradient of the layer's KCO inputs, weights
 //         transposed with the taps mirrored.
-template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM>
+// G workgroups share a tile, each computing KCO / G of its output channels
+// (blockIdx.y picks the group): the small grids of the coarse levels have few
+// tiles, and one workgroup per tile would leave most CUs idle.  Every output
+// channel accumulates in the same order whatever G is.
+template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM, int G = 1>
 __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in_, const uint16_t *__restrict__ w,
                                               const uint16_t *__restrict__ bias, const uint16_t *__restrict__ skip,
                                               void *__restrict__ out_) {
-  constexpr int CH = KCI < 8 ? KCI : 8;
+  static_assert(KCO % G == 0, "whole channel groups");
+  constexpr int CH = KCI < 8 ? KCI : 8, KCG = KCO / G;
+  const int cob = blockIdx.y * KCG;
   __shared__ float sx[CH * kHV];
-  __shared__ float sw[KCO * CH * 27];
+  __shared__ float sw[KCG * CH * 27];
   int x0, y0, z0;
   tile_origin(blockIdx.x, n, x0, y0, z0);
   const int tid = threadIdx.x, lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
   const int64_t n3 = (int64_t)n * n * n;
-  float acc[KCO];
+  float acc[KCG];
 #pragma unroll
-  for (int co = 0; co < KCO; ++co) acc[co] = MODE == 0 ? bf2f(bias[co]) : 0.0f;
+  for (int co = 0; co < KCG; ++co) acc[co] = MODE == 0 ? bf2f(bias[cob + co]) : 0.0f;
   for (int c0 = 0; c0 < KCI; c0 += CH) {
     __syncthreads();
     for (int i = tid; i < CH * kHV; i += kT) {
@@ -79,11 +85,12 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
       }
       sx[i] = v;
     }
-    for (int i = tid; i < KCO * CH * 27; i += kT) {
+    for (int i = tid; i < KCG * CH * 27; i += kT) {
       const int co = i / (CH * 27), r = i % (CH * 27), c = r / 27, t = r % 27;
       float v = 0.0f;
       if (c0 + c < KCI)
-        v = MODE == 0 ? bf2f(w[((int64_t)co * KCI + c0 + c) * 27 + t]) : bf2f(w[((int64_t)(c0 + c) * KCO + co) * 27 + 26 - t]);
+        v = MODE == 0 ? bf2f(w[((int64_t)(cob + co) * KCI + c0 + c) * 27 + t])
+                      : bf2f(w[((int64_t)(c0 + c) * KCO + cob + co) * 27 + 26 - t]);
       sw[i] = v;
     }
     __syncthreads();
@@ -93,7 +100,7 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
         const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
         const float v = sx[c * kHV + (lz + dz) * 100 + (ly + dy) * 10 + lx + dx];
 #pragma unroll
-        for (int co = 0; co < KCO; ++co) acc[co] = fmaf(v, sw[(co * CH + c) * 27 + t], acc[co]);
+        for (int co = 0; co < KCG; ++co) acc[co] = fmaf(v, sw[(co * CH + c) * 27 + t], acc[co]);
       }
     }
   }
@@ -101,8 +108,9 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
   if (gx >= n || gy >= n || gz >= n) return;
   const int64_t vi = ((int64_t)gz * n + gy) * n + gx;
 #pragma unroll
-  for (int co = 0; co < KCO; ++co) {
-    float v = acc[co];
+  for (int cg = 0; cg < KCG; ++cg) {
+    const int co = cob + cg;
+    float v = acc[cg];
     if (SKIP) v += bf2f(skip[co * n3 + vi]);
     if (RELU) v = fmaxf(v, 0.0f);
     if (OUTF32) {
@@ -352,17 +360,43 @@ int Net::get_params(float *h) {
   return hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
+// channel groups per tile: double while the grid has fewer than 512 workgroups
+// and the output channels split evenly (at most 8 groups)
+template <int KCO>
+static int channel_groups(int n) {
+  int g = 1;
+  while (g < 8 && KCO % (2 * g) == 0 && tiles_of(n) * g < 512) g *= 2;
+  return g;
+}
+
+template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM, int G>
+static void conv_launch(const Net &N, int n, const void *in, const uint16_t *w, const uint16_t *b,
+                        const uint16_t *skip, void *out) {
+  if constexpr (KCO % G == 0)
+    hipLaunchKernelGGL((k_conv3<KCI, KCO, MODE, RELU, SKIP, OUTF32, ACCUM, G>), dim3(tiles_of(n), G), dim3(kT), 0,
+                       N.st, n, in, w, b, skip, out);
+}
+
+template <int KCI, int KCO, int MODE, bool RELU, bool SKIP, bool OUTF32, bool ACCUM>
+static void conv_any(const Net &N, int n, const void *in, const uint16_t *w, const uint16_t *b, const uint16_t *skip,
+                     void *out) {
+  switch (channel_groups<KCO>(n)) {
+    case 8: conv_launch<KCI, KCO, MODE, RELU, SKIP, OUTF32, ACCUM, 8>(N, n, in, w, b, skip, out); break;
+    case 4: conv_launch<KCI, KCO, MODE, RELU, SKIP, OUTF32, ACCUM, 4>(N, n, in, w, b, skip, out); break;
+    case 2: conv_launch<KCI, KCO, MODE, RELU, SKIP, OUTF32, ACCUM, 2>(N, n, in, w, b, skip, out); break;
+    default: conv_launch<KCI, KCO, MODE, RELU, SKIP, OUTF32, ACCUM, 1>(N, n, in, w, b, skip, out); break;
+  }
+}
+
 template <int CI, int CO, bool RELU, bool SKIP, bool OUTF32>
 static void conv_fwd(const Net &N, int n, const uint16_t *in, int layer, const uint16_t *skip, void *out) {
-  hipLaunchKernelGGL((k_conv3<CI, CO, 0, RELU, SKIP, OUTF32, false>), dim3(tiles_of(n)), dim3(kT), 0, N.st, n, in,
-                     N.thb + N.woff[layer], N.thb + N.boff[layer], skip, out);
+  conv_any<CI, CO, 0, RELU, SKIP, OUTF32, false>(N, n, in, N.thb + N.woff[layer], N.thb + N.boff[layer], skip, out);
 }
 
 // dX of a layer (KCI = layer outputs, KCO = layer inputs)
 template <int LCO, int LCI, bool ACCUM>
 static void conv_bwd_data(const Net &N, int n, const float *g, int layer, float *dx) {
-  hipLaunchKernelGGL((k_conv3<LCO, LCI, 1, false, false, true, ACCUM>), dim3(tiles_of(n)), dim3(kT), 0, N.st, n, g,
-                     N.thb + N.woff[layer], nullptr, nullptr, dx);
+  conv_any<LCO, LCI, 1, false, false, true, ACCUM>(N, n, g, N.thb + N.woff[layer], nullptr, nullptr, dx);
 }
 
 template <int CI, int CO>
