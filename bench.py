@@ -320,8 +320,9 @@ def bench_learned(args):
     levels = net.set_multigrid(True, 1)
     # roofline of the solves' dominant kernel, the CSR SpMV, timed live with hipEvents
     nnz, nd = prob.owned_nnz(), prob.ngdofs
-    ms_spmv = prob.spmv_device_ms(50)
-    spmv_bytes = 12 * nnz + 28 * nd  # value + column id per entry; row start, length, x and y per row
+    # bytes: value (8 B) + column id (4 B, 2 B where a block holds 16-bit offsets) per entry;
+    # row start, length, x and y per row
+    ms_spmv, spmv_bytes = prob.spmv_device_timing(50)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_k_spmv_blk_learned_q1_{cells}.json")
     if os.path.exists(tpath):

@@ -640,13 +640,13 @@ int gdx_spmv(gdx_problem *p, const double *x, double *y) {
   return rc;
 }
 
-int gdx_spmv_timed(gdx_problem *p, int32_t reps, double *ms) {
+int gdx_spmv_timed(gdx_problem *p, int32_t reps, double *ms, double *bytes) {
   GDX_REQUIRE(p && ms, GDX_ERR_ARG, "NULL argument");
   GDX_REQUIRE(reps >= 1 && reps <= 100000, GDX_ERR_ARG, "reps must be in 1..100000");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
   GDX_HIP(hipSetDevice(p->ctx->device));
-  int rc = gdx::solve_spmv_timed(p, reps, ms);
+  int rc = gdx::solve_spmv_timed(p, reps, ms, bytes);
   if (rc) set_error("gdx_spmv_timed failed");
   return rc;
 }

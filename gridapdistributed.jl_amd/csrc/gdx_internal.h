@@ -220,6 +220,6 @@ double *flux_state(gdx_problem *p);
 int flux_set_state(gdx_problem *p, const double *xh);
 int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st);
 int solve_spmv(gdx_problem *p, const double *x, double *y);
-int solve_spmv_timed(gdx_problem *p, int reps, double *ms);
+int solve_spmv_timed(gdx_problem *p, int reps, double *ms, double *bytes);
 int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *x, int *iters, double *relres, double *ms);
 }  // namespace gdx

@@ -24,6 +24,7 @@ namespace gdx {
 
 constexpr int kRedBlocks = 1024;  // partial sums per reduction
 constexpr int kBlk = 256;
+constexpr int kSpmvRows = 128;  // rows per block of the block SpMV
 
 // y[g] = sum_k val[rstart[g]+k] * x[cgid[rstart[g]+k]] ; LPR lanes per row
 template <int LPR>
@@ -50,10 +51,15 @@ __global__ __launch_bounds__(kBlk) void k_spmv(int n, const long long *__restric
 // thread staging its U entries in registers before the gathers of x, and puts
 // the products in LDS; LPR = NT / RB lanes per row then sum their row in a
 // fixed order.  Other blocks (a part boundary or a ghost-row gap inside) read
-// their rows directly with LPR lanes each.
+// their rows directly with LPR lanes each.  A streamed block whose column ids
+// fall in at most four windows of 16384 ids (its own part's rows and the ghost
+// columns' ranges in up to three neighbour parts) reads them as 16-bit codes
+// c16 = window << 14 | offset against the four bases bbase[block] (first base
+// -1: 32-bit ids), 10 instead of 12 bytes per entry.
 template <int RB, int NT>
 __global__ __launch_bounds__(NT) void k_spmv_blk(int n, const long long *__restrict__ rstart,
                                                  const int *__restrict__ rlen, const int *__restrict__ cgid,
+                                                 const uint16_t *__restrict__ c16, const int4 *__restrict__ bbase,
                                                  const double *__restrict__ val, const double *__restrict__ x,
                                                  double *__restrict__ y) {
   constexpr int U = 8, CAP = U * NT, LPR = NT / RB;
@@ -76,13 +82,24 @@ __global__ __launch_bounds__(NT) void k_spmv_blk(int n, const long long *__restr
   double acc = 0.0;
   if (ok) {
     const int m = (int)(B - A);
+    const int4 bb = bbase[blockIdx.x];
     double vv[U];
     int cc[U];
+    if (bb.x >= 0) {
 #pragma unroll
-    for (int i = 0; i < U; ++i) {
-      const int k = t + NT * i;
-      vv[i] = k < m ? val[A + k] : 0.0;
-      cc[i] = k < m ? cgid[A + k] : 0;
+      for (int i = 0; i < U; ++i) {
+        const int k = t + NT * i;
+        vv[i] = k < m ? val[A + k] : 0.0;
+        const int c = k < m ? (int)c16[A + k] : 0, w = c >> 14;
+        cc[i] = (c & 0x3FFF) + (w == 0 ? bb.x : w == 1 ? bb.y : w == 2 ? bb.z : bb.w);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        const int k = t + NT * i;
+        vv[i] = k < m ? val[A + k] : 0.0;
+        cc[i] = k < m ? cgid[A + k] : 0;
+      }
     }
 #pragma unroll
     for (int i = 0; i < U; ++i) {
@@ -210,6 +227,12 @@ struct SolveData {
   int n = 0, lpr = 8;
   long long *rstart = nullptr, *bidx = nullptr, *dpos = nullptr;
   int *rlen = nullptr, *cgid = nullptr;
+  // block SpMV column ids as 16-bit (window, offset) codes against four bases
+  // per block (first base -1: the block keeps its 32-bit ids), and the SpMV's
+  // bytes per launch
+  uint16_t *c16 = nullptr;
+  int4 *bbase = nullptr;
+  double spmv_bytes = 0.0;
   double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
   double *part = nullptr, *sc = nullptr;
 };
@@ -256,7 +279,7 @@ int run_iteration(hipStream_t st, IterGraph &G, F &&body) {
 void solve_free(gdx_problem *p) {
   SolveData *s = solve_data(p);
   if (!s) return;
-  void *ptrs[] = {s->rstart, s->bidx, s->dpos, s->rlen, s->cgid, s->x, s->b, s->r, s->z, s->p, s->q, s->dinv,
+  void *ptrs[] = {s->rstart, s->bidx, s->dpos, s->rlen, s->cgid, s->c16, s->bbase, s->x, s->b, s->r, s->z, s->p, s->q, s->dinv,
                   s->part, s->sc};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -287,10 +310,57 @@ static int solve_setup(gdx_problem *p) {
       if (dpos[g] < 0) return GDX_ERR_STATE;
     }
   }
+  // 16-bit column codes of the block SpMV's streamed blocks (the kernel's own
+  // test: rows stored back to back, at most 8 * 512 entries).  Windows of
+  // 16384 ids are grown greedily over the block's entries; a block needing
+  // more than four keeps its 32-bit ids.
+  const int lpr = maxlen > 64 ? 32 : (maxlen > 32 ? 16 : 8);
+  const int64_t nblk = (n + kSpmvRows - 1) / kSpmvRows;
+  std::vector<uint16_t> c16;
+  std::vector<int4> bbase;
+  double spmv_bytes = 12.0 * (double)p->owned_nnz + 28.0 * (double)n;
+  if (lpr == 8) {
+    c16.assign(std::max<int64_t>(p->nnz_total, 1), 0);
+    bbase.assign(std::max<int64_t>(nblk, 1), int4{-1, 0, 0, 0});
+    spmv_bytes += 16.0 * (double)nblk;
+    for (int64_t b = 0; b < nblk; ++b) {
+      const int64_t r0 = b * kSpmvRows, r1 = std::min<int64_t>(n, r0 + kSpmvRows);
+      bool ok = true;
+      for (int64_t r = r0; r + 1 < r1 && ok; ++r) ok = rstart[r] + rlen[r] == rstart[r + 1];
+      const long long A = rstart[r0], B = rstart[r1 - 1] + rlen[r1 - 1];
+      if (!ok || B - A > 8 * 512 || B <= A) continue;
+      int wlo[4], whi[4], nw = 0;
+      for (long long k = A; k < B && nw <= 4; ++k) {
+        const int c = cgid[k];
+        int w = 0;
+        while (w < nw && std::max(whi[w], c) - std::min(wlo[w], c) > 16383) ++w;
+        if (w == nw) {
+          if (nw == 4) {
+            nw = 5;
+            break;
+          }
+          wlo[w] = whi[w] = c;
+          ++nw;
+        }
+        wlo[w] = std::min(wlo[w], c);
+        whi[w] = std::max(whi[w], c);
+      }
+      if (nw > 4) continue;
+      for (long long k = A; k < B; ++k) {
+        const int c = cgid[k];
+        int w = 0;
+        while (c < wlo[w] || c > whi[w]) ++w;  // every id lies in the window that took it
+        c16[k] = (uint16_t)(w << 14 | (c - wlo[w]));
+      }
+      bbase[b] = int4{wlo[0], nw > 1 ? wlo[1] : 0, nw > 2 ? wlo[2] : 0, nw > 3 ? wlo[3] : 0};
+      spmv_bytes -= 2.0 * (double)(B - A);
+    }
+  }
   SolveData *s = new SolveData;
   p->solve = s;
   s->n = (int)n;
-  s->lpr = maxlen > 64 ? 32 : (maxlen > 32 ? 16 : 8);
+  s->lpr = lpr;
+  s->spmv_bytes = spmv_bytes;
   auto up = [](auto **dst, const auto &v) -> hipError_t {
     hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(v.size(), 1) * sizeof(v[0]));
     if (e != hipSuccess) return e;
@@ -298,7 +368,8 @@ static int solve_setup(gdx_problem *p) {
   };
   hipError_t e = hipSuccess;
   if ((e = up(&s->rstart, rstart)) || (e = up(&s->bidx, bidx)) || (e = up(&s->dpos, dpos)) ||
-      (e = up(&s->rlen, rlen)) || (e = up(&s->cgid, cgid))) {
+      (e = up(&s->rlen, rlen)) || (e = up(&s->cgid, cgid)) || (lpr == 8 && ((e = up(&s->c16, c16)) ||
+                                                                           (e = up(&s->bbase, bbase))))) {
     solve_free(p);
     return GDX_ERR_ALLOC;
   }
@@ -320,8 +391,8 @@ static void spmv(gdx_problem *p, SolveData *s, const double *x, double *y, hipSt
   const int rows_per_block = kBlk / s->lpr;
   const dim3 g((s->n + rows_per_block - 1) / rows_per_block);
   if (s->lpr == 8)  // rows of at most 32 entries
-    hipLaunchKernelGGL((k_spmv_blk<128, 512>), dim3((s->n + 127) / 128), dim3(512), 0, st, s->n, s->rstart, s->rlen,
-                       s->cgid, p->dev.values, x, y);
+    hipLaunchKernelGGL((k_spmv_blk<kSpmvRows, 512>), dim3((s->n + kSpmvRows - 1) / kSpmvRows), dim3(512), 0, st, s->n,
+                       s->rstart, s->rlen, s->cgid, s->c16, s->bbase, p->dev.values, x, y);
   else if (s->lpr == 16)
     hipLaunchKernelGGL(k_spmv<16>, g, dim3(kBlk), 0, st, s->n, s->rstart, s->rlen, s->cgid, p->dev.values, x, y);
   else
@@ -337,7 +408,7 @@ static void dot2(SolveData *s, const double *a, const double *b, const double *c
 
 // device time of one SpMV of the assembled matrix (the solver's kernel on
 // its own vectors), averaged over reps launches between two hipEvents
-int solve_spmv_timed(gdx_problem *p, int reps, double *ms) {
+int solve_spmv_timed(gdx_problem *p, int reps, double *ms, double *bytes) {
   int rc = solve_setup(p);
   if (rc) return rc;
   SolveData *s = solve_data(p);
@@ -358,6 +429,7 @@ int solve_spmv_timed(gdx_problem *p, int reps, double *ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!rc) *ms = (double)t / reps;
+  if (!rc && bytes) *bytes = s->spmv_bytes;
   return rc;
 }
 

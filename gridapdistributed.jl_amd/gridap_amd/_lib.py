@@ -118,7 +118,7 @@ def lib():
         "gdx_phase_times": [vp, P(ctypes.c_double), i32],
         "gdx_owned_nnz": [vp, P(i64)],
         "gdx_spmv": [vp, vp, vp],
-        "gdx_spmv_timed": [vp, i32, P(ctypes.c_double)],
+        "gdx_spmv_timed": [vp, i32, P(ctypes.c_double), P(ctypes.c_double)],
         "gdx_pcg": [vp, ctypes.c_double, i32, vp, P(i32), P(ctypes.c_double), P(ctypes.c_double)],
         "gdx_set_state": [vp, vp],
         "gdx_set_cell_coefficient": [vp, vp],

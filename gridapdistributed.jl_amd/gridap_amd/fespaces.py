@@ -277,11 +277,16 @@ class GpuProblem:
         _lib.call("gdx_spmv", self.handle, x.ctypes.data, y.ctypes.data)
         return y
 
+    def spmv_device_timing(self, reps=20):
+        """(ms, bytes): device time of one SpMV of the assembled matrix (hipEvents,
+        averaged over reps) and its algorithmic bytes per launch."""
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        _lib.call("gdx_spmv_timed", self.handle, int(reps), ctypes.byref(ms), ctypes.byref(nb))
+        return ms.value, nb.value
+
     def spmv_device_ms(self, reps=20):
         """Device time (ms) of one SpMV of the assembled matrix (hipEvents, averaged over reps)."""
-        ms = ctypes.c_double()
-        _lib.call("gdx_spmv_timed", self.handle, int(reps), ctypes.byref(ms))
-        return ms.value
+        return self.spmv_device_timing(reps)[0]
 
     def pvector(self, which=_lib.PRANGE_ROWS):
         """A device PVector on A's row (default) or column PRange."""

@@ -179,8 +179,11 @@ int gdx_export_rhs(gdx_problem *prob, int32_t part, double *b);
 int gdx_spmv(gdx_problem *prob, const double *x, double *y);
 /* Device time (ms) of one SpMV of the assembled matrix, averaged over reps
    launches between two hipEvents on the library's stream (the solver's kernel
-   on its own vectors; a measurement helper with no counterpart in the reference) */
-int gdx_spmv_timed(gdx_problem *prob, int32_t reps, double *ms);
+   on its own vectors; a measurement helper with no counterpart in the reference).
+   bytes (may be NULL) receives the SpMV's algorithmic bytes per launch: values
+   (8 B), column ids (4 B, or 2 B in blocks read as 16-bit offsets) and 28 B per
+   row (row start, length, x and y). */
+int gdx_spmv_timed(gdx_problem *prob, int32_t reps, double *ms, double *bytes);
 /* Jacobi-preconditioned CG from x0 = 0 until ||r|| <= rtol ||b||; relres is the
    true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time */
 int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,

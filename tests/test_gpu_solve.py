@@ -61,12 +61,17 @@ def test_pcg_matches_direct_solve_and_repeats(gpu_ctx):
 
 
 def test_spmv_timed_runs_the_solver_kernel(gpu_ctx):
-    """gdx_spmv_timed: a positive device time per SpMV, and the SpMV it times
-    still gives the oracle's product afterwards."""
+    """gdx_spmv_timed: a positive device time per SpMV, bytes per launch between
+    the all-16-bit and all-32-bit column id forms, and the SpMV it times (rows
+    in 128-row blocks, 16-bit column offsets inside a part, 32-bit ids across
+    part boundaries) still gives the oracle's product afterwards."""
     p = W.poisson3d_q1(20, (2, 2, 1))
     prob = _prob(gpu_ctx, p)
-    ms = prob.spmv_device_ms(10)
+    ms, nbytes = prob.spmv_device_timing(10)
     assert 0.0 < ms < 100.0
+    nnz, n = prob.owned_nnz(), prob.ngdofs
+    nblk = -(-n // 128)
+    assert 10 * nnz + 28 * n + 16 * nblk <= nbytes < 12 * nnz + 28 * n + 16 * nblk
     r = oracle.run(p)
     A, _ = owned_global(r.parts, r.ngdofs)
     x = np.random.default_rng(2).standard_normal(prob.ngdofs)
