@@ -515,13 +515,17 @@ int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *xh, int *iters, 
   if (rc) return rc;
   SolveData *s = solve_data(p);
   hipStream_t st = p->ctx->stream;
+  // timed: the solve, from the assembled system on the device to the solution
+  // there (the host copy of the solution is outside)
+  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
   auto t0 = std::chrono::steady_clock::now();
   if ((rc = pcg_run(p, s, rtol, maxiter, iters, relres))) return rc;
+  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
+  auto t1 = std::chrono::steady_clock::now();
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (xh && (hipMemcpyAsync(xh, s->x, s->n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
              hipStreamSynchronize(st) != hipSuccess))
     return GDX_ERR_HIP;
-  auto t1 = std::chrono::steady_clock::now();
-  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   return GDX_OK;
 }
 
@@ -681,7 +685,6 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
   hipStream_t st = p->ctx->stream;
   const int n = s->n;
   using clk = std::chrono::steady_clock;
-  const auto t0 = clk::now();
   double *rhs_d = nullptr;
   if (rhs_h) {
     if (hipMalloc(reinterpret_cast<void **>(&rhs_d), std::max(n, 1) * sizeof(double)) != hipSuccess)
@@ -691,7 +694,13 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
       return GDX_ERR_HIP;
     }
   }
-  rc = fpcg_run(p, s, prec, rhs_d, rtol, maxiter, iters, relres, fellback);
+  // timed: the solve, from the right-hand side resident on the device to the
+  // solution there (the host copies of both are outside)
+  rc = hipStreamSynchronize(st) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+  const auto t0 = clk::now();
+  if (!rc) rc = fpcg_run(p, s, prec, rhs_d, rtol, maxiter, iters, relres, fellback);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = GDX_ERR_HIP;
+  if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
   if (!rc && xh &&
       (hipMemcpyAsync(xh, s->x, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
        hipStreamSynchronize(st) != hipSuccess))
@@ -700,7 +709,6 @@ int solve_fpcg(gdx_problem *p, const FpcgPrec *prec, const double *rhs_h, double
     (void)hipStreamSynchronize(st);
     (void)hipFree(rhs_d);
   }
-  if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
   return rc;
 }
 
@@ -724,13 +732,16 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
   double *xs = flux_state(p);
   using clk = std::chrono::steady_clock;
   auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
-  const auto t0 = clk::now();
   double ms_asm = 0.0, ms_sol = 0.0;
   if (xh) {
     if (hipMemcpyAsync(xs, xh, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) return GDX_ERR_HIP;
   } else if (n > 0 && hipMemsetAsync(xs, 0, n * sizeof(double), st) != hipSuccess) {
     return GDX_ERR_HIP;
   }
+  // timed (ms_total): from the initial guess resident on the device to the
+  // solution there (the host copies of x are outside)
+  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
+  const auto t0 = clk::now();
   // assemble at xs and return ||r||
   auto assemble = [&](bool linear, double *rnorm) -> int {
     const auto ta = clk::now();
@@ -792,6 +803,8 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
     if ((rc = correct(&li))) return rc;
     lin_total += li;
   }
+  if (hipStreamSynchronize(st) != hipSuccess) return GDX_ERR_HIP;
+  const double ms_total = ms_since(t0);
   if (xh && (hipMemcpyAsync(xh, xs, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
              hipStreamSynchronize(st) != hipSuccess))
     return GDX_ERR_HIP;
@@ -801,7 +814,7 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
     rep->converged = conv ? 1 : 0;
     rep->res_ref = res_ref;
     rep->res_final = res;
-    rep->ms_total = ms_since(t0);
+    rep->ms_total = ms_total;
     rep->ms_assembly = ms_asm;
     rep->ms_solve = ms_sol;
     rep->lin_fallbacks = fallbacks;

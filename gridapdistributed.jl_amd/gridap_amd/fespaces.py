@@ -299,7 +299,8 @@ class GpuProblem:
         return y
 
     def pcg(self, rtol=1e-8, maxiter=10000, want_x=True):
-        """Jacobi-PCG on the assembled system; returns (x, iterations, true relative residual, ms)."""
+        """Jacobi-PCG on the assembled system; returns (x, iterations, true relative residual, ms),
+        ms the solve's wall time on the device (the copy of x to the host is outside it)."""
         x = np.empty(self.ngdofs) if want_x else None
         it, rr, ms = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
         _lib.call("gdx_pcg", self.handle, float(rtol), int(maxiter), None if x is None else x.ctypes.data,

@@ -106,7 +106,9 @@ class LearnedCoarseSolver:
 
 def fpcg(prob, net=None, rhs=None, rtol=1e-8, maxiter=20000):
     """Flexible PCG with the learned preconditioner (net attached to prob), or
-    Jacobi-PCG (net=None).  Returns (x, iterations, true relative residual, ms, fell_back)."""
+    Jacobi-PCG (net=None).  Returns (x, iterations, true relative residual, ms, fell_back);
+    ms is the solve's wall time from rhs resident on the device to the solution
+    there (the host copies of rhs and x are outside it)."""
     x = np.empty(prob.ngdofs)
     r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
     it, rr, ms, fb = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()

@@ -185,7 +185,8 @@ int gdx_spmv(gdx_problem *prob, const double *x, double *y);
    row (row start, length, x and y). */
 int gdx_spmv_timed(gdx_problem *prob, int32_t reps, double *ms, double *bytes);
 /* Jacobi-preconditioned CG from x0 = 0 until ||r|| <= rtol ||b||; relres is the
-   true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time */
+   true FP64 residual ||b - A x|| / ||b|| recomputed at the end, ms the wall time
+   of the solve on the device (the copy of the solution to xsol is outside it) */
 int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32_t *iters, double *relres,
             double *ms);
 
@@ -230,7 +231,9 @@ typedef struct gdx_newton_report {
   int32_t converged;
   double res_ref;      /* ||r(x_given)|| (2-norm over free dofs) */
   double res_final;    /* ||r(x)|| at the returned x */
-  double ms_total, ms_assembly, ms_solve;
+  double ms_total, ms_assembly, ms_solve; /* wall times; ms_total from the initial guess
+                                             on the device to the solution there (the
+                                             host copies of x are outside it) */
   int32_t lin_fallbacks; /* corrections in which FPCG fell back to Jacobi-PCG */
 } gdx_newton_report;
 /* Newton's method on the flux operator: NLSolver(method=:newton) +
@@ -293,7 +296,9 @@ int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
    or Jacobi-PCG when nn is NULL; falls back to Jacobi-PCG from the current
    iterate when the flexible iteration stagnates (fellback = 1).  rhs: host
    array of ngdofs (NULL = the assembled RHS); stops at ||r|| <= rtol ||b||,
-   relres = true FP64 residual recomputed at the end. */
+   relres = true FP64 residual recomputed at the end; ms = wall time of the
+   solve from the right-hand side on the device to the solution there (the
+   host copies of rhs and xsol are outside it). */
 int gdx_fpcg(gdx_problem *prob, gdx_nn *nn, const double *rhs, double rtol, int32_t maxiter, double *xsol,
              int32_t *iters, double *relres, double *ms, int32_t *fellback);
 

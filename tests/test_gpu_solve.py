@@ -33,6 +33,25 @@ def test_spmv_matches_oracle_matrix(gpu_ctx, p):
     prob.close()
 
 
+def test_spmv_mixed_column_code_blocks(gpu_ctx):
+    """64^3 Q1 on parts (2,2,2): 250,047 rows span more than 16384 ids, so the
+    block SpMV reads most 128-row blocks as 16-bit (window, offset) codes and
+    the few whose columns need more than four windows (part corners) as 32-bit
+    ids; the product matches the assembled matrix's either way."""
+    p = W.poisson3d_q1(64, (2, 2, 2))
+    prob = _prob(gpu_ctx, p)
+    parts = [prob.export_part(k) for k in range(prob.nparts)]
+    A, _ = owned_global(parts, prob.ngdofs)
+    x = np.random.default_rng(5).standard_normal(prob.ngdofs)
+    assert rel_err(prob.spmv(x), A @ x) < 1e-13
+    _, nbytes = prob.spmv_device_timing(2)
+    nnz, n = prob.owned_nnz(), prob.ngdofs
+    nblk = -(-n // 128)
+    lo = 10 * nnz + 28 * n + 16 * nblk  # every entry 16-bit
+    assert lo < nbytes < lo + 0.05 * (2 * nnz)  # some entries 32-bit, at most 5%
+    prob.close()
+
+
 def test_pcg_linear_solution_2d(gpu_ctx):
     # u = x + y is in the Q1 space: the solve reproduces it at the nodes
     p = W.poisson2d_q1(64, (2, 2))
