@@ -65,7 +65,9 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
   constexpr int CH = KCI < 8 ? KCI : 8, KCG = KCO / G;
   const int cob = blockIdx.y * KCG;
   __shared__ float sx[CH * kHV];
-  __shared__ float sw[KCG * CH * 27];
+  // weights [c][t][co]: the KCG weights of one input channel and tap are
+  // contiguous, read as 128-bit LDS loads in the FMA loop
+  __shared__ __align__(16) float sw[KCG * CH * 27];
   int x0, y0, z0;
   tile_origin(blockIdx.x, n, x0, y0, z0);
   const int tid = threadIdx.x, lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
       if (c0 + c < KCI)
         v = MODE == 0 ? bf2f(w[((int64_t)(cob + co) * KCI + c0 + c) * 27 + t])
                       : bf2f(w[((int64_t)(c0 + c) * KCO + cob + co) * 27 + 26 - t]);
-      sw[i] = v;
+      sw[(c * 27 + t) * KCG + co] = v;
     }
     __syncthreads();
     for (int c = 0; c < CH; ++c) {
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
         const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
         const float v = sx[c * kHV + (lz + dz) * 100 + (ly + dy) * 10 + lx + dx];
 #pragma unroll
-        for (int co = 0; co < KCG; ++co) acc[co] = fmaf(v, sw[(co * CH + c) * 27 + t], acc[co]);
+        for (int co = 0; co < KCG; ++co) acc[co] = fmaf(v, sw[(c * 27 + t) * KCG + co], acc[co]);
       }
     }
   }
