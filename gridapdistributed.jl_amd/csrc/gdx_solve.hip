@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kBlk) void k_spmv(int n, const long long *__restric
 // fall in at most four windows of 16384 ids (its own part's rows and the ghost
 // columns' ranges in up to three neighbour parts) reads them as 16-bit codes
 // c16 = window << 14 | offset against the four bases bbase[block] (first base
-// -1: 32-bit ids), 10 instead of 12 bytes per entry.
+// -1: 32-bit ids), 10 instead of 12 bytes per entry.  Values and codes, read
+// once per product, are loaded non-temporally so that L2 keeps the gathered x.
 template <int RB, int NT>
 __global__ __launch_bounds__(NT) void k_spmv_blk(int n, const long long *__restrict__ rstart,
                                                  const int *__restrict__ rlen, const int *__restrict__ cgid,
@@ -89,8 +90,8 @@ __global__ __launch_bounds__(NT) void k_spmv_blk(int n, const long long *__restr
 #pragma unroll
       for (int i = 0; i < U; ++i) {
         const int k = t + NT * i;
-        vv[i] = k < m ? val[A + k] : 0.0;
-        const int c = k < m ? (int)c16[A + k] : 0, w = c >> 14;
+        vv[i] = k < m ? __builtin_nontemporal_load(val + A + k) : 0.0;
+        const int c = k < m ? (int)__builtin_nontemporal_load(c16 + A + k) : 0, w = c >> 14;
         cc[i] = (c & 0x3FFF) + (w == 0 ? bb.x : w == 1 ? bb.y : w == 2 ? bb.z : bb.w);
       }
     } else {
