@@ -114,6 +114,7 @@ int gdx_batch_autotune(gdx_batch *b, int32_t reps, int32_t *chosen) {
     gdx_problem *p = b->probs[i];
     int best = p->tileR[2], best_var = p->kvariant3d;
     p->precision = GDX_PREC_FP64;  // the choice is made for (and applies to) the FP64 form
+    p->hbm_patterns = false;       // and for the tiles' usual pattern placement
     if (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION) {
       float best_ms = 1e30f;
       // candidates: z-marching columns of 16, 32, 64 z rows; the per-tile kernel
@@ -221,6 +222,10 @@ void release_batch_device(gdx_batch *b) {
   free_branches(b);
   b->probs.clear();
   b->ctx = nullptr;
+}
+void batch_invalidate_problem(gdx_ctx *ctx, gdx_problem *p) {
+  for (gdx_batch *b : ctx->batches)
+    if (std::find(b->probs.begin(), b->probs.end(), p) != b->probs.end()) drop_graph(b);
 }
 void batch_forget_problem(gdx_ctx *ctx, gdx_problem *p) {
   for (gdx_batch *b : ctx->batches)

@@ -116,14 +116,18 @@ int check_desc(const gdx_problem_desc *d) {
 
 // tiles whose position patterns fit in the matrix kernel's LDS go first
 void order_tiles(gdx_problem *p) {
-  const int cap = p->D != 3             ? 0
+  const int cap = p->hbm_patterns       ? -1
+                  : p->D != 3             ? 0
                   : p->kvariant3d == 1 ? gdx::pattern_capacity_tile3d(p->K, p->NQ)
                   : p->kvariant3d == 2 ? gdx::pattern_capacity_tile3d_direct(p->K, p->NQ)
                   : p->kvariant3d == 3 ? gdx::pattern_capacity_tile3d_wslab(p->K, p->NQ)
                                        : gdx::pattern_capacity_3d(p->K, p->NQ);
-  std::stable_partition(p->tiles.begin(), p->tiles.end(), [cap](const gdx::Tile &t) { return t.npat <= cap; });
+  // (a tile without free rows has no pattern list to read from HBM: it stays in
+  // the LDS instance, which then stores nothing)
+  auto fast = [cap](const gdx::Tile &t) { return t.npat <= cap || t.npat == 0; };
+  std::stable_partition(p->tiles.begin(), p->tiles.end(), fast);
   p->n_fast_tiles = 0;
-  for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += t.npat <= cap ? 1 : 0;
+  for (const gdx::Tile &t : p->tiles) p->n_fast_tiles += fast(t) ? 1 : 0;
 }
 
 int part_check(gdx_problem *p, int32_t part) {
@@ -263,8 +267,12 @@ int gdx_symbolic(gdx_problem *p) {
     p->symbolic_done = true;
     return GDX_OK;
   }
+  // device buffers are re-made: graphs that captured the old ones are dropped
+  gdx::batch_invalidate_problem(p->ctx, p);
+  p->symbolic_done = false;
   free_device(p);
   gdx::elast_free(p);
+  gdx::flux_free(p);
   gdx::Device &d = p->dev;
   const bool elast = p->desc.op_kind == GDX_OP_ELASTICITY;
   // kappa and f at the quadrature points (constant-coefficient elasticity has none)
@@ -295,6 +303,12 @@ int gdx_symbolic(gdx_problem *p) {
     set_error("uploading the elasticity tables failed");
     return rc;
   }
+  // the flux operator's lattice state and id map, set up here so that no
+  // allocation, copy or fill ever runs inside a batch capture
+  if (p->desc.op_kind == GDX_OP_FLUX && (rc = gdx::flux_setup(p))) {
+    set_error("setting up the flux operator's device state failed");
+    return rc;
+  }
   for (hipEvent_t &e : p->ev)
     if (!e) GDX_HIP(hipEventCreate(&e));
   p->symbolic_done = true;
@@ -305,6 +319,8 @@ extern "C++" namespace gdx {
 // rebuild the tiles for extent R (the row table's tile-local pattern bits
 // change with them) and upload tiles, tile patterns and row table
 int retile(gdx_problem *p, const int R[3]) {
+  // the tile buffers are re-allocated: graphs holding the old ones are dropped
+  if (p->ctx) batch_invalidate_problem(p->ctx, p);
   build_tiles(p, R);
   order_tiles(p);
   int rc;
@@ -353,6 +369,7 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(precision == GDX_PREC_FP64 || (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION && p->kvariant3d == 0),
               GDX_ERR_UNSUPPORTED,
               "the FP32 form runs with the 3D z-marching kernel (gdx_problem_set_kernel(prob, 0, rows))");
+  if (precision != p->precision) gdx::batch_invalidate_problem(p->ctx, p);  // captured kernels change
   p->precision = precision;
   GDX_HIP(hipSetDevice(p->ctx->device));
   hipStream_t st = p->ctx->stream;
@@ -402,6 +419,8 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
 
 int gdx_problem_set_kernel(gdx_problem *p, int32_t variant, int32_t rows_z) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  const bool hbm = (variant & GDX_KERNEL_HBM_PATTERNS) != 0;
+  variant &= ~GDX_KERNEL_HBM_PATTERNS;
   GDX_REQUIRE(p->symbolic_done && p->ctx, GDX_ERR_STATE, "needs gdx_symbolic and a device context");
   GDX_REQUIRE(variant >= 0 && variant <= 3, GDX_ERR_ARG,
               "variant must be 0 (z-marching), 1 (per tile), 2 (per tile, direct stores) or 3 (per tile, wave slabs)");
@@ -412,12 +431,13 @@ int gdx_problem_set_kernel(gdx_problem *p, int32_t variant, int32_t rows_z) {
               "rows_z must be a positive multiple of the order");
   GDX_HIP(hipSetDevice(p->ctx->device));
   GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+  p->hbm_patterns = hbm;
   return gdx::set_kernel_variant(p, variant, rows_z);
 }
 
 int gdx_problem_get_kernel(gdx_problem *p, int32_t *variant, int32_t *rows_z) {
   GDX_REQUIRE(p && variant && rows_z, GDX_ERR_ARG, "NULL argument");
-  *variant = p->kvariant3d;
+  *variant = p->kvariant3d | (p->hbm_patterns ? GDX_KERNEL_HBM_PATTERNS : 0);
   *rows_z = p->tileR[2];
   return GDX_OK;
 }
@@ -440,6 +460,9 @@ int gdx_set_cell_coefficient(gdx_problem *p, const double *kappa) {
   GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
               "cell-wise kappa needs the diffusion operator with a constant source");
   GDX_HIP(hipSetDevice(p->ctx->device));
+  // a change of coefficient mode changes the captured coefficient launch: graphs
+  // holding p are dropped (new values in the same buffer replay as they are)
+  if (p->coef_cell != (kappa != nullptr)) gdx::batch_invalidate_problem(p->ctx, p);
   if (!kappa) {
     p->coef_cell = false;
     return GDX_OK;
@@ -462,6 +485,7 @@ int gdx_cell_coefficient_grf(gdx_problem *p, uint64_t seed, double corr_len, dou
               "cell-wise kappa needs the diffusion operator with a constant source");
   GDX_REQUIRE(corr_len > 0.0 && log_scale >= 0.0 && log_scale < 50.0, GDX_ERR_ARG, "invalid field parameters");
   GDX_HIP(hipSetDevice(p->ctx->device));
+  if (!p->coef_cell) gdx::batch_invalidate_problem(p->ctx, p);  // the coefficient mode changes
   int rc = gdx::grf_cell_coefficient(p, seed, corr_len, log_scale);
   if (rc) set_error("random field generation failed");
   return rc;

@@ -237,6 +237,13 @@ void launch_flux_t(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hip
   }
 }
 
+// zero fill (a kernel rather than a memset: flux_setup may run on a stream
+// that a caller later captures)
+__global__ void k_fill_zero(int64_t n, double *__restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
 }  // namespace
 
 bool flux_supported(int dim, int order, int nq) { return order == 1 && (dim == 2 || dim == 3) && nq >= 1 && nq <= 3; }
@@ -252,9 +259,10 @@ void flux_free(gdx_problem *p) {
 }
 
 // device buffers of the flux operator: lattice state with Dirichlet data, the
-// free-id -> lattice map, and the state vector (zero)
+// free-id -> lattice map, and the state vector (zero).  Run by gdx_symbolic,
+// never inside a capture; any failure leaves p->flux unset.
 int flux_setup(gdx_problem *p) {
-  if (p->flux) return GDX_OK;
+  flux_free(p);
   const int64_t nnodes = (int64_t)p->kp.nn[0] * p->kp.nn[1] * p->kp.nn[2];
   std::vector<long long> gid_node(std::max<int64_t>(p->ngdofs, 1), 0);
   for (const Part &P : p->P)
@@ -265,30 +273,37 @@ int flux_setup(gdx_problem *p) {
       const int64_t g2 = p->K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1]);
       gid_node[P.ldof_gid[i]] = g0 + (int64_t)p->kp.nn[0] * (g1 + (int64_t)p->kp.nn[1] * g2);
     }
-  FluxData *fd = new FluxData;
+  FluxData *fd = new (std::nothrow) FluxData;
+  if (!fd) return GDX_ERR_ALLOC;
   p->flux = fd;
   fd->nnodes = nnodes;
   hipStream_t st = p->ctx->stream;
+  const int64_t nx = std::max<int64_t>(p->ngdofs, 1);
   if (hipMalloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&fd->gid_node), gid_node.size() * sizeof(long long)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&fd->xs), std::max<int64_t>(p->ngdofs, 1) * sizeof(double)) != hipSuccess) {
+      hipMalloc(reinterpret_cast<void **>(&fd->xs), nx * sizeof(double)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }
   if (hipMemcpy(fd->gid_node, gid_node.data(), gid_node.size() * sizeof(long long), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      hipMemsetAsync(fd->xs, 0, std::max<int64_t>(p->ngdofs, 1) * sizeof(double), st) != hipSuccess)
+      hipSuccess) {
+    flux_free(p);
     return GDX_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, nx, fd->xs);
   hipLaunchKernelGGL(k_lattice_init, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, nnodes, fd->ulat,
                      p->kp);
-  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    flux_free(p);
+    return GDX_ERR_HIP;
+  }
+  return GDX_OK;
 }
 
 double *flux_state(gdx_problem *p) { return p->flux ? flux_data(p)->xs : nullptr; }
 
 int flux_set_state(gdx_problem *p, const double *xh) {
-  int rc = flux_setup(p);
-  if (rc) return rc;
+  if (!p->flux) return GDX_ERR_STATE;
   FluxData *fd = flux_data(p);
   hipStream_t st = p->ctx->stream;
   const size_t bytes = std::max<int64_t>(p->ngdofs, 1) * sizeof(double);
@@ -303,8 +318,7 @@ int flux_set_state(gdx_problem *p, const double *xh) {
 // by global id; nullptr = the stored state).  linear: p = 2, alpha = 1, beta = 0
 // (the Poisson operator with the same source), used for Newton's initial guess.
 int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st) {
-  int rc = flux_setup(p);
-  if (rc) return rc;
+  if (!p->flux) return GDX_ERR_STATE;  // flux_setup runs in gdx_symbolic
   FluxData *fd = flux_data(p);
   const int64_t n = p->ngdofs;
   if (n > 0)

@@ -148,6 +148,7 @@ struct gdx_problem {
   int n_fast_tiles = 0;  // leading tiles whose position patterns fit in LDS
   int kvariant3d = 0;    // 3D matrix kernel: 0 z-marching columns, 1 per-tile, 2 per-tile direct-store,
                          // 3 per-tile wave-slab (Q2); chosen by the autotuner
+  bool hbm_patterns = false;  // every tile takes the HBM-pattern kernel instance (GDX_KERNEL_HBM_PATTERNS)
   int precision = 0;     // contraction precision of the current numeric phase (GDX_PREC_*)
   int tileR[3] = {1, 1, 1};
   int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
@@ -170,6 +171,9 @@ void release_nn_device(gdx_nn *nn);
 void release_batch_device(gdx_batch *b);
 void pvector_release(gdx_problem *p);  // PRange maps; the problem's PVectors lose their device memory
 void batch_forget_problem(gdx_ctx *ctx, gdx_problem *p);  // a destroyed problem leaves every batch
+// every batch holding p drops its graph (its launch returns GDX_ERR_STATE until
+// captured again): called whenever p's buffers, kernels or coefficient mode change
+void batch_invalidate_problem(gdx_ctx *ctx, gdx_problem *p);
 template <class T>
 void unregister(std::vector<T *> &v, T *x) {
   for (size_t i = 0; i < v.size(); ++i)

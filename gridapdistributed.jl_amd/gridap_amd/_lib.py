@@ -14,6 +14,7 @@ GDX_OK = 0
 ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ERR_UNSUPPORTED", -5: "GDX_ERR_ALLOC"}
 SUB_ASSEMBLED_ROWS, FULLY_ASSEMBLED_ROWS = 0, 1
 PREC_FP64 = 0
+KERNEL_HBM_PATTERNS = 0x100  # gdx_problem_set_kernel flag: position patterns read from HBM in every tile
 RHS_CONST, RHS_MANUFACTURED = 0, 1
 OP_DIFFUSION, OP_FLUX, OP_ELASTICITY = 0, 1, 2
 DIR_CLAMPED_X0 = 2

@@ -156,6 +156,10 @@ int gdx_numeric(gdx_problem *prob, int32_t precision);
    rows (Q2 only; rows_z ignored for 1-3).  All give the same matrix to
    rounding; gdx_batch_autotune picks the fastest. */
 int gdx_problem_set_kernel(gdx_problem *prob, int32_t variant, int32_t rows_z);
+/* variant flag: every tile reads its position patterns from HBM, as the tiles
+   with more distinct patterns than the kernel's LDS holds always do (same
+   values; lets tests run those kernel instances on small meshes) */
+#define GDX_KERNEL_HBM_PATTERNS 0x100
 int gdx_problem_get_kernel(gdx_problem *prob, int32_t *variant, int32_t *rows_z);
 
 /* queries / exports (copies device data into caller arrays) */

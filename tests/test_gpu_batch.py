@@ -9,6 +9,7 @@ import oracle
 from _helpers import STRUCT_KEYS, rel_err
 from gridap_amd import AssemblyBatch, problem_from_workload
 from gridap_amd import workloads as W
+from gridap_amd._lib import KERNEL_HBM_PATTERNS
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +76,31 @@ def test_tile_kernel_matches_oracle(gpu_ctx, p, variant):
     for k, P in enumerate(r.parts):
         Q = prob.export_part(k)
         assert rel_err(Q["val"], P["val"]) <= 1e-12
+    prob.close()
+
+
+@pytest.mark.parametrize("p", [W.poisson3d_q2_fourier(7, (2, 1, 2)), W.poisson3d_q1(9, (2, 2, 1)),
+                               W.poisson3d_q2_fourier(6, (1, 2, 2), strategy="full")],
+                         ids=["q2", "q1", "q2-full"])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3], ids=["z-marching", "per-tile", "direct-store", "wave-slab"])
+def test_hbm_pattern_instances_match_oracle(gpu_ctx, p, variant):
+    """Tiles with more distinct position patterns than LDS holds run kernel
+    instances that read the patterns from HBM; the flag sends every tile there."""
+    print("hbm-pattern instance, variant", variant, p["order"], p["cells"], p["parts"], flush=True)
+    if variant == 3 and p["order"] != 2:
+        pytest.skip("the wave-slab kernel is built for Q2")
+    r = oracle.run(p)
+    assem, prob = make(gpu_ctx, p)
+    prob.set_kernel(variant | KERNEL_HBM_PATTERNS, 8)
+    assert prob.get_kernel()[0] == variant | KERNEL_HBM_PATTERNS
+    prob.numeric()
+    prob.sync()
+    for k, P in enumerate(r.parts):
+        Q = prob.export_part(k)
+        assert rel_err(Q["val"], P["val"]) <= 1e-12
+        assert rel_err(Q["b"], P["b"]) <= 1e-12
+        no = P["n_own"]
+        assert np.all(Q["val"][Q["rowptr"][no]:] == 0.0) and np.all(Q["b"][no:] == 0.0)
     prob.close()
 
 

@@ -240,6 +240,73 @@ def test_q2_numbering_rule_2d():
     assert list(P["ldof_gid"]) == list(range(15))
 
 
+# Hand-enumerated 3D Q2 numbering (one part, no Dirichlet), written out as
+# lattice coordinates without the polytope tables of the oracle or the
+# library.  The rule written down: all vertices lexicographically (x fastest);
+# then the edges at first encounter over the cells in order, each cell's
+# edges in the HEX's local order (the four x-parallel edges at (y,z) = (0,0),
+# (1,0), (0,1), (1,1), then the y-parallel edges at (x,z) in the same order,
+# then the z-parallel ones at (x,y)); then the faces likewise (z = 0, z = 1,
+# y = 0, y = 1, x = 0, x = 1); then the cell interiors.
+Q2_3D_NUMBERING = {
+    (2, 1, 1): [(0, 0, 0), (2, 0, 0), (4, 0, 0), (0, 2, 0), (2, 2, 0), (4, 2, 0),
+                (0, 0, 2), (2, 0, 2), (4, 0, 2), (0, 2, 2), (2, 2, 2), (4, 2, 2),
+                # cell 0 edges: x-parallel, y-parallel, z-parallel
+                (1, 0, 0), (1, 2, 0), (1, 0, 2), (1, 2, 2), (0, 1, 0), (2, 1, 0), (0, 1, 2), (2, 1, 2),
+                (0, 0, 1), (2, 0, 1), (0, 2, 1), (2, 2, 1),
+                # cell 1 edges not yet numbered
+                (3, 0, 0), (3, 2, 0), (3, 0, 2), (3, 2, 2), (4, 1, 0), (4, 1, 2), (4, 0, 1), (4, 2, 1),
+                # faces: cell 0, then cell 1's new ones
+                (1, 1, 0), (1, 1, 2), (1, 0, 1), (1, 2, 1), (0, 1, 1), (2, 1, 1),
+                (3, 1, 0), (3, 1, 2), (3, 0, 1), (3, 2, 1), (4, 1, 1),
+                (1, 1, 1), (3, 1, 1)],
+    (1, 1, 2): [(0, 0, 0), (2, 0, 0), (0, 2, 0), (2, 2, 0), (0, 0, 2), (2, 0, 2), (0, 2, 2), (2, 2, 2),
+                (0, 0, 4), (2, 0, 4), (0, 2, 4), (2, 2, 4),
+                (1, 0, 0), (1, 2, 0), (1, 0, 2), (1, 2, 2), (0, 1, 0), (2, 1, 0), (0, 1, 2), (2, 1, 2),
+                (0, 0, 1), (2, 0, 1), (0, 2, 1), (2, 2, 1),
+                (1, 0, 4), (1, 2, 4), (0, 1, 4), (2, 1, 4), (0, 0, 3), (2, 0, 3), (0, 2, 3), (2, 2, 3),
+                (1, 1, 0), (1, 1, 2), (1, 0, 1), (1, 2, 1), (0, 1, 1), (2, 1, 1),
+                (1, 1, 4), (1, 0, 3), (1, 2, 3), (0, 1, 3), (2, 1, 3),
+                (1, 1, 1), (1, 1, 3)],
+}
+
+
+@pytest.mark.parametrize("cells", sorted(Q2_3D_NUMBERING))
+def test_q2_numbering_rule_3d(cells, host_ctx):
+    from gridap_amd import fespaces, problem_from_workload
+    p = dict(dim=3, order=2, qdeg=4, cells=cells, parts=(1, 1, 1), domain=(0, cells[0], 0, cells[1], 0, cells[2]),
+             coef_const=1.0, modes=np.zeros((0, 5)), rhs="const", rhs_const=0.0, dirichlet=False, uD=(0, 0, 0, 0))
+    nnx, nny = 2 * cells[0] + 1, 2 * cells[1] + 1
+    expect = [x + nnx * (y + nny * z) for x, y, z in Q2_3D_NUMBERING[cells]]
+    assert len(set(expect)) == len(expect) == nnx * nny * (2 * cells[2] + 1)
+    P = oracle.run(p).parts[0]
+    assert list(P["ldof_gnode"]) == expect
+    assert list(P["ldof_gid"]) == list(range(len(expect)))
+    # the library's host symbolic phase numbers the same way
+    assem, data = problem_from_workload(host_ctx, p)
+    desc, keep = fespaces.problem_desc(data, assem.strategy)
+    gp = fespaces.GpuProblem(host_ctx, desc, keep)
+    gp.symbolic()
+    Q = gp.export_part(0)
+    assert list(Q["ldof_gnode"]) == expect
+    gp.close()
+
+
+def test_c1_full_size_against_closed_forms():
+    # C1 itself: 2D Q1 256^2, parts (2,2); counts, the interior stencil and A x* = b for u = x + y
+    p = W.poisson2d_q1(256, (2, 2))
+    r, A, b = _global(p)
+    assert r.ngdofs == 255 ** 2 and A.nnz == 763 ** 2
+    g2n = np.zeros(r.ngdofs, np.int64)
+    for P in r.parts:
+        g2n[P["ldof_gid"]] = P["ldof_gnode"]
+    xs, ys = (g2n % 257) / 256.0, (g2n // 257) / 256.0
+    assert np.abs(A @ (xs + ys) - b).max() <= 1e-12 * np.abs(b).max()
+    d = A.diagonal()
+    assert np.allclose(d, 8.0 / 3.0, rtol=1e-13, atol=0)
+    assert abs(A - A.T).max() <= 1e-15 * abs(A).max()
+
+
 @pytest.mark.parametrize("p", [
     W.poisson2d_q1(5, (2, 2)),
     dict(W.poisson2d_q1(4, (1, 2)), modes=W.fourier_modes(3, nmodes=4, kmax=3), rhs="manufactured",
