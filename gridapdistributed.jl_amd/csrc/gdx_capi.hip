@@ -167,6 +167,13 @@ int gdx_init(int32_t device, gdx_ctx **out) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     return GDX_ERR_HIP;
   }
+  // the device address of the no-alloc entry point's staging array
+  if (gdx::noalloc_setup(device) != GDX_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    set_error("hipGetSymbolAddress of the no-alloc staging array failed");
+    return GDX_ERR_HIP;
+  }
   *out = c;
   return GDX_OK;
 }

@@ -183,6 +183,10 @@ void unregister(std::vector<T *> &v, T *x) {
     }
 }
 int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
+// device address of the no-alloc staging array (gdx_twogrid.hip): resolved once
+// per device with that device current (gdx_init), then looked up
+int noalloc_setup(int device);
+double *noalloc_scratch(int device);
 void fill_kparams(gdx_problem *p);
 void tile_shape(int dim, int K, int R[3]);
 void build_tiles(gdx_problem *p, const int R[3]);

@@ -24,9 +24,17 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "gdx_nn.h"
+
+// Staging of r and z for gdx_nn_apply_noalloc: one file-scope array per GPU
+// (each device that loads the library holds its own copy until the process
+// ends).  Host code never passes the array's name to a kernel or a copy: it
+// uses the per-device address that noalloc_setup obtains.
+static constexpr size_t kNoAllocDoubles = (size_t)4 << 20;  // 32 MiB: r and z of up to 2,097,152 dofs
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 namespace gdx {
 
@@ -1115,6 +1123,30 @@ int twogrid_train_step(gdx_nn *nn, uint64_t seed, float lr, int smooth_passes, f
   return GDX_OK;
 }
 
+// per-device address of the no-alloc staging array, set once with that device current
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return GDX_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return GDX_OK;
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return GDX_ERR_STATE;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return GDX_ERR_HIP;
+  g_noalloc_ptr[device] = static_cast<double *>(ptr);
+  return GDX_OK;
+}
+
+double *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return device >= 0 && device < kMaxDevices ? g_noalloc_ptr[device] : nullptr;
+}
+
 }  // namespace gdx
 
 // ---------------------------------------------------------------------------
@@ -1244,6 +1276,42 @@ int gdx_nn_apply(gdx_nn *nn, const double *r, double *z) {
     rc = GDX_ERR_HIP;
   (void)hipFree(rd);
   (void)hipFree(zd);
+  return rc;
+}
+
+int gdx_nn_apply_noalloc(gdx_nn *nn, const double *r, double *z) {
+  if (!nn || !r || !z) {
+    gdx::set_error("NULL argument");
+    return GDX_ERR_ARG;
+  }
+  if (!nn->tg || !nn->ctx) {
+    gdx::set_error("gdx_nn_attach has not run");
+    return GDX_ERR_STATE;
+  }
+  const int dev = nn->ctx->device;
+  if (hipSetDevice(dev) != hipSuccess) return GDX_ERR_HIP;
+  gdx_problem *p = gdx::twogrid_problem(nn);
+  const int64_t n = p->ngdofs;
+  if (2 * n > (int64_t)kNoAllocDoubles) {
+    gdx::set_error("gdx_nn_apply_noalloc: the problem has more than 2,097,152 free dofs (the staging limit)");
+    return GDX_ERR_ARG;
+  }
+  int rc = gdx::noalloc_setup(dev);  // normally done by gdx_init on this device
+  double *base = gdx::noalloc_scratch(dev);
+  if (rc || !base) {
+    gdx::set_error("gdx_nn_apply_noalloc: no device address for the staging array");
+    return rc ? rc : GDX_ERR_HIP;
+  }
+  // r is copied in whole and z written in whole: nothing has to start at zero
+  double *rd = base, *zd = base + n;
+  hipStream_t st = p->ctx->stream;
+  rc = n == 0 || hipMemcpyAsync(rd, r, n * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess ? GDX_OK
+                                                                                                   : GDX_ERR_HIP;
+  if (!rc) rc = gdx::twogrid_apply(nn, rd, zd, st);
+  if (!rc && ((n > 0 && hipMemcpyAsync(z, zd, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) ||
+              hipStreamSynchronize(st) != hipSuccess))
+    rc = GDX_ERR_HIP;
+  if (rc) gdx::set_error("gdx_nn_apply_noalloc failed");
   return rc;
 }
 

@@ -29,7 +29,7 @@ EXPORTED = [
     "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg", "gdx_set_state", "gdx_newton",
     "gdx_set_cell_coefficient", "gdx_cell_coefficient_grf", "gdx_export_cell_coefficient",
     "gdx_nn_create", "gdx_nn_destroy", "gdx_nn_num_params", "gdx_nn_save", "gdx_nn_load", "gdx_nn_forward",
-    "gdx_nn_backward", "gdx_nn_attach", "gdx_nn_train_step", "gdx_nn_apply", "gdx_fpcg",
+    "gdx_nn_backward", "gdx_nn_attach", "gdx_nn_train_step", "gdx_nn_apply", "gdx_nn_apply_noalloc", "gdx_fpcg",
     "gdx_nn_set_options", "gdx_nn_coarse_operator", "gdx_nn_set_multigrid", "gdx_nn_level_operator",
     "gdx_batch_create", "gdx_batch_add", "gdx_batch_autotune", "gdx_batch_capture", "gdx_batch_launch",
     "gdx_batch_destroy", "gdx_problem_set_kernel", "gdx_problem_get_kernel",
@@ -135,6 +135,7 @@ def lib():
         "gdx_nn_attach": [vp, vp],
         "gdx_nn_train_step": [vp, ctypes.c_uint64, ctypes.c_float, i32, P(ctypes.c_float)],
         "gdx_nn_apply": [vp, vp, vp],
+        "gdx_nn_apply_noalloc": [vp, vp, vp],
         "gdx_nn_set_options": [vp, i32, i32],
         "gdx_nn_coarse_operator": [vp, vp],
         "gdx_nn_set_multigrid": [vp, i32, i32, vp],

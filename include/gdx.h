@@ -296,6 +296,15 @@ int gdx_nn_level_operator(gdx_nn *nn, int32_t level, int32_t *m, double *stencil
 int gdx_nn_coarse_operator(gdx_nn *nn, double *stencil);
 /* z = M r on host vectors (global-id order) */
 int gdx_nn_apply(gdx_nn *nn, const double *r, double *z);
+/* gdx_nn_apply without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate (serving loops, harnesses
+   that forbid hipMalloc).  The device staging of r and z lives in a static
+   array of 32 MiB that each GPU running the library holds until the process
+   ends, so the attached problem may have at most 2,097,152 free dofs (larger:
+   GDX_ERR_ARG).  Every call on a GPU shares that array: calls that overlap on
+   one GPU (several contexts or threads) race.  Same kernels and results as
+   gdx_nn_apply. */
+int gdx_nn_apply_noalloc(gdx_nn *nn, const double *r, double *z);
 /* Flexible PCG with the learned multilevel preconditioner (nn attached to prob),
    or Jacobi-PCG when nn is NULL; falls back to Jacobi-PCG from the current
    iterate when the flexible iteration stagnates (fellback = 1).  rhs: host
