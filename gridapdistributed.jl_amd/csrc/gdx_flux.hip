@@ -8,16 +8,15 @@
 // (test/PLaplacianTests.jl:24-31 is p = 4, alpha = beta = 1; the p-Laplacian
 // is p = 3 with alpha = eps^2 regularising grad(u) = 0).
 //
-// Owner computes rows: 2^D lanes per owned free row, one per adjacent cell,
-// evaluate grad(u) at the cell's Gauss points from the state on the global
-// node lattice (free values scattered there by global id, Dirichlet nodes hold
-// uD) and accumulate the cell's contributions to the row; the row's 3^D
-// Jacobian entries and its residual are then summed over the cells in a fixed
-// order.  Each Gauss point is evaluated by the 2^D rows of its cell:
-// recomputation instead of a scatter, so there are no atomics and two
-// assemblies are bit-identical.  Rows are written through their position
-// patterns (Dirichlet columns dropped, as in the affine path); ghost rows are
-// never written and stay zero.
+// Two launches per assembly.  The cell stage evaluates every Gauss point of
+// every mesh cell once, from the state on the global node lattice (free values
+// scattered there by global id, Dirichlet nodes hold uD), into the cell's
+// symmetric Jacobian and residual (HBM, entry-major).  The row stage is owner
+// computes rows: 2^D lanes per owned free row gather the row's entries from its
+// adjacent cells and sum them in a fixed cell order, so there are no atomics
+// and two assemblies are bit-identical.  Rows are written through their
+// position patterns (Dirichlet columns dropped, as in the affine path); ghost
+// rows are never written and stay zero.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +38,9 @@ struct FluxData {
   double *ulat = nullptr;         // u on the global node lattice
   long long *gid_node = nullptr;  // free global id -> lattice node
   double *xs = nullptr;           // state: free values by global id
+  double *cm = nullptr;           // cell Jacobians and residuals [NC(NC+1)/2 + NC][ncell]
+  double *tab = nullptr;          // reference-cell tables per Gauss point (FluxTab layout)
+  int64_t ncell = 0;
   bool state_set = false;
 };
 
@@ -85,56 +87,199 @@ __global__ void k_state_scatter(int64_t n, const long long *__restrict__ gid_nod
   if (g < n) ulat[gid_node[g]] = x ? x[g] : 0.0;
 }
 
-// 2^D lanes per row, one per adjacent cell: a lane evaluates grad(u) at its
-// cell's Gauss points and accumulates the cell's contributions to the row's
-// 2^D columns and to the residual; the row's lanes then sum the 3^D slots from
-// LDS in a fixed cell order.  Each Gauss point is still evaluated once per row
-// of its cell (no scatter, no atomics), but a lane holds only 2^D accumulators.
+// Reference-cell tables at every Gauss point, identical for every cell of the
+// uniform mesh, built once on the host (flux_setup) into a small read-only
+// device buffer: per Gauss point q a record of QS doubles
+//   [0]                 w_q x cell volume
+//   [1 + a]             phi_a(q)
+//   [1 + NC + a D + d]  d phi_a / d x_d (q)   (1/h applied)
+//   [1 + NC + NC D + k] grad(phi_a).grad(phi_b) (q), k = sym_index(a, b), a <= b
+// Every lane reads the same entries, so the kernels load them with scalar
+// loads (uniform addresses) instead of LDS reads.
+template <int D>
+struct FluxTab {
+  static constexpr int NC = 1 << D, TW = 0, TP = 1, TG = 1 + NC, TGAB = 1 + NC + NC * D;
+  static constexpr int QS = (TGAB + NC * (NC + 1) / 2 + 7) & ~7;
+};
+
+// position of cell-matrix entry (a, b), a <= b, in the packed upper triangle
+template <int NC>
+__host__ __device__ constexpr int sym_index(int a, int b) {
+  return a * NC - a * (a - 1) / 2 + (b - a);
+}
+
+// cell Jacobian and residual of one cell (the stage-1 arithmetic, shared by the
+// two-launch and the fused forms): J packed upper triangle, then r
 template <int D, int NQ, int MODE>
-__global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles, int ntiles,
-                                                 const RowInfo *__restrict__ rowinfo,
-                                                 const uint8_t *__restrict__ pat_fwd, int pstride,
-                                                 const double *__restrict__ ulat, double *__restrict__ values,
-                                                 double *__restrict__ b, KParams kp, FluxParams fp) {
-  constexpr int S = D == 3 ? 27 : 9, NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ;
+__device__ __forceinline__ void flux_cell(const double *__restrict__ tab, const double *__restrict__ ulat, int64_t c0,
+                                          int64_t c1, int64_t c2, const KParams &kp, const FluxParams &fp,
+                                          double (&J)[(1 << D) * ((1 << D) + 1) / 2], double (&R)[1 << D]) {
+  using F = FluxTab<D>;
+  constexpr int NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ, NJ = NC * (NC + 1) / 2;
+  const int64_t sy = kp.nn[0], sz = (int64_t)kp.nn[0] * kp.nn[1];
+  const double *ub = ulat + c0 + sy * c1 + sz * c2;
+  double uc[NC];
+#pragma unroll
+  for (int bb = 0; bb < NC; ++bb) uc[bb] = ub[(bb & 1) + sy * ((bb >> 1) & 1) + (D == 3 ? sz * ((bb >> 2) & 1) : 0)];
+#pragma unroll
+  for (int k = 0; k < NJ; ++k) J[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < NC; ++a) R[a] = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < NQD; ++q) {
+    const double *T = tab + q * F::QS;
+    double gu0 = 0.0, gu1 = 0.0, gu2 = 0.0;
+#pragma unroll
+    for (int bb = 0; bb < NC; ++bb) {
+      gu0 = fma(uc[bb], T[F::TG + bb * D], gu0);
+      gu1 = fma(uc[bb], T[F::TG + bb * D + 1], gu1);
+      if constexpr (D == 3) gu2 = fma(uc[bb], T[F::TG + bb * D + 2], gu2);
+    }
+    const double s2 = fp.alpha + fp.beta * (gu0 * gu0 + gu1 * gu1 + gu2 * gu2);
+    const double kap = flux_kappa<MODE>(s2, fp);
+    const double gam = (MODE == 1 || s2 <= 0.0) ? 0.0 : (fp.p - 2.0) * fp.beta * kap / s2;
+    const double wd = T[F::TW];
+    double ga[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+      double v = gu0 * T[F::TG + a * D] + gu1 * T[F::TG + a * D + 1];
+      if constexpr (D == 3) v += gu2 * T[F::TG + a * D + 2];
+      ga[a] = v;
+    }
+    const double wk = wd * kap, wg = wd * gam;
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+      R[a] = fma(wd, kap * ga[a] - fp.f * T[F::TP + a], R[a]);
+      const double wga = wg * ga[a];
+#pragma unroll
+      for (int bb = a; bb < NC; ++bb) {
+        const int k = sym_index<NC>(a, bb);
+        J[k] = fma(wk, T[F::TGAB + k], fma(wga, ga[bb], J[k]));
+      }
+    }
+  }
+}
+
+// Stage 1: one lane per mesh cell evaluates each of the cell's Gauss points
+// once (grad u, kappa, gamma) and accumulates the cell Jacobian
+//   J_ab = sum_q w_q (kappa grad(phi_a).grad(phi_b) + gamma (grad(u).grad(phi_a)) (grad(u).grad(phi_b)))
+// (symmetric: the upper triangle is kept) and the cell residual
+//   r_a = sum_q w_q (kappa grad(u).grad(phi_a) - f phi_a).
+// Output, entry-major for coalesced stores and loads: cm[e * ncell + cell],
+// e < NC(NC+1)/2 the packed J, then the NC residual entries.
+template <int D, int NQ, int MODE>
+__global__ __launch_bounds__(256) void k_flux_cells(int64_t ncell, const double *__restrict__ tab,
+                                                    const double *__restrict__ ulat, double *__restrict__ cm,
+                                                    KParams kp, FluxParams fp) {
+  constexpr int NC = 1 << D, NJ = NC * (NC + 1) / 2;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int64_t nc0 = kp.ncells[0], nc1 = kp.ncells[1];
+  const int64_t c0 = c % nc0, c1 = (c / nc0) % nc1, c2 = D == 3 ? c / (nc0 * nc1) : 0;
+  double J[NJ], R[NC];
+  flux_cell<D, NQ, MODE>(tab, ulat, c0, c1, c2, kp, fp, J, R);
+#pragma unroll
+  for (int k = 0; k < NJ; ++k) cm[k * ncell + c] = J[k];
+#pragma unroll
+  for (int a = 0; a < NC; ++a) cm[(NJ + a) * ncell + c] = R[a];
+}
+
+// Fused 3D form: one 512-lane workgroup per tile of 8 x 8 x 4 rows.  The
+// tile's (ext+1)^3 cells (at most 9 x 9 x 5) compute their Jacobian and
+// residual into LDS (one lane per cell, the stage-1 arithmetic), then two
+// lanes per row sum the row's 27 slots from LDS in the fixed cell order of
+// the row stage and store them through the row's position pattern.  Cells on
+// the tile boundary are recomputed by the neighbouring tiles (9 x 9 x 5 cells
+// per 256 rows) instead of a round trip of every cell matrix through HBM.
+template <int NQ, int MODE>
+__global__ __launch_bounds__(512) void k_flux_fused(const Tile *__restrict__ tiles, int ntiles,
+                                                    const RowInfo *__restrict__ rowinfo,
+                                                    const uint8_t *__restrict__ pat_fwd, int pstride,
+                                                    const double *__restrict__ tab, const double *__restrict__ ulat,
+                                                    double *__restrict__ values, double *__restrict__ b, KParams kp,
+                                                    FluxParams fp) {
+  constexpr int D = 3, NC = 8, NJ = 36, NE = NJ + NC, ES = NE + 1;  // odd record stride against bank conflicts
+  constexpr int R1 = 8, R2 = 8, R3 = 4, CX = R1 + 1, CY = R2 + 1, CZ = R3 + 1, NCELL = CX * CY * CZ;
+  constexpr int LPR = 512 / (R1 * R2 * R3);  // lanes per row
+  static_assert(NCELL <= 512, "one lane per cell");
+  __shared__ double cj[NCELL * ES];
+  const int tb = tile_of_block(ntiles);
+  if (tb >= ntiles) return;
+  const Tile t = tiles[tb];
+  const int tid = threadIdx.x;
+  // stage 1: the cells the tile's rows touch (zero outside the mesh)
+  if (tid < NCELL) {
+    const int ix = tid % CX, iy = (tid / CX) % CY, iz = tid / (CX * CY);
+    const int64_t cx = t.org[0] - 1 + ix, cy = t.org[1] - 1 + iy, cz = t.org[2] - 1 + iz;
+    double *dst = cj + tid * ES;
+    if (ix <= t.ext[0] && iy <= t.ext[1] && iz <= t.ext[2] && cx >= 0 && cx < kp.ncells[0] && cy >= 0 &&
+        cy < kp.ncells[1] && cz >= 0 && cz < kp.ncells[2]) {
+      double J[NJ], R[NC];
+      flux_cell<D, NQ, MODE>(tab, ulat, cx, cy, cz, kp, fp, J, R);
+#pragma unroll
+      for (int k = 0; k < NJ; ++k) dst[k] = J[k];
+#pragma unroll
+      for (int a = 0; a < NC; ++a) dst[NJ + a] = R[a];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NE; ++k) dst[k] = 0.0;
+    }
+  }
+  __syncthreads();
+  // stage 2: LPR lanes per row split its 27 slots
+  const int r = tid / LPR, q = tid % LPR;
+  const int u1 = r % R1, u2 = (r / R1) % R2, u3 = r / (R1 * R2);
+  if (u1 >= t.ext[0] || u2 >= t.ext[1] || u3 >= t.ext[2]) return;
+  const int g0 = t.org[0] + u1, g1 = t.org[1] + u2, g2 = t.org[2] + u3;
+  const RowInfo ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
+  if (ri.pid < 0) return;
+  const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
+  double *dstv = values + ri.rowstart;
+  for (int sl = q; sl < 27; sl += LPR) {
+    const int s0 = sl % 3, s1 = (sl / 3) % 3, s2 = sl / 9;
+    double acc = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      const int c0 = cc & 1, c1 = (cc >> 1) & 1, c2 = (cc >> 2) & 1;
+      const int b0 = s0 - c0, b1 = s1 - c1, b2 = s2 - c2;
+      if (b0 < 0 || b0 > 1 || b1 < 0 || b1 > 1 || b2 < 0 || b2 > 1) continue;
+      const int ai = (NC - 1) - cc, bb = b0 | (b1 << 1) | (b2 << 2);
+      const int e = bb < ai ? sym_index<NC>(bb, ai) : sym_index<NC>(ai, bb);
+      acc += cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ES + e];
+    }
+    const int pos = fwd[sl];
+    if (pos < kSlotDirichlet) dstv[pos] = acc;
+  }
+  if (q == 0) {
+    double rr = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      const int c0 = cc & 1, c1 = (cc >> 1) & 1, c2 = (cc >> 2) & 1;
+      rr += cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ES + NJ + (NC - 1) - cc];
+    }
+    b[ri.bidx] = rr;
+  }
+}
+
+// Stage 2: 2^D lanes per owned free row, one per adjacent cell, read the cell's
+// row of J (the row node's local index ai against the 2^D columns) and r_ai;
+// the row's lanes then sum the 3^D slots from LDS over the cells holding both
+// nodes in a fixed cell order, and write them through the row's position
+// pattern (Dirichlet columns dropped).  No atomics: two assemblies are
+// bit-identical.  Ghost rows are never written and stay zero.
+template <int D>
+__global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tiles, int ntiles,
+                                                   const RowInfo *__restrict__ rowinfo,
+                                                   const uint8_t *__restrict__ pat_fwd, int pstride,
+                                                   const double *__restrict__ cm, int64_t ncell,
+                                                   double *__restrict__ values, double *__restrict__ b, KParams kp) {
+  constexpr int S = D == 3 ? 27 : 9, NC = 1 << D, NJ = NC * (NC + 1) / 2;
   constexpr int RPB = 256 / NC;  // rows per pass of the workgroup
   __shared__ double part[256][NC + 1];
-  // reference-cell tables at every Gauss point, identical for every cell of the
-  // uniform mesh: gradients of the 2^D local basis functions (1/h applied),
-  // values, weight x cell volume, and grad(phi_a).grad(phi_b) (rows padded to
-  // NC + 1 doubles against bank conflicts).  Filled once from the 1D tables of
-  // the kernel argument, so the Gauss-point loop only reads LDS.
-  __shared__ double tg[NQD][NC][D], tp[NQD][NC], tw[NQD], tgab[NQD][NC][NC + 1];
-  {
-    const double ih[3] = {1.0 / kp.h[0], 1.0 / kp.h[1], D == 3 ? 1.0 / kp.h[2] : 0.0};
-    const double vol = kp.h[0] * kp.h[1] * (D == 3 ? kp.h[2] : 1.0);
-    for (int i = threadIdx.x; i < NQD * NC; i += blockDim.x) {
-      const int q = i / NC, bb = i % NC;
-      const int q0 = q % NQ, q1 = (q / NQ) % NQ, q2 = D == 3 ? q / (NQ * NQ) : 0;
-      const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
-      const double p0 = kp.phi[b0][q0], p1 = kp.phi[b1][q1], p2 = D == 3 ? kp.phi[b2][q2] : 1.0;
-      const double d0 = kp.dphi[b0][q0] * ih[0], d1 = kp.dphi[b1][q1] * ih[1];
-      const double d2 = D == 3 ? kp.dphi[b2][q2] * ih[2] : 0.0;
-      tg[q][bb][0] = d0 * p1 * p2;
-      tg[q][bb][1] = p0 * d1 * p2;
-      if constexpr (D == 3) tg[q][bb][2] = p0 * p1 * d2;
-      tp[q][bb] = p0 * p1 * p2;
-      if (bb == 0) tw[q] = kp.qw[q0] * kp.qw[q1] * (D == 3 ? kp.qw[q2] : 1.0) * vol;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < NQD * NC * NC; i += blockDim.x) {
-      const int q = i / (NC * NC), aa = (i / NC) % NC, bb = i % NC;
-      double v = tg[q][aa][0] * tg[q][bb][0] + tg[q][aa][1] * tg[q][bb][1];
-      if constexpr (D == 3) v += tg[q][aa][2] * tg[q][bb][2];
-      tgab[q][aa][bb] = v;
-    }
-    __syncthreads();
-  }
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
   const Tile t = tiles[tb];
   const int nrows = t.ext[0] * t.ext[1] * t.ext[2];
-  const int64_t sy = kp.nn[0], sz = (int64_t)kp.nn[0] * kp.nn[1];
   const int tid = threadIdx.x, e = tid % NC, rl = tid / NC;
   const int e0 = e & 1, e1 = (e >> 1) & 1, e2 = D == 3 ? (e >> 2) & 1 : 0;
   const int ai = (NC - 1) - e;  // the row node's index in the cell: (1 - e0, 1 - e1, 1 - e2)
@@ -148,70 +293,45 @@ __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles,
       ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
     }
     const bool live = ri.pid >= 0;
-    double acc8[NC];
-#pragma unroll
-    for (int bb = 0; bb < NC; ++bb) acc8[bb] = 0.0;
-    double res = 0.0;
     const int c0 = g0 - 1 + e0, c1 = g1 - 1 + e1, c2 = D == 3 ? g2 - 1 + e2 : 0;
     const bool incell = live && c0 >= 0 && c0 < kp.ncells[0] && c1 >= 0 && c1 < kp.ncells[1] &&
                         (D == 2 || (c2 >= 0 && c2 < kp.ncells[2]));
+    double v[NC + 1];
     if (incell) {
-      const double *ub = ulat + c0 + sy * c1 + sz * c2;
-      double uc[NC];
+      const double *src = cm + c0 + (int64_t)kp.ncells[0] * (c1 + (int64_t)kp.ncells[1] * c2);
 #pragma unroll
       for (int bb = 0; bb < NC; ++bb)
-        uc[bb] = ub[(bb & 1) + sy * ((bb >> 1) & 1) + (D == 3 ? sz * ((bb >> 2) & 1) : 0)];
-#pragma unroll 1
-      for (int q = 0; q < NQD; ++q) {
-        double gu0 = 0.0, gu1 = 0.0, gu2 = 0.0;
+        v[bb] = src[(int64_t)(bb < ai ? sym_index<NC>(bb, ai) : sym_index<NC>(ai, bb)) * ncell];
+      v[NC] = src[(int64_t)(NJ + ai) * ncell];
+    } else {
 #pragma unroll
-        for (int bb = 0; bb < NC; ++bb) {
-          gu0 = fma(uc[bb], tg[q][bb][0], gu0);
-          gu1 = fma(uc[bb], tg[q][bb][1], gu1);
-          if constexpr (D == 3) gu2 = fma(uc[bb], tg[q][bb][2], gu2);
-        }
-        const double s2 = fp.alpha + fp.beta * (gu0 * gu0 + gu1 * gu1 + gu2 * gu2);
-        const double kap = flux_kappa<MODE>(s2, fp);
-        const double gam = (MODE == 1 || s2 <= 0.0) ? 0.0 : (fp.p - 2.0) * fp.beta * kap / s2;
-        const double wd = tw[q];
-        double gua = gu0 * tg[q][ai][0] + gu1 * tg[q][ai][1];
-        if constexpr (D == 3) gua += gu2 * tg[q][ai][2];
-        res = fma(wd, kap * gua - fp.f * tp[q][ai], res);
-        const double wk = wd * kap, wg = wd * gam * gua;
-#pragma unroll
-        for (int bb = 0; bb < NC; ++bb) {
-          double gub = gu0 * tg[q][bb][0] + gu1 * tg[q][bb][1];
-          if constexpr (D == 3) gub += gu2 * tg[q][bb][2];
-          acc8[bb] = fma(wk, tgab[q][ai][bb], fma(wg, gub, acc8[bb]));
-        }
-      }
+      for (int bb = 0; bb <= NC; ++bb) v[bb] = 0.0;
     }
 #pragma unroll
-    for (int bb = 0; bb < NC; ++bb) part[tid][bb] = acc8[bb];
-    part[tid][NC] = res;
+    for (int bb = 0; bb <= NC; ++bb) part[tid][bb] = v[bb];
     __syncthreads();
     if (live) {
       // slot (s0, s1, s2) of the row = sum over the cells c holding both nodes of
-      // c's contribution to column (s - c), cells in a fixed order
+      // c's entry for column (s - c), cells in a fixed order
       const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
       double *dst = values + ri.rowstart;
       const int base = rl * NC;
       for (int sl = e; sl < S; sl += NC) {
         const int s0 = sl % 3, s1 = (sl / 3) % 3, s2 = D == 3 ? sl / 9 : 0;
-        double v = 0.0;
+        double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const int b0 = s0 - (c & 1), b1 = s1 - ((c >> 1) & 1), b2 = D == 3 ? s2 - ((c >> 2) & 1) : 0;
+        for (int cc = 0; cc < NC; ++cc) {
+          const int b0 = s0 - (cc & 1), b1 = s1 - ((cc >> 1) & 1), b2 = D == 3 ? s2 - ((cc >> 2) & 1) : 0;
           if (b0 < 0 || b0 > 1 || b1 < 0 || b1 > 1 || b2 < 0 || b2 > 1) continue;
-          v += part[base + c][b0 | (b1 << 1) | (b2 << 2)];
+          acc += part[base + cc][b0 | (b1 << 1) | (b2 << 2)];
         }
         const int pos = fwd[sl];
-        if (pos < kSlotDirichlet) dst[pos] = v;
+        if (pos < kSlotDirichlet) dst[pos] = acc;
       }
       if (e == 0) {
         double rr = 0.0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) rr += part[base + c][NC];
+        for (int cc = 0; cc < NC; ++cc) rr += part[base + cc][NC];
         b[ri.bidx] = rr;
       }
     }
@@ -221,10 +341,21 @@ __global__ __launch_bounds__(256) void k_flux_q1(const Tile *__restrict__ tiles,
 
 template <int D, int NQ, int MODE>
 void launch_flux_m(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
+  if constexpr (D == 3 && NQ <= 2) {  // (3 Gauss points per direction: the tables leave no room in LDS)
+    if (p->tileR[0] == 8 && p->tileR[1] == 8 && p->tileR[2] == 4) {  // the fused form's tile shape
+      const int nt = (int)p->tiles.size();
+      hipLaunchKernelGGL((k_flux_fused<NQ, MODE>), dim3(8u * ((nt + 7) / 8)), dim3(512), 0, st, p->dev.tiles, nt,
+                         p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->tab, fd->ulat,
+                         p->dev.values, p->dev.b, p->kp, fp);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_flux_cells<D, NQ, MODE>), dim3((unsigned)((fd->ncell + 255) / 256)), dim3(256), 0, st,
+                     fd->ncell, fd->tab, fd->ulat, fd->cm, p->kp, fp);
   const int nt = (int)p->tiles.size();
-  hipLaunchKernelGGL((k_flux_q1<D, NQ, MODE>), dim3(8u * ((nt + 7) / 8)), dim3(256), 0, st, p->dev.tiles, nt,
-                     p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->ulat, p->dev.values, p->dev.b,
-                     p->kp, fp);
+  hipLaunchKernelGGL((k_flux_rows<D>), dim3(8u * ((nt + 7) / 8)), dim3(256), 0, st, p->dev.tiles, nt,
+                     p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->cm, fd->ncell, p->dev.values,
+                     p->dev.b, p->kp);
 }
 
 template <int D, int NQ>
@@ -251,7 +382,7 @@ bool flux_supported(int dim, int order, int nq) { return order == 1 && (dim == 2
 void flux_free(gdx_problem *p) {
   FluxData *fd = flux_data(p);
   if (!fd) return;
-  void *ptrs[] = {fd->ulat, fd->gid_node, fd->xs};
+  void *ptrs[] = {fd->ulat, fd->gid_node, fd->xs, fd->cm, fd->tab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete fd;
@@ -261,6 +392,37 @@ void flux_free(gdx_problem *p) {
 // device buffers of the flux operator: lattice state with Dirichlet data, the
 // free-id -> lattice map, and the state vector (zero).  Run by gdx_symbolic,
 // never inside a capture; any failure leaves p->flux unset.
+// the FluxTab records of every Gauss point (tensor-product rule, x fastest)
+static std::vector<double> flux_tables_host(const KParams &kp, int D, int NQ) {
+  const int NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ;
+  const int TP = 1, TG = 1 + NC, TGAB = 1 + NC + NC * D, QS = (TGAB + NC * (NC + 1) / 2 + 7) & ~7;
+  std::vector<double> t((size_t)NQD * QS, 0.0);
+  const double ih[3] = {1.0 / kp.h[0], 1.0 / kp.h[1], D == 3 ? 1.0 / kp.h[2] : 0.0};
+  const double vol = kp.h[0] * kp.h[1] * (D == 3 ? kp.h[2] : 1.0);
+  for (int q = 0; q < NQD; ++q) {
+    double *T = t.data() + (size_t)q * QS;
+    const int q0 = q % NQ, q1 = (q / NQ) % NQ, q2 = D == 3 ? q / (NQ * NQ) : 0;
+    T[0] = kp.qw[q0] * kp.qw[q1] * (D == 3 ? kp.qw[q2] : 1.0) * vol;
+    for (int bb = 0; bb < NC; ++bb) {
+      const int b0 = bb & 1, b1 = (bb >> 1) & 1, b2 = D == 3 ? (bb >> 2) & 1 : 0;
+      const double p0 = kp.phi[b0][q0], p1 = kp.phi[b1][q1], p2 = D == 3 ? kp.phi[b2][q2] : 1.0;
+      const double d0 = kp.dphi[b0][q0] * ih[0], d1 = kp.dphi[b1][q1] * ih[1];
+      const double d2 = D == 3 ? kp.dphi[b2][q2] * ih[2] : 0.0;
+      T[TP + bb] = p0 * p1 * p2;
+      T[TG + bb * D] = d0 * p1 * p2;
+      T[TG + bb * D + 1] = p0 * d1 * p2;
+      if (D == 3) T[TG + bb * D + 2] = p0 * p1 * d2;
+    }
+    for (int a = 0; a < NC; ++a)
+      for (int bb = a; bb < NC; ++bb) {
+        double v = 0.0;
+        for (int d = 0; d < D; ++d) v += T[TG + a * D + d] * T[TG + bb * D + d];
+        T[TGAB + a * NC - a * (a - 1) / 2 + (bb - a)] = v;
+      }
+  }
+  return t;
+}
+
 int flux_setup(gdx_problem *p) {
   flux_free(p);
   const int64_t nnodes = (int64_t)p->kp.nn[0] * p->kp.nn[1] * p->kp.nn[2];
@@ -277,16 +439,26 @@ int flux_setup(gdx_problem *p) {
   if (!fd) return GDX_ERR_ALLOC;
   p->flux = fd;
   fd->nnodes = nnodes;
+  fd->ncell = num_cells(p);
+  const int nc = 1 << p->D;
+  const int64_t ncm = std::max<int64_t>((int64_t)(nc * (nc + 1) / 2 + nc) * fd->ncell, 1);
   hipStream_t st = p->ctx->stream;
   const int64_t nx = std::max<int64_t>(p->ngdofs, 1);
-  if (hipMalloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
+  if (hipMalloc(reinterpret_cast<void **>(&fd->cm), ncm * sizeof(double)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&fd->gid_node), gid_node.size() * sizeof(long long)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&fd->xs), nx * sizeof(double)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }
+  const std::vector<double> tab = flux_tables_host(p->kp, p->D, p->NQ);
+  if (hipMalloc(reinterpret_cast<void **>(&fd->tab), tab.size() * sizeof(double)) != hipSuccess) {
+    flux_free(p);
+    return GDX_ERR_ALLOC;
+  }
   if (hipMemcpy(fd->gid_node, gid_node.data(), gid_node.size() * sizeof(long long), hipMemcpyHostToDevice) !=
-      hipSuccess) {
+          hipSuccess ||
+      hipMemcpy(fd->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_HIP;
   }

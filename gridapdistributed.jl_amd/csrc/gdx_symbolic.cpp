@@ -781,9 +781,10 @@ int symbolic(gdx_problem *p) {
   p->rows_total = row_base;
   p->owned_nnz = owned_nnz;
 
-  // tiles of owned rows
+  // tiles of owned rows (the flux operator's fused kernel takes 8 x 8 x 4 rows)
   int R[3];
   tile_shape(c.D, K, R);
+  if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
   build_tiles(p, R);
   return GDX_OK;
 }
