@@ -952,6 +952,12 @@ static void assemble_fresh(orc_ctx *C) {
   }
 }
 
+/* threads of the in-place re-assembly (orc_set_threads; 1 = the reference's
+   debug mode, all parts in one process one after another) */
+static int g_orc_threads = 1;
+void orc_set_threads(int n) { g_orc_threads = n < 1 ? 1 : n; }
+static int orc_threads_for(int nparts) { return g_orc_threads < nparts ? g_orc_threads : nparts; }
+
 /* in-place numeric re-assembly (LocalView path + assemble!) into the CSR built
    by assemble_fresh.  Returns elapsed seconds. */
 static double assemble_inplace(orc_ctx *C) {
@@ -978,6 +984,11 @@ static double assemble_inplace(orc_ctx *C) {
     colof_all[r] = gid_lookup(C->P[r].col_gid, C->P[r].ncols, C->ngdofs);
   }
   double t0 = now_s();
+  /* the parts are independent (as the reference's parts, one per process):
+     one OpenMP thread per part; assemble!(A) below writes each owner's rows
+     from one thread, senders in ascending order, so the result does not
+     depend on the thread count */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads_for(np))
   for (int r = 0; r < np; r++) {
     part_t *P = &C->P[r];
     memset(P->val, 0, sizeof(double) * P->rowptr[P->nrows]);
@@ -985,6 +996,7 @@ static double assemble_inplace(orc_ctx *C) {
     numeric_loop(C, P, 1, fe_to_row[r], fe_to_col[r]);
   }
   if (!FA) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads_for(np))
     for (int dst = 0; dst < np; dst++) {
       part_t *Pd = &C->P[dst];
       for (int src = 0; src < np; src++) {
@@ -1006,6 +1018,7 @@ static double assemble_inplace(orc_ctx *C) {
         }
       }
     }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads_for(np))
     for (int r = 0; r < np; r++) {
       part_t *P = &C->P[r];
       for (i64 g = P->n_own; g < P->nrows; g++) {

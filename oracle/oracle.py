@@ -54,6 +54,8 @@ def lib():
         L.orc_part_fe.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.orc_part_matrix.argtypes = [vp, ctypes.c_int] + [vp] * 8
         L.orc_free.argtypes = [vp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
         L.orc_energy.restype = ctypes.c_double
         L.orc_energy.argtypes = [vp]
         _LIB = L
@@ -131,8 +133,11 @@ def energy(p):
         L.orc_free(h)
 
 
-def run(p, inplace_reps=0):
+def run(p, inplace_reps=0, threads=1):
+    """threads: OpenMP threads of the in-place re-assembly (one part per thread;
+    1 = the parts one after another, as in the reference's debug mode)."""
     L = lib()
+    L.orc_set_threads(int(threads))
     d = make_desc(p)
     h = L.orc_run(ctypes.byref(d), inplace_reps)
     try:
