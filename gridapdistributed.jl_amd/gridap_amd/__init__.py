@@ -16,6 +16,7 @@ from .fespaces import (CLAMPED_X0_TAGS, AffineFEOperator, BodyForce, ConstantCoe
 from .geometry import CartesianDiscreteModel, DeviceContext, Measure, Triangulation, with_gpu
 from .batch import AssemblyBatch
 from .learned import LearnedCoarseSolver, fpcg
+from .visualization import interpolate, writevtk
 from .nonlinear import (FEFunction, FEOperator, FESolver, FluxJacobian, FluxResidual, NLSolver, PowerLawFlux,
                         jacobian, residual, residual_and_jacobian, residual_and_jacobian_, solve, zero)
 
