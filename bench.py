@@ -144,9 +144,11 @@ def cpu_baseline_solve(n=28):
 
 
 def bench_mixed(prob, args, nnz, ms_fp64, variant, rows_z):
-    """The mixed form (FP32 sum-factorised contractions, FP64 CSR values) on
-    the same problem: best z-marching column length, its Gnnz/s over the FP64
-    form's, and its error against the FP64 values (max and Frobenius, relative)."""
+    """The mixed form (FP32 sum-factorised contractions, FP64 CSR values, RHS
+    and lifting) on the same problem and the same kernel family as the FP64
+    form: the direct-store kernel, and the z-marching kernel at its best column
+    length beside it; Gnnz/s over the FP64 form's, and the error against the
+    FP64 values (max and Frobenius, relative)."""
     import numpy as np
 
     def values():
@@ -154,11 +156,7 @@ def bench_mixed(prob, args, nnz, ms_fp64, variant, rows_z):
         parts = [prob.export_part(k) for k in range(prob.nparts)]
         return np.concatenate([P["val"] for P in parts]), np.concatenate([P["b"] for P in parts])
 
-    prob.numeric()
-    a64, b64 = values()
-    best = None
-    for rz in (16, 32, 64):
-        prob.set_kernel(0, rz)
+    def timed():
         for _ in range(args.warmup):
             prob.numeric(precision=1)
         prob.sync()
@@ -166,19 +164,36 @@ def bench_mixed(prob, args, nnz, ms_fp64, variant, rows_z):
         for _ in range(args.steps):
             prob.numeric(precision=1)
         prob.sync()
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    prob.numeric()
+    a64, b64 = values()
+    forms = {}
+    prob.set_kernel(2)
+    forms["direct-store"] = (timed(), 2, 0)
+    best = None
+    for rz in (16, 32, 64):
+        prob.set_kernel(0, rz)
+        ms = timed()
         if best is None or ms < best[0]:
-            best = (ms, rz)
-    prob.set_kernel(0, best[1])
-    prob.numeric(precision=1)
-    a32, b32 = values()
+            best = (ms, 0, rz)
+    forms["z-marching"] = best
+    out = {}
+    for name, (ms, var, rz) in forms.items():
+        prob.set_kernel(var, rz if rz else 32)
+        prob.numeric(precision=1)
+        a32, b32 = values()
+        d = a32 - a64
+        out[name] = {"value_gnnz_s": round(nnz / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 4),
+                     "speedup_vs_fp64": round(ms_fp64 / ms, 3),
+                     "max_rel_err": float(np.abs(d).max() / np.abs(a64).max()),
+                     "fro_rel_err": float(np.linalg.norm(d) / np.linalg.norm(a64)),
+                     "rhs_max_rel_err": float(np.abs(b32 - b64).max() / np.abs(b64).max())}
+        if var == 0:
+            out[name]["z_rows_per_workgroup"] = rz
     prob.set_kernel(variant, rows_z)  # back to the FP64 form's tuned kernel
-    d = a32 - a64
-    return {"value_gnnz_s": round(nnz / (best[0] * 1e-3) / 1e9, 3), "ms_per_step": round(best[0], 4),
-            "z_rows_per_workgroup": best[1], "speedup_vs_fp64": round(ms_fp64 / best[0], 3),
-            "max_rel_err": float(np.abs(d).max() / np.abs(a64).max()),
-            "fro_rel_err": float(np.linalg.norm(d) / np.linalg.norm(a64)),
-            "rhs_max_rel_err": float(np.abs(b32 - b64).max() / np.abs(b64).max())}
+    prob.numeric()
+    return out
 
 
 def bench_plaplacian(args):

@@ -373,9 +373,10 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
   GDX_REQUIRE(precision == GDX_PREC_FP64 || precision == GDX_PREC_FP32, GDX_ERR_ARG, "unknown precision");
-  GDX_REQUIRE(precision == GDX_PREC_FP64 || (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION && p->kvariant3d == 0),
+  GDX_REQUIRE(precision == GDX_PREC_FP64 || (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION &&
+                                              (p->kvariant3d == 0 || p->kvariant3d == 2)),
               GDX_ERR_UNSUPPORTED,
-              "the FP32 form runs with the 3D z-marching kernel (gdx_problem_set_kernel(prob, 0, rows))");
+              "the FP32 form runs with the 3D z-marching or direct-store kernel (gdx_problem_set_kernel(prob, 0|2, rows))");
   if (precision != p->precision) gdx::batch_invalidate_problem(p->ctx, p);  // captured kernels change
   p->precision = precision;
   GDX_HIP(hipSetDevice(p->ctx->device));

@@ -354,7 +354,7 @@ int launch_matrix(gdx_problem *p, hipStream_t st) {
   if (p->tiles.empty()) return GDX_OK;
   const int K = p->K, NQ = p->NQ;
   if (p->D == 3) {
-    // the FP32 form exists for the z-marching kernel only (its tiles and pattern capacity)
+    // the FP32 form exists for the z-marching and the direct-store kernels
     if (p->precision == GDX_PREC_FP32 && p->kvariant3d == 0) return launch_matrix3d(p, st, GDX_PREC_FP32);
     if (p->kvariant3d == 2) return launch_matrix_tile3d_direct(p, st);
     if (p->kvariant3d == 3) return launch_matrix_tile3d_wslab(p, st);

@@ -323,9 +323,11 @@ __device__ __forceinline__ int y_pair_index(int u, int j) {
   return ND + 7 * (u / 2) + 4 + j;                                // s = -1, 0, 1
 }
 
-template <int K, int NQ, int W, int Q3, int U3>
+// CT: type of the contractions (double, or float for the mixed form); the
+// values are stored and the lifting accumulated in FP64 either way
+template <int K, int NQ, int W, int Q3, int U3, class CT>
 __device__ __forceinline__ double z_row_direct(const RowT &e, const uint8_t *__restrict__ fwd, int slot12,
-                                               const double (&Y1)[Q3], const double (&Y2)[Q3], double *__restrict__ values,
+                                               const CT (&Y1)[Q3], const CT (&Y2)[Q3], double *__restrict__ values,
                                                int g0, int g1, int g2, const KParams &kp) {
   double lift = 0.0;
 #pragma unroll
@@ -333,48 +335,48 @@ __device__ __forceinline__ double z_row_direct(const RowT &e, const uint8_t *__r
     if (K == 2 && (U3 & 1) && (s3 == -2 || s3 == 2)) continue;
     int lc0, nc, a0, b0;
     pair_cells<K>(U3, s3, lc0, nc, a0, b0);
-    double v = 0.0;
+    CT v = 0;
 #pragma unroll
     for (int tt = 0; tt < 2 * NQ; ++tt) {
       if (tt >= nc * NQ) break;
       double G, M;
       pair_factors<K, NQ>(kp, 2, a0, b0, tt, G, M);
-      v = fma(Y1[lc0 * NQ + tt], M, fma(Y2[lc0 * NQ + tt], G, v));
+      v = fma(Y1[lc0 * NQ + tt], (CT)M, fma(Y2[lc0 * NQ + tt], (CT)G, v));
     }
     const int pos = fwd[slot12 + (s3 + K) * W * W];
     if (pos < kSlotDirichlet) {
-      values[e.rowstart + pos] = v;
+      values[e.rowstart + pos] = (double)v;
     } else if (pos == kSlotDirichlet) {
       const int s1 = slot12 % W - K, s2 = slot12 / W - K;
-      lift = fma(v, uD_at(kp, g0 + s1, g1 + s2, g2 + s3), lift);
+      lift = fma((double)v, uD_at(kp, g0 + s1, g1 + s2, g2 + s3), lift);
     }
   }
   return lift;
 }
 
-template <int K, int NQ, int W, int Q3, int R3, int U = 0>
+template <int K, int NQ, int W, int Q3, int R3, class CT, int U = 0>
 __device__ __forceinline__ double z_row_direct_dispatch(int u3, const RowT &e, const uint8_t *__restrict__ fwd,
-                                                        int slot12, const double (&Y1)[Q3], const double (&Y2)[Q3],
+                                                        int slot12, const CT (&Y1)[Q3], const CT (&Y2)[Q3],
                                                         double *__restrict__ values, int g0, int g1, int g2,
                                                         const KParams &kp) {
   if constexpr (U < R3) {
-    if (u3 == U) return z_row_direct<K, NQ, W, Q3, U>(e, fwd, slot12, Y1, Y2, values, g0, g1, g2, kp);
-    return z_row_direct_dispatch<K, NQ, W, Q3, R3, U + 1>(u3, e, fwd, slot12, Y1, Y2, values, g0, g1, g2, kp);
+    if (u3 == U) return z_row_direct<K, NQ, W, Q3, U, CT>(e, fwd, slot12, Y1, Y2, values, g0, g1, g2, kp);
+    return z_row_direct_dispatch<K, NQ, W, Q3, R3, CT, U + 1>(u3, e, fwd, slot12, Y1, Y2, values, g0, g1, g2, kp);
   }
   return 0.0;
 }
 
 // the second bound is waves per SIMD: 4 keeps two 8-wave workgroups per CU (<= 128 VGPRs)
-template <int K, int NQ, int R1, int R2, int R3, bool PATG>
+template <int K, int NQ, int R1, int R2, int R3, bool PATG, class CT>
 __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3d_direct(
     const Tile *__restrict__ tiles, int tile0, int ntiles, const RowInfo *__restrict__ rowinfo,
     const double *__restrict__ Wq, const double *__restrict__ Fq, const uint8_t *__restrict__ pat_fwd,
     const int *__restrict__ tile_pats, double *__restrict__ values, double *__restrict__ b, KParams kp) {
   using T = Tile3D<K, NQ, R1, R2, R3>;
   __shared__ double lds[T::LDS_ALL];
-  double *kap = lds;                  // [Q3][Q2][Q1]
-  double *XG = lds + T::LDS_KAPPA;    // [NP1][XS]
-  double *XM = XG + T::NP1 * T::XS;   // [NP1][XS]
+  CT *kap = reinterpret_cast<CT *>(lds);  // [Q3][Q2][Q1]
+  CT *XG = kap + T::LDS_KAPPA;            // [NP1][XS]
+  CT *XM = XG + T::NP1 * T::XS;           // [NP1][XS]
   double *Fl = lds;                   // after stage 2: f [Q3][Q2][Q1]
   double *Z1 = lds + T::OFF_Z1;       // [u1][q3][q2]
   double *Z2 = lds + T::OFF_Z2;       // [u1][u2][q3]
@@ -438,7 +440,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
       v = Wq[qi];
       f = Fq[qi];
     }
-    if (i < T::LDS_KAPPA) kap[i] = v;
+    if (i < T::LDS_KAPPA) kap[i] = (CT)v;
     fpre[j] = f;
   }
   __syncthreads();
@@ -448,12 +450,17 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     int u, s, lc0, nc, a0, b0;
     pair_natural<K>(p1, u, s);
     pair_cells<K>(u, s, lc0, nc, a0, b0);
-    double G[2 * NQ], M[2 * NQ];
+    CT G[2 * NQ], M[2 * NQ];
 #pragma unroll
-    for (int tt = 0; tt < 2 * NQ; ++tt) pair_factors<K, NQ>(kp, 0, a0, b0, tt, G[tt], M[tt]);
+    for (int tt = 0; tt < 2 * NQ; ++tt) {
+      double g, m;
+      pair_factors<K, NQ>(kp, 0, a0, b0, tt, g, m);
+      G[tt] = (CT)g;
+      M[tt] = (CT)m;
+    }
     for (int it = lane; it < T::Q2 * T::Q3; it += 64) {
-      const double *kr = kap + it * T::Q1 + lc0 * NQ;
-      double xg = 0.0, xm = 0.0;
+      const CT *kr = kap + it * T::Q1 + lc0 * NQ;
+      CT xg = 0, xm = 0;
       if (nc == 2) {
 #pragma unroll
         for (int tt = 0; tt < 2 * NQ; ++tt) {
@@ -487,27 +494,27 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     pair_natural<K>(p1, u1, s1);
     pair_doubles_first<K, R2>(tid / T::NP1, u2, s2);
   }
-  double Y1[T::Q3], Y2[T::Q3];
+  CT Y1[T::Q3], Y2[T::Q3];
   {
     int lc0, nc, a0, b0;
     pair_cells<K>(u2, s2, lc0, nc, a0, b0);
-    double G[2 * NQ], M[2 * NQ];
+    CT G[2 * NQ], M[2 * NQ];
 #pragma unroll
     for (int tt = 0; tt < 2 * NQ; ++tt) {
       const int ci = tt / NQ, q = tt % NQ;
       const int j = ((a0 - K * ci) * (K + 1) + (b0 - K * ci)) * NQ + q;
-      G[tt] = pfy[j];
-      M[tt] = pfy[T::PFY + j];
+      G[tt] = (CT)pfy[j];
+      M[tt] = (CT)pfy[T::PFY + j];
     }
-    const double *xg = XG + p1 * T::XS + lc0 * NQ;
-    const double *xm = XM + p1 * T::XS + lc0 * NQ;
+    const CT *xg = XG + p1 * T::XS + lc0 * NQ;
+    const CT *xm = XM + p1 * T::XS + lc0 * NQ;
     if (nc == 2) {
 #pragma unroll
       for (int q3 = 0; q3 < T::Q3; ++q3) {
-        double y1 = 0.0, y2 = 0.0;
+        CT y1 = 0, y2 = 0;
 #pragma unroll
         for (int tt = 0; tt < 2 * NQ; ++tt) {
-          const double g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
+          const CT g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
           y1 = fma(g, M[tt], fma(m, G[tt], y1));
           y2 = fma(m, M[tt], y2);
         }
@@ -517,10 +524,10 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     } else {
 #pragma unroll
       for (int q3 = 0; q3 < T::Q3; ++q3) {
-        double y1 = 0.0, y2 = 0.0;
+        CT y1 = 0, y2 = 0;
 #pragma unroll
         for (int tt = 0; tt < NQ; ++tt) {
-          const double g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
+          const CT g = xg[q3 * T::Q2 + tt], m = xm[q3 * T::Q2 + tt];
           y1 = fma(g, M[tt], fma(m, G[tt], y1));
           y2 = fma(m, M[tt], y2);
         }
@@ -587,7 +594,7 @@ __global__ __launch_bounds__((Tile3<K, NQ, R1, R2, R3>::NT), 4) void k_assemble3
     const uint8_t *fwd = PATG ? pat_fwd + (size_t)tile_pats[t.pat_off + lpid] * T::SP
                               : reinterpret_cast<const uint8_t *>(lpat) + lpid * T::SP;
     const double lift =
-        z_row_direct_dispatch<K, NQ, T::W, T::Q3, R3>(u3, e, fwd, slot12, Y1, Y2, values, g0, g1, t.org[2] + u3, kp);
+        z_row_direct_dispatch<K, NQ, T::W, T::Q3, R3, CT>(u3, e, fwd, slot12, Y1, Y2, values, g0, g1, t.org[2] + u3, kp);
     if (e.lenf & 0x10000) liftp[u3 * T::NT + tid] = lift;
   }
   __syncthreads();
@@ -948,18 +955,25 @@ int launch_matrix_tile3d_wslab(gdx_problem *p, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 
-template <int K, int NQ, int R1, int R2, int R3>
-static void run3_direct(gdx_problem *p, hipStream_t st) {
+template <int K, int NQ, int R1, int R2, int R3, class CT>
+static void run3_direct_t(gdx_problem *p, hipStream_t st) {
   using T = Tile3D<K, NQ, R1, R2, R3>;
   const int nt = (int)p->tiles.size(), nf = p->n_fast_tiles;
   if (nf > 0)
-    hipLaunchKernelGGL((k_assemble3d_direct<K, NQ, R1, R2, R3, false>), dim3(8u * ((nf + 7) / 8)), dim3(T::NT), 0,
+    hipLaunchKernelGGL((k_assemble3d_direct<K, NQ, R1, R2, R3, false, CT>), dim3(8u * ((nf + 7) / 8)), dim3(T::NT), 0,
                        st, p->dev.tiles, 0, nf, p->dev.rowinfo, p->dev.W, p->dev.F, p->dev.pat_fwd,
                        p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
   if (nt > nf)
-    hipLaunchKernelGGL((k_assemble3d_direct<K, NQ, R1, R2, R3, true>), dim3(8u * ((nt - nf + 7) / 8)), dim3(T::NT), 0,
-                       st, p->dev.tiles, nf, nt - nf, p->dev.rowinfo, p->dev.W, p->dev.F, p->dev.pat_fwd,
-                       p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
+    hipLaunchKernelGGL((k_assemble3d_direct<K, NQ, R1, R2, R3, true, CT>), dim3(8u * ((nt - nf + 7) / 8)),
+                       dim3(T::NT), 0, st, p->dev.tiles, nf, nt - nf, p->dev.rowinfo, p->dev.W, p->dev.F,
+                       p->dev.pat_fwd, p->dev.tile_pats, p->dev.values, p->dev.b, p->kp);
+}
+
+// FP64 or the mixed form (FP32 contractions), by the problem's precision
+template <int K, int NQ, int R1, int R2, int R3>
+static void run3_direct(gdx_problem *p, hipStream_t st) {
+  if (p->precision == GDX_PREC_FP32) run3_direct_t<K, NQ, R1, R2, R3, float>(p, st);
+  else run3_direct_t<K, NQ, R1, R2, R3, double>(p, st);
 }
 
 int pattern_capacity_tile3d_direct(int K, int NQ) {

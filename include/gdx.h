@@ -46,7 +46,7 @@ extern "C" {
 
 /* numeric precision of the cell contractions; the CSR values are FP64 */
 #define GDX_PREC_FP64 0
-#define GDX_PREC_FP32 1 /* mixed form: sum-factorised contractions in FP32 (3D, z-marching kernel) */
+#define GDX_PREC_FP32 1 /* mixed form: sum-factorised contractions in FP32 (3D, z-marching or direct-store kernel) */
 
 /* operator kinds */
 #define GDX_OP_DIFFUSION 0 /* affine: a(u,v) and l(v) below */

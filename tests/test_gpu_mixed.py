@@ -1,5 +1,6 @@
 """The mixed form (GDX_PREC_FP32: sum-factorised contractions in FP32, CSR
-values in FP64) on device 0: same structure as the FP64 form and the oracle,
+values in FP64) on device 0, with the z-marching kernel and the direct-store
+kernel (the one the FP64 bench times): same structure as the oracle,
 max relative matrix error <= 1e-6 (max|dA| / max|A|) and Frobenius relative
 error <= 1e-6, RHS likewise; the FP64 form afterwards is unchanged."""
 import numpy as np
@@ -16,11 +17,14 @@ CASES = [W.poisson3d_q2_fourier(8, (2, 2, 2)), W.poisson3d_q2_fourier(9, (1, 2, 
          W.poisson3d_q1(12, (2, 2, 1)), dict(W.poisson3d_q2_fourier(6, (2, 1, 1)), qdeg=2)]
 
 
+@pytest.mark.parametrize("variant", [0, 2], ids=["z-marching", "direct-store"])
 @pytest.mark.parametrize("p", CASES, ids=["q2", "q2-full", "q1", "q2-qd2"])
-def test_fp32_form_error_bound(gpu_ctx, p):
+def test_fp32_form_error_bound(gpu_ctx, p, variant):
+    print("fp32 form, kernel", variant, p["order"], p["cells"], p["parts"], flush=True)
     r = oracle.run(p)
     assem, data = problem_from_workload(gpu_ctx, p)
     prob = assem._problem_for(data)
+    prob.set_kernel(variant, 16)
     prob.numeric(precision=1)
     prob.sync()
     for k, P in enumerate(r.parts):

@@ -128,8 +128,8 @@ def test_golden_fixture(gpu_ctx):
     prob.close()
 
 
-@pytest.mark.parametrize("kernel", [(2, 8, 0, TOL), (0, 32, 0, TOL), (0, 16, 1, 1e-6)],
-                         ids=["direct-store-fp64", "z-marching-fp64", "z-marching-fp32"])
+@pytest.mark.parametrize("kernel", [(2, 8, 0, TOL), (0, 32, 0, TOL), (0, 16, 1, 1e-6), (2, 8, 1, 1e-6)],
+                         ids=["direct-store-fp64", "z-marching-fp64", "z-marching-fp32", "direct-store-fp32"])
 def test_full_size_q2_64(gpu_ctx, kernel):
     """3D Q2 64^3, parts (2,2,2), kappa = exp(16 Fourier modes), run by the
     kernel the bench times (direct store), the z-marching kernel and the FP32
