@@ -370,6 +370,13 @@ class GpuProblem:
         _lib.call("gdx_part_sizes", self.handle, int(part), s)
         return list(s)
 
+    def export_fe(self, part):
+        """The part's FE dof PRange only: (ldof_gid, ldof_owner, ldof_gnode)."""
+        nl = self.sizes(part)[_lib.SZ_NLDOFS]
+        gid, owner, gnode = np.empty(nl, np.int64), np.empty(nl, np.int32), np.empty(nl, np.int64)
+        _lib.call("gdx_export_fe", self.handle, int(part), gid.ctypes.data, owner.ctypes.data, gnode.ctypes.data)
+        return gid, owner, gnode
+
     def export_part(self, part):
         nl, no, ngr, ngc, nnz = self.sizes(part)
         P = dict(n_ldofs=nl, n_own=no, n_ghost_rows=ngr, n_ghost_cols=ngc, nnz=nnz)
