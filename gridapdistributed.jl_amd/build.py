@@ -13,7 +13,7 @@ SRC = ["csrc/gdx_capi.hip", "csrc/gdx_kernels.hip", "csrc/gdx_march3d.hip", "csr
        "csrc/gdx_solve.hip",
        "csrc/gdx_flux.hip", "csrc/gdx_coef.hip",
        "csrc/gdx_nn.hip", "csrc/gdx_twogrid.hip", "csrc/gdx_batch.hip", "csrc/gdx_elast.hip", "csrc/gdx_pvector.hip",
-       "csrc/gdx_symbolic.cpp"]
+       "csrc/gdx_symbolic.cpp", "csrc/gdx_symbolic_dev.hip"]
 OUT = os.path.join(HERE, "libgdx.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -59,7 +59,7 @@ int upload(T **dst, const std::vector<T> &src) {
 void free_device(gdx_problem *p) {
   gdx::Device &d = p->dev;
   void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_invr, d.pat_len,
-                  d.pat_dir, d.tiles, d.tile_pats, d.cellk};
+                  d.pat_dir, d.tiles, d.tile_pats, d.cellk, d.csr_rowptr, d.csr_colidx};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   d = gdx::Device{};
@@ -257,7 +257,18 @@ int gdx_problem_destroy(gdx_problem *p) {
 
 int gdx_symbolic(gdx_problem *p) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
-  if (p->ctx) GDX_HIP(hipSetDevice(p->ctx->device));
+  if (p->ctx) {
+    GDX_HIP(hipSetDevice(p->ctx->device));
+    // device buffers are re-made (the device stage of the symbolic phase
+    // writes some of them): graphs that captured the old ones are dropped
+    gdx::batch_invalidate_problem(p->ctx, p);
+    p->symbolic_done = false;
+    GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+    free_device(p);
+    gdx::solve_free(p);  // its row tables are rebuilt from the new structure
+    gdx::elast_free(p);
+    gdx::flux_free(p);
+  }
   try {
     int rc = gdx::symbolic(p);
     if (rc) return rc;
@@ -274,12 +285,6 @@ int gdx_symbolic(gdx_problem *p) {
     p->symbolic_done = true;
     return GDX_OK;
   }
-  // device buffers are re-made: graphs that captured the old ones are dropped
-  gdx::batch_invalidate_problem(p->ctx, p);
-  p->symbolic_done = false;
-  free_device(p);
-  gdx::elast_free(p);
-  gdx::flux_free(p);
   gdx::Device &d = p->dev;
   const bool elast = p->desc.op_kind == GDX_OP_ELASTICITY;
   // kappa and f at the quadrature points (constant-coefficient elasticity has none)
@@ -298,14 +303,16 @@ int gdx_symbolic(gdx_problem *p) {
   GDX_HIP(hipMemset(d.values, 0, std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
   GDX_HIP(hipMemset(d.b, 0, std::max<int64_t>(p->rows_total, 1) * sizeof(double)));
   int rc;
-  if ((rc = upload(&d.rowinfo, p->rowinfo))) return rc;
+  // the row table and tile pattern lists are in HBM already when the device
+  // stage built them
+  if (!p->dev_structure && (rc = upload(&d.rowinfo, p->rowinfo))) return rc;
   if ((rc = upload(&d.pat_fwd, p->pat_fwd))) return rc;
   if ((rc = upload(&d.pat_inv, p->pat_inv))) return rc;
   if ((rc = upload(&d.pat_invr, p->pat_invr))) return rc;
   if ((rc = upload(&d.pat_len, p->pat_len))) return rc;
   if ((rc = upload(&d.pat_dir, p->pat_dir))) return rc;
   if ((rc = upload(&d.tiles, p->tiles))) return rc;
-  if ((rc = upload(&d.tile_pats, p->tile_pats))) return rc;
+  if (!p->dev_structure && (rc = upload(&d.tile_pats, p->tile_pats))) return rc;
   if (elast && (rc = gdx::elast_setup(p))) {
     set_error("uploading the elasticity tables failed");
     return rc;
@@ -328,9 +335,15 @@ extern "C++" namespace gdx {
 int retile(gdx_problem *p, const int R[3]) {
   // the tile buffers are re-allocated: graphs holding the old ones are dropped
   if (p->ctx) batch_invalidate_problem(p->ctx, p);
+  int rc;
+  if (p->dev_structure) {
+    // the row table is in HBM: the tiles are built there as well
+    if ((rc = build_tiles_device(p, R))) return rc;
+    order_tiles(p);
+    return upload(&p->dev.tiles, p->tiles);
+  }
   build_tiles(p, R);
   order_tiles(p);
-  int rc;
   if ((rc = upload(&p->dev.rowinfo, p->rowinfo))) return rc;
   if ((rc = upload(&p->dev.tiles, p->tiles))) return rc;
   if ((rc = upload(&p->dev.tile_pats, p->tile_pats))) return rc;
@@ -588,7 +601,7 @@ int gdx_part_sizes(gdx_problem *p, int32_t part, int64_t *s) {
   s[GDX_SZ_NOWN] = P.n_own;
   s[GDX_SZ_NGHOST_ROWS] = P.n_ghost_rows;
   s[GDX_SZ_NGHOST_COLS] = P.n_ghost_cols;
-  s[GDX_SZ_NNZ] = P.rowptr[P.nrows];
+  s[GDX_SZ_NNZ] = P.nnz;
   return GDX_OK;
 }
 
@@ -633,17 +646,21 @@ int gdx_export_csr(gdx_problem *p, int32_t part, int64_t *rowptr, int32_t *colid
   int rc = part_check(p, part);
   if (rc) return rc;
   GDX_REQUIRE(rowptr && colidx && values, GDX_ERR_ARG, "NULL argument");
+  if (p->ctx && (rc = gdx::host_csr(p))) {
+    set_error("copying the local CSR to the host failed");
+    return rc;
+  }
   const gdx::Part &P = p->P[part];
   std::memcpy(rowptr, P.rowptr.data(), (P.nrows + 1) * sizeof(int64_t));
-  std::memcpy(colidx, P.colidx.data(), P.rowptr[P.nrows] * sizeof(int32_t));
+  std::memcpy(colidx, P.colidx.data(), P.nnz * sizeof(int32_t));
   if (!p->ctx) {
-    std::memset(values, 0, P.rowptr[P.nrows] * sizeof(double));
+    std::memset(values, 0, P.nnz * sizeof(double));
     return GDX_OK;
   }
   GDX_HIP(hipSetDevice(p->ctx->device));
   GDX_HIP(hipStreamSynchronize(p->ctx->stream));
-  if (P.rowptr[P.nrows] > 0)
-    GDX_HIP(hipMemcpy(values, p->dev.values + P.val_base, P.rowptr[P.nrows] * sizeof(double), hipMemcpyDeviceToHost));
+  if (P.nnz > 0)
+    GDX_HIP(hipMemcpy(values, p->dev.values + P.val_base, P.nnz * sizeof(double), hipMemcpyDeviceToHost));
   return GDX_OK;
 }
 

@@ -166,6 +166,7 @@ void elast_free(gdx_problem *p) {
 // (p->kp: points, weights, basis values and derivatives on [0,1], spacing h)
 int elast_setup(gdx_problem *p) {
   elast_free(p);
+  if (int rc = host_rowinfo(p)) return rc;
   const KParams &kp = p->kp;
   const int K = p->K, W = 2 * K + 1, NQ = p->NQ;
   int nnmax = 0;
@@ -197,9 +198,11 @@ int elast_setup(gdx_problem *p) {
       }
   }
   // owned row nodes in the order of their rows in the values buffer, so that
-  // consecutive waves write consecutive memory
-  std::vector<ElastRow> rows;
-  rows.reserve(p->rowinfo.size());
+  // consecutive waves write consecutive memory: a node's place is its owned
+  // row group (parts in order, then owned rows, whose offsets increase)
+  std::vector<int64_t> grp0(p->nparts + 1, 0);
+  for (int r = 0; r < p->nparts; ++r) grp0[r + 1] = grp0[r] + p->P[r].n_own / 3;
+  std::vector<ElastRow> rows(grp0[p->nparts]);
   for (int r = 0; r < p->nparts; ++r) {
     const Part &P = p->P[r];
     const int64_t n = P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
@@ -214,11 +217,11 @@ int elast_setup(gdx_problem *p) {
       er.g[0] = (unsigned short)(P.onb_lo[0] + li % P.onb_n[0]);
       er.g[1] = (unsigned short)(P.onb_lo[1] + (li / P.onb_n[0]) % P.onb_n[1]);
       er.g[2] = (unsigned short)(P.onb_lo[2] + li / (P.onb_n[0] * P.onb_n[1]));
-      rows.push_back(er);
+      const int64_t o = ri.bidx - P.row_base;  // the node's first owned row
+      if (o < 0 || o >= P.n_own || o % 3) return GDX_ERR_STATE;
+      rows[grp0[r] + o / 3] = er;
     }
   }
-  std::sort(rows.begin(), rows.end(),
-            [](const ElastRow &x, const ElastRow &y) { return x.rowstart < y.rowstart; });
   // position -> packed (stencil offsets, column component): slot * 3 + component
   // with slot = o0 + W (o1 + W o2); padding entries stay as they are
   std::vector<uint16_t> code(p->pat16_inv.size());

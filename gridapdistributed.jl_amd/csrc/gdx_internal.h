@@ -33,9 +33,11 @@ struct Part {
   std::vector<int32_t> row_owner, col_owner;
   int64_t nrows = 0, ncols = 0, n_ghost_rows = 0, n_ghost_cols = 0;
   std::vector<int64_t> row_ldof;  // local row -> ldof
-  // local CSR pattern
+  // local CSR pattern (host copies: made on first host use when the device
+  // symbolic phase built them)
   std::vector<int64_t> rowptr;
   std::vector<int32_t> colidx;
+  int64_t nnz = 0;  // stored entries of the local CSR
   int64_t val_base = 0, row_base = 0;
   // owned node box (global node coords)
   int64_t onb_lo[3], onb_n[3];
@@ -106,6 +108,10 @@ struct Device {
   Tile *tiles = nullptr;
   int *tile_pats = nullptr;    // per tile: its distinct pattern ids
   double *cellk = nullptr;     // kappa per cell (cell-wise coefficient mode)
+  // local CSR of every part, concatenated (device symbolic phase): global row
+  // offsets (rows_total + 1 entries) and local column ids
+  long long *csr_rowptr = nullptr;
+  int *csr_colidx = nullptr;
 };
 
 }  // namespace gdx
@@ -152,6 +158,10 @@ struct gdx_problem {
   int precision = 0;     // contraction precision of the current numeric phase (GDX_PREC_*)
   int tileR[3] = {1, 1, 1};
   int64_t nnz_total = 0, rows_total = 0, owned_nnz = 0;
+  // the device symbolic phase built the CSR, row table and tiles in HBM; their
+  // host copies (P.rowptr/colidx, rowinfo) are made on first host use
+  bool dev_structure = false;
+  bool host_csr_stale = false, host_rowinfo_stale = false;
   gdx::KParams kp{};
   gdx::Device dev;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -182,7 +192,24 @@ void unregister(std::vector<T *> &v, T *x) {
       return;
     }
 }
-int symbolic(gdx_problem *p);  // host symbolic phase (gdx_symbolic.cpp)
+int symbolic(gdx_problem *p);  // symbolic phase (gdx_symbolic.cpp)
+// device stage of the symbolic phase (gdx_symbolic_dev.hip): CSR, position
+// patterns and row table from the host numbering and PRanges.  Returns
+// GDX_OK, a GDX_ERR_* code, or kUseHost when the host stage must build them.
+constexpr int kUseHost = 1;
+int structure_device(gdx_problem *p);
+int build_tiles_device(gdx_problem *p, const int R[3]);
+int host_csr(gdx_problem *p);      // host copies of the local CSRs (no-op when current)
+int host_rowinfo(gdx_problem *p);  // host copy of the row table (no-op when current)
+// tables of one position pattern from its key (slot -> offset, kSlotAbsent or
+// kSlotDirichlet) and the node coordinates of a row that has it; returns
+// whether the pattern has Dirichlet slots
+bool scalar_pattern_tables(int D, int K, const uint8_t *key, int S, int SP, const int64_t g[3], int len,
+                           uint8_t *fwd, uint8_t *inv, uint8_t *invr);
+// vector-valued rows: key over slot * ncomp + component (16 bit: offset, 0xFFFE
+// Dirichlet, 0xFFFF absent); inv = position -> slot * ncomp + component;
+// returns whether the pattern has Dirichlet slots
+bool vector_pattern_inv(const uint16_t *key, int SN, int SP, uint16_t *inv);
 // device address of the no-alloc staging array (gdx_twogrid.hip): resolved once
 // per device with that device current (gdx_init), then looked up
 int noalloc_setup(int device);

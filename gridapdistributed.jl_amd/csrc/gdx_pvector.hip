@@ -228,6 +228,7 @@ int build_mul(gdx_problem *p) {
   if (D->mul_built) return GDX_OK;
   int rc = build_range(p, GDX_PRANGE_COLS);
   if (rc) return rc;
+  if ((rc = host_csr(p))) return rc;
   std::vector<long long> rstart, rcb;
   std::vector<int> rlen, colidx(std::max<int64_t>(p->nnz_total, 0));
   rstart.reserve(p->rows_total);

@@ -292,8 +292,7 @@ void vector_csr(gdx_problem *p, const Ctx &c) {
           ids.emplace(skey, pid);
           L.keys.push_back(std::move(skey));
           std::vector<uint16_t> inv(SP, 0);
-          for (int s3 = 0; s3 < SN; ++s3)
-            if (key[s3] < kDir16) inv[key[s3]] = (uint16_t)s3;
+          (void)vector_pattern_inv(key.data(), SN, SP, inv.data());
           L.inv.insert(L.inv.end(), inv.begin(), inv.end());
           L.len.push_back(e);
           L.dir.push_back(has_dir ? 1 : 0);
@@ -334,6 +333,7 @@ void vector_csr(gdx_problem *p, const Ctx &c) {
     P.val_base = val_base;
     P.row_base = row_base;
     owned_nnz += P.rowptr[P.n_own];
+    P.nnz = P.rowptr[P.nrows];
     val_base += P.rowptr[P.nrows];
     row_base += P.nrows;
     P.rowinfo_base = rinfo_n;
@@ -371,6 +371,47 @@ void vector_csr(gdx_problem *p, const Ctx &c) {
 
 }  // namespace
 
+bool scalar_pattern_tables(int D, int K, const uint8_t *key, int S, int SP, const int64_t g[3], int len,
+                           uint8_t *fwd, uint8_t *inv, uint8_t *invr) {
+  bool has_dir = false;
+  for (int s = 0; s < SP; ++s) fwd[s] = kSlotAbsent, inv[s] = 0, invr[s] = 0;
+  for (int s = 0; s < S; ++s) {
+    const uint8_t v = key[s];
+    if (v == kSlotDirichlet) has_dir = true;
+    else if (v != kSlotAbsent) inv[v] = (uint8_t)s;
+    fwd[s] = v;
+  }
+  // reduced numbering (the pattern fixes the node parity of its rows); the
+  // Dirichlet slots follow the row's entries, their count in the last byte
+  const int Wd = 2 * K + 1;
+  auto reduced = [&](int s) {
+    const int sx = s % Wd - K, sy = (s / Wd) % Wd - K, sz = D == 3 ? s / (Wd * Wd) - K : 0;
+    int r = sx - stencil_lo(K, (int)g[0]);
+    r += stencil_width(K, (int)g[0]) * (sy - stencil_lo(K, (int)g[1]));
+    if (D == 3) r += stencil_width(K, (int)g[0]) * stencil_width(K, (int)g[1]) * (sz - stencil_lo(K, (int)g[2]));
+    return r;
+  };
+  int ndir = 0;
+  for (int s = 0; s < S; ++s) {
+    const uint8_t v = fwd[s];
+    if (v != kSlotDirichlet && v != kSlotAbsent) invr[v] = (uint8_t)reduced(s);
+  }
+  for (int s = 0; s < S; ++s)
+    if (fwd[s] == kSlotDirichlet) invr[len + ndir++] = (uint8_t)reduced(s);
+  invr[SP - 1] = (uint8_t)ndir;
+  return has_dir;
+}
+
+bool vector_pattern_inv(const uint16_t *key, int SN, int SP, uint16_t *inv) {
+  bool has_dir = false;
+  for (int s = 0; s < SP; ++s) inv[s] = 0;
+  for (int s3 = 0; s3 < SN; ++s3) {
+    if (key[s3] < 0xFFFE) inv[key[s3]] = (uint16_t)s3;
+    else if (key[s3] == 0xFFFE) has_dir = true;
+  }
+  return has_dir;
+}
+
 void tile_shape(int dim, int K, int R[3]) {
   // must match the template instances launched by launch_matrix
   if (dim == 3) {
@@ -384,6 +425,8 @@ void tile_shape(int dim, int K, int R[3]) {
 }
 
 int symbolic(gdx_problem *p) {
+  p->dev_structure = false;
+  p->host_csr_stale = p->host_rowinfo_stale = false;
   Ctx c;
   c.p = p;
   c.D = p->D;
@@ -607,6 +650,20 @@ int symbolic(gdx_problem *p) {
     P.n_ghost_cols = P.ncols - P.n_own;
   });
 
+  // CSR, position patterns, row table and tiles: on the device unless the
+  // problem has none (host-only problems) or GDX_HOST_SYMBOLIC=1 asks for the
+  // host stage; the device stage hands back to the host one when its pattern
+  // table overflows or two keys collide in a hash
+  if (p->ctx && !(std::getenv("GDX_HOST_SYMBOLIC") && std::atoi(std::getenv("GDX_HOST_SYMBOLIC")) > 0)) {
+    const int rc = structure_device(p);
+    if (rc != kUseHost) {
+      if (rc) return rc;
+      int R[3];
+      tile_shape(c.D, K, R);
+      if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
+      return build_tiles_device(p, R);
+    }
+  }
   if (nc > 1) {
     vector_csr(p, c);
     int R[3];
@@ -677,36 +734,11 @@ int symbolic(gdx_problem *p) {
           pid = (int)L.len.size();
           ids.emplace(key, pid);
           L.keys.push_back(key);
-          std::vector<uint8_t> inv(SP, 0), fwd(SP, kSlotAbsent);
-          bool has_dir = false;
-          for (int s = 0; s < S; ++s) {
-            uint8_t v = (uint8_t)key[s];
-            if (v == kSlotDirichlet) has_dir = true;
-            else if (v != kSlotAbsent) inv[v] = (uint8_t)s;
-            fwd[s] = v;
-          }
+          std::vector<uint8_t> inv(SP, 0), fwd(SP, kSlotAbsent), invr(SP, 0);
+          const bool has_dir = scalar_pattern_tables(c.D, K, reinterpret_cast<const uint8_t *>(key.data()), S, SP, g,
+                                                     (int)ents.size(), fwd.data(), inv.data(), invr.data());
           L.fwd.insert(L.fwd.end(), fwd.begin(), fwd.end());
           L.inv.insert(L.inv.end(), inv.begin(), inv.end());
-          // reduced numbering (the pattern fixes the node parity of its rows)
-          std::vector<uint8_t> invr(SP, 0);
-          int ndir = 0;
-          const int Wd = 2 * K + 1;
-          auto reduced = [&](int s) {
-            const int sx = s % Wd - K, sy = (s / Wd) % Wd - K, sz = c.D == 3 ? s / (Wd * Wd) - K : 0;
-            int r = sx - stencil_lo(K, (int)g[0]);
-            r += stencil_width(K, (int)g[0]) * (sy - stencil_lo(K, (int)g[1]));
-            if (c.D == 3)
-              r += stencil_width(K, (int)g[0]) * stencil_width(K, (int)g[1]) * (sz - stencil_lo(K, (int)g[2]));
-            return r;
-          };
-          for (int s = 0; s < S; ++s) {
-            const uint8_t v = fwd[s];
-            if (v == kSlotDirichlet) continue;
-            if (v != kSlotAbsent) invr[v] = (uint8_t)reduced(s);
-          }
-          for (int s = 0; s < S; ++s)
-            if (fwd[s] == kSlotDirichlet) invr[ents.size() + ndir++] = (uint8_t)reduced(s);
-          invr[SP - 1] = (uint8_t)ndir;
           L.invr.insert(L.invr.end(), invr.begin(), invr.end());
           L.len.push_back((int)ents.size());
           L.dir.push_back(has_dir ? 1 : 0);
@@ -747,6 +779,7 @@ int symbolic(gdx_problem *p) {
     P.val_base = val_base;
     P.row_base = row_base;
     owned_nnz += P.rowptr[P.n_own];
+    P.nnz = P.rowptr[P.nrows];
     val_base += P.rowptr[P.nrows];
     row_base += P.nrows;
     P.rowinfo_base = rinfo_n;

@@ -1,0 +1,785 @@
+// Device stage of libgdx's symbolic phase.  From the host's conforming dof
+// numbering and matrix PRanges (gdx_symbolic.cpp) it builds in HBM:
+//   - every part's local CSR: row lengths counted over each row's stencil,
+//     offsets by a scan, and the columns of a row placed by rank counting
+//     (a column's position is the number of smaller columns in its row, so the
+//     row comes out sorted, as the reference's CSR from sorted COO triplets,
+//     src/Algebra.jl:1136-1257);
+//   - the position pattern of every owned row (stencil slot -> position),
+//     deduplicated through a hash table whose slots keep the first owned row
+//     (over parts, then rows) that has the key; pattern ids follow those first
+//     occurrences, which is the host stage's sequential first-encounter order,
+//     and every row's key is compared with its representative's, so that a
+//     hash collision is found (the host stage then builds the structure);
+//   - the row table over each part's owned node box, and the tiles.
+// Everything equals the host stage's output bit for bit.  Host copies of the
+// CSRs and the row table are made only when host code asks for them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "gdx_internal.h"
+
+namespace gdx {
+namespace {
+
+constexpr uint16_t kAbs16 = 0xFFFF, kDir16 = 0xFFFE;
+constexpr int kTableBits = 18;
+constexpr unsigned kTableSize = 1u << kTableBits;
+constexpr int kBlock = 256;
+
+struct MeshG {
+  long long n[3], nn[3];
+  int D, K, nc, dir_kind;
+};
+
+// one part, by value; offsets index the concatenation over parts in part order
+struct PartG {
+  long long loc_lo[3], lnn[3], own_lo[3], own_hi[3], onb_lo[3], onb_n[3];
+  long long n_own, nrows;
+  long long row0;    // first row of the part among all parts' rows
+  long long ogrp0;   // first owned row group (nc consecutive rows of one node)
+  long long rinfo0;  // first row-table entry
+  long long lnode0, ldof0, own0;
+};
+
+__device__ inline bool dirichlet(const MeshG &m, long long g0, long long g1, long long g2) {
+  if (m.dir_kind == 0) return false;
+  if (m.dir_kind == GDX_DIR_CLAMPED_X0) return g0 == 0;
+  if (g0 == 0 || g0 == m.nn[0] - 1 || g1 == 0 || g1 == m.nn[1] - 1) return true;
+  return m.D == 3 && (g2 == 0 || g2 == m.nn[2] - 1);
+}
+
+// global nodes sharing a cell of [clo, chi) with node i along one direction
+__device__ inline void shared_range(int K, long long i, long long clo, long long chi, long long &jlo, long long &jhi) {
+  const long long cmin = max((i + K - 1) / K - 1, clo), cmax = min(i / K, chi - 1);
+  jlo = K * cmin;
+  jhi = K * cmax + K;
+  if (cmin > cmax) jlo = 1, jhi = 0;
+}
+
+__device__ inline void node_of(const MeshG &m, const PartG &P, long long l, long long g[3]) {
+  g[0] = m.K * P.loc_lo[0] + l % P.lnn[0];
+  g[1] = m.K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
+  g[2] = m.K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1]);
+}
+
+__device__ inline long long lnode_in(const MeshG &m, const PartG &P, long long g0, long long g1, long long g2) {
+  const long long l0 = g0 - m.K * P.loc_lo[0], l1 = g1 - m.K * P.loc_lo[1], l2 = g2 - m.K * P.loc_lo[2];
+  if (l0 < 0 || l0 >= P.lnn[0] || l1 < 0 || l1 >= P.lnn[1] || l2 < 0 || l2 >= P.lnn[2]) return -1;
+  return l0 + P.lnn[0] * (l1 + P.lnn[1] * l2);
+}
+
+// the row's stencil range per direction: owned rows couple over the whole
+// mesh, ghost rows only through the part's owned cells
+__device__ inline void row_ranges(const MeshG &m, const PartG &P, const long long g[3], bool own, long long jlo[3],
+                                  long long jhi[3]) {
+  for (int d = 0; d < 3; ++d) jlo[d] = jhi[d] = 0;
+  for (int d = 0; d < m.D; ++d) {
+    if (own) shared_range(m.K, g[d], 0, m.n[d], jlo[d], jhi[d]);
+    else shared_range(m.K, g[d], P.own_lo[d], P.own_hi[d], jlo[d], jhi[d]);
+  }
+}
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014): spreads an entry's
+// (index, key) pair over 64 bits before the order-free sum of a row's hash
+__device__ inline unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_fill_i64(int64_t *x, long long n, int64_t v) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *x, long long n, unsigned long long v) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// owned row of every owned dof (-1 elsewhere) and the node of every row group
+__global__ void k_ldof_orow(PartG P, const int64_t *own_ldofs, int64_t *ldof_orow) {
+  const long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (o < P.n_own) ldof_orow[own_ldofs[o]] = o;
+}
+
+__global__ void k_row_lnode(PartG P, int nc, const int64_t *row_ldof, const int64_t *ldof_lnode, int64_t *row_lnode,
+                            int *err) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= P.nrows / nc) return;
+  const long long ld = row_ldof[q * nc];
+  if (ld % nc != 0) atomicOr(err, 1);  // the rows of a node must be grouped
+  row_lnode[q] = ldof_lnode[ld];
+}
+
+// row lengths: nc columns per free stencil node of the row's range
+__global__ void k_rowlen(MeshG m, PartG P, const int64_t *row_lnode, const int64_t *lnode_dof, const int64_t *ldof_col,
+                         long long *rowlen, int *err) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= P.nrows / m.nc) return;
+  long long g[3], jlo[3], jhi[3];
+  node_of(m, P, row_lnode[q], g);
+  row_ranges(m, P, g, q * m.nc < P.n_own, jlo, jhi);
+  long long cnt = 0;
+  for (long long jz = jlo[2]; jz <= jhi[2]; ++jz)
+    for (long long jy = jlo[1]; jy <= jhi[1]; ++jy)
+      for (long long jx = jlo[0]; jx <= jhi[0]; ++jx) {
+        if (dirichlet(m, jx, jy, jz)) continue;
+        const long long lj = lnode_in(m, P, jx, jy, jz);
+        const long long j0 = lj >= 0 ? lnode_dof[lj] : -1;
+        if (j0 < 0 || ldof_col[j0] < 0) {
+          atomicOr(err, 1);  // a stencil column missing from the col PRange
+          continue;
+        }
+        ++cnt;
+      }
+  for (int k = 0; k < m.nc; ++k) rowlen[P.row0 + q * m.nc + k] = cnt * m.nc;
+}
+
+// Exclusive scan of n int64 values into out[0..n] (out[n] = total): blocks of
+// kBlock threads x kItems values scan in LDS; the block totals are scanned the
+// same way and added back.
+constexpr int kItems = 8;
+constexpr long long kScanTile = (long long)kBlock * kItems;
+
+__global__ __launch_bounds__(kBlock) void k_scan_tile(const long long *in, long long *out, long long n, long long *sums) {
+  __shared__ long long part[kBlock];
+  const long long base = blockIdx.x * kScanTile + (long long)threadIdx.x * kItems;
+  long long v[kItems], acc = 0;
+  for (int i = 0; i < kItems; ++i) {
+    v[i] = base + i < n ? in[base + i] : 0;
+    acc += v[i];
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele over the thread totals
+    const long long t = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += t;
+    __syncthreads();
+  }
+  long long run = part[threadIdx.x] - acc;
+  for (int i = 0; i < kItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = part[kBlock - 1];
+}
+
+__global__ void k_scan_add(long long *out, long long n, const long long *offs) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] += offs[i / kScanTile];
+}
+
+__global__ void k_copy1(long long *dst, const long long *src) { *dst = *src; }
+
+int scan_exclusive(const long long *in, long long *out, long long n, hipStream_t st) {
+  const long long nb = std::max<long long>((n + kScanTile - 1) / kScanTile, 1);
+  long long *sums = nullptr, *offs = nullptr;
+  if (hipMalloc(&sums, nb * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+  if (hipMalloc(&offs, (nb + 1) * sizeof(long long)) != hipSuccess) {
+    (void)hipFree(sums);
+    return GDX_ERR_ALLOC;
+  }
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kBlock), 0, st, in, out, n, sums);
+  int rc = GDX_OK;
+  if (nb > 1) {
+    rc = scan_exclusive(sums, offs, nb, st);
+    if (!rc) hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, out, n, offs);
+  }
+  // total: the single block's sum, or the scan of the block sums' total
+  if (!rc) hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, st, out + n, nb > 1 ? offs + nb : sums);
+  rc = rc ? rc : (hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(sums);
+  (void)hipFree(offs);
+  return rc;
+}
+
+// One row group (the nc rows of a node) per LP lanes, one lane per (stencil
+// slot, component) entry: its column, its rank among the row's columns, the
+// column ids of the group's rows and, for owned rows, the position key and
+// its order-free hash.
+template <int LP, int BS>
+__global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_t *row_lnode,
+                                                  const int64_t *lnode_dof, const int64_t *ldof_col,
+                                                  const long long *rowptr, int *colidx, uint16_t *keys, int KS,
+                                                  unsigned long long *hash, int *err) {
+  constexpr int GPB = BS / LP;
+  __shared__ int colv[BS];
+  __shared__ unsigned long long hs[GPB];
+  const int grp = threadIdx.x / LP, e = threadIdx.x % LP;
+  const long long q = (long long)blockIdx.x * GPB + grp;
+  const bool live = q < P.nrows / m.nc;
+  const int W = 2 * m.K + 1, S = m.D == 3 ? W * W * W : W * W, SN = S * m.nc;
+  if (threadIdx.x < GPB) hs[threadIdx.x] = 0;
+  int col = INT_MAX;
+  uint16_t key = kAbs16;
+  bool own = false;
+  if (live) {
+    own = q * m.nc < P.n_own;
+    if (e < SN) {
+      long long g[3], jlo[3], jhi[3];
+      node_of(m, P, row_lnode[q], g);
+      row_ranges(m, P, g, own, jlo, jhi);
+      const int s = e / m.nc, k = e % m.nc;
+      const long long j0 = g[0] + s % W - m.K, j1 = g[1] + (s / W) % W - m.K,
+                      j2 = g[2] + (m.D == 3 ? s / (W * W) - m.K : 0);
+      if (j0 >= jlo[0] && j0 <= jhi[0] && j1 >= jlo[1] && j1 <= jhi[1] && j2 >= jlo[2] && j2 <= jhi[2]) {
+        if (dirichlet(m, j0, j1, j2)) {
+          key = kDir16;
+        } else {
+          // the same test as the row length's (the node's first column), so a
+          // row never holds more columns than were counted for it
+          const long long lj = lnode_in(m, P, j0, j1, j2);
+          const long long d0 = lj >= 0 ? lnode_dof[lj] : -1;
+          const long long c0 = d0 >= 0 ? ldof_col[d0] : -1;
+          const long long c = c0 >= 0 ? ldof_col[d0 + k] : -1;
+          if (c < 0 || c > INT_MAX - 1) atomicOr(err, 1);
+          else col = (int)c;
+        }
+      }
+    }
+  }
+  colv[threadIdx.x] = col;
+  __syncthreads();
+  if (col != INT_MAX) {
+    const int *cv = colv + grp * LP;
+    int rank = 0;
+    for (int i = 0; i < SN; ++i) rank += cv[i] < col;
+    key = (uint16_t)rank;
+    const long long r0 = P.row0 + q * m.nc;
+    for (int kk = 0; kk < m.nc; ++kk) colidx[rowptr[r0 + kk] + rank] = col;
+  }
+  if (live && own && e < KS) {
+    keys[(P.ogrp0 + q) * KS + e] = key;  // padding entries read as absent
+    if (e < SN) atomicAdd(&hs[grp], mix64(((unsigned long long)e << 16) | key));
+  }
+  __syncthreads();
+  if (live && own && e == 0) hash[P.ogrp0 + q] = hs[grp] | 1ull;  // 0 marks an empty table slot
+}
+
+// first owned row group of every distinct hash (open addressing, linear probing)
+__global__ void k_insert(const unsigned long long *hash, long long n, unsigned long long *tkey,
+                         unsigned long long *tmin, unsigned *slot, int *err) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const unsigned long long h = hash[q];
+  unsigned s = (unsigned)(h >> 1) & (kTableSize - 1);
+  for (unsigned t = 0; t < kTableSize; ++t) {
+    const unsigned long long old = atomicCAS(&tkey[s], 0ull, h);
+    if (old == 0ull || old == h) {
+      atomicMin(&tmin[s], (unsigned long long)q);
+      slot[q] = s;
+      return;
+    }
+    s = (s + 1) & (kTableSize - 1);
+  }
+  atomicOr(err, 2);  // table full
+}
+
+__global__ void k_collect(const unsigned long long *tkey, const unsigned long long *tmin, unsigned *cnt,
+                          unsigned long long *out_min, unsigned *out_slot) {
+  const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= kTableSize || tkey[s] == 0ull) return;
+  const unsigned i = atomicAdd(cnt, 1u);
+  out_min[i] = tmin[s];
+  out_slot[i] = s;
+}
+
+__global__ void k_gather_keys(const uint16_t *keys, int KS, const long long *rep, int npat, uint16_t *out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)npat * KS) return;
+  out[i] = keys[rep[i / KS] * KS + i % KS];
+}
+
+// pattern id of every owned row group; its key must equal its representative's
+__global__ void k_pattern_ids(const uint16_t *keys, int KS, long long n, const unsigned *slot,
+                              const unsigned long long *tmin, const int *tpid, int *pid, int *err) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const unsigned s = slot[q];
+  const long long rep = (long long)tmin[s];
+  const uint64_t *a = reinterpret_cast<const uint64_t *>(keys + q * KS);
+  const uint64_t *b = reinterpret_cast<const uint64_t *>(keys + rep * KS);
+  bool same = true;
+  for (int w = 0; w < KS / 4; ++w) same = same && a[w] == b[w];
+  if (!same) atomicOr(err, 4);  // hash collision
+  pid[q] = tpid[s];
+}
+
+__global__ void k_rowinfo(MeshG m, PartG P, const int64_t *lnode_dof, const int64_t *ldof_orow,
+                          const long long *rowptr, const int *pid_grp, const int *pat_dir, RowInfo *out, int *err) {
+  const long long li = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (li >= P.onb_n[0] * P.onb_n[1] * P.onb_n[2]) return;
+  const long long g0 = P.onb_lo[0] + li % P.onb_n[0], g1 = P.onb_lo[1] + (li / P.onb_n[0]) % P.onb_n[1],
+                  g2 = P.onb_lo[2] + li / (P.onb_n[0] * P.onb_n[1]);
+  const long long l = lnode_in(m, P, g0, g1, g2);
+  const long long ld = l >= 0 ? lnode_dof[l] : -1;
+  RowInfo ri{0, 0, -1, 0};
+  if (ld >= 0) {
+    const long long o = ldof_orow[ld];
+    if (o < 0) {
+      atomicOr(err, 8);  // the owned node box holds a non-owned dof
+    } else {
+      const long long grow = P.row0 + o;
+      const int pid = pid_grp[P.ogrp0 + o / m.nc];
+      ri.rowstart = rowptr[grow];
+      ri.bidx = grow;
+      ri.pid = pid;
+      ri.lenf = (int)(rowptr[grow + 1] - rowptr[grow]) | (pat_dir[pid] << 16);
+    }
+  }
+  out[P.rinfo0 + li] = ri;
+}
+
+// distinct patterns of each candidate tile in first-occurrence order (rows z,
+// y, x); the rows get their tile-local index in bits 17.. of lenf
+__global__ void k_tile_patterns(const Tile *cand, int ncand, int cap, RowInfo *rowinfo, int *lists, int *npat,
+                                int *any, int *err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncand) return;
+  const Tile T = cand[t];
+  int *list = lists + (long long)t * cap;
+  int n = 0, has = 0;
+  for (int z = 0; z < T.ext[2]; ++z)
+    for (int y = 0; y < T.ext[1]; ++y)
+      for (int x = 0; x < T.ext[0]; ++x) {
+        const long long li = ((long long)(T.org[2] - T.onb_lo[2] + z) * T.onb_n[1] + (T.org[1] - T.onb_lo[1] + y)) *
+                                 T.onb_n[0] +
+                             (T.org[0] - T.onb_lo[0] + x);
+        RowInfo &ri = rowinfo[T.rowinfo_base + li];
+        if (ri.pid < 0) continue;
+        has = 1;
+        int l = 0;
+        while (l < n && list[l] != ri.pid) ++l;
+        if (l == n) list[n++] = ri.pid;
+        if (l >= (1 << 14)) atomicOr(err, 16);
+        ri.lenf = (ri.lenf & 0x1FFFF) | (l << 17);
+      }
+  npat[t] = n;
+  any[t] = has;
+}
+
+__global__ void k_tile_write(const int *lists, int cap, const int *npat, const long long *off, int ncand,
+                             int *tile_pats) {
+  const int t = blockIdx.x;
+  if (t >= ncand || off[t] < 0) return;
+  for (int i = threadIdx.x; i < npat[t]; i += blockDim.x) tile_pats[off[t] + i] = lists[(long long)t * cap + i];
+}
+
+unsigned grid_of(long long n) { return (unsigned)std::max<long long>((n + kBlock - 1) / kBlock, 1); }
+
+// device allocations of one call, freed together
+struct Scratch {
+  std::vector<void *> ptrs;
+  ~Scratch() {
+    for (void *q : ptrs) (void)hipFree(q);
+  }
+  template <class T>
+  T *get(long long n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<long long>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    ptrs.push_back(q);
+    return static_cast<T *>(q);
+  }
+};
+
+template <class T>
+bool put(T *dst, const std::vector<T> &src) {
+  return src.empty() || hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+template <int LP>
+void launch_fill(const MeshG &m, const PartG &G, hipStream_t st, const int64_t *row_lnode, const int64_t *lnode_dof,
+                 const int64_t *ldof_col, const long long *rowptr, int *colidx, uint16_t *keys, int KS,
+                 unsigned long long *hash, int *err) {
+  constexpr int BS = LP > kBlock ? LP : kBlock;
+  constexpr int GPB = BS / LP;
+  const long long ng = G.nrows / m.nc;
+  if (ng == 0) return;
+  hipLaunchKernelGGL((k_fill_rows<LP, BS>), dim3((unsigned)((ng + GPB - 1) / GPB)), dim3(BS), 0, st, m, G, row_lnode,
+                     lnode_dof, ldof_col, rowptr, colidx, keys, KS, hash, err);
+}
+
+}  // namespace
+
+int structure_device(gdx_problem *p) {
+  hipStream_t st = p->ctx->stream;
+  const int nparts = p->nparts, nc = p->ncomp, D = p->D, K = p->K;
+  MeshG m{};
+  for (int k = 0; k < 3; ++k) {
+    m.n[k] = k < D ? p->desc.cells[k] : 1;
+    m.nn[k] = k < D ? K * m.n[k] + 1 : 1;
+  }
+  m.D = D;
+  m.K = K;
+  m.nc = nc;
+  m.dir_kind = p->desc.dir_kind;
+  const int W = 2 * K + 1, S = D == 3 ? W * W * W : W * W, SN = S * nc, KS = (SN + 3) & ~3;
+  int LP = 32;
+  while (LP < KS) LP *= 2;
+  if (LP > 512) return kUseHost;
+  std::vector<PartG> G(nparts);
+  long long rows = 0, ogrps = 0, rinfo = 0, lnodes = 0, ldofs = 0, owns = 0;
+  for (int r = 0; r < nparts; ++r) {
+    const Part &P = p->P[r];
+    PartG &g = G[r];
+    for (int k = 0; k < 3; ++k) {
+      g.loc_lo[k] = P.loc_lo[k];
+      g.lnn[k] = P.lnn[k];
+      g.own_lo[k] = P.own_lo[k];
+      g.own_hi[k] = P.own_hi[k];
+      g.onb_lo[k] = P.onb_lo[k];
+      g.onb_n[k] = P.onb_n[k];
+    }
+    if (P.nrows % nc || P.n_own % nc) return kUseHost;
+    g.n_own = P.n_own;
+    g.nrows = P.nrows;
+    g.row0 = rows;
+    g.ogrp0 = ogrps;
+    g.rinfo0 = rinfo;
+    g.lnode0 = lnodes;
+    g.ldof0 = ldofs;
+    g.own0 = owns;
+    rows += P.nrows;
+    ogrps += P.n_own / nc;
+    rinfo += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+    lnodes += (long long)P.lnode_dof.size();
+    ldofs += P.n_ldofs;
+    owns += P.n_own;
+  }
+  Scratch X;
+  int64_t *d_lnode_dof = X.get<int64_t>(lnodes), *d_ldof_col = X.get<int64_t>(ldofs),
+          *d_ldof_lnode = X.get<int64_t>(ldofs), *d_ldof_orow = X.get<int64_t>(ldofs), *d_row_ldof = X.get<int64_t>(rows),
+          *d_own = X.get<int64_t>(owns), *d_row_lnode = X.get<int64_t>(rows / nc);
+  long long *d_rowlen = X.get<long long>(rows);
+  int *d_err = X.get<int>(1);
+  if (!d_lnode_dof || !d_ldof_col || !d_ldof_lnode || !d_ldof_orow || !d_row_ldof || !d_own || !d_row_lnode ||
+      !d_rowlen || !d_err)
+    return GDX_ERR_ALLOC;
+  for (int r = 0; r < nparts; ++r) {
+    const Part &P = p->P[r];
+    if (!put(d_lnode_dof + G[r].lnode0, P.lnode_dof) || !put(d_ldof_col + G[r].ldof0, P.ldof_col) ||
+        !put(d_ldof_lnode + G[r].ldof0, P.ldof_lnode) || !put(d_row_ldof + G[r].row0, P.row_ldof) ||
+        !put(d_own + G[r].own0, P.own_ldofs))
+      return GDX_ERR_HIP;
+  }
+  if (hipMemsetAsync(d_err, 0, sizeof(int), st) != hipSuccess) return GDX_ERR_HIP;
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(ldofs)), dim3(kBlock), 0, st, d_ldof_orow, ldofs, (int64_t)-1);
+  for (int r = 0; r < nparts; ++r) {
+    const PartG &g = G[r];
+    hipLaunchKernelGGL(k_ldof_orow, dim3(grid_of(g.n_own)), dim3(kBlock), 0, st, g, d_own + g.own0,
+                       d_ldof_orow + g.ldof0);
+    hipLaunchKernelGGL(k_row_lnode, dim3(grid_of(g.nrows / nc)), dim3(kBlock), 0, st, g, nc, d_row_ldof + g.row0,
+                       d_ldof_lnode + g.ldof0, d_row_lnode + g.row0 / nc, d_err);
+    hipLaunchKernelGGL(k_rowlen, dim3(grid_of(g.nrows / nc)), dim3(kBlock), 0, st, m, g, d_row_lnode + g.row0 / nc,
+                       d_lnode_dof + g.lnode0, d_ldof_col + g.ldof0, d_rowlen, d_err);
+  }
+  if (hipGetLastError() != hipSuccess) return GDX_ERR_HIP;
+  // row offsets: the parts' CSRs back to back, so part r's local rowptr is the
+  // global one minus its first value
+  long long *rowptr = nullptr;
+  if (hipMalloc(&rowptr, (rows + 1) * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+  int rc = scan_exclusive(d_rowlen, rowptr, rows, st);
+  if (rc) {
+    (void)hipFree(rowptr);
+    return rc;
+  }
+  // the part boundaries of the offsets: first row, first ghost row, end
+  std::vector<long long> bound(3 * nparts);
+  for (int r = 0; r < nparts && !rc; ++r) {
+    const long long at[3] = {G[r].row0, G[r].row0 + G[r].n_own, G[r].row0 + G[r].nrows};
+    for (int k = 0; k < 3 && !rc; ++k)
+      if (hipMemcpy(&bound[3 * r + k], rowptr + at[k], sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = GDX_ERR_HIP;
+  }
+  long long nnz = 0;
+  if (!rc && hipMemcpy(&nnz, rowptr + rows, sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) rc = GDX_ERR_HIP;
+  int *colidx = nullptr;
+  if (!rc && hipMalloc(&colidx, std::max<long long>(nnz, 1) * sizeof(int)) != hipSuccess) rc = GDX_ERR_ALLOC;
+  uint16_t *keys = rc ? nullptr : X.get<uint16_t>(ogrps * KS);
+  unsigned long long *hash = rc ? nullptr : X.get<unsigned long long>(ogrps);
+  unsigned long long *tkey = rc ? nullptr : X.get<unsigned long long>(kTableSize);
+  unsigned long long *tmin = rc ? nullptr : X.get<unsigned long long>(kTableSize);
+  unsigned *slot = rc ? nullptr : X.get<unsigned>(ogrps);
+  if (!rc && (!keys || !hash || !tkey || !tmin || !slot)) rc = GDX_ERR_ALLOC;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    if (colidx) (void)hipFree(colidx);
+    (void)hipFree(rowptr);
+    return code;
+  };
+  if (rc) return fail(rc);
+  for (int r = 0; r < nparts; ++r) {
+    const PartG &g = G[r];
+    const int64_t *rl = d_row_lnode + g.row0 / nc, *ldn = d_lnode_dof + g.lnode0, *ldc = d_ldof_col + g.ldof0;
+    switch (LP) {
+      case 32: launch_fill<32>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+      case 64: launch_fill<64>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+      case 128: launch_fill<128>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+      case 256: launch_fill<256>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+      default: launch_fill<512>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+    }
+  }
+  // distinct keys: first occurrences through the hash table
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_of(kTableSize)), dim3(kBlock), 0, st, tkey, (long long)kTableSize, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_of(kTableSize)), dim3(kBlock), 0, st, tmin, (long long)kTableSize, ~0ull);
+  if (ogrps > 0)
+    hipLaunchKernelGGL(k_insert, dim3(grid_of(ogrps)), dim3(kBlock), 0, st, hash, ogrps, tkey, tmin, slot, d_err);
+  unsigned *d_cnt = X.get<unsigned>(1);
+  unsigned long long *l_min = X.get<unsigned long long>(kTableSize);
+  unsigned *l_slot = X.get<unsigned>(kTableSize);
+  if (!d_cnt || !l_min || !l_slot) return fail(GDX_ERR_ALLOC);
+  if (hipMemsetAsync(d_cnt, 0, sizeof(unsigned), st) != hipSuccess) return fail(GDX_ERR_HIP);
+  hipLaunchKernelGGL(k_collect, dim3(grid_of(kTableSize)), dim3(kBlock), 0, st, tkey, tmin, d_cnt, l_min, l_slot);
+  int err = 0;
+  unsigned npat = 0;
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&npat, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(GDX_ERR_HIP);
+  if (err) return fail(kUseHost);
+  std::vector<unsigned long long> hmin(npat);
+  std::vector<unsigned> hslot(npat);
+  if (npat && (hipMemcpy(hmin.data(), l_min, npat * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess ||
+               hipMemcpy(hslot.data(), l_slot, npat * sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess))
+    return fail(GDX_ERR_HIP);
+  // pattern ids in order of first occurrence
+  std::vector<unsigned> order(npat);
+  for (unsigned i = 0; i < npat; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](unsigned a, unsigned b) { return hmin[a] < hmin[b]; });
+  std::vector<int> tpid(kTableSize, -1);
+  std::vector<long long> rep(npat);
+  for (unsigned i = 0; i < npat; ++i) {
+    tpid[hslot[order[i]]] = (int)i;
+    rep[i] = (long long)hmin[order[i]];
+  }
+  int *d_tpid = X.get<int>(kTableSize), *d_pid = X.get<int>(ogrps);
+  long long *d_rep = X.get<long long>(npat);
+  uint16_t *d_repkeys = X.get<uint16_t>((long long)npat * KS);
+  if (!d_tpid || !d_pid || !d_rep || !d_repkeys) return fail(GDX_ERR_ALLOC);
+  if (!put(d_tpid, tpid) || !put(d_rep, rep)) return fail(GDX_ERR_HIP);
+  if (ogrps > 0)
+    hipLaunchKernelGGL(k_pattern_ids, dim3(grid_of(ogrps)), dim3(kBlock), 0, st, keys, KS, ogrps, slot, tmin, d_tpid,
+                       d_pid, d_err);
+  if (npat > 0)
+    hipLaunchKernelGGL(k_gather_keys, dim3(grid_of((long long)npat * KS)), dim3(kBlock), 0, st, keys, KS, d_rep,
+                       (int)npat, d_repkeys);
+  std::vector<uint16_t> repkeys((size_t)npat * KS);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      (npat && hipMemcpy(repkeys.data(), d_repkeys, repkeys.size() * sizeof(uint16_t), hipMemcpyDeviceToHost) !=
+                   hipSuccess))
+    return fail(GDX_ERR_HIP);
+  if (err) return fail(kUseHost);
+  // pattern tables from the representatives' keys, as the host stage makes them
+  p->pat_fwd.clear();
+  p->pat_inv.clear();
+  p->pat_invr.clear();
+  p->pat_len.assign(npat, 0);
+  p->pat_dir.assign(npat, 0);
+  p->pat16_inv.clear();
+  std::vector<long long> ogrp0(nparts);
+  for (int r = 0; r < nparts; ++r) ogrp0[r] = G[r].ogrp0;
+  if (nc == 1) {
+    const int SP = pattern_stride(S);
+    p->pat_fwd.resize((size_t)npat * SP);
+    p->pat_inv.resize((size_t)npat * SP);
+    p->pat_invr.resize((size_t)npat * SP);
+    std::vector<uint8_t> key8(S);
+    for (unsigned i = 0; i < npat; ++i) {
+      const uint16_t *k16 = &repkeys[(size_t)i * KS];
+      int len = 0;
+      for (int s = 0; s < S; ++s) {
+        key8[s] = k16[s] == kDir16 ? kSlotDirichlet : k16[s] == kAbs16 ? kSlotAbsent : (uint8_t)k16[s];
+        len += k16[s] < kDir16;
+      }
+      // the representative row's node (its parity fixes the reduced numbering)
+      const int r = (int)(std::upper_bound(ogrp0.begin(), ogrp0.end(), rep[i]) - ogrp0.begin()) - 1;
+      const Part &P = p->P[r];
+      const int64_t l = P.ldof_lnode[P.row_ldof[rep[i] - ogrp0[r]]];
+      const int64_t g[3] = {K * P.loc_lo[0] + l % P.lnn[0], K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1],
+                            K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1])};
+      p->pat_dir[i] = scalar_pattern_tables(D, K, key8.data(), S, SP, g, len, &p->pat_fwd[(size_t)i * SP],
+                                            &p->pat_inv[(size_t)i * SP], &p->pat_invr[(size_t)i * SP])
+                          ? 1
+                          : 0;
+      p->pat_len[i] = len;
+    }
+  } else {
+    const int SP = (SN + 8) & ~7;
+    p->pat16_stride = SP;
+    p->pat16_inv.resize((size_t)npat * SP);
+    for (unsigned i = 0; i < npat; ++i) {
+      const uint16_t *k16 = &repkeys[(size_t)i * KS];
+      int len = 0;
+      for (int s = 0; s < SN; ++s) len += k16[s] < kDir16;
+      p->pat_dir[i] = vector_pattern_inv(k16, SN, SP, &p->pat16_inv[(size_t)i * SP]) ? 1 : 0;
+      p->pat_len[i] = len;
+    }
+  }
+  // row table
+  int *d_dir = X.get<int>(npat);
+  RowInfo *rinfo_d = nullptr;
+  if (!d_dir || !put(d_dir, p->pat_dir)) return fail(GDX_ERR_ALLOC);
+  if (hipMalloc(&rinfo_d, std::max<long long>(rinfo, 1) * sizeof(RowInfo)) != hipSuccess) return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r) {
+    const PartG &g = G[r];
+    hipLaunchKernelGGL(k_rowinfo, dim3(grid_of(g.onb_n[0] * g.onb_n[1] * g.onb_n[2])), dim3(kBlock), 0, st, m, g,
+                       d_lnode_dof + g.lnode0, d_ldof_orow + g.ldof0, rowptr, d_pid, d_dir, rinfo_d, d_err);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipFree(rinfo_d);
+    return fail(GDX_ERR_HIP);
+  }
+  if (err) {
+    (void)hipFree(rinfo_d);
+    return fail(kUseHost);
+  }
+  // sizes and offsets of the parts; host copies of the CSRs and the row table
+  // are made on demand
+  int64_t owned_nnz = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.val_base = bound[3 * r];
+    P.row_base = G[r].row0;
+    P.nnz = bound[3 * r + 2] - bound[3 * r];
+    P.rowinfo_base = G[r].rinfo0;
+    P.rowptr.clear();
+    P.colidx.clear();
+    owned_nnz += bound[3 * r + 1] - bound[3 * r];
+  }
+  p->rowinfo.clear();
+  p->npat = (int)npat;
+  p->nnz_total = nnz;
+  p->rows_total = rows;
+  p->owned_nnz = owned_nnz;
+  Device &d = p->dev;
+  if (d.csr_rowptr) (void)hipFree(d.csr_rowptr);
+  if (d.csr_colidx) (void)hipFree(d.csr_colidx);
+  if (d.rowinfo) (void)hipFree(d.rowinfo);
+  d.csr_rowptr = rowptr;
+  d.csr_colidx = colidx;
+  d.rowinfo = rinfo_d;
+  p->dev_structure = true;
+  p->host_csr_stale = p->host_rowinfo_stale = true;
+  return GDX_OK;
+}
+
+int build_tiles_device(gdx_problem *p, const int R[3]) {
+  hipStream_t st = p->ctx->stream;
+  for (int k = 0; k < 3; ++k) p->tileR[k] = R[k];
+  // candidate tiles in the host stage's order: parts, then tz, ty, tx
+  std::vector<Tile> cand;
+  for (int r = 0; r < p->nparts; ++r) {
+    const Part &P = p->P[r];
+    int64_t nt[3];
+    for (int k = 0; k < 3; ++k) nt[k] = (P.onb_n[k] + R[k] - 1) / R[k];
+    for (int64_t tz = 0; tz < nt[2]; ++tz)
+      for (int64_t ty = 0; ty < nt[1]; ++ty)
+        for (int64_t tx = 0; tx < nt[0]; ++tx) {
+          Tile t{};
+          t.part = r;
+          const int64_t tt[3] = {tx, ty, tz};
+          for (int k = 0; k < 3; ++k) {
+            t.org[k] = (int)(P.onb_lo[k] + tt[k] * R[k]);
+            t.ext[k] = (int)std::min<int64_t>(R[k], P.onb_n[k] - tt[k] * R[k]);
+            t.onb_lo[k] = (int)P.onb_lo[k];
+            t.onb_n[k] = (int)P.onb_n[k];
+          }
+          t.rowinfo_base = P.rowinfo_base;
+          cand.push_back(t);
+        }
+  }
+  const int nc_ = (int)cand.size(), cap = R[0] * R[1] * R[2];
+  Scratch X;
+  Tile *d_cand = X.get<Tile>(nc_);
+  int *lists = X.get<int>((long long)nc_ * cap), *d_npat = X.get<int>(nc_), *d_any = X.get<int>(nc_),
+      *d_err = X.get<int>(1);
+  long long *d_off = X.get<long long>(nc_);
+  if (!d_cand || !lists || !d_npat || !d_any || !d_err || !d_off) return GDX_ERR_ALLOC;
+  if (!put(d_cand, cand) || hipMemsetAsync(d_err, 0, sizeof(int), st) != hipSuccess) return GDX_ERR_HIP;
+  if (nc_ > 0)
+    hipLaunchKernelGGL(k_tile_patterns, dim3((nc_ + 63) / 64), dim3(64), 0, st, d_cand, nc_, cap, p->dev.rowinfo,
+                       lists, d_npat, d_any, d_err);
+  std::vector<int> npat(nc_), any(nc_);
+  int err = 0;
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      (nc_ && (hipMemcpy(npat.data(), d_npat, nc_ * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+               hipMemcpy(any.data(), d_any, nc_ * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)))
+    return GDX_ERR_HIP;
+  if (err) {
+    set_error("internal: too many patterns in one tile");
+    return GDX_ERR_ARG;
+  }
+  std::vector<long long> off(nc_, -1);
+  p->tiles.clear();
+  long long total = 0;
+  for (int t = 0; t < nc_; ++t) {
+    if (!any[t]) continue;
+    Tile T = cand[t];
+    T.npat = npat[t];
+    T.pat_off = total;
+    off[t] = total;
+    total += npat[t];
+    p->tiles.push_back(T);
+  }
+  int *tp = nullptr;
+  if (hipMalloc(&tp, std::max<long long>(total, 1) * sizeof(int)) != hipSuccess) return GDX_ERR_ALLOC;
+  p->tile_pats.assign(total, 0);
+  if (!put(d_off, off)) {
+    (void)hipFree(tp);
+    return GDX_ERR_HIP;
+  }
+  if (nc_ > 0) hipLaunchKernelGGL(k_tile_write, dim3(nc_), dim3(64), 0, st, lists, cap, d_npat, d_off, nc_, tp);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      (total && hipMemcpy(p->tile_pats.data(), tp, total * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)) {
+    (void)hipFree(tp);
+    return GDX_ERR_HIP;
+  }
+  if (p->dev.tile_pats) (void)hipFree(p->dev.tile_pats);
+  p->dev.tile_pats = tp;
+  p->host_rowinfo_stale = true;
+  return GDX_OK;
+}
+
+int host_csr(gdx_problem *p) {
+  if (!p->host_csr_stale) return GDX_OK;
+  if (hipSetDevice(p->ctx->device) != hipSuccess || hipStreamSynchronize(p->ctx->stream) != hipSuccess)
+    return GDX_ERR_HIP;
+  for (Part &P : p->P) {
+    P.rowptr.resize(P.nrows + 1);
+    P.colidx.resize(P.nnz);
+    if (hipMemcpy(P.rowptr.data(), p->dev.csr_rowptr + P.row_base, (P.nrows + 1) * sizeof(long long),
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        (P.nnz && hipMemcpy(P.colidx.data(), p->dev.csr_colidx + P.val_base, P.nnz * sizeof(int),
+                            hipMemcpyDeviceToHost) != hipSuccess))
+      return GDX_ERR_HIP;
+    for (int64_t &v : P.rowptr) v -= P.val_base;
+  }
+  p->host_csr_stale = false;
+  return GDX_OK;
+}
+
+int host_rowinfo(gdx_problem *p) {
+  if (!p->host_rowinfo_stale) return GDX_OK;
+  int64_t n = 0;
+  for (const Part &P : p->P) n += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+  p->rowinfo.resize(n);
+  if (hipSetDevice(p->ctx->device) != hipSuccess || hipStreamSynchronize(p->ctx->stream) != hipSuccess ||
+      (n && hipMemcpy(p->rowinfo.data(), p->dev.rowinfo, n * sizeof(RowInfo), hipMemcpyDeviceToHost) != hipSuccess))
+    return GDX_ERR_HIP;
+  p->host_rowinfo_stale = false;
+  return GDX_OK;
+}
+
+}  // namespace gdx
