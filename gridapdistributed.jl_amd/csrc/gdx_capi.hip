@@ -63,6 +63,7 @@ void free_device(gdx_problem *p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   d = gdx::Device{};
+  gdx::free_numbering(p);
 }
 
 int check_desc(const gdx_problem_desc *d) {
@@ -134,6 +135,18 @@ int part_check(gdx_problem *p, int32_t part) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(part >= 0 && part < p->nparts, GDX_ERR_ARG, "part index out of range");
+  return GDX_OK;
+}
+
+// host copies of the numbering and PRanges (made on first use after a device
+// symbolic phase)
+int fe_on_host(gdx_problem *p) {
+  if (p->ctx && p->host_fe_stale) {
+    GDX_HIP(hipSetDevice(p->ctx->device));
+    const int rc = gdx::host_fe(p);
+    if (rc) set_error("copying the numbering to the host failed");
+    return rc;
+  }
   return GDX_OK;
 }
 
@@ -608,6 +621,7 @@ int gdx_part_sizes(gdx_problem *p, int32_t part, int64_t *s) {
 int gdx_export_fe(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner, int64_t *gnode) {
   int rc = part_check(p, part);
   if (rc) return rc;
+  if ((rc = fe_on_host(p))) return rc;
   GDX_REQUIRE(gid && owner && gnode, GDX_ERR_ARG, "NULL argument");
   const gdx::Part &P = p->P[part];
   for (int64_t i = 0; i < P.n_ldofs; ++i) {
@@ -625,6 +639,7 @@ int gdx_export_fe(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner, in
 int gdx_export_rows(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner) {
   int rc = part_check(p, part);
   if (rc) return rc;
+  if ((rc = fe_on_host(p))) return rc;
   GDX_REQUIRE(gid && owner, GDX_ERR_ARG, "NULL argument");
   const gdx::Part &P = p->P[part];
   std::memcpy(gid, P.row_gid.data(), P.nrows * sizeof(int64_t));
@@ -635,6 +650,7 @@ int gdx_export_rows(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner) 
 int gdx_export_cols(gdx_problem *p, int32_t part, int64_t *gid, int32_t *owner) {
   int rc = part_check(p, part);
   if (rc) return rc;
+  if ((rc = fe_on_host(p))) return rc;
   GDX_REQUIRE(gid && owner, GDX_ERR_ARG, "NULL argument");
   const gdx::Part &P = p->P[part];
   std::memcpy(gid, P.col_gid.data(), P.ncols * sizeof(int64_t));

@@ -426,9 +426,10 @@ static std::vector<double> flux_tables_host(const KParams &kp, int D, int NQ) {
 int flux_setup(gdx_problem *p) {
   flux_free(p);
   const int64_t nnodes = (int64_t)p->kp.nn[0] * p->kp.nn[1] * p->kp.nn[2];
-  std::vector<long long> gid_node(std::max<int64_t>(p->ngdofs, 1), 0);
+  // free global id -> lattice node: built on the device when the numbering is there
+  std::vector<long long> gid_node(p->dev_numbering ? 0 : std::max<int64_t>(p->ngdofs, 1), 0);
   for (const Part &P : p->P)
-    for (int64_t i = 0; i < P.n_ldofs; ++i) {
+    for (int64_t i = 0; i < P.n_ldofs && !p->dev_numbering; ++i) {
       const int64_t l = P.ldof_lnode[i];
       const int64_t g0 = p->K * P.loc_lo[0] + l % P.lnn[0];
       const int64_t g1 = p->K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
@@ -446,7 +447,8 @@ int flux_setup(gdx_problem *p) {
   const int64_t nx = std::max<int64_t>(p->ngdofs, 1);
   if (hipMalloc(reinterpret_cast<void **>(&fd->cm), ncm * sizeof(double)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&fd->gid_node), gid_node.size() * sizeof(long long)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&fd->gid_node), std::max<int64_t>(p->ngdofs, 1) * sizeof(long long)) !=
+          hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&fd->xs), nx * sizeof(double)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
@@ -456,8 +458,9 @@ int flux_setup(gdx_problem *p) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }
-  if (hipMemcpy(fd->gid_node, gid_node.data(), gid_node.size() * sizeof(long long), hipMemcpyHostToDevice) !=
-          hipSuccess ||
+  if ((p->dev_numbering ? gid_node_device(p, fd->gid_node) != GDX_OK
+                         : hipMemcpy(fd->gid_node, gid_node.data(), gid_node.size() * sizeof(long long),
+                                     hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(fd->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_HIP;

@@ -39,6 +39,8 @@ struct Part {
   std::vector<int32_t> colidx;
   int64_t nnz = 0;  // stored entries of the local CSR
   int64_t val_base = 0, row_base = 0;
+  // offsets of the part's arrays among all parts' (device symbolic phase)
+  int64_t lnode_base = 0, ldof_base = 0, own_base = 0, col_base = 0;
   // owned node box (global node coords)
   int64_t onb_lo[3], onb_n[3];
   int64_t rowinfo_base = 0;
@@ -90,6 +92,15 @@ struct KParams {
   double rhs_const;
   int dir_kind;
   double dir_coef[4];
+};
+
+// Numbering, global ids and PRanges made by the device symbolic phase: each
+// array holds every part's array back to back in part order (offsets in
+// Part::lnode_base, ldof_base, own_base, row_base, col_base).
+struct DevNumbering {
+  int64_t *lnode_dof = nullptr, *ldof_lnode = nullptr, *ldof_gid = nullptr, *own_ldofs = nullptr,
+          *ldof_col = nullptr, *row_ldof = nullptr, *row_gid = nullptr, *col_gid = nullptr;
+  int32_t *ldof_owner = nullptr, *row_owner = nullptr, *col_owner = nullptr;
 };
 
 struct Device {
@@ -162,6 +173,10 @@ struct gdx_problem {
   // host copies (P.rowptr/colidx, rowinfo) are made on first host use
   bool dev_structure = false;
   bool host_csr_stale = false, host_rowinfo_stale = false;
+  // the numbering, ids and PRanges were made on the device (in dn); their host
+  // copies in P are made on first host use
+  bool dev_numbering = false, host_fe_stale = false;
+  gdx::DevNumbering dn;
   gdx::KParams kp{};
   gdx::Device dev;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -198,7 +213,20 @@ int symbolic(gdx_problem *p);  // symbolic phase (gdx_symbolic.cpp)
 // GDX_OK, a GDX_ERR_* code, or kUseHost when the host stage must build them.
 constexpr int kUseHost = 1;
 int structure_device(gdx_problem *p);
+// GDX_TRACE_SYMBOLIC=1: the wall time of each symbolic stage since the last
+// mark (or since `start`) goes to stderr
+void symbolic_mark(const char *stage, bool start = false);
 int build_tiles_device(gdx_problem *p, const int R[3]);
+// device numbering, owners, global ids and row/col PRanges (the part geometry
+// in P is set): owner_dim[d][N] = part index along d owning lattice node N;
+// off = the reference cell's dofs as order-K lattice offsets (ncd of them);
+// E, F = edges and faces of the cell polytope as doubled-lattice offsets
+int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const int off[27][3], int ncd,
+                     const int (*E)[3], int nE, const int (*F)[3], int nF);
+void free_numbering(gdx_problem *p);
+int host_fe(gdx_problem *p);  // host copies of the numbering, ids and PRanges (no-op when current)
+// free global id -> lattice node (x fastest), on the device, from the device numbering
+int gid_node_device(gdx_problem *p, long long *gid_node);
 int host_csr(gdx_problem *p);      // host copies of the local CSRs (no-op when current)
 int host_rowinfo(gdx_problem *p);  // host copy of the row table (no-op when current)
 // tables of one position pattern from its key (slot -> offset, kSlotAbsent or

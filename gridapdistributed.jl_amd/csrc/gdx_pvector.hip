@@ -143,6 +143,7 @@ int build_range(gdx_problem *p, int which) {
   PvData *D = pvd(p);
   RangeMaps &R = D->r[which];
   if (R.built) return GDX_OK;
+  if (int rc = host_fe(p)) return rc;
   const int np = p->nparts;
   std::vector<int64_t> base(np + 1, 0), first(np, 0);
   for (int r = 0; r < np; ++r) {

@@ -292,6 +292,7 @@ static int solve_setup(gdx_problem *p) {
   if (p->solve) return GDX_OK;
   const int64_t n = p->ngdofs;
   if (n >= (int64_t)1 << 31) return GDX_ERR_UNSUPPORTED;
+  if (int rc = host_fe(p)) return rc;
   if (int rc = host_csr(p)) return rc;
   std::vector<long long> rstart(n), bidx(n), dpos(n);
   std::vector<int> rlen(n), cgid(std::max<int64_t>(p->nnz_total, 1), 0);

@@ -4,7 +4,9 @@
 // uploaded to HBM by gdx_capi.hip.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -412,6 +414,16 @@ bool vector_pattern_inv(const uint16_t *key, int SN, int SP, uint16_t *inv) {
   return has_dir;
 }
 
+void symbolic_mark(const char *stage, bool start) {
+  static thread_local std::chrono::steady_clock::time_point last;
+  const auto now = std::chrono::steady_clock::now();
+  const char *e = std::getenv("GDX_TRACE_SYMBOLIC");
+  if (!start && e && std::atoi(e) > 0)
+    std::fprintf(stderr, "gdx symbolic: %-16s %8.2f ms\n", stage,
+                 std::chrono::duration<double, std::milli>(now - last).count());
+  last = now;
+}
+
 void tile_shape(int dim, int K, int R[3]) {
   // must match the template instances launched by launch_matrix
   if (dim == 3) {
@@ -424,57 +436,15 @@ void tile_shape(int dim, int K, int R[3]) {
   }
 }
 
-int symbolic(gdx_problem *p) {
-  p->dev_structure = false;
-  p->host_csr_stale = p->host_rowinfo_stale = false;
-  Ctx c;
-  c.p = p;
-  c.D = p->D;
-  c.K = p->K;
-  const gdx_problem_desc &d = p->desc;
-  for (int k = 0; k < 3; ++k) {
-    c.n[k] = k < c.D ? d.cells[k] : 1;
-    c.nn[k] = k < c.D ? c.K * c.n[k] + 1 : 1;
-    int np = k < c.D ? d.parts[k] : 1;
-    c.own_lo_dim[k].resize(np);
-    c.own_hi_dim[k].resize(np);
-    for (int q = 0; q < np; ++q) local_range(q, np, c.n[k], 0, c.own_lo_dim[k][q], c.own_hi_dim[k][q]);
-    // owner of node N along k: the highest part whose first node is <= N
-    // (generate_gids: owner = max owner of the cells touching the dof)
-    c.node_owner_dim[k].assign(c.nn[k], 0);
-    for (int64_t N = 0; N < c.nn[k]; ++N) {
-      int o = 0;
-      for (int q = 0; q < np; ++q)
-        if (c.K * c.own_lo_dim[k][q] <= N) o = q;
-      c.node_owner_dim[k][N] = o;
-    }
-  }
+// Host numbering, owners, global ids and row/col PRanges of every part.
+static void host_fe_stage(gdx_problem *p, const Ctx &c) {
   const int nparts = p->nparts;
-  p->P.assign(nparts, Part{});
+  const gdx_problem_desc &d = p->desc;
   const bool FA = d.strategy == GDX_FULLY_ASSEMBLED_ROWS;
-  for_parts(nparts, [&](int r) {
-    Part &P = p->P[r];
-    P.rank = r;
-    P.pc[0] = r % d.parts[0];
-    P.pc[1] = (r / d.parts[0]) % d.parts[1];
-    P.pc[2] = c.D == 3 ? r / (d.parts[0] * d.parts[1]) : 0;
-    for (int k = 0; k < 3; ++k) {
-      if (k < c.D) {
-        local_range(P.pc[k], d.parts[k], c.n[k], 0, P.own_lo[k], P.own_hi[k]);
-        local_range(P.pc[k], d.parts[k], c.n[k], 1, P.loc_lo[k], P.loc_hi[k]);
-        P.lnn[k] = c.K * (P.loc_hi[k] - P.loc_lo[k]) + 1;
-        P.onb_lo[k] = c.K * P.own_lo[k];
-        P.onb_n[k] = c.K * (P.own_hi[k] - P.own_lo[k]) + (P.pc[k] == d.parts[k] - 1 ? 1 : 0);
-      } else {
-        P.own_lo[k] = P.loc_lo[k] = 0;
-        P.own_hi[k] = P.loc_hi[k] = 1;
-        P.lnn[k] = 1;
-        P.onb_lo[k] = 0;
-        P.onb_n[k] = 1;
-      }
-    }
-    number_part(c, P);
-  });
+  (void)FA;
+  (void)d;
+  for_parts(nparts, [&](int r) { number_part(c, p->P[r]); });
+  symbolic_mark("numbering");
   // owners and rank-blocked global ids (src/FESpaces.jl:139-261): owned dofs of
   // part r get consecutive ids after those of parts < r
   for_parts(nparts, [&](int r) {
@@ -539,6 +509,7 @@ int symbolic(gdx_problem *p) {
     p->ngdofs *= nc;
   }
 
+  symbolic_mark("global ids");
   int off[27][3];
   const int ncd = cell_dof_offsets(c.D, c.K, off);
   const int K = c.K;
@@ -650,20 +621,17 @@ int symbolic(gdx_problem *p) {
     P.n_ghost_cols = P.ncols - P.n_own;
   });
 
-  // CSR, position patterns, row table and tiles: on the device unless the
-  // problem has none (host-only problems) or GDX_HOST_SYMBOLIC=1 asks for the
-  // host stage; the device stage hands back to the host one when its pattern
-  // table overflows or two keys collide in a hash
-  if (p->ctx && !(std::getenv("GDX_HOST_SYMBOLIC") && std::atoi(std::getenv("GDX_HOST_SYMBOLIC")) > 0)) {
-    const int rc = structure_device(p);
-    if (rc != kUseHost) {
-      if (rc) return rc;
-      int R[3];
-      tile_shape(c.D, K, R);
-      if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
-      return build_tiles_device(p, R);
-    }
-  }
+  symbolic_mark("PRanges");
+}
+
+// Host CSR, position patterns, row table and tiles of every part.
+static int host_csr_stage(gdx_problem *p, const Ctx &c) {
+  const int nparts = p->nparts;
+  const gdx_problem_desc &d = p->desc;
+  const bool FA = d.strategy == GDX_FULLY_ASSEMBLED_ROWS;
+  (void)FA;
+  (void)d;
+  const int K = c.K, nc = p->ncomp;
   if (nc > 1) {
     vector_csr(p, c);
     int R[3];
@@ -820,6 +788,93 @@ int symbolic(gdx_problem *p) {
   if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
   build_tiles(p, R);
   return GDX_OK;
+}
+
+int symbolic(gdx_problem *p) {
+  p->dev_structure = p->dev_numbering = false;
+  p->host_csr_stale = p->host_rowinfo_stale = p->host_fe_stale = false;
+  symbolic_mark("", true);
+  Ctx c;
+  c.p = p;
+  c.D = p->D;
+  c.K = p->K;
+  const gdx_problem_desc &d = p->desc;
+  for (int k = 0; k < 3; ++k) {
+    c.n[k] = k < c.D ? d.cells[k] : 1;
+    c.nn[k] = k < c.D ? c.K * c.n[k] + 1 : 1;
+    int np = k < c.D ? d.parts[k] : 1;
+    c.own_lo_dim[k].resize(np);
+    c.own_hi_dim[k].resize(np);
+    for (int q = 0; q < np; ++q) local_range(q, np, c.n[k], 0, c.own_lo_dim[k][q], c.own_hi_dim[k][q]);
+    // owner of node N along k: the highest part whose first node is <= N
+    // (generate_gids: owner = max owner of the cells touching the dof)
+    c.node_owner_dim[k].assign(c.nn[k], 0);
+    for (int64_t N = 0; N < c.nn[k]; ++N) {
+      int o = 0;
+      for (int q = 0; q < np; ++q)
+        if (c.K * c.own_lo_dim[k][q] <= N) o = q;
+      c.node_owner_dim[k][N] = o;
+    }
+  }
+  const int nparts = p->nparts;
+  p->P.assign(nparts, Part{});
+  for_parts(nparts, [&](int r) {
+    Part &P = p->P[r];
+    P.rank = r;
+    P.pc[0] = r % d.parts[0];
+    P.pc[1] = (r / d.parts[0]) % d.parts[1];
+    P.pc[2] = c.D == 3 ? r / (d.parts[0] * d.parts[1]) : 0;
+    for (int k = 0; k < 3; ++k) {
+      if (k < c.D) {
+        local_range(P.pc[k], d.parts[k], c.n[k], 0, P.own_lo[k], P.own_hi[k]);
+        local_range(P.pc[k], d.parts[k], c.n[k], 1, P.loc_lo[k], P.loc_hi[k]);
+        P.lnn[k] = c.K * (P.loc_hi[k] - P.loc_lo[k]) + 1;
+        P.onb_lo[k] = c.K * P.own_lo[k];
+        P.onb_n[k] = c.K * (P.own_hi[k] - P.own_lo[k]) + (P.pc[k] == d.parts[k] - 1 ? 1 : 0);
+      } else {
+        P.own_lo[k] = P.loc_lo[k] = 0;
+        P.own_hi[k] = P.loc_hi[k] = 1;
+        P.lnn[k] = 1;
+        P.onb_lo[k] = 0;
+        P.onb_n[k] = 1;
+      }
+    }
+  });
+  // numbering, ids and PRanges, then CSR, position patterns, row table and
+  // tiles: on the device unless the problem has none (host-only problems) or
+  // GDX_HOST_SYMBOLIC=1 asks for the host stages; the device CSR stage hands
+  // back to the host one when its pattern table overflows or two keys collide
+  // in a hash
+  const char *hs = std::getenv("GDX_HOST_SYMBOLIC");
+  const bool device = p->ctx && !(hs && std::atoi(hs) > 0);
+  bool numbered = false;
+  if (device) {
+    int off[27][3];
+    const int ncd = cell_dof_offsets(c.D, c.K, off);
+    const int rc = numbering_device(p, c.node_owner_dim, off, ncd, c.D == 3 ? kHexE : kQuadE, c.D == 3 ? 12 : 4,
+                                    kHexF, c.D == 3 ? 6 : 0);
+    if (rc != kUseHost && rc != GDX_OK) return rc;
+    numbered = rc == GDX_OK;
+    symbolic_mark("numbering+PRanges (device)");
+  }
+  if (!numbered) host_fe_stage(p, c);
+  if (device) {
+    const int rc = structure_device(p);
+    if (rc != kUseHost) {
+      if (rc) return rc;
+      int R[3];
+      tile_shape(c.D, c.K, R);
+      if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
+      const int rt = build_tiles_device(p, R);
+      symbolic_mark("tiles (device)");
+      return rt;
+    }
+    if (numbered) {
+      const int rf = host_fe(p);  // the host stage reads the numbering from the host copies
+      if (rf) return rf;
+    }
+  }
+  return host_csr_stage(p, c);
 }
 
 // Tiles of owned rows of extent R (rows per direction; multiples of the order),

@@ -293,6 +293,11 @@ __global__ void k_collect(const unsigned long long *tkey, const unsigned long lo
   out_slot[i] = s;
 }
 
+__global__ void k_gather_i64(const int64_t *src, const long long *idx, long long n, long long *out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
 __global__ void k_gather_keys(const uint16_t *keys, int KS, const long long *rep, int npat, uint16_t *out) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= (long long)npat * KS) return;
@@ -410,6 +415,624 @@ void launch_fill(const MeshG &m, const PartG &G, hipStream_t st, const int64_t *
 
 }  // namespace
 
+namespace {
+
+// ---- numbering, owners, global ids and PRanges on the device ----
+
+struct PartN {
+  long long loc_lo[3], lnn[3], lc[3], own_lo[3], own_hi[3];
+  long long nlnode, lnode0;  // local nodes; offset among all parts' nodes
+  long long ncl, ef0;        // local cells; offset of the edge/face flags
+  long long ldof0, nldof, own0, n_own, gid0;
+  long long key0, nkey;      // ghost-column key space
+  long long row0, col0;
+  int rank;
+};
+
+// the reference cell: its dofs as order-K lattice offsets; its edges and faces
+// as doubled-lattice offsets in polytope order; a doubled offset's index in
+// its dimension's list
+struct NumTabs {
+  int ncd, nE, nF;
+  int off[27][3];
+  int E[12][3], F[6][3];
+  int fidx[27];
+};
+
+struct OwnerTabs {
+  const int *dim[3];  // part index along d owning lattice node N
+  int np[3];
+};
+
+// the first local cell holding doubled-lattice coordinate pt along a direction
+__device__ inline long long first_cell(long long pt) { return (pt & 1) ? (pt - 1) / 2 : max(pt / 2 - 1, 0LL); }
+
+__device__ inline long long pn_lnode_in(const MeshG &m, const PartN &P, long long g0, long long g1, long long g2) {
+  const long long l0 = g0 - m.K * P.loc_lo[0], l1 = g1 - m.K * P.loc_lo[1], l2 = g2 - m.K * P.loc_lo[2];
+  if (l0 < 0 || l0 >= P.lnn[0] || l1 < 0 || l1 >= P.lnn[1] || l2 < 0 || l2 >= P.lnn[2]) return -1;
+  return l0 + P.lnn[0] * (l1 + P.lnn[1] * l2);
+}
+
+__device__ inline void pn_node(const MeshG &m, const PartN &P, long long l, long long g[3]) {
+  g[0] = m.K * P.loc_lo[0] + l % P.lnn[0];
+  g[1] = m.K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
+  g[2] = m.K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1]);
+}
+
+// whether (cell c, edge or face t) is where the edge or face is numbered: the
+// first local cell holding it (edges block, then faces block)
+__global__ void k_ef_flags(MeshG m, PartN P, NumTabs T, long long *flag) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long nE = P.ncl * T.nE;
+  if (t >= nE + P.ncl * T.nF) return;
+  long long c;
+  const int *o;
+  if (t < nE) {
+    c = t / T.nE;
+    o = T.E[t % T.nE];
+  } else {
+    c = (t - nE) / T.nF;
+    o = T.F[(t - nE) % T.nF];
+  }
+  const long long cc[3] = {c % P.lc[0], (c / P.lc[0]) % P.lc[1], c / (P.lc[0] * P.lc[1])};
+  bool first = true;
+  for (int d = 0; d < m.D; ++d) first = first && first_cell(2 * cc[d] + o[d]) == cc[d];
+  flag[P.ef0 + t] = first ? 1 : 0;
+}
+
+// a local node's position in the numbering order (vertices lexicographic, then
+// first edges, first faces, cell interiors; every node for order 1) and
+// whether it is free; free flags are stored by position
+__global__ void k_node_pos(MeshG m, PartN P, NumTabs T, const long long *efscan, long long nef, long long *pos,
+                           long long *freef) {
+  const long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (l >= P.nlnode) return;
+  const long long x = l % P.lnn[0], y = (l / P.lnn[0]) % P.lnn[1], z = l / (P.lnn[0] * P.lnn[1]);
+  long long ps = l;
+  if (m.K == 2) {
+    const int odd = (int)(x & 1) + (int)(y & 1) + (m.D == 3 ? (int)(z & 1) : 0);
+    const long long nv = (P.lc[0] + 1) * (P.lc[1] + 1) * (m.D == 3 ? P.lc[2] + 1 : 1);
+    if (odd == 0) {
+      ps = x / 2 + (P.lc[0] + 1) * (y / 2 + (P.lc[1] + 1) * (z / 2));
+    } else {
+      const long long c0 = first_cell(x), c1 = first_cell(y), c2 = m.D == 3 ? first_cell(z) : 0;
+      const long long cell = c0 + P.lc[0] * (c1 + P.lc[1] * c2);
+      if (odd == m.D) {
+        ps = nv + nef + cell;
+      } else {
+        const int o = (int)(x - 2 * c0) + 3 * (int)(y - 2 * c1) + 9 * (int)(m.D == 3 ? z - 2 * c2 : 0);
+        const int f = T.fidx[o];
+        const long long t = odd == 1 ? cell * T.nE + f : P.ncl * T.nE + cell * T.nF + f;
+        ps = nv + efscan[P.ef0 + t] - efscan[P.ef0];
+      }
+    }
+  }
+  pos[P.lnode0 + l] = ps;
+  long long g[3];
+  pn_node(m, P, l, g);
+  freef[P.lnode0 + ps] = dirichlet(m, g[0], g[1], g[2]) ? 0 : 1;
+}
+
+__global__ void k_lnode_dof(MeshG m, PartN P, const long long *pos, const long long *freef, const long long *fscan,
+                            int64_t *lnode_dof, int64_t *ldof_lnode) {
+  const long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (l >= P.nlnode) return;
+  const long long ps = P.lnode0 + pos[P.lnode0 + l];
+  if (!freef[ps]) {
+    lnode_dof[P.lnode0 + l] = -1;
+    return;
+  }
+  const long long dd = (long long)m.nc * (fscan[ps] - fscan[P.lnode0]);
+  lnode_dof[P.lnode0 + l] = dd;
+  for (int k = 0; k < m.nc; ++k) ldof_lnode[P.ldof0 + dd + k] = l;
+}
+
+// owner of every dof (the owner of its node: generate_gids' max-owner rule)
+// every dof has one node of the part (the order covered every node once)
+__global__ void k_check_ldofs(PartN P, const int64_t *ldof_lnode, int *err) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  const long long l = ldof_lnode[P.ldof0 + i];
+  if (l < 0 || l >= P.nlnode) atomicOr(err, 4);
+}
+
+__global__ void k_owners(MeshG m, PartN P, OwnerTabs O, const int64_t *ldof_lnode, int32_t *ldof_owner,
+                         long long *ownf) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  long long g[3];
+  pn_node(m, P, ldof_lnode[P.ldof0 + i], g);
+  const int o = (O.dim[2][g[2]] * O.np[1] + O.dim[1][g[1]]) * O.np[0] + O.dim[0][g[0]];
+  ldof_owner[P.ldof0 + i] = o;
+  ownf[P.ldof0 + i] = o == P.rank ? 1 : 0;
+}
+
+// owned dofs: own index, global id, column id
+__global__ void k_own_ids(PartN P, const long long *ownf, const long long *oscan, int64_t *own_ldofs,
+                          int64_t *ldof_gid, int64_t *ldof_col) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  if (!ownf[P.ldof0 + i]) {
+    ldof_col[P.ldof0 + i] = -1;
+    return;
+  }
+  const long long o = oscan[P.ldof0 + i] - oscan[P.ldof0];
+  own_ldofs[P.own0 + o] = i;
+  ldof_gid[P.ldof0 + i] = P.gid0 + o;
+  ldof_col[P.ldof0 + i] = o;
+}
+
+// ghost dofs: global id from the owner's numbering of the node; ghost rows
+// (SubAssembledRows) are the ghosts on the closure of the owned cells
+__global__ void k_ghost_ids(MeshG m, const PartN *parts, int r, int fa, const int32_t *ldof_owner,
+                            const int64_t *ldof_lnode, const int64_t *lnode_dof, int64_t *ldof_gid, long long *grf,
+                            int *err) {
+  const PartN P = parts[r];
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  const int ow = ldof_owner[P.ldof0 + i];
+  if (ow == P.rank) {
+    grf[P.ldof0 + i] = 0;
+    return;
+  }
+  long long g[3];
+  pn_node(m, P, ldof_lnode[P.ldof0 + i], g);
+  const PartN &Q = parts[ow];
+  const long long lq = pn_lnode_in(m, Q, g[0], g[1], g[2]);
+  const long long dq = lq >= 0 ? lnode_dof[Q.lnode0 + lq] : -1;
+  if (dq < 0) atomicOr(err, 1);  // a ghost dof missing in its owner
+  else ldof_gid[P.ldof0 + i] = ldof_gid[Q.ldof0 + dq + i % m.nc];
+  bool in = !fa;
+  for (int d = 0; d < m.D; ++d) in = in && g[d] >= m.K * P.own_lo[d] && g[d] <= m.K * P.own_hi[d];
+  grf[P.ldof0 + i] = in ? 1 : 0;
+}
+
+__global__ void k_rows(PartN P, const long long *ownf, const long long *oscan, const long long *grf,
+                       const long long *grscan, const int64_t *ldof_gid, const int32_t *ldof_owner, int64_t *row_ldof,
+                       int64_t *row_gid, int32_t *row_owner) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  long long row;
+  if (ownf[P.ldof0 + i]) row = oscan[P.ldof0 + i] - oscan[P.ldof0];
+  else if (grf[P.ldof0 + i]) row = P.n_own + grscan[P.ldof0 + i] - grscan[P.ldof0];
+  else return;
+  row_ldof[P.row0 + row] = i;
+  row_gid[P.row0 + row] = ldof_gid[P.ldof0 + i];
+  row_owner[P.row0 + row] = ldof_owner[P.ldof0 + i];
+}
+
+// One pass of the ghost-column scan: the cells of a box (of part s), those
+// holding a row of interest (any free dof when want < 0, else a dof owned by
+// want) visited in order; each of their non-owned dofs (in part r's
+// numbering) keeps its first visit, as a key ordered like the visits
+// (SubAssembledRows: cells, then components, then cell dofs), or its own id
+// (FullyAssembledRows: ordered by local id).
+__global__ void k_visit(MeshG m, NumTabs T, const PartN *parts, int r, int s, int want, int fa, long long lo0,
+                        long long lo1, long long lo2, long long n0, long long n1, long long n2, long long koff,
+                        const int64_t *lnode_dof, const int32_t *ldof_owner, unsigned long long *minkey, int *err) {
+  const long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (ci >= n0 * n1 * n2) return;
+  const long long c[3] = {lo0 + ci % n0, lo1 + (ci / n0) % n1, lo2 + ci / (n0 * n1)};
+  const PartN &S = parts[s];
+  bool has = false;
+  for (int a = 0; a < T.ncd && !has; ++a) {
+    const long long ls = pn_lnode_in(m, S, m.K * c[0] + T.off[a][0], m.K * c[1] + T.off[a][1], m.K * c[2] + T.off[a][2]);
+    if (ls < 0) {
+      atomicOr(err, 2);
+      return;
+    }
+    const long long i = lnode_dof[S.lnode0 + ls];
+    has = i >= 0 && (want < 0 || ldof_owner[S.ldof0 + i] == want);
+  }
+  if (!has) return;
+  const PartN &P = parts[r];
+  for (int k = 0; k < m.nc; ++k)
+    for (int b = 0; b < T.ncd; ++b) {
+      const long long lp = pn_lnode_in(m, P, m.K * c[0] + T.off[b][0], m.K * c[1] + T.off[b][1], m.K * c[2] + T.off[b][2]);
+      if (lp < 0) {
+        atomicOr(err, 2);  // a visited cell outside the part's local cells
+        continue;
+      }
+      long long j = lnode_dof[P.lnode0 + lp];
+      if (j < 0) continue;
+      j += k;
+      if (j >= P.nldof) {
+        atomicOr(err, 2);
+        continue;
+      }
+      if (ldof_owner[P.ldof0 + j] == P.rank) continue;
+      const unsigned long long key = fa ? (unsigned long long)j : (unsigned long long)(koff + (ci * m.nc + k) * T.ncd + b);
+      atomicMin(&minkey[P.ldof0 + j], key);
+    }
+}
+
+__global__ void k_key_flags(PartN P, const unsigned long long *minkey, long long *kf) {
+  const long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (j >= P.nldof) return;
+  const unsigned long long k = minkey[P.ldof0 + j];
+  if (k != ~0ull) kf[P.key0 + (long long)k] = 1;
+}
+
+__global__ void k_cols(PartN P, const long long *ownf, const long long *oscan, const unsigned long long *minkey,
+                       const long long *kscan, const int64_t *ldof_gid, const int32_t *ldof_owner, int64_t *ldof_col,
+                       int64_t *col_gid, int32_t *col_owner) {
+  const long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (j >= P.nldof) return;
+  long long col;
+  if (ownf[P.ldof0 + j]) {
+    col = oscan[P.ldof0 + j] - oscan[P.ldof0];
+  } else {
+    const unsigned long long k = minkey[P.ldof0 + j];
+    if (k == ~0ull) return;
+    col = P.n_own + kscan[P.key0 + (long long)k] - kscan[P.key0];
+    ldof_col[P.ldof0 + j] = col;
+  }
+  col_gid[P.col0 + col] = ldof_gid[P.ldof0 + j];
+  col_owner[P.col0 + col] = ldof_owner[P.ldof0 + j];
+}
+
+__global__ void k_gid_node(MeshG m, PartN P, const int64_t *ldof_lnode, const int64_t *ldof_gid, long long ngdofs,
+                           long long *gid_node) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P.nldof) return;
+  long long g[3];
+  pn_node(m, P, ldof_lnode[P.ldof0 + i], g);
+  const long long gid = ldof_gid[P.ldof0 + i];
+  if (gid >= 0 && gid < ngdofs) gid_node[gid] = g[0] + m.nn[0] * (g[1] + m.nn[1] * g[2]);
+}
+
+template <class T>
+int get_at(const T *src, long long i, T &v) {
+  return hipMemcpy(&v, src + i, sizeof(T), hipMemcpyDeviceToHost) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+MeshG mesh_of(const gdx_problem *p) {
+  MeshG m{};
+  for (int k = 0; k < 3; ++k) {
+    m.n[k] = k < p->D ? p->desc.cells[k] : 1;
+    m.nn[k] = k < p->D ? p->K * m.n[k] + 1 : 1;
+  }
+  m.D = p->D;
+  m.K = p->K;
+  m.nc = p->ncomp;
+  m.dir_kind = p->desc.dir_kind;
+  return m;
+}
+
+// the per-part view of the device numbering
+PartN part_view(const gdx_problem *p, int r) {
+  const Part &P = p->P[r];
+  PartN n{};
+  for (int k = 0; k < 3; ++k) {
+    n.loc_lo[k] = P.loc_lo[k];
+    n.lnn[k] = P.lnn[k];
+    n.lc[k] = k < p->D ? P.loc_hi[k] - P.loc_lo[k] : 1;
+    n.own_lo[k] = P.own_lo[k];
+    n.own_hi[k] = P.own_hi[k];
+  }
+  n.nlnode = P.lnn[0] * P.lnn[1] * P.lnn[2];
+  n.lnode0 = P.lnode_base;
+  n.ncl = n.lc[0] * n.lc[1] * n.lc[2];
+  n.ldof0 = P.ldof_base;
+  n.nldof = P.n_ldofs;
+  n.own0 = P.own_base;
+  n.n_own = P.n_own;
+  n.row0 = P.row_base;
+  n.col0 = P.col_base;
+  n.rank = r;
+  return n;
+}
+
+}  // namespace
+
+void free_numbering(gdx_problem *p) {
+  DevNumbering &d = p->dn;
+  void *ptrs[] = {d.lnode_dof, d.ldof_lnode, d.ldof_gid, d.own_ldofs, d.ldof_col, d.row_ldof,
+                  d.row_gid,   d.col_gid,    d.ldof_owner, d.row_owner, d.col_owner};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  d = DevNumbering{};
+  p->dev_numbering = false;
+  p->host_fe_stale = false;
+}
+
+int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const int off[27][3], int ncd,
+                     const int (*E)[3], int nE, const int (*F)[3], int nF) {
+  hipStream_t st = p->ctx->stream;
+  free_numbering(p);
+  const int nparts = p->nparts, nc = p->ncomp, K = p->K, D = p->D;
+  const bool fa = p->desc.strategy == GDX_FULLY_ASSEMBLED_ROWS;
+  const MeshG m = mesh_of(p);
+  NumTabs T{};
+  T.ncd = ncd;
+  T.nE = K == 2 ? nE : 0;
+  T.nF = K == 2 ? nF : 0;
+  for (int a = 0; a < ncd; ++a)
+    for (int d = 0; d < 3; ++d) T.off[a][d] = off[a][d];
+  for (int f = 0; f < 27; ++f) T.fidx[f] = -1;
+  for (int f = 0; f < nE; ++f) {
+    for (int d = 0; d < 3; ++d) T.E[f][d] = E[f][d];
+    T.fidx[E[f][0] + 3 * E[f][1] + 9 * E[f][2]] = f;
+  }
+  for (int f = 0; f < nF; ++f) {
+    for (int d = 0; d < 3; ++d) T.F[f][d] = F[f][d];
+    T.fidx[F[f][0] + 3 * F[f][1] + 9 * F[f][2]] = f;
+  }
+  Scratch X;
+  int *d_err = X.get<int>(1);
+  std::vector<int> odim;
+  long long odoff[3];
+  for (int d = 0; d < 3; ++d) {
+    odoff[d] = (long long)odim.size();
+    odim.insert(odim.end(), owner_dim[d].begin(), owner_dim[d].end());
+  }
+  int *d_odim = X.get<int>((long long)odim.size());
+  if (!d_err || !d_odim || !put(d_odim, odim) || hipMemsetAsync(d_err, 0, sizeof(int), st) != hipSuccess)
+    return GDX_ERR_ALLOC;
+  OwnerTabs O{};
+  for (int d = 0; d < 3; ++d) {
+    O.dim[d] = d_odim + odoff[d];
+    O.np[d] = d < D ? p->desc.parts[d] : 1;
+  }
+  // 1. node positions in the numbering order and free flags
+  std::vector<PartN> V(nparts);
+  long long nodes = 0, efs = 0;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.lnode_base = nodes;
+    V[r] = part_view(p, r);
+    V[r].ef0 = efs;
+    nodes += V[r].nlnode;
+    efs += V[r].ncl * (T.nE + T.nF);
+  }
+  long long *efl = X.get<long long>(efs), *efscan = X.get<long long>(efs + 1), *pos = X.get<long long>(nodes),
+            *freef = X.get<long long>(nodes), *fscan = X.get<long long>(nodes + 1);
+  if (!efl || !efscan || !pos || !freef || !fscan) return GDX_ERR_ALLOC;
+  int rc;
+  if (efs > 0) {
+    for (int r = 0; r < nparts; ++r)
+      hipLaunchKernelGGL(k_ef_flags, dim3(grid_of(V[r].ncl * (T.nE + T.nF))), dim3(kBlock), 0, st, m, V[r], T, efl);
+    if ((rc = scan_exclusive(efl, efscan, efs, st))) return rc;
+  }
+  std::vector<long long> nef(nparts, 0);
+  for (int r = 0; r < nparts && efs > 0; ++r) {
+    long long a, b;
+    if ((rc = get_at(efscan, V[r].ef0, a)) || (rc = get_at(efscan, V[r].ef0 + V[r].ncl * (T.nE + T.nF), b))) return rc;
+    nef[r] = b - a;
+    const long long nv = (V[r].lc[0] + 1) * (V[r].lc[1] + 1) * (D == 3 ? V[r].lc[2] + 1 : 1);
+    if (nv + nef[r] + V[r].ncl != V[r].nlnode) return kUseHost;  // the order must cover every node once
+  }
+  // positions not hit stay non-free (a numbering that missed a node is caught below)
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(nodes)), dim3(kBlock), 0, st, reinterpret_cast<int64_t *>(freef), nodes,
+                     (int64_t)0);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_node_pos, dim3(grid_of(V[r].nlnode)), dim3(kBlock), 0, st, m, V[r], T, efscan, nef[r], pos,
+                       freef);
+  if (hipGetLastError() != hipSuccess) return GDX_ERR_HIP;
+  if ((rc = scan_exclusive(freef, fscan, nodes, st))) return rc;
+  // 2. dofs of the free nodes (nc consecutive per node)
+  long long ldofs = 0;
+  std::vector<long long> fb(nparts + 1);
+  for (int r = 0; r <= nparts; ++r)
+    if ((rc = get_at(fscan, r < nparts ? V[r].lnode0 : nodes, fb[r]))) return rc;
+  for (int r = 0; r < nparts; ++r) {
+    Part &P = p->P[r];
+    P.n_ldofs = nc * (fb[r + 1] - fb[r]);
+    P.ldof_base = ldofs;
+    V[r].ldof0 = ldofs;
+    V[r].nldof = P.n_ldofs;
+    ldofs += P.n_ldofs;
+  }
+  DevNumbering &dn = p->dn;
+  auto alloc = [](auto **q, long long n) {
+    return hipMalloc(reinterpret_cast<void **>(q), std::max<long long>(n, 1) * sizeof(**q)) == hipSuccess;
+  };
+  if (!alloc(&dn.lnode_dof, nodes) || !alloc(&dn.ldof_lnode, ldofs) || !alloc(&dn.ldof_owner, ldofs) ||
+      !alloc(&dn.ldof_gid, ldofs) || !alloc(&dn.ldof_col, ldofs)) {
+    free_numbering(p);
+    return GDX_ERR_ALLOC;
+  }
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    free_numbering(p);
+    return code;
+  };
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(ldofs)), dim3(kBlock), 0, st, dn.ldof_lnode, ldofs, (int64_t)-1);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_lnode_dof, dim3(grid_of(V[r].nlnode)), dim3(kBlock), 0, st, m, V[r], pos, freef, fscan,
+                       dn.lnode_dof, dn.ldof_lnode);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_check_ldofs, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], dn.ldof_lnode, d_err);
+  {
+    int err = 0;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(GDX_ERR_HIP);
+    if (err) return fail(kUseHost);
+  }
+  // 3. owners, owned dofs, global ids (rank-blocked: part r's owned dofs after
+  // those of parts < r, src/FESpaces.jl:139-261)
+  long long *ownf = X.get<long long>(ldofs), *oscan = X.get<long long>(ldofs + 1), *grf = X.get<long long>(ldofs),
+            *grscan = X.get<long long>(ldofs + 1);
+  if (!ownf || !oscan || !grf || !grscan) return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_owners, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, m, V[r], O, dn.ldof_lnode,
+                       dn.ldof_owner, ownf);
+  if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
+  if ((rc = scan_exclusive(ownf, oscan, ldofs, st))) return fail(rc);
+  long long owns = 0;
+  for (int r = 0; r < nparts; ++r) {
+    long long a, b;
+    if ((rc = get_at(oscan, V[r].ldof0, a)) || (rc = get_at(oscan, V[r].ldof0 + V[r].nldof, b))) return fail(rc);
+    Part &P = p->P[r];
+    P.n_own = b - a;
+    P.own_base = owns;
+    V[r].n_own = P.n_own;
+    V[r].own0 = owns;
+    V[r].gid0 = owns;  // owned dofs of all parts in part order: the global ids
+    owns += P.n_own;
+  }
+  p->ngdofs = owns;
+  if (!alloc(&dn.own_ldofs, owns)) return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_own_ids, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], ownf, oscan, dn.own_ldofs,
+                       dn.ldof_gid, dn.ldof_col);
+  PartN *d_parts = X.get<PartN>(nparts);
+  if (!d_parts || !put(d_parts, V)) return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_ghost_ids, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, m, d_parts, r, fa ? 1 : 0,
+                       dn.ldof_owner, dn.ldof_lnode, dn.lnode_dof, dn.ldof_gid, grf, d_err);
+  if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
+  if ((rc = scan_exclusive(grf, grscan, ldofs, st))) return fail(rc);
+  // 4. row PRange: owned dofs, then the ghost rows, each in local id order
+  long long rows = 0;
+  for (int r = 0; r < nparts; ++r) {
+    long long a, b;
+    if ((rc = get_at(grscan, V[r].ldof0, a)) || (rc = get_at(grscan, V[r].ldof0 + V[r].nldof, b))) return fail(rc);
+    Part &P = p->P[r];
+    P.n_ghost_rows = b - a;
+    P.nrows = P.n_own + P.n_ghost_rows;
+    P.row_base = rows;
+    V[r].row0 = rows;
+    rows += P.nrows;
+  }
+  if (!alloc(&dn.row_ldof, rows) || !alloc(&dn.row_gid, rows) || !alloc(&dn.row_owner, rows))
+    return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_rows, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], ownf, oscan, grf, grscan,
+                       dn.ldof_gid, dn.ldof_owner, dn.row_ldof, dn.row_gid, dn.row_owner);
+  // 5. col PRange: owned dofs, then the ghost columns in first-encounter order
+  // over the local COO (src/Algebra.jl:1136-1257): the own cells, then the
+  // cells whose triplets each sender ships, senders in ascending rank order
+  struct Pass {
+    int r, s, want;
+    long long lo[3], n[3], koff;
+  };
+  std::vector<Pass> passes;
+  long long keys = 0;
+  for (int r = 0; r < nparts; ++r) {
+    const Part &P = p->P[r];
+    V[r].key0 = keys;
+    long long nk = 0;
+    auto add = [&](int s, int want, const long long lo[3], const long long hi[3]) {
+      Pass q{r, s, want, {lo[0], lo[1], lo[2]}, {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]}, nk};
+      const long long cells = q.n[0] * q.n[1] * q.n[2];
+      if (cells <= 0) return;
+      passes.push_back(q);
+      nk += cells * nc * ncd;
+    };
+    if (fa) {
+      passes.push_back(Pass{r, r, r, {P.loc_lo[0], P.loc_lo[1], P.loc_lo[2]},
+                            {P.loc_hi[0] - P.loc_lo[0], P.loc_hi[1] - P.loc_lo[1], P.loc_hi[2] - P.loc_lo[2]}, 0});
+      nk = P.n_ldofs;
+    } else {
+      const long long lo[3] = {P.own_lo[0], P.own_lo[1], P.own_lo[2]}, hi[3] = {P.own_hi[0], P.own_hi[1], P.own_hi[2]};
+      add(r, -1, lo, hi);
+      for (int s = 0; s < nparts; ++s) {
+        if (s == r) continue;
+        const Part &S = p->P[s];
+        long long a[3], b[3];
+        bool empty = false;
+        for (int k = 0; k < 3; ++k) {
+          // sender cells whose closure meets this part's owned node box
+          a[k] = std::max<long long>(S.own_lo[k], (P.onb_lo[k] + K - 1) / K - 1);
+          b[k] = std::min<long long>(S.own_hi[k], (P.onb_lo[k] + P.onb_n[k] - 1) / K + 1);
+          if (k >= D) a[k] = 0, b[k] = 1;
+          if (a[k] >= b[k]) empty = true;
+        }
+        if (!empty) add(s, r, a, b);
+      }
+    }
+    V[r].nkey = nk;
+    keys += nk;
+  }
+  unsigned long long *minkey = X.get<unsigned long long>(ldofs);
+  long long *kf = X.get<long long>(keys), *kscan = X.get<long long>(keys + 1);
+  if (!minkey || !kf || !kscan || !put(d_parts, V)) return fail(GDX_ERR_ALLOC);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_of(ldofs)), dim3(kBlock), 0, st, minkey, ldofs, ~0ull);
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(keys)), dim3(kBlock), 0, st, reinterpret_cast<int64_t *>(kf), keys,
+                     (int64_t)0);
+  for (const Pass &q : passes)
+    hipLaunchKernelGGL(k_visit, dim3(grid_of(q.n[0] * q.n[1] * q.n[2])), dim3(kBlock), 0, st, m, T, d_parts, q.r, q.s,
+                       q.want, fa ? 1 : 0, q.lo[0], q.lo[1], q.lo[2], q.n[0], q.n[1], q.n[2], q.koff, dn.lnode_dof,
+                       dn.ldof_owner, minkey, d_err);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_key_flags, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], minkey, kf);
+  if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
+  if ((rc = scan_exclusive(kf, kscan, keys, st))) return fail(rc);
+  long long cols = 0;
+  for (int r = 0; r < nparts; ++r) {
+    long long a, b;
+    if ((rc = get_at(kscan, V[r].key0, a)) || (rc = get_at(kscan, V[r].key0 + V[r].nkey, b))) return fail(rc);
+    Part &P = p->P[r];
+    P.n_ghost_cols = b - a;
+    P.ncols = P.n_own + P.n_ghost_cols;
+    P.col_base = cols;
+    V[r].col0 = cols;
+    cols += P.ncols;
+  }
+  if (!alloc(&dn.col_gid, cols) || !alloc(&dn.col_owner, cols)) return fail(GDX_ERR_ALLOC);
+  for (int r = 0; r < nparts; ++r)
+    hipLaunchKernelGGL(k_cols, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], ownf, oscan, minkey, kscan,
+                       dn.ldof_gid, dn.ldof_owner, dn.ldof_col, dn.col_gid, dn.col_owner);
+  int err = 0;
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(GDX_ERR_HIP);
+  if (err) return fail(kUseHost);
+  // host copies of the numbering are made on first host use
+  for (Part &P : p->P) {
+    P.lnode_dof.clear();
+    P.ldof_lnode.clear();
+    P.ldof_owner.clear();
+    P.ldof_gid.clear();
+    P.own_ldofs.clear();
+    P.ldof_col.clear();
+    P.row_ldof.clear();
+    P.row_gid.clear();
+    P.row_owner.clear();
+    P.col_gid.clear();
+    P.col_owner.clear();
+  }
+  p->dev_numbering = true;
+  p->host_fe_stale = true;
+  return GDX_OK;
+}
+
+int host_fe(gdx_problem *p) {
+  if (!p->host_fe_stale) return GDX_OK;
+  if (hipSetDevice(p->ctx->device) != hipSuccess || hipStreamSynchronize(p->ctx->stream) != hipSuccess)
+    return GDX_ERR_HIP;
+  const DevNumbering &d = p->dn;
+  auto get = [](auto &v, const auto *src, long long n) {
+    v.resize(n);
+    return n == 0 || hipMemcpy(v.data(), src, n * sizeof(v[0]), hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  for (Part &P : p->P) {
+    const long long nl = P.lnn[0] * P.lnn[1] * P.lnn[2];
+    if (!get(P.lnode_dof, d.lnode_dof + P.lnode_base, nl) || !get(P.ldof_lnode, d.ldof_lnode + P.ldof_base, P.n_ldofs) ||
+        !get(P.ldof_owner, d.ldof_owner + P.ldof_base, P.n_ldofs) ||
+        !get(P.ldof_gid, d.ldof_gid + P.ldof_base, P.n_ldofs) || !get(P.own_ldofs, d.own_ldofs + P.own_base, P.n_own) ||
+        !get(P.ldof_col, d.ldof_col + P.ldof_base, P.n_ldofs) || !get(P.row_ldof, d.row_ldof + P.row_base, P.nrows) ||
+        !get(P.row_gid, d.row_gid + P.row_base, P.nrows) || !get(P.row_owner, d.row_owner + P.row_base, P.nrows) ||
+        !get(P.col_gid, d.col_gid + P.col_base, P.ncols) || !get(P.col_owner, d.col_owner + P.col_base, P.ncols))
+      return GDX_ERR_HIP;
+  }
+  p->host_fe_stale = false;
+  return GDX_OK;
+}
+
+int gid_node_device(gdx_problem *p, long long *gid_node) {
+  const MeshG m = mesh_of(p);
+  for (int r = 0; r < p->nparts; ++r) {
+    const PartN V = part_view(p, r);
+    if (V.nldof > 0)
+      hipLaunchKernelGGL(k_gid_node, dim3(grid_of(V.nldof)), dim3(kBlock), 0, p->ctx->stream, m, V, p->dn.ldof_lnode,
+                         p->dn.ldof_gid, (long long)p->ngdofs, gid_node);
+  }
+  return hipGetLastError() == hipSuccess && hipStreamSynchronize(p->ctx->stream) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
 int structure_device(gdx_problem *p) {
   hipStream_t st = p->ctx->stream;
   const int nparts = p->nparts, nc = p->ncomp, D = p->D, K = p->K;
@@ -451,26 +1074,31 @@ int structure_device(gdx_problem *p) {
     rows += P.nrows;
     ogrps += P.n_own / nc;
     rinfo += P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
-    lnodes += (long long)P.lnode_dof.size();
+    lnodes += P.lnn[0] * P.lnn[1] * P.lnn[2];
     ldofs += P.n_ldofs;
     owns += P.n_own;
   }
   Scratch X;
-  int64_t *d_lnode_dof = X.get<int64_t>(lnodes), *d_ldof_col = X.get<int64_t>(ldofs),
-          *d_ldof_lnode = X.get<int64_t>(ldofs), *d_ldof_orow = X.get<int64_t>(ldofs), *d_row_ldof = X.get<int64_t>(rows),
-          *d_own = X.get<int64_t>(owns), *d_row_lnode = X.get<int64_t>(rows / nc);
+  // the numbering and PRanges: made on the device already, or uploaded
+  const bool dn = p->dev_numbering;
+  int64_t *d_lnode_dof = dn ? p->dn.lnode_dof : X.get<int64_t>(lnodes),
+          *d_ldof_col = dn ? p->dn.ldof_col : X.get<int64_t>(ldofs),
+          *d_ldof_lnode = dn ? p->dn.ldof_lnode : X.get<int64_t>(ldofs),
+          *d_row_ldof = dn ? p->dn.row_ldof : X.get<int64_t>(rows), *d_own = dn ? p->dn.own_ldofs : X.get<int64_t>(owns),
+          *d_ldof_orow = X.get<int64_t>(ldofs), *d_row_lnode = X.get<int64_t>(rows / nc);
   long long *d_rowlen = X.get<long long>(rows);
   int *d_err = X.get<int>(1);
   if (!d_lnode_dof || !d_ldof_col || !d_ldof_lnode || !d_ldof_orow || !d_row_ldof || !d_own || !d_row_lnode ||
       !d_rowlen || !d_err)
     return GDX_ERR_ALLOC;
-  for (int r = 0; r < nparts; ++r) {
+  for (int r = 0; r < nparts && !dn; ++r) {
     const Part &P = p->P[r];
     if (!put(d_lnode_dof + G[r].lnode0, P.lnode_dof) || !put(d_ldof_col + G[r].ldof0, P.ldof_col) ||
         !put(d_ldof_lnode + G[r].ldof0, P.ldof_lnode) || !put(d_row_ldof + G[r].row0, P.row_ldof) ||
         !put(d_own + G[r].own0, P.own_ldofs))
       return GDX_ERR_HIP;
   }
+  symbolic_mark("uploads");
   if (hipMemsetAsync(d_err, 0, sizeof(int), st) != hipSuccess) return GDX_ERR_HIP;
   hipLaunchKernelGGL(k_fill_i64, dim3(grid_of(ldofs)), dim3(kBlock), 0, st, d_ldof_orow, ldofs, (int64_t)-1);
   for (int r = 0; r < nparts; ++r) {
@@ -492,6 +1120,7 @@ int structure_device(gdx_problem *p) {
     (void)hipFree(rowptr);
     return rc;
   }
+  symbolic_mark("row lengths+scan");
   // the part boundaries of the offsets: first row, first ghost row, end
   std::vector<long long> bound(3 * nparts);
   for (int r = 0; r < nparts && !rc; ++r) {
@@ -546,6 +1175,7 @@ int structure_device(gdx_problem *p) {
       hipMemcpy(&npat, d_cnt, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
     return fail(GDX_ERR_HIP);
   if (err) return fail(kUseHost);
+  symbolic_mark("CSR + hash");
   std::vector<unsigned long long> hmin(npat);
   std::vector<unsigned> hslot(npat);
   if (npat && (hipMemcpy(hmin.data(), l_min, npat * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -589,6 +1219,21 @@ int structure_device(gdx_problem *p) {
   std::vector<long long> ogrp0(nparts);
   for (int r = 0; r < nparts; ++r) ogrp0[r] = G[r].ogrp0;
   if (nc == 1) {
+    // the representatives' nodes (their parity fixes the reduced numbering)
+    std::vector<long long> rgrp(npat), rnode(npat);
+    std::vector<int> rpart(npat);
+    for (unsigned i = 0; i < npat; ++i) {
+      const int r = (int)(std::upper_bound(ogrp0.begin(), ogrp0.end(), rep[i]) - ogrp0.begin()) - 1;
+      rpart[i] = r;
+      rgrp[i] = G[r].row0 / nc + (rep[i] - ogrp0[r]);
+    }
+    long long *d_rg = X.get<long long>(npat), *d_rn = X.get<long long>(npat);
+    if (!d_rg || !d_rn) return fail(GDX_ERR_ALLOC);
+    if (!put(d_rg, rgrp)) return fail(GDX_ERR_HIP);
+    if (npat) hipLaunchKernelGGL(k_gather_i64, dim3(grid_of(npat)), dim3(kBlock), 0, st, d_row_lnode, d_rg, (long long)npat, d_rn);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+        (npat && hipMemcpy(rnode.data(), d_rn, npat * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess))
+      return fail(GDX_ERR_HIP);
     const int SP = pattern_stride(S);
     p->pat_fwd.resize((size_t)npat * SP);
     p->pat_inv.resize((size_t)npat * SP);
@@ -601,10 +1246,8 @@ int structure_device(gdx_problem *p) {
         key8[s] = k16[s] == kDir16 ? kSlotDirichlet : k16[s] == kAbs16 ? kSlotAbsent : (uint8_t)k16[s];
         len += k16[s] < kDir16;
       }
-      // the representative row's node (its parity fixes the reduced numbering)
-      const int r = (int)(std::upper_bound(ogrp0.begin(), ogrp0.end(), rep[i]) - ogrp0.begin()) - 1;
-      const Part &P = p->P[r];
-      const int64_t l = P.ldof_lnode[P.row_ldof[rep[i] - ogrp0[r]]];
+      const Part &P = p->P[rpart[i]];
+      const int64_t l = rnode[i];
       const int64_t g[3] = {K * P.loc_lo[0] + l % P.lnn[0], K * P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1],
                             K * P.loc_lo[2] + l / (P.lnn[0] * P.lnn[1])};
       p->pat_dir[i] = scalar_pattern_tables(D, K, key8.data(), S, SP, g, len, &p->pat_fwd[(size_t)i * SP],
@@ -625,6 +1268,7 @@ int structure_device(gdx_problem *p) {
       p->pat_len[i] = len;
     }
   }
+  symbolic_mark("pattern ids");
   // row table
   int *d_dir = X.get<int>(npat);
   RowInfo *rinfo_d = nullptr;
@@ -644,6 +1288,7 @@ int structure_device(gdx_problem *p) {
     (void)hipFree(rinfo_d);
     return fail(kUseHost);
   }
+  symbolic_mark("row table");
   // sizes and offsets of the parts; host copies of the CSRs and the row table
   // are made on demand
   int64_t owned_nnz = 0;

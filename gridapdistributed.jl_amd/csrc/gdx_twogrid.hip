@@ -980,9 +980,11 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
     T->n = n;
     T->nf = nf;
     T->ng = (int)p->ngdofs;
-    std::vector<long long> gid_node(std::max<int64_t>(p->ngdofs, 1), 0);
+    // free global id -> fine lattice node (the problem's lattice: nf = 2n + 1
+    // nodes per direction), built on the device when the numbering is there
+    std::vector<long long> gid_node(p->dev_numbering ? 0 : std::max<int64_t>(p->ngdofs, 1), 0);
     for (const Part &P : p->P)
-      for (int64_t i = 0; i < P.n_ldofs; ++i) {
+      for (int64_t i = 0; i < P.n_ldofs && !p->dev_numbering; ++i) {
         const int64_t l = P.ldof_lnode[i];
         const int64_t g0 = P.loc_lo[0] + l % P.lnn[0];
         const int64_t g1 = P.loc_lo[1] + (l / P.lnn[0]) % P.lnn[1];
@@ -990,7 +992,8 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
         gid_node[P.ldof_gid[i]] = g0 + (int64_t)nf * (g1 + (int64_t)nf * g2);
       }
     const size_t gsz = std::max<int64_t>(p->ngdofs, 1) * sizeof(double);
-    bool ok = hipMalloc(reinterpret_cast<void **>(&T->gid_node), gid_node.size() * sizeof(long long)) == hipSuccess &&
+    bool ok = hipMalloc(reinterpret_cast<void **>(&T->gid_node), std::max<int64_t>(p->ngdofs, 1) * sizeof(long long)) ==
+                  hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->flat), nfl * sizeof(double)) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->dlat), nfl * sizeof(double)) == hipSuccess &&
               hipMalloc(reinterpret_cast<void **>(&T->t1), gsz) == hipSuccess &&
@@ -1010,8 +1013,9 @@ int twogrid_attach(gdx_nn *nn, gdx_problem *p) {
       twogrid_free(nn);
       return GDX_ERR_ALLOC;
     }
-    if (hipMemcpy(T->gid_node, gid_node.data(), gid_node.size() * sizeof(long long), hipMemcpyHostToDevice) !=
-        hipSuccess)
+    if (p->dev_numbering ? gid_node_device(p, T->gid_node) != GDX_OK
+                         : hipMemcpy(T->gid_node, gid_node.data(), gid_node.size() * sizeof(long long),
+                                     hipMemcpyHostToDevice) != hipSuccess)
       return GDX_ERR_HIP;
   }
   int ng = 0;
