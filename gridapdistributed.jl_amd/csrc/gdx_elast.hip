@@ -55,6 +55,30 @@ struct ElastRow {
 };
 static_assert(sizeof(ElastRow) == 32, "ElastRow is 32 bytes");
 
+// the owned row nodes of one part's owned node box, each at its owned row group
+__global__ void k_elast_rows(const RowInfo *__restrict__ rowinfo, long long rinfo0, long long row_base, long long n_own,
+                             long long grp0, long long lo0, long long lo1, long long lo2, long long n0, long long n1,
+                             long long n2, ElastRow *__restrict__ rows, int *err) {
+  const long long li = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (li >= n0 * n1 * n2) return;
+  const RowInfo ri = rowinfo[rinfo0 + li];
+  if (ri.pid < 0) return;
+  const long long o = ri.bidx - row_base;  // the node's first owned row
+  if (o < 0 || o >= n_own || o % 3) {
+    atomicOr(err, 1);
+    return;
+  }
+  ElastRow er{};
+  er.rowstart = ri.rowstart;
+  er.bidx = ri.bidx;
+  er.pid = ri.pid;
+  er.len = ri.lenf & 0xFFFF;
+  er.g[0] = (unsigned short)(lo0 + li % n0);
+  er.g[1] = (unsigned short)(lo1 + (li / n0) % n1);
+  er.g[2] = (unsigned short)(lo2 + li / (n0 * n1));
+  rows[grp0 + o / 3] = er;
+}
+
 struct ElastData {
   double *fac = nullptr;    // [3][nnmax][W][4]: K, M, P, Q at (node, offset + K)
   double *mass = nullptr;   // [3][nnmax]
@@ -166,7 +190,6 @@ void elast_free(gdx_problem *p) {
 // (p->kp: points, weights, basis values and derivatives on [0,1], spacing h)
 int elast_setup(gdx_problem *p) {
   elast_free(p);
-  if (int rc = host_rowinfo(p)) return rc;
   const KParams &kp = p->kp;
   const int K = p->K, W = 2 * K + 1, NQ = p->NQ;
   int nnmax = 0;
@@ -202,8 +225,9 @@ int elast_setup(gdx_problem *p) {
   // row group (parts in order, then owned rows, whose offsets increase)
   std::vector<int64_t> grp0(p->nparts + 1, 0);
   for (int r = 0; r < p->nparts; ++r) grp0[r + 1] = grp0[r] + p->P[r].n_own / 3;
-  std::vector<ElastRow> rows(grp0[p->nparts]);
-  for (int r = 0; r < p->nparts; ++r) {
+  // the row table is in HBM after the device symbolic phase: the list is made there
+  std::vector<ElastRow> rows(p->dev_structure ? 0 : grp0[p->nparts]);
+  for (int r = 0; r < p->nparts && !p->dev_structure; ++r) {
     const Part &P = p->P[r];
     const int64_t n = P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
     for (int64_t li = 0; li < n; ++li) {
@@ -234,7 +258,7 @@ int elast_setup(gdx_problem *p) {
   p->elast = E;
   E->nnmax = nnmax;
   E->stride = p->pat16_stride;
-  E->nrownodes = (long long)rows.size();
+  E->nrownodes = (long long)grp0[p->nparts];
   auto up = [](void **dst, const void *src, size_t bytes) {
     if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
     return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -243,6 +267,32 @@ int elast_setup(gdx_problem *p) {
             up(reinterpret_cast<void **>(&E->mass), mass.data(), mass.size() * sizeof(double)) &&
             up(reinterpret_cast<void **>(&E->pinv), code.data(), code.size() * sizeof(uint16_t)) &&
             up(reinterpret_cast<void **>(&E->rows), rows.data(), rows.size() * sizeof(ElastRow));
+  if (ok && p->dev_structure) {
+    (void)hipFree(E->rows);
+    E->rows = nullptr;
+    int *err = nullptr;
+    ok = hipMalloc(reinterpret_cast<void **>(&E->rows), std::max<int64_t>(grp0[p->nparts], 1) * sizeof(ElastRow)) ==
+             hipSuccess &&
+         hipMalloc(reinterpret_cast<void **>(&err), sizeof(int)) == hipSuccess &&
+         hipMemsetAsync(err, 0, sizeof(int), p->ctx->stream) == hipSuccess;
+    for (int r = 0; r < p->nparts && ok; ++r) {
+      const Part &P = p->P[r];
+      const long long n = P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+      if (n > 0)
+        hipLaunchKernelGGL(k_elast_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->ctx->stream,
+                           p->dev.rowinfo, (long long)P.rowinfo_base, (long long)P.row_base, (long long)P.n_own,
+                           (long long)grp0[r], (long long)P.onb_lo[0], (long long)P.onb_lo[1], (long long)P.onb_lo[2],
+                           (long long)P.onb_n[0], (long long)P.onb_n[1], (long long)P.onb_n[2], E->rows, err);
+    }
+    int herr = 0;
+    ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(p->ctx->stream) == hipSuccess &&
+         hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    if (err) (void)hipFree(err);
+    if (ok && herr) {
+      elast_free(p);
+      return GDX_ERR_STATE;
+    }
+  }
   if (!ok) {
     elast_free(p);
     return GDX_ERR_ALLOC;

@@ -179,109 +179,180 @@ __global__ void k_scan_add(long long *out, long long n, const long long *offs) {
 
 __global__ void k_copy1(long long *dst, const long long *src) { *dst = *src; }
 
-int scan_exclusive(const long long *in, long long *out, long long n, hipStream_t st) {
+// Launches of the scan; ws holds the block sums and their scans of every
+// level (scan_workspace(n) values).  Stream-ordered, no synchronisation.
+long long scan_workspace(long long n) {
   const long long nb = std::max<long long>((n + kScanTile - 1) / kScanTile, 1);
-  long long *sums = nullptr, *offs = nullptr;
-  if (hipMalloc(&sums, nb * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
-  if (hipMalloc(&offs, (nb + 1) * sizeof(long long)) != hipSuccess) {
-    (void)hipFree(sums);
-    return GDX_ERR_ALLOC;
-  }
-  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kBlock), 0, st, in, out, n, sums);
-  int rc = GDX_OK;
-  if (nb > 1) {
-    rc = scan_exclusive(sums, offs, nb, st);
-    if (!rc) hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, out, n, offs);
-  }
-  // total: the single block's sum, or the scan of the block sums' total
-  if (!rc) hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, st, out + n, nb > 1 ? offs + nb : sums);
-  rc = rc ? rc : (hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP);
-  (void)hipStreamSynchronize(st);
-  (void)hipFree(sums);
-  (void)hipFree(offs);
-  return rc;
+  return nb > 1 ? 2 * nb + 1 + scan_workspace(nb) : 2;
 }
 
-// One row group (the nc rows of a node) per LP lanes, one lane per (stencil
-// slot, component) entry: its column, its rank among the row's columns, the
-// column ids of the group's rows and, for owned rows, the position key and
-// its order-free hash.
+void scan_launch(const long long *in, long long *out, long long n, long long *ws, hipStream_t st) {
+  const long long nb = std::max<long long>((n + kScanTile - 1) / kScanTile, 1);
+  long long *sums = ws, *offs = ws + nb;
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kBlock), 0, st, in, out, n, sums);
+  if (nb > 1) {
+    scan_launch(sums, offs, nb, ws + 2 * nb + 1, st);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, out, n, offs);
+  }
+  // total: the single block's sum, or the scan of the block sums' total
+  hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, st, out + n, nb > 1 ? offs + nb : sums);
+}
+
+// workspaces of the scans of one symbolic phase, grown as needed
+struct ScanWs {
+  long long *ws = nullptr;
+  long long cap = 0;
+  ~ScanWs() {
+    if (ws) (void)hipFree(ws);
+  }
+};
+
+int scan_exclusive(const long long *in, long long *out, long long n, hipStream_t st, ScanWs &W) {
+  const long long need = scan_workspace(n);
+  if (need > W.cap) {
+    // the previous workspace may still be read by queued scans
+    if (W.ws && (hipStreamSynchronize(st) != hipSuccess || hipFree(W.ws) != hipSuccess)) return GDX_ERR_HIP;
+    W.ws = nullptr;
+    W.cap = 0;
+    if (hipMalloc(&W.ws, need * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+    W.cap = need;
+  }
+  scan_launch(in, out, n, W.ws, st);
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+// One row group (the nc rows of a node) per LP lanes, one lane per stencil
+// node: its columns, their positions in the row, the column ids of the group's
+// rows and, for owned rows, the position key and its order-free hash.  The
+// own columns come first, in the order of their nodes (an own node's nc
+// columns are consecutive), then the ghost columns by id; a column's position
+// counts the smaller columns of its kind in the row's compact lists.
+constexpr int kMaxComp = 3;
+
 template <int LP, int BS>
 __global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_t *row_lnode,
                                                   const int64_t *lnode_dof, const int64_t *ldof_col,
                                                   const long long *rowptr, int *colidx, uint16_t *keys, int KS,
                                                   unsigned long long *hash, int *err) {
   constexpr int GPB = BS / LP;
-  __shared__ int colv[BS];
+  __shared__ int ownl[BS];             // per group: first columns of the own nodes
+  __shared__ int ghl[BS * kMaxComp];   // per group: ghost columns
+  __shared__ int nown[GPB], ngh[GPB];
   __shared__ unsigned long long hs[GPB];
-  const int grp = threadIdx.x / LP, e = threadIdx.x % LP;
+  const int grp = threadIdx.x / LP, s = threadIdx.x % LP, nc = m.nc;
   const long long q = (long long)blockIdx.x * GPB + grp;
-  const bool live = q < P.nrows / m.nc;
-  const int W = 2 * m.K + 1, S = m.D == 3 ? W * W * W : W * W, SN = S * m.nc;
-  if (threadIdx.x < GPB) hs[threadIdx.x] = 0;
-  int col = INT_MAX;
-  uint16_t key = kAbs16;
-  bool own = false;
-  if (live) {
-    own = q * m.nc < P.n_own;
-    if (e < SN) {
-      long long g[3], jlo[3], jhi[3];
-      node_of(m, P, row_lnode[q], g);
-      row_ranges(m, P, g, own, jlo, jhi);
-      const int s = e / m.nc, k = e % m.nc;
-      const long long j0 = g[0] + s % W - m.K, j1 = g[1] + (s / W) % W - m.K,
-                      j2 = g[2] + (m.D == 3 ? s / (W * W) - m.K : 0);
-      if (j0 >= jlo[0] && j0 <= jhi[0] && j1 >= jlo[1] && j1 <= jhi[1] && j2 >= jlo[2] && j2 <= jhi[2]) {
-        if (dirichlet(m, j0, j1, j2)) {
-          key = kDir16;
+  const bool live = q < P.nrows / nc;
+  const int W = 2 * m.K + 1, S = m.D == 3 ? W * W * W : W * W, SN = S * nc;
+  if (threadIdx.x < GPB) {
+    hs[threadIdx.x] = 0;
+    nown[threadIdx.x] = 0;
+    ngh[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  int kind = 0;  // 0 absent, 1 Dirichlet, 2 own node, 3 ghost node
+  int c0 = 0, ck[kMaxComp] = {0, 0, 0};
+  const bool own_row = live && q * nc < P.n_own;
+  if (live && s < S) {
+    long long g[3], jlo[3], jhi[3];
+    node_of(m, P, row_lnode[q], g);
+    row_ranges(m, P, g, own_row, jlo, jhi);
+    const long long j0 = g[0] + s % W - m.K, j1 = g[1] + (s / W) % W - m.K,
+                    j2 = g[2] + (m.D == 3 ? s / (W * W) - m.K : 0);
+    if (j0 >= jlo[0] && j0 <= jhi[0] && j1 >= jlo[1] && j1 <= jhi[1] && j2 >= jlo[2] && j2 <= jhi[2]) {
+      if (dirichlet(m, j0, j1, j2)) {
+        kind = 1;
+      } else {
+        // the same test as the row length's (the node's first column), so a
+        // row never holds more columns than were counted for it
+        const long long lj = lnode_in(m, P, j0, j1, j2);
+        const long long d0 = lj >= 0 ? lnode_dof[lj] : -1;
+        const long long cc = d0 >= 0 ? ldof_col[d0] : -1;
+        if (cc < 0 || cc > INT_MAX - kMaxComp) {
+          atomicOr(err, 1);
+        } else if (cc < P.n_own) {
+          kind = 2;
+          c0 = (int)cc;
+          for (int k = 1; k < nc; ++k)
+            if (ldof_col[d0 + k] != cc + k) atomicOr(err, 1);
+          ownl[grp * LP + atomicAdd(&nown[grp], 1)] = c0;
         } else {
-          // the same test as the row length's (the node's first column), so a
-          // row never holds more columns than were counted for it
-          const long long lj = lnode_in(m, P, j0, j1, j2);
-          const long long d0 = lj >= 0 ? lnode_dof[lj] : -1;
-          const long long c0 = d0 >= 0 ? ldof_col[d0] : -1;
-          const long long c = c0 >= 0 ? ldof_col[d0 + k] : -1;
-          if (c < 0 || c > INT_MAX - 1) atomicOr(err, 1);
-          else col = (int)c;
+          kind = 3;
+          for (int k = 0; k < nc; ++k) {
+            const long long c = ldof_col[d0 + k];
+            if (c < P.n_own || c > INT_MAX - 1) atomicOr(err, 1);
+            ck[k] = (int)c;
+            ghl[grp * LP * kMaxComp + atomicAdd(&ngh[grp], 1)] = ck[k];
+          }
         }
       }
     }
   }
-  colv[threadIdx.x] = col;
   __syncthreads();
-  if (col != INT_MAX) {
-    const int *cv = colv + grp * LP;
-    int rank = 0;
-    for (int i = 0; i < SN; ++i) rank += cv[i] < col;
-    key = (uint16_t)rank;
-    const long long r0 = P.row0 + q * m.nc;
-    for (int kk = 0; kk < m.nc; ++kk) colidx[rowptr[r0 + kk] + rank] = col;
+  uint16_t key[kMaxComp] = {kAbs16, kAbs16, kAbs16};
+  const long long r0 = P.row0 + q * nc;
+  if (kind == 1) {
+    for (int k = 0; k < nc; ++k) key[k] = kDir16;
+  } else if (kind == 2) {
+    const int *ol = ownl + grp * LP;
+    int r = 0;
+    for (int i = 0; i < nown[grp]; ++i) r += ol[i] < c0;
+    for (int k = 0; k < nc; ++k) {
+      key[k] = (uint16_t)(nc * r + k);
+      for (int kk = 0; kk < nc; ++kk) colidx[rowptr[r0 + kk] + nc * r + k] = c0 + k;
+    }
+  } else if (kind == 3) {
+    const int *gl = ghl + grp * LP * kMaxComp, ng = ngh[grp], base = nc * nown[grp];
+    for (int k = 0; k < nc; ++k) {
+      int r = 0;
+      for (int i = 0; i < ng; ++i) r += gl[i] < ck[k];
+      key[k] = (uint16_t)(base + r);
+      for (int kk = 0; kk < nc; ++kk) colidx[rowptr[r0 + kk] + base + r] = ck[k];
+    }
   }
-  if (live && own && e < KS) {
-    keys[(P.ogrp0 + q) * KS + e] = key;  // padding entries read as absent
-    if (e < SN) atomicAdd(&hs[grp], mix64(((unsigned long long)e << 16) | key));
+  if (own_row && s < S) {
+    unsigned long long h = 0;
+    for (int k = 0; k < nc; ++k) {
+      const int e = s * nc + k;
+      keys[(P.ogrp0 + q) * KS + e] = key[k];
+      h += mix64(((unsigned long long)e << 16) | key[k]);
+    }
+    atomicAdd(&hs[grp], h);
+    if (s == 0)
+      for (int e = SN; e < KS; ++e) keys[(P.ogrp0 + q) * KS + e] = kAbs16;  // padding reads as absent
   }
   __syncthreads();
-  if (live && own && e == 0) hash[P.ogrp0 + q] = hs[grp] | 1ull;  // 0 marks an empty table slot
+  if (own_row && s == 0) hash[P.ogrp0 + q] = hs[grp] | 1ull;  // 0 marks an empty table slot
 }
 
-// first owned row group of every distinct hash (open addressing, linear probing)
-__global__ void k_insert(const unsigned long long *hash, long long n, unsigned long long *tkey,
-                         unsigned long long *tmin, unsigned *slot, int *err) {
+// first owned row group of every distinct hash (open addressing, linear
+// probing).  Neighbouring rows mostly share a pattern: the lanes of a wave with
+// the same hash elect their first lane, which alone probes the table and takes
+// the minimum (the wave's smallest row of that hash), and passes the slot on.
+__global__ __launch_bounds__(256) void k_insert(const unsigned long long *hash, long long n, unsigned long long *tkey,
+                                                unsigned long long *tmin, unsigned *slot, int *err) {
   const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const unsigned long long h = hash[q];
+  const bool live = q < n;
+  const unsigned long long h = live ? hash[q] : 0ull;
+  const int lane = (int)(threadIdx.x & 63);
+  int leader = lane;
+  for (int l = 63; l >= 0; --l)
+    if (__shfl(h, l) == h) leader = l;  // the lowest lane with this hash
   unsigned s = (unsigned)(h >> 1) & (kTableSize - 1);
-  for (unsigned t = 0; t < kTableSize; ++t) {
-    const unsigned long long old = atomicCAS(&tkey[s], 0ull, h);
-    if (old == 0ull || old == h) {
-      atomicMin(&tmin[s], (unsigned long long)q);
-      slot[q] = s;
-      return;
+  if (live && leader == lane) {
+    bool placed = false;
+    for (unsigned t = 0; t < kTableSize && !placed; ++t) {
+      const unsigned long long old = atomicCAS(&tkey[s], 0ull, h);
+      if (old == 0ull || old == h) {
+        if (tmin[s] > (unsigned long long)q) atomicMin(&tmin[s], (unsigned long long)q);
+        placed = true;
+      } else {
+        s = (s + 1) & (kTableSize - 1);
+      }
     }
-    s = (s + 1) & (kTableSize - 1);
+    if (!placed) atomicOr(err, 2);  // table full
   }
-  atomicOr(err, 2);  // table full
+  s = (unsigned)__shfl((int)s, leader);
+  if (live) slot[q] = s;
 }
 
 __global__ void k_collect(const unsigned long long *tkey, const unsigned long long *tmin, unsigned *cnt,
@@ -370,6 +441,59 @@ __global__ void k_tile_patterns(const Tile *cand, int ncand, int cap, RowInfo *r
       }
   npat[t] = n;
   any[t] = has;
+}
+
+// The same with one workgroup per candidate tile, for pattern counts that fit
+// in LDS: every pattern's first row in the tile (atomic minimum), then each
+// distinct pattern's rank among those first rows is its tile-local index.
+constexpr int kTileThreads = 256;
+
+__global__ __launch_bounds__(kTileThreads) void k_tile_patterns_wg(const Tile *cand, int ncand, int cap, int npat_all,
+                                                                   RowInfo *rowinfo, int *lists, int *npat, int *any,
+                                                                   int *err) {
+  extern __shared__ int lds[];
+  int *first = lds, *dl = lds + npat_all, *rk = lds + 2 * npat_all;
+  __shared__ int cnt;
+  const int t = blockIdx.x;
+  if (t >= ncand) return;
+  const Tile T = cand[t];
+  const int nrow = T.ext[0] * T.ext[1] * T.ext[2];
+  for (int p = threadIdx.x; p < npat_all; p += kTileThreads) first[p] = INT_MAX;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  auto entry = [&](int i) -> RowInfo & {
+    const int x = i % T.ext[0], y = (i / T.ext[0]) % T.ext[1], z = i / (T.ext[0] * T.ext[1]);
+    const long long li = ((long long)(T.org[2] - T.onb_lo[2] + z) * T.onb_n[1] + (T.org[1] - T.onb_lo[1] + y)) *
+                             T.onb_n[0] +
+                         (T.org[0] - T.onb_lo[0] + x);
+    return rowinfo[T.rowinfo_base + li];
+  };
+  for (int i = threadIdx.x; i < nrow; i += kTileThreads) {
+    const int pid = entry(i).pid;
+    if (pid >= 0) atomicMin(&first[pid], i);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < npat_all; p += kTileThreads)
+    if (first[p] != INT_MAX) dl[atomicAdd(&cnt, 1)] = p;
+  __syncthreads();
+  const int n = cnt;
+  for (int u = threadIdx.x; u < n; u += kTileThreads) {
+    const int pu = dl[u], fu = first[pu];
+    int r = 0;
+    for (int v = 0; v < n; ++v) r += first[dl[v]] < fu;
+    rk[pu] = r;
+    lists[(long long)t * cap + r] = pu;
+  }
+  __syncthreads();
+  if (n > (1 << 14)) atomicOr(err, 16);
+  for (int i = threadIdx.x; i < nrow; i += kTileThreads) {
+    RowInfo &ri = entry(i);
+    if (ri.pid >= 0) ri.lenf = (ri.lenf & 0x1FFFF) | (rk[ri.pid] << 17);
+  }
+  if (threadIdx.x == 0) {
+    npat[t] = n;
+    any[t] = n > 0;
+  }
 }
 
 __global__ void k_tile_write(const int *lists, int cap, const int *npat, const long long *off, int ncand,
@@ -681,9 +805,27 @@ __global__ void k_gid_node(MeshG m, PartN P, const int64_t *ldof_lnode, const in
   if (gid >= 0 && gid < ngdofs) gid_node[gid] = g[0] + m.nn[0] * (g[1] + m.nn[1] * g[2]);
 }
 
-template <class T>
-int get_at(const T *src, long long i, T &v) {
-  return hipMemcpy(&v, src + i, sizeof(T), hipMemcpyDeviceToHost) == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+__global__ void k_gather_ll(const long long *src, const long long *idx, int n, long long *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+// src[idx[i]] for every i, read back with one copy (stream-ordered)
+int read_at(const long long *src, const std::vector<long long> &idx, std::vector<long long> &out, hipStream_t st) {
+  const int n = (int)idx.size();
+  out.assign(n, 0);
+  if (n == 0) return GDX_OK;
+  long long *d = nullptr;
+  if (hipMalloc(&d, 2 * n * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+  int rc = GDX_OK;
+  if (hipMemcpyAsync(d, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice, st) != hipSuccess) rc = GDX_ERR_HIP;
+  if (!rc) hipLaunchKernelGGL(k_gather_ll, dim3((n + 255) / 256), dim3(256), 0, st, src, d, n, d + n);
+  if (!rc && (hipGetLastError() != hipSuccess ||
+              hipMemcpyAsync(out.data(), d + n, n * sizeof(long long), hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess))
+    rc = GDX_ERR_HIP;
+  (void)hipFree(d);
+  return rc;
 }
 
 MeshG mesh_of(const gdx_problem *p) {
@@ -759,6 +901,7 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
     T.fidx[F[f][0] + 3 * F[f][1] + 9 * F[f][2]] = f;
   }
   Scratch X;
+  ScanWs SW;
   int *d_err = X.get<int>(1);
   std::vector<int> odim;
   long long odoff[3];
@@ -792,13 +935,16 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
   if (efs > 0) {
     for (int r = 0; r < nparts; ++r)
       hipLaunchKernelGGL(k_ef_flags, dim3(grid_of(V[r].ncl * (T.nE + T.nF))), dim3(kBlock), 0, st, m, V[r], T, efl);
-    if ((rc = scan_exclusive(efl, efscan, efs, st))) return rc;
+    if ((rc = scan_exclusive(efl, efscan, efs, st, SW))) return rc;
   }
-  std::vector<long long> nef(nparts, 0);
+  std::vector<long long> nef(nparts, 0), at, val;
+  for (int r = 0; r < nparts && efs > 0; ++r) at.push_back(V[r].ef0);
+  if (efs > 0) {
+    at.push_back(efs);
+    if ((rc = read_at(efscan, at, val, st))) return rc;
+  }
   for (int r = 0; r < nparts && efs > 0; ++r) {
-    long long a, b;
-    if ((rc = get_at(efscan, V[r].ef0, a)) || (rc = get_at(efscan, V[r].ef0 + V[r].ncl * (T.nE + T.nF), b))) return rc;
-    nef[r] = b - a;
+    nef[r] = val[r + 1] - val[r];
     const long long nv = (V[r].lc[0] + 1) * (V[r].lc[1] + 1) * (D == 3 ? V[r].lc[2] + 1 : 1);
     if (nv + nef[r] + V[r].ncl != V[r].nlnode) return kUseHost;  // the order must cover every node once
   }
@@ -809,12 +955,13 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
     hipLaunchKernelGGL(k_node_pos, dim3(grid_of(V[r].nlnode)), dim3(kBlock), 0, st, m, V[r], T, efscan, nef[r], pos,
                        freef);
   if (hipGetLastError() != hipSuccess) return GDX_ERR_HIP;
-  if ((rc = scan_exclusive(freef, fscan, nodes, st))) return rc;
+  if ((rc = scan_exclusive(freef, fscan, nodes, st, SW))) return rc;
   // 2. dofs of the free nodes (nc consecutive per node)
   long long ldofs = 0;
-  std::vector<long long> fb(nparts + 1);
-  for (int r = 0; r <= nparts; ++r)
-    if ((rc = get_at(fscan, r < nparts ? V[r].lnode0 : nodes, fb[r]))) return rc;
+  std::vector<long long> fb;
+  at.clear();
+  for (int r = 0; r <= nparts; ++r) at.push_back(r < nparts ? V[r].lnode0 : nodes);
+  if ((rc = read_at(fscan, at, fb, st))) return rc;
   for (int r = 0; r < nparts; ++r) {
     Part &P = p->P[r];
     P.n_ldofs = nc * (fb[r + 1] - fb[r]);
@@ -859,13 +1006,14 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
     hipLaunchKernelGGL(k_owners, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, m, V[r], O, dn.ldof_lnode,
                        dn.ldof_owner, ownf);
   if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
-  if ((rc = scan_exclusive(ownf, oscan, ldofs, st))) return fail(rc);
+  if ((rc = scan_exclusive(ownf, oscan, ldofs, st, SW))) return fail(rc);
   long long owns = 0;
+  at.clear();
+  for (int r = 0; r <= nparts; ++r) at.push_back(r < nparts ? V[r].ldof0 : ldofs);
+  if ((rc = read_at(oscan, at, val, st))) return fail(rc);
   for (int r = 0; r < nparts; ++r) {
-    long long a, b;
-    if ((rc = get_at(oscan, V[r].ldof0, a)) || (rc = get_at(oscan, V[r].ldof0 + V[r].nldof, b))) return fail(rc);
     Part &P = p->P[r];
-    P.n_own = b - a;
+    P.n_own = val[r + 1] - val[r];
     P.own_base = owns;
     V[r].n_own = P.n_own;
     V[r].own0 = owns;
@@ -883,14 +1031,13 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
     hipLaunchKernelGGL(k_ghost_ids, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, m, d_parts, r, fa ? 1 : 0,
                        dn.ldof_owner, dn.ldof_lnode, dn.lnode_dof, dn.ldof_gid, grf, d_err);
   if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
-  if ((rc = scan_exclusive(grf, grscan, ldofs, st))) return fail(rc);
+  if ((rc = scan_exclusive(grf, grscan, ldofs, st, SW))) return fail(rc);
   // 4. row PRange: owned dofs, then the ghost rows, each in local id order
   long long rows = 0;
+  if ((rc = read_at(grscan, at, val, st))) return fail(rc);
   for (int r = 0; r < nparts; ++r) {
-    long long a, b;
-    if ((rc = get_at(grscan, V[r].ldof0, a)) || (rc = get_at(grscan, V[r].ldof0 + V[r].nldof, b))) return fail(rc);
     Part &P = p->P[r];
-    P.n_ghost_rows = b - a;
+    P.n_ghost_rows = val[r + 1] - val[r];
     P.nrows = P.n_own + P.n_ghost_rows;
     P.row_base = rows;
     V[r].row0 = rows;
@@ -959,13 +1106,14 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
   for (int r = 0; r < nparts; ++r)
     hipLaunchKernelGGL(k_key_flags, dim3(grid_of(V[r].nldof)), dim3(kBlock), 0, st, V[r], minkey, kf);
   if (hipGetLastError() != hipSuccess) return fail(GDX_ERR_HIP);
-  if ((rc = scan_exclusive(kf, kscan, keys, st))) return fail(rc);
+  if ((rc = scan_exclusive(kf, kscan, keys, st, SW))) return fail(rc);
   long long cols = 0;
+  at.clear();
+  for (int r = 0; r <= nparts; ++r) at.push_back(r < nparts ? V[r].key0 : keys);
+  if ((rc = read_at(kscan, at, val, st))) return fail(rc);
   for (int r = 0; r < nparts; ++r) {
-    long long a, b;
-    if ((rc = get_at(kscan, V[r].key0, a)) || (rc = get_at(kscan, V[r].key0 + V[r].nkey, b))) return fail(rc);
     Part &P = p->P[r];
-    P.n_ghost_cols = b - a;
+    P.n_ghost_cols = val[r + 1] - val[r];
     P.ncols = P.n_own + P.n_ghost_cols;
     P.col_base = cols;
     V[r].col0 = cols;
@@ -1046,9 +1194,9 @@ int structure_device(gdx_problem *p) {
   m.nc = nc;
   m.dir_kind = p->desc.dir_kind;
   const int W = 2 * K + 1, S = D == 3 ? W * W * W : W * W, SN = S * nc, KS = (SN + 3) & ~3;
-  int LP = 32;
-  while (LP < KS) LP *= 2;
-  if (LP > 512) return kUseHost;
+  int LP = 32;  // lanes per row group: one per stencil node
+  while (LP < S) LP *= 2;
+  if (LP > 128 || nc > kMaxComp) return kUseHost;
   std::vector<PartG> G(nparts);
   long long rows = 0, ogrps = 0, rinfo = 0, lnodes = 0, ldofs = 0, owns = 0;
   for (int r = 0; r < nparts; ++r) {
@@ -1115,22 +1263,20 @@ int structure_device(gdx_problem *p) {
   // global one minus its first value
   long long *rowptr = nullptr;
   if (hipMalloc(&rowptr, (rows + 1) * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
-  int rc = scan_exclusive(d_rowlen, rowptr, rows, st);
+  ScanWs SW;
+  int rc = scan_exclusive(d_rowlen, rowptr, rows, st, SW);
   if (rc) {
     (void)hipFree(rowptr);
     return rc;
   }
   symbolic_mark("row lengths+scan");
   // the part boundaries of the offsets: first row, first ghost row, end
-  std::vector<long long> bound(3 * nparts);
-  for (int r = 0; r < nparts && !rc; ++r) {
-    const long long at[3] = {G[r].row0, G[r].row0 + G[r].n_own, G[r].row0 + G[r].nrows};
-    for (int k = 0; k < 3 && !rc; ++k)
-      if (hipMemcpy(&bound[3 * r + k], rowptr + at[k], sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = GDX_ERR_HIP;
-  }
-  long long nnz = 0;
-  if (!rc && hipMemcpy(&nnz, rowptr + rows, sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) rc = GDX_ERR_HIP;
+  std::vector<long long> bound, at;
+  for (int r = 0; r < nparts; ++r)
+    for (long long a : {G[r].row0, G[r].row0 + G[r].n_own, G[r].row0 + G[r].nrows}) at.push_back(a);
+  at.push_back(rows);
+  rc = read_at(rowptr, at, bound, st);
+  const long long nnz = rc ? 0 : bound[3 * nparts];
   int *colidx = nullptr;
   if (!rc && hipMalloc(&colidx, std::max<long long>(nnz, 1) * sizeof(int)) != hipSuccess) rc = GDX_ERR_ALLOC;
   uint16_t *keys = rc ? nullptr : X.get<uint16_t>(ogrps * KS);
@@ -1152,9 +1298,7 @@ int structure_device(gdx_problem *p) {
     switch (LP) {
       case 32: launch_fill<32>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
       case 64: launch_fill<64>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
-      case 128: launch_fill<128>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
-      case 256: launch_fill<256>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
-      default: launch_fill<512>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
+      default: launch_fill<128>(m, g, st, rl, ldn, ldc, rowptr, colidx, keys, KS, hash, d_err); break;
     }
   }
   // distinct keys: first occurrences through the hash table
@@ -1352,7 +1496,12 @@ int build_tiles_device(gdx_problem *p, const int R[3]) {
   long long *d_off = X.get<long long>(nc_);
   if (!d_cand || !lists || !d_npat || !d_any || !d_err || !d_off) return GDX_ERR_ALLOC;
   if (!put(d_cand, cand) || hipMemsetAsync(d_err, 0, sizeof(int), st) != hipSuccess) return GDX_ERR_HIP;
-  if (nc_ > 0)
+  // the workgroup form when three ints per pattern fit in 64 KB of LDS
+  const size_t lds = (size_t)3 * p->npat * sizeof(int);
+  if (nc_ > 0 && lds <= 65536)
+    hipLaunchKernelGGL(k_tile_patterns_wg, dim3(nc_), dim3(kTileThreads), lds, st, d_cand, nc_, cap, p->npat,
+                       p->dev.rowinfo, lists, d_npat, d_any, d_err);
+  else if (nc_ > 0)
     hipLaunchKernelGGL(k_tile_patterns, dim3((nc_ + 63) / 64), dim3(64), 0, st, d_cand, nc_, cap, p->dev.rowinfo,
                        lists, d_npat, d_any, d_err);
   std::vector<int> npat(nc_), any(nc_);
