@@ -21,6 +21,22 @@ bool matrix_kernel_supported(int dim, int K, int NQ);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+static thread_local long g_fault_alloc = -1;  // allocations left before the injected failure
+FaultScope::FaultScope() {
+  const char *e = std::getenv("GDX_FAULT_ALLOC");
+  g_fault_alloc = (e && *e) ? std::atol(e) : -1;
+}
+FaultScope::~FaultScope() { g_fault_alloc = -1; }
+hipError_t dev_malloc(void **p, size_t bytes) {
+  if (g_fault_alloc == 0) {
+    g_fault_alloc = -1;
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  if (g_fault_alloc > 0) --g_fault_alloc;
+  return hipMalloc(p, bytes);
+}
 }  // namespace gdx
 
 using gdx::set_error;
@@ -51,7 +67,7 @@ int upload(T **dst, const std::vector<T> &src) {
     *dst = nullptr;
   }
   size_t n = src.size() ? src.size() : 1;
-  GDX_HIP(hipMalloc(reinterpret_cast<void **>(dst), n * sizeof(T)));
+  GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(dst), n * sizeof(T)));
   if (!src.empty()) GDX_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
   return GDX_OK;
 }
@@ -270,6 +286,7 @@ int gdx_problem_destroy(gdx_problem *p) {
 
 int gdx_symbolic(gdx_problem *p) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  const gdx::FaultScope fault_scope;
   if (p->ctx) {
     GDX_HIP(hipSetDevice(p->ctx->device));
     // device buffers are re-made (the device stage of the symbolic phase
@@ -303,15 +320,15 @@ int gdx_symbolic(gdx_problem *p) {
   // kappa and f at the quadrature points (constant-coefficient elasticity has none)
   const size_t nq = elast ? 1 : (size_t)p->kp.qn[0] * p->kp.qn[1] * p->kp.qn[2];
   int qnmax = std::max(p->kp.qn[0], std::max(p->kp.qn[1], p->kp.qn[2]));
-  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.W), nq * sizeof(double)));
-  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.F), nq * sizeof(double)));
+  GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(&d.W), nq * sizeof(double)));
+  GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(&d.F), nq * sizeof(double)));
   if (p->desc.nmodes > 0) {
-    GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.E), (size_t)2 * 3 * p->desc.nmodes * qnmax * sizeof(double)));
+    GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(&d.E), (size_t)2 * 3 * p->desc.nmodes * qnmax * sizeof(double)));
     int rc = upload(&d.modes, p->modes);
     if (rc) return rc;
   }
-  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.values), std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
-  GDX_HIP(hipMalloc(reinterpret_cast<void **>(&d.b), std::max<int64_t>(p->rows_total, 1) * sizeof(double)));
+  GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(&d.values), std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
+  GDX_HIP(gdx::dev_malloc(reinterpret_cast<void **>(&d.b), std::max<int64_t>(p->rows_total, 1) * sizeof(double)));
   // ghost rows are never written by the numeric phase: they stay zero
   GDX_HIP(hipMemset(d.values, 0, std::max<int64_t>(p->nnz_total, 1) * sizeof(double)));
   GDX_HIP(hipMemset(d.b, 0, std::max<int64_t>(p->rows_total, 1) * sizeof(double)));

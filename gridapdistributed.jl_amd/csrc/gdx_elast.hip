@@ -260,7 +260,7 @@ int elast_setup(gdx_problem *p) {
   E->stride = p->pat16_stride;
   E->nrownodes = (long long)grp0[p->nparts];
   auto up = [](void **dst, const void *src, size_t bytes) {
-    if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    if (dev_malloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
     return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
   bool ok = up(reinterpret_cast<void **>(&E->fac), fac.data(), fac.size() * sizeof(double)) &&
@@ -271,9 +271,9 @@ int elast_setup(gdx_problem *p) {
     (void)hipFree(E->rows);
     E->rows = nullptr;
     int *err = nullptr;
-    ok = hipMalloc(reinterpret_cast<void **>(&E->rows), std::max<int64_t>(grp0[p->nparts], 1) * sizeof(ElastRow)) ==
+    ok = dev_malloc(reinterpret_cast<void **>(&E->rows), std::max<int64_t>(grp0[p->nparts], 1) * sizeof(ElastRow)) ==
              hipSuccess &&
-         hipMalloc(reinterpret_cast<void **>(&err), sizeof(int)) == hipSuccess &&
+         dev_malloc(reinterpret_cast<void **>(&err), sizeof(int)) == hipSuccess &&
          hipMemsetAsync(err, 0, sizeof(int), p->ctx->stream) == hipSuccess;
     for (int r = 0; r < p->nparts && ok; ++r) {
       const Part &P = p->P[r];

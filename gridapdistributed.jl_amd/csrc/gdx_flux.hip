@@ -445,16 +445,16 @@ int flux_setup(gdx_problem *p) {
   const int64_t ncm = std::max<int64_t>((int64_t)(nc * (nc + 1) / 2 + nc) * fd->ncell, 1);
   hipStream_t st = p->ctx->stream;
   const int64_t nx = std::max<int64_t>(p->ngdofs, 1);
-  if (hipMalloc(reinterpret_cast<void **>(&fd->cm), ncm * sizeof(double)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&fd->gid_node), std::max<int64_t>(p->ngdofs, 1) * sizeof(long long)) !=
+  if (dev_malloc(reinterpret_cast<void **>(&fd->cm), ncm * sizeof(double)) != hipSuccess ||
+      dev_malloc(reinterpret_cast<void **>(&fd->ulat), nnodes * sizeof(double)) != hipSuccess ||
+      dev_malloc(reinterpret_cast<void **>(&fd->gid_node), std::max<int64_t>(p->ngdofs, 1) * sizeof(long long)) !=
           hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&fd->xs), nx * sizeof(double)) != hipSuccess) {
+      dev_malloc(reinterpret_cast<void **>(&fd->xs), nx * sizeof(double)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }
   const std::vector<double> tab = flux_tables_host(p->kp, p->D, p->NQ);
-  if (hipMalloc(reinterpret_cast<void **>(&fd->tab), tab.size() * sizeof(double)) != hipSuccess) {
+  if (dev_malloc(reinterpret_cast<void **>(&fd->tab), tab.size() * sizeof(double)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }

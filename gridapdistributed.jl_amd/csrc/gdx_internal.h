@@ -208,6 +208,19 @@ void unregister(std::vector<T *> &v, T *x) {
     }
 }
 int symbolic(gdx_problem *p);  // symbolic phase (gdx_symbolic.cpp)
+// The device allocations of the symbolic phase go through dev_malloc: it is
+// hipMalloc, except that GDX_FAULT_ALLOC=n, read when gdx_symbolic starts
+// (FaultScope), makes the n-th of that call's allocations (from 0) fail as
+// exhausted HBM would, once, so that the error paths can be tested.
+hipError_t dev_malloc(void **p, size_t bytes);
+template <class T>
+hipError_t dev_malloc(T **p, size_t bytes) {
+  return dev_malloc(reinterpret_cast<void **>(p), bytes);
+}
+struct FaultScope {
+  FaultScope();
+  ~FaultScope();
+};
 // device stage of the symbolic phase (gdx_symbolic_dev.hip): CSR, position
 // patterns and row table from the host numbering and PRanges.  Returns
 // GDX_OK, a GDX_ERR_* code, or kUseHost when the host stage must build them.

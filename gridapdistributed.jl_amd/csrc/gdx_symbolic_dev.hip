@@ -200,10 +200,14 @@ void scan_launch(const long long *in, long long *out, long long n, long long *ws
 
 // workspaces of the scans of one symbolic phase, grown as needed
 struct ScanWs {
+  hipStream_t st;
   long long *ws = nullptr;
   long long cap = 0;
+  explicit ScanWs(hipStream_t s) : st(s) {}
   ~ScanWs() {
-    if (ws) (void)hipFree(ws);
+    if (!ws) return;
+    (void)hipStreamSynchronize(st);  // queued scans may still read it
+    (void)hipFree(ws);
   }
 };
 
@@ -214,7 +218,7 @@ int scan_exclusive(const long long *in, long long *out, long long n, hipStream_t
     if (W.ws && (hipStreamSynchronize(st) != hipSuccess || hipFree(W.ws) != hipSuccess)) return GDX_ERR_HIP;
     W.ws = nullptr;
     W.cap = 0;
-    if (hipMalloc(&W.ws, need * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+    if (dev_malloc(&W.ws, need * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
     W.cap = need;
   }
   scan_launch(in, out, n, W.ws, st);
@@ -505,16 +509,20 @@ __global__ void k_tile_write(const int *lists, int cap, const int *npat, const l
 
 unsigned grid_of(long long n) { return (unsigned)std::max<long long>((n + kBlock - 1) / kBlock, 1); }
 
-// device allocations of one call, freed together
+// device allocations of one call, freed together once the stream's queued
+// work (which may still read them, after an early error return) is done
 struct Scratch {
+  hipStream_t st;
   std::vector<void *> ptrs;
+  explicit Scratch(hipStream_t s) : st(s) {}
   ~Scratch() {
+    (void)hipStreamSynchronize(st);
     for (void *q : ptrs) (void)hipFree(q);
   }
   template <class T>
   T *get(long long n) {
     void *q = nullptr;
-    if (hipMalloc(&q, std::max<long long>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    if (dev_malloc(&q, std::max<long long>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
     ptrs.push_back(q);
     return static_cast<T *>(q);
   }
@@ -816,7 +824,7 @@ int read_at(const long long *src, const std::vector<long long> &idx, std::vector
   out.assign(n, 0);
   if (n == 0) return GDX_OK;
   long long *d = nullptr;
-  if (hipMalloc(&d, 2 * n * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+  if (dev_malloc(&d, 2 * n * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
   int rc = GDX_OK;
   if (hipMemcpyAsync(d, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice, st) != hipSuccess) rc = GDX_ERR_HIP;
   if (!rc) hipLaunchKernelGGL(k_gather_ll, dim3((n + 255) / 256), dim3(256), 0, st, src, d, n, d + n);
@@ -900,8 +908,8 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
     for (int d = 0; d < 3; ++d) T.F[f][d] = F[f][d];
     T.fidx[F[f][0] + 3 * F[f][1] + 9 * F[f][2]] = f;
   }
-  Scratch X;
-  ScanWs SW;
+  Scratch X(st);
+  ScanWs SW(st);
   int *d_err = X.get<int>(1);
   std::vector<int> odim;
   long long odoff[3];
@@ -972,7 +980,7 @@ int numbering_device(gdx_problem *p, const std::vector<int> owner_dim[3], const 
   }
   DevNumbering &dn = p->dn;
   auto alloc = [](auto **q, long long n) {
-    return hipMalloc(reinterpret_cast<void **>(q), std::max<long long>(n, 1) * sizeof(**q)) == hipSuccess;
+    return dev_malloc(reinterpret_cast<void **>(q), std::max<long long>(n, 1) * sizeof(**q)) == hipSuccess;
   };
   if (!alloc(&dn.lnode_dof, nodes) || !alloc(&dn.ldof_lnode, ldofs) || !alloc(&dn.ldof_owner, ldofs) ||
       !alloc(&dn.ldof_gid, ldofs) || !alloc(&dn.ldof_col, ldofs)) {
@@ -1226,7 +1234,7 @@ int structure_device(gdx_problem *p) {
     ldofs += P.n_ldofs;
     owns += P.n_own;
   }
-  Scratch X;
+  Scratch X(st);
   // the numbering and PRanges: made on the device already, or uploaded
   const bool dn = p->dev_numbering;
   int64_t *d_lnode_dof = dn ? p->dn.lnode_dof : X.get<int64_t>(lnodes),
@@ -1262,8 +1270,8 @@ int structure_device(gdx_problem *p) {
   // row offsets: the parts' CSRs back to back, so part r's local rowptr is the
   // global one minus its first value
   long long *rowptr = nullptr;
-  if (hipMalloc(&rowptr, (rows + 1) * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
-  ScanWs SW;
+  if (dev_malloc(&rowptr, (rows + 1) * sizeof(long long)) != hipSuccess) return GDX_ERR_ALLOC;
+  ScanWs SW(st);
   int rc = scan_exclusive(d_rowlen, rowptr, rows, st, SW);
   if (rc) {
     (void)hipFree(rowptr);
@@ -1278,7 +1286,7 @@ int structure_device(gdx_problem *p) {
   rc = read_at(rowptr, at, bound, st);
   const long long nnz = rc ? 0 : bound[3 * nparts];
   int *colidx = nullptr;
-  if (!rc && hipMalloc(&colidx, std::max<long long>(nnz, 1) * sizeof(int)) != hipSuccess) rc = GDX_ERR_ALLOC;
+  if (!rc && dev_malloc(&colidx, std::max<long long>(nnz, 1) * sizeof(int)) != hipSuccess) rc = GDX_ERR_ALLOC;
   uint16_t *keys = rc ? nullptr : X.get<uint16_t>(ogrps * KS);
   unsigned long long *hash = rc ? nullptr : X.get<unsigned long long>(ogrps);
   unsigned long long *tkey = rc ? nullptr : X.get<unsigned long long>(kTableSize);
@@ -1417,7 +1425,7 @@ int structure_device(gdx_problem *p) {
   int *d_dir = X.get<int>(npat);
   RowInfo *rinfo_d = nullptr;
   if (!d_dir || !put(d_dir, p->pat_dir)) return fail(GDX_ERR_ALLOC);
-  if (hipMalloc(&rinfo_d, std::max<long long>(rinfo, 1) * sizeof(RowInfo)) != hipSuccess) return fail(GDX_ERR_ALLOC);
+  if (dev_malloc(&rinfo_d, std::max<long long>(rinfo, 1) * sizeof(RowInfo)) != hipSuccess) return fail(GDX_ERR_ALLOC);
   for (int r = 0; r < nparts; ++r) {
     const PartG &g = G[r];
     hipLaunchKernelGGL(k_rowinfo, dim3(grid_of(g.onb_n[0] * g.onb_n[1] * g.onb_n[2])), dim3(kBlock), 0, st, m, g,
@@ -1489,7 +1497,7 @@ int build_tiles_device(gdx_problem *p, const int R[3]) {
         }
   }
   const int nc_ = (int)cand.size(), cap = R[0] * R[1] * R[2];
-  Scratch X;
+  Scratch X(st);
   Tile *d_cand = X.get<Tile>(nc_);
   int *lists = X.get<int>((long long)nc_ * cap), *d_npat = X.get<int>(nc_), *d_any = X.get<int>(nc_),
       *d_err = X.get<int>(1);
@@ -1528,7 +1536,7 @@ int build_tiles_device(gdx_problem *p, const int R[3]) {
     p->tiles.push_back(T);
   }
   int *tp = nullptr;
-  if (hipMalloc(&tp, std::max<long long>(total, 1) * sizeof(int)) != hipSuccess) return GDX_ERR_ALLOC;
+  if (dev_malloc(&tp, std::max<long long>(total, 1) * sizeof(int)) != hipSuccess) return GDX_ERR_ALLOC;
   p->tile_pats.assign(total, 0);
   if (!put(d_off, off)) {
     (void)hipFree(tp);
