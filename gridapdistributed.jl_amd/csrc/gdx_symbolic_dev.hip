@@ -243,23 +243,33 @@ __global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_
   __shared__ int ghl[BS * kMaxComp];   // per group: ghost columns
   __shared__ int nown[GPB], ngh[GPB];
   __shared__ unsigned long long hs[GPB];
+  __shared__ long long geo[GPB][9];  // the group's node and stencil ranges (one lane computes them)
   const int grp = threadIdx.x / LP, s = threadIdx.x % LP, nc = m.nc;
   const long long q = (long long)blockIdx.x * GPB + grp;
   const bool live = q < P.nrows / nc;
   const int W = 2 * m.K + 1, S = m.D == 3 ? W * W * W : W * W, SN = S * nc;
+  const bool own_row = live && q * nc < P.n_own;
   if (threadIdx.x < GPB) {
     hs[threadIdx.x] = 0;
     nown[threadIdx.x] = 0;
     ngh[threadIdx.x] = 0;
   }
-  __syncthreads();
-  int kind = 0;  // 0 absent, 1 Dirichlet, 2 own node, 3 ghost node
-  int c0 = 0, ck[kMaxComp] = {0, 0, 0};
-  const bool own_row = live && q * nc < P.n_own;
-  if (live && s < S) {
+  if (live && s == 0) {
     long long g[3], jlo[3], jhi[3];
     node_of(m, P, row_lnode[q], g);
     row_ranges(m, P, g, own_row, jlo, jhi);
+    for (int d = 0; d < 3; ++d) {
+      geo[grp][d] = g[d];
+      geo[grp][3 + d] = jlo[d];
+      geo[grp][6 + d] = jhi[d];
+    }
+  }
+  __syncthreads();
+  int kind = 0;  // 0 absent, 1 Dirichlet, 2 own node, 3 ghost node
+  int c0 = 0, ck[kMaxComp] = {0, 0, 0};
+  if (live && s < S) {
+    const long long g[3] = {geo[grp][0], geo[grp][1], geo[grp][2]};
+    const long long jlo[3] = {geo[grp][3], geo[grp][4], geo[grp][5]}, jhi[3] = {geo[grp][6], geo[grp][7], geo[grp][8]};
     const long long j0 = g[0] + s % W - m.K, j1 = g[1] + (s / W) % W - m.K,
                     j2 = g[2] + (m.D == 3 ? s / (W * W) - m.K : 0);
     if (j0 >= jlo[0] && j0 <= jhi[0] && j1 >= jlo[1] && j1 <= jhi[1] && j2 >= jlo[2] && j2 <= jhi[2]) {
