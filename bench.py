@@ -562,6 +562,8 @@ def main():
                          "learned: FPCG with the learned preconditioner vs Jacobi-PCG on 3D Q1 128^3; batched: 32 Q1 "
                          "problems + elasticity Q2 in one hipGraph; elasticity: the elasticity Q2 assembly alone")
     ap.add_argument("--no-elasticity", action="store_true", help="batched workload without the elasticity problem")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="default workload only: skip the other configurations' runs summarised in other_configs")
     ap.add_argument("--fields", type=int, default=2000, help="training fields (learned workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (scale reference for HBM byte counters)")
@@ -685,7 +687,52 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
     prob.close()
     ctx.close()
+    if not args.no_other_configs:
+        out["other_configs"] = other_configs()
     print(json.dumps(out), flush=True)
+
+
+# the key fields of each other configuration's bench line
+OTHER = {
+    "plaplacian": ("C5", ["newton_iters", "fpcg_iters", "converged", "speedup_vs_jacobi_pcg_newton", "jacobian_assembly_ms",
+                          "symbolic_s"]),
+    "learned": ("C4", ["training_fields", "training_s", "time_ratio_learned_over_jacobi"]),
+    "elasticity": ("C3 elasticity alone", ["nnz_owned", "symbolic_s", "kernel_ms"]),
+    "batched": ("C3", ["nnz_owned_total", "ms_eager_one_by_one", "speedup_graph_vs_eager"]),
+}
+
+
+def other_configs():
+    """The other configurations, each run as its own bench process (default
+    steps, no CPU baseline) after the headline measurement, summarised."""
+    import subprocess
+
+    res = {}
+    for w, (name, keys) in OTHER.items():
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", w, "--no-cpu-baseline"]
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=400)
+            line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1] if r.returncode == 0 else None
+        except (subprocess.TimeoutExpired, IndexError):
+            line, r = None, None
+        if line is None:
+            res[w] = {"config": name, "error": (r.stderr[-400:] if r is not None else "timed out or no result")}
+            continue
+        d = json.loads(line)
+        item = {k: d[k] for k in ("metric", "value", "unit", "ms_per_step", "higher_is_better") if k in d}
+        item["config"] = name
+        item["workload"] = d.get("config", {}).get("workload")
+        cfg = d.get("config", {})
+        for k in keys:
+            if k in cfg:
+                item[k] = cfg[k]
+        if w == "learned":
+            item["solvers"] = {k: {kk: v.get(kk) for kk in ("s", "iters", "time_ratio_over_jacobi") if kk in v}
+                               for k, v in cfg.get("solvers", {}).items()} if isinstance(cfg.get("solvers"), dict) else None
+        if "roofline" in d:
+            item["roofline"] = {k: d["roofline"].get(k) for k in ("bound", "achieved", "peak", "unit", "frac", "kernel")}
+        res[w] = item
+    return res
 
 
 if __name__ == "__main__":

@@ -294,6 +294,10 @@ int gdx_symbolic(gdx_problem *p) {
     gdx::batch_invalidate_problem(p->ctx, p);
     p->symbolic_done = false;
     GDX_HIP(hipStreamSynchronize(p->ctx->stream));
+    // networks attached to the problem hold its solver's diagonal: they lose
+    // the attachment (their next use returns GDX_ERR_STATE until re-attached)
+    for (gdx_nn *nn : p->ctx->nns)
+      if (gdx::twogrid_problem(nn) == p) gdx::twogrid_free(nn);
     free_device(p);
     gdx::solve_free(p);  // its row tables are rebuilt from the new structure
     gdx::elast_free(p);

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1361,6 +1362,12 @@ int structure_device(gdx_problem *p) {
   if (ogrps > 0)
     hipLaunchKernelGGL(k_pattern_ids, dim3(grid_of(ogrps)), dim3(kBlock), 0, st, keys, KS, ogrps, slot, tmin, d_tpid,
                        d_pid, d_err);
+  // GDX_FAULT_HASH=1: report a hash collision, so that the host stage's
+  // take-over after a collision can be tested
+  {
+    const char *fh = std::getenv("GDX_FAULT_HASH");
+    if (fh && std::atoi(fh) > 0 && hipMemsetAsync(d_err, 4, 1, st) != hipSuccess) return fail(GDX_ERR_HIP);
+  }
   if (npat > 0)
     hipLaunchKernelGGL(k_gather_keys, dim3(grid_of((long long)npat * KS)), dim3(kBlock), 0, st, keys, KS, d_rep,
                        (int)npat, d_repkeys);

@@ -111,3 +111,51 @@ def test_device_stage_twice_and_after_host_copies(gpu_ctx):
     x2, it2, rr2, _ = gp.pcg(1e-10, 2000)
     assert it2 == it and np.array_equal(x, x2)
     gp.close()
+
+
+@pytest.mark.parametrize("p", [CASES[3], CASES[9], CASES[10]], ids=ident)
+def test_host_stage_takes_over_after_a_hash_collision(gpu_ctx, p):
+    """GDX_FAULT_HASH=1 reports a collision in the device pattern table: the
+    host CSR stage then builds the structure from the device numbering's host
+    copies, with the same result."""
+    dev = build(gpu_ctx, p, False)
+    os.environ["GDX_FAULT_HASH"] = "1"
+    try:
+        fb = build(gpu_ctx, p, False)
+    finally:
+        os.environ.pop("GDX_FAULT_HASH", None)
+    for gp in (dev, fb):
+        gp.numeric()
+    for Pa, Pb in zip(exported(dev), exported(fb)):
+        for key in STRUCT_KEYS:
+            assert np.array_equal(Pa[key], Pb[key]), key
+        assert np.array_equal(Pa["val"].view(np.int64), Pb["val"].view(np.int64))
+        assert np.array_equal(Pa["b"].view(np.int64), Pb["b"].view(np.int64))
+    dev.close()
+    fb.close()
+
+
+def test_resymbolic_detaches_networks(gpu_ctx):
+    """A second symbolic phase re-makes the solver's tables, which an attached
+    network refers to: the network is detached (FPCG with it is refused until
+    it is attached again), and works after re-attachment."""
+    from gridap_amd import LearnedCoarseSolver, fpcg
+    from gridap_amd._lib import GdxError
+
+    p = W.poisson3d_q1(8, (2, 1, 1))
+    gp = build(gpu_ctx, p, False)
+    gp.numeric()
+    net = LearnedCoarseSolver(gpu_ctx, 4, 0)
+    net.attach(gp)
+    net.set_options(8, use_network=False)
+    x1, it1, r1, _, _ = fpcg(gp, net, None, 1e-10, 500)
+    gp.symbolic()
+    gp.numeric()
+    with pytest.raises(GdxError):
+        fpcg(gp, net, None, 1e-10, 500)
+    net.attach(gp)
+    net.set_options(8, use_network=False)  # options belong to the attachment
+    x2, it2, r2, _, _ = fpcg(gp, net, None, 1e-10, 500)
+    assert r1 <= 1e-10 and r2 <= 1e-10 and it1 == it2 and np.array_equal(x1, x2)
+    net.close()
+    gp.close()
