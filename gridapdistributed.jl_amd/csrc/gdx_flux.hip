@@ -184,24 +184,29 @@ __global__ __launch_bounds__(256) void k_flux_cells(int64_t ncell, const double 
   for (int a = 0; a < NC; ++a) cm[(NJ + a) * ncell + c] = R[a];
 }
 
-// Fused 3D form: one 512-lane workgroup per tile of 8 x 8 x 4 rows.  The
-// tile's (ext+1)^3 cells (at most 9 x 9 x 5) compute their Jacobian and
-// residual into LDS (one lane per cell, the stage-1 arithmetic), then two
-// lanes per row sum the row's 27 slots from LDS in the fixed cell order of
-// the row stage and store them through the row's position pattern.  Cells on
-// the tile boundary are recomputed by the neighbouring tiles (9 x 9 x 5 cells
-// per 256 rows) instead of a round trip of every cell matrix through HBM.
-template <int NQ, int MODE>
-__global__ __launch_bounds__(512) void k_flux_fused(const Tile *__restrict__ tiles, int ntiles,
+// Fused 3D form: one workgroup of NT lanes per tile of 8 x R2 x R3 rows.  The
+// tile's (ext+1)^3 cells (at most 9 x (R2+1) x (R3+1)) compute their Jacobian
+// and residual into LDS (one lane per cell, the stage-1 arithmetic), then
+// NT / rows lanes per row sum the row's 27 slots from LDS in the fixed cell
+// order of the row stage and store them through the row's position pattern.
+// Cells on the tile boundary are recomputed by the neighbouring tiles instead
+// of a round trip of every cell matrix through HBM.  With 8 x 4 x 4 rows (225
+// cells, 81 KB of records, 256 lanes) two workgroups share a CU, so one's
+// loads and stores overlap the other's cell arithmetic (8 x 8 x 4 rows with
+// 405 cells in 146 KB left one workgroup per CU: 0.29 against 0.25 ms at 96^3).
+// The values do not depend on the tile shape (same cell arithmetic, same
+// summation order per row).
+template <int NQ, int MODE, int R2, int R3, int NT>
+__global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tiles, int ntiles,
                                                     const RowInfo *__restrict__ rowinfo,
                                                     const uint8_t *__restrict__ pat_fwd, int pstride,
                                                     const double *__restrict__ tab, const double *__restrict__ ulat,
                                                     double *__restrict__ values, double *__restrict__ b, KParams kp,
                                                     FluxParams fp) {
   constexpr int D = 3, NC = 8, NJ = 36, NE = NJ + NC, ES = NE + 1;  // odd record stride against bank conflicts
-  constexpr int R1 = 8, R2 = 8, R3 = 4, CX = R1 + 1, CY = R2 + 1, CZ = R3 + 1, NCELL = CX * CY * CZ;
-  constexpr int LPR = 512 / (R1 * R2 * R3);  // lanes per row
-  static_assert(NCELL <= 512, "one lane per cell");
+  constexpr int R1 = 8, CX = R1 + 1, CY = R2 + 1, CZ = R3 + 1, NCELL = CX * CY * CZ;
+  constexpr int LPR = NT / (R1 * R2 * R3);  // lanes per row
+  static_assert(NCELL <= NT && LPR >= 1, "one lane per cell");
   __shared__ double cj[NCELL * ES];
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
@@ -342,10 +347,10 @@ __global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tile
 template <int D, int NQ, int MODE>
 void launch_flux_m(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
   if constexpr (D == 3 && NQ <= 2) {  // (3 Gauss points per direction: the tables leave no room in LDS)
-    if (p->tileR[0] == 8 && p->tileR[1] == 8 && p->tileR[2] == 4) {  // the fused form's tile shape
-      const int nt = (int)p->tiles.size();
-      hipLaunchKernelGGL((k_flux_fused<NQ, MODE>), dim3(8u * ((nt + 7) / 8)), dim3(512), 0, st, p->dev.tiles, nt,
-                         p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->tab, fd->ulat,
+    const int nt = (int)p->tiles.size();
+    if (p->tileR[0] == 8 && p->tileR[1] == kFluxR2 && p->tileR[2] == kFluxR3) {  // the fused form's tile shape
+      hipLaunchKernelGGL((k_flux_fused<NQ, MODE, kFluxR2, kFluxR3, kFluxNT>), dim3(8u * ((nt + 7) / 8)), dim3(kFluxNT), 0, st,
+                         p->dev.tiles, nt, p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->tab, fd->ulat,
                          p->dev.values, p->dev.b, p->kp, fp);
       return;
     }

@@ -14,6 +14,9 @@ constexpr int kMaxQ = 4;     // Gauss points per direction
 constexpr int kMaxK = 2;     // polynomial order
 constexpr uint8_t kSlotAbsent = 0xFF;
 constexpr uint8_t kSlotDirichlet = 0xFE;
+// tiles of the fused 3D flux kernel: 8 x kFluxR2 x kFluxR3 owned rows, kFluxNT
+// lanes (8 x 4 x 4: the cell records of two workgroups fit in one CU's LDS)
+constexpr int kFluxR2 = 4, kFluxR3 = 4, kFluxNT = 256;
 
 // One part (subdomain) of the Cartesian partition; all parts live on one GPU.
 struct Part {

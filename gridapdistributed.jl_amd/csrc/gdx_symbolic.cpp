@@ -782,10 +782,10 @@ static int host_csr_stage(gdx_problem *p, const Ctx &c) {
   p->rows_total = row_base;
   p->owned_nnz = owned_nnz;
 
-  // tiles of owned rows (the flux operator's fused kernel takes 8 x 8 x 4 rows)
+  // tiles of owned rows (the flux operator's fused kernel takes its own shape)
   int R[3];
   tile_shape(c.D, K, R);
-  if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
+  if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[1] = kFluxR2, R[2] = kFluxR3;
   build_tiles(p, R);
   return GDX_OK;
 }
@@ -864,7 +864,7 @@ int symbolic(gdx_problem *p) {
       if (rc) return rc;
       int R[3];
       tile_shape(c.D, c.K, R);
-      if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[2] = 4;
+      if (p->desc.op_kind == GDX_OP_FLUX && c.D == 3) R[1] = kFluxR2, R[2] = kFluxR3;
       const int rt = build_tiles_device(p, R);
       symbolic_mark("tiles (device)");
       return rt;
