@@ -240,8 +240,8 @@ __global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_
                                                   const long long *rowptr, int *colidx, uint16_t *keys, int KS,
                                                   unsigned long long *hash, int *err) {
   constexpr int GPB = BS / LP;
-  __shared__ int ownl[BS];             // per group: first columns of the own nodes
-  __shared__ int ghl[BS * kMaxComp];   // per group: ghost columns
+  __shared__ __align__(16) int ownl[BS];            // per group: first columns of the own nodes
+  __shared__ __align__(16) int ghl[BS * kMaxComp];  // per group: ghost columns
   __shared__ int nown[GPB], ngh[GPB];
   __shared__ unsigned long long hs[GPB];
   __shared__ long long geo[GPB][9];  // the group's node and stencil ranges (one lane computes them)
@@ -255,6 +255,9 @@ __global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_
     nown[threadIdx.x] = 0;
     ngh[threadIdx.x] = 0;
   }
+  // unused list entries read as larger than every column (the rank loops read four at a time)
+  ownl[threadIdx.x] = INT_MAX;
+  for (int k = 0; k < kMaxComp; ++k) ghl[threadIdx.x * kMaxComp + k] = INT_MAX;
   if (live && s == 0) {
     long long g[3], jlo[3], jhi[3];
     node_of(m, P, row_lnode[q], g);
@@ -308,18 +311,25 @@ __global__ __launch_bounds__(BS) void k_fill_rows(MeshG m, PartG P, const int64_
   if (kind == 1) {
     for (int k = 0; k < nc; ++k) key[k] = kDir16;
   } else if (kind == 2) {
-    const int *ol = ownl + grp * LP;
+    const int4 *ol = reinterpret_cast<const int4 *>(ownl + grp * LP);
     int r = 0;
-    for (int i = 0; i < nown[grp]; ++i) r += ol[i] < c0;
+    for (int i = 0; i < (nown[grp] + 3) / 4; ++i) {
+      const int4 v = ol[i];
+      r += (v.x < c0) + (v.y < c0) + (v.z < c0) + (v.w < c0);
+    }
     for (int k = 0; k < nc; ++k) {
       key[k] = (uint16_t)(nc * r + k);
       for (int kk = 0; kk < nc; ++kk) colidx[rowptr[r0 + kk] + nc * r + k] = c0 + k;
     }
   } else if (kind == 3) {
-    const int *gl = ghl + grp * LP * kMaxComp, ng = ngh[grp], base = nc * nown[grp];
+    const int4 *gl = reinterpret_cast<const int4 *>(ghl + grp * LP * kMaxComp);
+    const int ng = ngh[grp], base = nc * nown[grp];
     for (int k = 0; k < nc; ++k) {
       int r = 0;
-      for (int i = 0; i < ng; ++i) r += gl[i] < ck[k];
+      for (int i = 0; i < (ng + 3) / 4; ++i) {
+        const int4 v = gl[i];
+        r += (v.x < ck[k]) + (v.y < ck[k]) + (v.z < ck[k]) + (v.w < ck[k]);
+      }
       key[k] = (uint16_t)(base + r);
       for (int kk = 0; kk < nc; ++kk) colidx[rowptr[r0 + kk] + base + r] = ck[k];
     }
