@@ -26,6 +26,13 @@ struct Net {
   int adam_t = 0;
   float *theta = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *thb = nullptr;
+  // the layers' weights again, as the forward convolutions read them: layer l
+  // at pkoff[l], [output channel, padded to 16][k, padded to a multiple of
+  // 32] with k = tap * inputs + input channel when the layer's inputs are a
+  // multiple of 8, else input channel * 27 + tap; zeros in the padding;
+  // re-made after every update
+  uint16_t *wpk = nullptr;
+  int64_t pkoff[kLayers + 1];
   // activations (BF16)
   uint16_t *a0 = nullptr, *x1a = nullptr, *x1 = nullptr, *p1 = nullptr, *x2a = nullptr, *x2 = nullptr, *p2 = nullptr,
            *x3a = nullptr, *x3 = nullptr, *u2 = nullptr, *y2a = nullptr, *y2 = nullptr, *u1 = nullptr, *y1a = nullptr;
@@ -45,6 +52,7 @@ struct Net {
   int forward_dev();                        // a0 -> out
   int backward_dev(bool want_dinp);         // dout -> grad (+ dinp)
   int adam_step(float lr, float clip);
+  void pack_weights();  // thb -> wpk
 };
 
 }  // namespace nn

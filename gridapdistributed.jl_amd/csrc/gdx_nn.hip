@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
     const int co = cob + cg;
     float v = acc[cg];
     if (SKIP) v += bf2f(skip[co * n3 + vi]);
-    if (RELU) v = fmaxf(v, 0.0f);
+    if (RELU) v = v < 0.0f ? 0.0f : v;  // NaN passes, as max(v, 0) in the oracle
     if (OUTF32) {
       float *o = reinterpret_cast<float *>(out_) + co * n3 + vi;
       *o = ACCUM ? *o + v : v;
@@ -122,6 +122,239 @@ __global__ __launch_bounds__(kT) void k_conv3(int n, const void *__restrict__ in
       reinterpret_cast<uint16_t *>(out_)[co * n3 + vi] = f2bf(v);
     }
   }
+}
+
+// Forward convolution on the matrix cores: an implicit GEMM per tile of
+// 8 x 8 x TZ outputs.  For a wave, M = its 16 TZ outputs (TZ blocks of 16: two
+// x rows each), N = the output channels (blocks of 16, zero-padded), K =
+// (input channel, tap) pairs (27 KCI, padded to a multiple of 32).  A
+// fragments come from the BF16 input halo in LDS through a table of (channel,
+// tap) offsets, B fragments from the weights staged in LDS as [output
+// channel][k]; v_mfma_f32_16x16x32_bf16 accumulates in FP32.
+// Lane layout of that instruction (checked on the device): lane l holds
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16] for j < 8, and
+// returns D[4 (l / 16) + i][l % 16] for i < 4.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int KCI, int KCO, int TZ, int NBW>
+struct ConvMfma {
+  static constexpr int KT = KCI * 27, KP = (KT + 31) / 32 * 32, NB = (KCO + 15) / 16, NCO = NB * 16;
+  static constexpr int HZ = TZ + 2, HV = 100 * HZ;
+  // CM: k = tap KCI + channel and the halo stored [voxel][channel], so that
+  // 8 consecutive k are one 16-byte LDS read; otherwise k = channel 27 + tap,
+  // the halo [channel][voxel] followed by a zero block for the padded k
+  static constexpr bool CM = KCI % 8 == 0;
+  static constexpr int SX = CM ? KCI * HV : KCI * HV + HV;
+  static constexpr int SW = NBW * 16 * KP;  // this workgroup's output channels of the packed weights
+  static constexpr int ROWS = KCI * HZ * 10, RU = (ROWS + 255) / 256;  // halo rows of 10, per thread
+  static constexpr int PO = 64 * TZ + 4;                                // output tile pitch per channel (floats)
+  static constexpr int SXO = SX > 2 * NBW * 16 * PO ? SX : 2 * NBW * 16 * PO;  // halo, then output tile
+};
+
+__device__ __forceinline__ bool al16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// A workgroup of four waves computes the 8 x 8 x TZ outputs of a tile for NBW
+// blocks of 16 output channels (blockIdx.y selects the first one); its 4 TZ
+// voxel blocks of 16 (two x rows at one z) are split TZ per wave.  Weights
+// come packed per layer ([output channel][k], both zero-padded) and are
+// staged with 16-byte copies; on grids with n % 8 == 0 each halo row is one
+// 16-byte load and two edge values, all of a thread's loads issued before its
+// LDS stores, and the outputs go out through LDS as rows of 8 voxels.
+template <int KCI, int KCO, bool RELU, bool SKIP, bool OUTF32, int TZ, int NBW>
+__global__ __launch_bounds__(256) void k_conv3_mfma(int n, const uint16_t *__restrict__ in,
+                                                    const uint16_t *__restrict__ wpk,
+                                                    const uint16_t *__restrict__ bias,
+                                                    const uint16_t *__restrict__ skip, void *__restrict__ out_) {
+  using C = ConvMfma<KCI, KCO, TZ, NBW>;
+  __shared__ __align__(16) uint16_t sx[C::SXO];
+  __shared__ __align__(16) uint16_t sw[C::SW];
+  __shared__ __align__(16) int koff[C::KP];
+  const int ntx = (n + 7) / 8, nty = (n + 7) / 8;
+  const int x0 = (blockIdx.x % ntx) * 8, y0 = ((blockIdx.x / ntx) % nty) * 8, z0 = (blockIdx.x / (ntx * nty)) * TZ;
+  const int nb0 = blockIdx.y * NBW;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t n3 = (int64_t)n * n * n;
+  const bool rows8 = (n & 7) == 0 && al16(in) && al16(out_) && (!SKIP || al16(skip));
+  {
+    const u16x8_t *src = reinterpret_cast<const u16x8_t *>(wpk + (int64_t)nb0 * 16 * C::KP);
+    u16x8_t *dst = reinterpret_cast<u16x8_t *>(sw);
+#pragma unroll 4
+    for (int i = tid; i < C::SW / 8; i += 256) dst[i] = src[i];
+  }
+  if (rows8) {
+    // halo rows of 10 (x0 + 8 <= n): row r is (c HZ + hz) 10 + hy, at sx[10 r]
+    // (CM: (hz 10 + hy) KCI + c, at sx[10 KCI (r / KCI) + c] with stride KCI)
+    u16x8_t mid[C::RU];
+    uint16_t lf[C::RU], rt[C::RU];
+#pragma unroll
+    for (int u = 0; u < C::RU; ++u) {
+      const int r = tid + 256 * u;
+      mid[u] = u16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+      lf[u] = rt[u] = 0;
+      const int c = C::CM ? r % KCI : r / (C::HZ * 10);
+      const int hy = C::CM ? (r / KCI) % 10 : r % 10, hz = C::CM ? r / (KCI * 10) : (r / 10) % C::HZ;
+      const int gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      if (r < C::ROWS && gy >= 0 && gy < n && gz >= 0 && gz < n) {
+        const uint16_t *row = in + (int64_t)c * n3 + ((int64_t)gz * n + gy) * n + x0;
+        mid[u] = *reinterpret_cast<const u16x8_t *>(row);
+        if (x0 > 0) lf[u] = row[-1];
+        if (x0 + 8 < n) rt[u] = row[8];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < C::RU; ++u) {
+      const int r = tid + 256 * u;
+      if (r < C::ROWS) {
+        constexpr int st = C::CM ? KCI : 1;
+        uint16_t *d = sx + (C::CM ? 10 * KCI * (r / KCI) + r % KCI : 10 * r);
+        d[0] = lf[u];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[st * (1 + j)] = mid[u][j];
+        d[9 * st] = rt[u];
+      }
+    }
+    if (!C::CM)
+      for (int i = tid; i < C::HV; i += 256) sx[KCI * C::HV + i] = 0;
+  } else {
+    for (int i = tid; i < C::SX; i += 256) {
+      const int c = C::CM ? i % KCI : i / C::HV, h = C::CM ? i / KCI : i % C::HV;
+      const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      uint16_t v = 0;
+      if (c < KCI && gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
+        v = in[(int64_t)c * n3 + ((int64_t)gz * n + gy) * n + gx];
+      sx[i] = v;
+    }
+  }
+  if (C::CM) {
+    // per group of 8 k: the group's tap offset times KCI plus its first channel, -1 past KT
+    for (int g = tid; g < C::KP / 8; g += 256) {
+      const int t = 8 * g / KCI, c0 = 8 * g % KCI;
+      koff[g] = t < 27 ? ((t / 9) * 100 + ((t / 3) % 3) * 10 + t % 3) * KCI + c0 : -1;
+    }
+  } else {
+    for (int k = tid; k < C::KP; k += 256) {
+      const int ci = k / 27, t = k % 27;
+      koff[k] = k < C::KT ? ci * C::HV + (t / 9) * 100 + ((t / 3) % 3) * 10 + t % 3 : KCI * C::HV;
+    }
+  }
+  __syncthreads();
+  f32x4_t acc[TZ][NBW];
+#pragma unroll
+  for (int mb = 0; mb < TZ; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) acc[mb][nb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  const int m = lane & 15, kq = (lane >> 4) * 8;
+  for (int kc = 0; kc < C::KP; kc += 32) {
+    const int *ko = koff + kc + kq;
+    bf16x8_t b[NBW];
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb)
+      b[nb] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t *>(sw + (nb * 16 + m) * C::KP + kc + kq));
+    const int kg = C::CM ? koff[(kc + kq) >> 3] : 0, kgc = kg < 0 ? 0 : kg;
+#pragma unroll
+    for (int mb = 0; mb < TZ; ++mb) {
+      const int q = wave * TZ + mb;  // voxel block: z = q / 4, rows 2 (q % 4) and 2 (q % 4) + 1
+      const int voff = (q >> 2) * 100 + (2 * (q & 3) + (m >> 3)) * 10 + (m & 7);
+      u16x8_t a;
+      if (C::CM) {
+        a = *reinterpret_cast<const u16x8_t *>(sx + voff * KCI + kgc);
+        if (kg < 0) a = u16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = sx[ko[j] + voff];
+      }
+      const bf16x8_t av = __builtin_bit_cast(bf16x8_t, a);
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[nb], acc[mb][nb], 0, 0, 0);
+    }
+  }
+  if (rows8) {
+    // sums to LDS as [channel][z][y][x] (pitch PO), then rows of 8 voxels out
+    __syncthreads();
+    float *so = reinterpret_cast<float *>(sx);
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < TZ; ++mb) {
+        const int q = wave * TZ + mb;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int mm = (lane >> 4) * 4 + i;
+          so[(nb * 16 + m) * C::PO + (q >> 2) * 64 + (2 * (q & 3) + (mm >> 3)) * 8 + (mm & 7)] = acc[mb][nb][i];
+        }
+      }
+    __syncthreads();
+    for (int r = tid; r < NBW * 16 * TZ * 8; r += 256) {
+      const int col = r / (TZ * 8), zl = (r / 8) % TZ, yl = r % 8;
+      const int co = nb0 * 16 + col, gz = z0 + zl, gy = y0 + yl;
+      if (co >= KCO || gz >= n || gy >= n) continue;
+      const float *src = so + col * C::PO + zl * 64 + yl * 8;
+      const f32x4_t lo = *reinterpret_cast<const f32x4_t *>(src), hi = *reinterpret_cast<const f32x4_t *>(src + 4);
+      const int64_t o = (int64_t)co * n3 + ((int64_t)gz * n + gy) * n + x0;
+      u16x8_t sk = u16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+      if (SKIP) sk = *reinterpret_cast<const u16x8_t *>(skip + o);
+      const float bco = bf2f(bias[co]);
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        v[j] = (j < 4 ? lo[j] : hi[j - 4]) + bco;
+        if (SKIP) v[j] += bf2f(sk[j]);
+        if (RELU) v[j] = v[j] < 0.0f ? 0.0f : v[j];  // NaN passes, as max(v, 0) in the oracle
+      }
+      if (OUTF32) {
+        f32x4_t *d = reinterpret_cast<f32x4_t *>(reinterpret_cast<float *>(out_) + o);
+        d[0] = f32x4_t{v[0], v[1], v[2], v[3]};
+        d[1] = f32x4_t{v[4], v[5], v[6], v[7]};
+      } else {
+        u16x8_t h;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) h[j] = f2bf(v[j]);
+        *reinterpret_cast<u16x8_t *>(reinterpret_cast<uint16_t *>(out_) + o) = h;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int nb = 0; nb < NBW; ++nb) {
+    const int co = (nb0 + nb) * 16 + m;
+    if (co >= KCO) continue;
+    const float bco = bf2f(bias[co]);
+#pragma unroll
+    for (int mb = 0; mb < TZ; ++mb) {
+      const int q = wave * TZ + mb, gz = z0 + (q >> 2);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int mm = (lane >> 4) * 4 + i, gx = x0 + (mm & 7), gy = y0 + 2 * (q & 3) + (mm >> 3);
+        if (gx >= n || gy >= n || gz >= n) continue;
+        const int64_t vi = ((int64_t)gz * n + gy) * n + gx;
+        float v = acc[mb][nb][i] + bco;
+        if (SKIP) v += bf2f(skip[co * n3 + vi]);
+        if (RELU) v = v < 0.0f ? 0.0f : v;  // NaN passes, as max(v, 0) in the oracle
+        if (OUTF32) reinterpret_cast<float *>(out_)[co * n3 + vi] = v;
+        else reinterpret_cast<uint16_t *>(out_)[co * n3 + vi] = f2bf(v);
+      }
+    }
+  }
+}
+
+// thb -> wpk for every layer (one thread per packed element)
+struct PackTab {
+  int64_t woff[kLayers], pkoff[kLayers + 1];
+  int kt[kLayers], kp[kLayers], co[kLayers], ci[kLayers];  // ci: input channels, k tap-major if ci % 8 == 0
+};
+
+__global__ void k_pack_weights(PackTab T, const uint16_t *__restrict__ thb, uint16_t *__restrict__ wpk) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= T.pkoff[kLayers]) return;
+  int l = 0;
+  while (i >= T.pkoff[l + 1]) ++l;
+  const int64_t j = i - T.pkoff[l];
+  const int co = (int)(j / T.kp[l]), k = (int)(j % T.kp[l]);
+  const int src = T.ci[l] % 8 == 0 ? (k % T.ci[l]) * 27 + k / T.ci[l] : k;  // thb holds [co][ci][tap]
+  wpk[i] = (co < T.co[l] && k < T.kt[l]) ? thb[T.woff[l] + (int64_t)co * T.kt[l] + src] : (uint16_t)0;
 }
 
 // weight and bias gradient of a layer: per-workgroup partial sums over its
@@ -324,6 +557,10 @@ int Net::init(int grid, uint64_t seed, hipStream_t s) {
   auto F = [&](float **p, int64_t cnt) { if (!rc) rc = alloc(reinterpret_cast<void **>(p), cnt * sizeof(float)); };
   auto B = [&](uint16_t **p, int64_t cnt) { if (!rc) rc = alloc(reinterpret_cast<void **>(p), cnt * sizeof(uint16_t)); };
   F(&theta, np); F(&grad, np); F(&m, np); F(&v, np); B(&thb, np);
+  pkoff[0] = 0;
+  for (int l = 0; l < kLayers; ++l)
+    pkoff[l + 1] = pkoff[l] + (int64_t)(sh[l][0] + 15) / 16 * 16 * ((sh[l][1] * 27 + 31) / 32 * 32);
+  B(&wpk, pkoff[kLayers]);
   B(&a0, kCin * v0); B(&x1a, kC0 * v0); B(&x1, kC0 * v0); B(&p1, kC0 * v1); B(&x2a, kC1 * v1); B(&x2, kC1 * v1);
   B(&p2, kC1 * v2); B(&x3a, kC2 * v2); B(&x3, kC2 * v2); B(&u2, kC2 * v1); B(&y2a, kC1 * v1); B(&y2, kC1 * v1);
   B(&u1, kC1 * v0); B(&y1a, kC0 * v0);
@@ -350,6 +587,7 @@ int Net::init(int grid, uint64_t seed, hipStream_t s) {
 int Net::set_params(const float *h) {
   if (hipMemcpyAsync(theta, h, np * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess) return GDX_ERR_HIP;
   hipLaunchKernelGGL(k_to_bf16, dim3(nblk(np)), dim3(kT), 0, st, np, theta, thb);
+  pack_weights();
   if (hipMemsetAsync(m, 0, np * sizeof(float), st) != hipSuccess ||
       hipMemsetAsync(v, 0, np * sizeof(float), st) != hipSuccess)
     return GDX_ERR_HIP;
@@ -390,9 +628,23 @@ static void conv_any(const Net &N, int n, const void *in, const uint16_t *w, con
   }
 }
 
+// forward convolutions on the matrix cores
 template <int CI, int CO, bool RELU, bool SKIP, bool OUTF32>
 static void conv_fwd(const Net &N, int n, const uint16_t *in, int layer, const uint16_t *skip, void *out) {
-  conv_any<CI, CO, 0, RELU, SKIP, OUTF32, false>(N, n, in, N.thb + N.woff[layer], N.thb + N.boff[layer], skip, out);
+  constexpr int NB = (CO + 15) / 16;
+  const uint16_t *w = N.wpk + N.pkoff[layer], *b = N.thb + N.boff[layer];
+  const unsigned txy = (unsigned)(((n + 7) / 8) * ((n + 7) / 8));
+  // 64^3 and up: tiles of 8 x 8 x 4 with every output channel; below, thinner
+  // tiles and one workgroup per block of 16 output channels, for more workgroups
+  if (n >= 64)
+    hipLaunchKernelGGL((k_conv3_mfma<CI, CO, RELU, SKIP, OUTF32, 4, NB>), dim3(txy * ((n + 3) / 4)), dim3(256), 0,
+                       N.st, n, in, w, b, skip, out);
+  else if (n >= 32)
+    hipLaunchKernelGGL((k_conv3_mfma<CI, CO, RELU, SKIP, OUTF32, 2, 1>), dim3(txy * ((n + 1) / 2), NB), dim3(256), 0,
+                       N.st, n, in, w, b, skip, out);
+  else
+    hipLaunchKernelGGL((k_conv3_mfma<CI, CO, RELU, SKIP, OUTF32, 1, 1>), dim3(txy * n, NB), dim3(256), 0, N.st, n, in,
+                       w, b, skip, out);
 }
 
 // dX of a layer (KCI = layer outputs, KCO = layer inputs)
@@ -487,7 +739,23 @@ int Net::adam_step(float lr, float clip) {
   const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
   hipLaunchKernelGGL(k_adam, dim3(nblk(np)), dim3(kT), 0, st, np, theta, thb, grad, m, v, lr, b1, b2, 1e-8f, c1, c2,
                      clip);
+  pack_weights();
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+void Net::pack_weights() {
+  PackTab T;
+  int sh[kLayers][2];
+  layer_shapes(sh);
+  for (int l = 0; l < kLayers; ++l) {
+    T.woff[l] = woff[l];
+    T.kt[l] = sh[l][1] * 27;
+    T.kp[l] = (T.kt[l] + 31) / 32 * 32;
+    T.co[l] = sh[l][0];
+    T.ci[l] = sh[l][1];
+  }
+  for (int l = 0; l <= kLayers; ++l) T.pkoff[l] = pkoff[l];
+  hipLaunchKernelGGL(k_pack_weights, dim3(nblk(pkoff[kLayers])), dim3(kT), 0, st, T, thb, wpk);
 }
 
 }  // namespace nn
