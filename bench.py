@@ -203,7 +203,10 @@ def bench_plaplacian(args):
     step = one full Newton solve (every Jacobian re-assembled in place), its
     corrections solved by FPCG with the multilevel preconditioner rebuilt from
     each Jacobian (coarse network off: it does not gain, see the learned
-    workload); the same solve with Jacobi-PCG corrections is timed beside it."""
+    workload); the same solve with Jacobi-PCG corrections is timed beside it,
+    and so are the forms the configuration names: FP32-contracted Jacobians
+    (mixed-precision re-assembly), and those with the coarse network on,
+    trained on this problem's own Jacobian."""
     import numpy as np
 
     import gridap_amd as G
@@ -222,14 +225,14 @@ def bench_plaplacian(args):
     net.set_options(8, use_network=False)
     kw = dict(rtol=1e-10, maxiter=40, lin_rtol=1e-8, lin_maxiter=20000, init=G._lib.NEWTON_INIT_POISSON)
 
-    def timed(precond):
+    def timed(precond, jac_precision=G._lib.PREC_FP64):
         for _ in range(args.warmup):
-            prob.newton(None, precond=precond, **kw)
+            prob.newton(None, precond=precond, jac_precision=jac_precision, **kw)
         prob.sync()
         reps = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            x, rep = prob.newton(None, precond=precond, **kw)
+            x, rep = prob.newton(None, precond=precond, jac_precision=jac_precision, **kw)
             reps.append(rep)
         prob.sync()
         wall = (time.perf_counter() - t0) / args.steps
@@ -239,12 +242,31 @@ def bench_plaplacian(args):
     x_j, rep_j, s_jacobi, wall_jacobi = timed(None)
     x, rep, s_per_solve, wall_per_solve = timed(net)
     sol_rel_diff = float(np.linalg.norm(x - x_j) / np.linalg.norm(x_j))
+    # mixed-precision re-assembly: FP32 cell Jacobians (residual and solves FP64)
+    x32, rep32, s32, _ = timed(net, G._lib.PREC_FP32)
     prob.set_state(x)
-    acc = 0.0
-    for _ in range(10):
-        prob.numeric()
-        acc += prob.phase_times()[3]
-    asm_ms = acc / 10
+
+    def asm_time(precision):
+        acc = 0.0
+        for _ in range(10):
+            prob.numeric(precision)
+            acc += prob.phase_times()[3]
+        return acc / 10
+
+    asm_ms = asm_time(G._lib.PREC_FP64)
+    asm32_ms = asm_time(G._lib.PREC_FP32)
+    # the coarse network on, trained on the Jacobian at the solution (smooth
+    # coarse errors of several widths), with FP32 Jacobians
+    prob.numeric()
+    net.attach(prob)
+    t0 = time.perf_counter()
+    for k in range(C5_TRAIN_STEPS):
+        net.train_step(31 * k + 7, 2e-3 if k < C5_TRAIN_STEPS // 2 else 1e-3, smooth_passes=(8, 16, 32, 64)[k % 4])
+    prob.sync()
+    t_train = time.perf_counter() - t0
+    net.set_options(8, use_network=True)
+    x_nn, rep_nn, s_nn, _ = timed(net, G._lib.PREC_FP32)
+    net.set_options(8, use_network=False)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_k_flux_fused_plaplacian_{n}.json")
     if os.path.exists(tpath):
@@ -272,7 +294,20 @@ def bench_plaplacian(args):
                    "speedup_vs_jacobi_pcg_newton": round(s_jacobi / s_per_solve, 3),
                    "solution_rel_diff_vs_jacobi": sol_rel_diff,
                    "jacobian_assembly_ms": round(asm_ms, 4),
-                   "jacobian_assembly_gnnz_s": round(nnz / (asm_ms * 1e-3) / 1e9, 2)},
+                   "jacobian_assembly_gnnz_s": round(nnz / (asm_ms * 1e-3) / 1e9, 2),
+                   "mixed_jacobians": {"s": round(s32, 4), "newton_iters": rep32["iters"],
+                                       "fpcg_iters": rep32["lin_iters"], "converged": rep32["converged"],
+                                       "res_final": rep32["res_final"],
+                                       "solution_rel_diff_vs_fp64": float(np.linalg.norm(x32 - x) / np.linalg.norm(x)),
+                                       "jacobian_assembly_ms": round(asm32_ms, 4),
+                                       "speedup_assembly_vs_fp64": round(asm_ms / asm32_ms, 3)},
+                   "network_on_mixed_jacobians": {"s": round(s_nn, 4), "newton_iters": rep_nn["iters"],
+                                                  "fpcg_iters": rep_nn["lin_iters"], "converged": rep_nn["converged"],
+                                                  "fpcg_fallbacks": rep_nn["lin_fallbacks"],
+                                                  "res_final": rep_nn["res_final"],
+                                                  "solution_rel_diff_vs_fp64": float(np.linalg.norm(x_nn - x) /
+                                                                                     np.linalg.norm(x)),
+                                                  "training_steps": C5_TRAIN_STEPS, "training_s": round(t_train, 2)}},
         "roofline": {"bound": "hbm", "achieved": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_flux_fused<2,2,4,4,256>", "alg_bytes_per_launch": int(8 * nnz)},
@@ -284,6 +319,10 @@ def bench_plaplacian(args):
     ctx.close()
     print(json.dumps(out), flush=True)
 
+
+# Adam steps of the p-Laplacian workload's coarse network (on the Jacobian at
+# the solution)
+C5_TRAIN_STEPS = 600
 
 # damped-Jacobi sweeps before and after each coarse-grid correction of the
 # learned workload's V-cycle (at 128^3: 15 FPCG iterations with 1, 13 with 2,
@@ -702,7 +741,7 @@ def main():
 # the key fields of each other configuration's bench line
 OTHER = {
     "plaplacian": ("C5", ["newton_iters", "fpcg_iters", "converged", "speedup_vs_jacobi_pcg_newton", "jacobian_assembly_ms",
-                          "symbolic_s"]),
+                          "symbolic_s", "mixed_jacobians", "network_on_mixed_jacobians"]),
     "learned": ("C4", ["training_fields", "training_s", "time_ratio_learned_over_jacobi"]),
     "elasticity": ("C3 elasticity alone", ["nnz_owned", "symbolic_s", "kernel_ms"]),
     "batched": ("C3", ["nnz_owned_total", "ms_eager_one_by_one", "speedup_graph_vs_eager"]),

@@ -406,7 +406,7 @@ extern "C++" namespace gdx {
 // the numeric phase's kernel launches on st (no events, no memsets: safe to
 // capture in a graph)
 int numeric_launches(gdx_problem *p, hipStream_t st) {
-  if (p->desc.op_kind == GDX_OP_FLUX) return flux_assemble(p, nullptr, false, st);
+  if (p->desc.op_kind == GDX_OP_FLUX) return flux_assemble(p, nullptr, false, p->precision == GDX_PREC_FP32, st);
   if (p->desc.op_kind == GDX_OP_ELASTICITY) return elast_assemble(p, st);
   int rc = launch_coefficients(p, st);
   if (!rc) rc = launch_rhs(p, st);
@@ -420,10 +420,12 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
   GDX_REQUIRE(precision == GDX_PREC_FP64 || precision == GDX_PREC_FP32, GDX_ERR_ARG, "unknown precision");
-  GDX_REQUIRE(precision == GDX_PREC_FP64 || (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION &&
-                                              (p->kvariant3d == 0 || p->kvariant3d == 2)),
+  GDX_REQUIRE(precision == GDX_PREC_FP64 ||
+                  (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION && (p->kvariant3d == 0 || p->kvariant3d == 2)) ||
+                  (p->desc.op_kind == GDX_OP_FLUX && gdx::flux_mixed_supported(p)),
               GDX_ERR_UNSUPPORTED,
-              "the FP32 form runs with the 3D z-marching or direct-store kernel (gdx_problem_set_kernel(prob, 0|2, rows))");
+              "the FP32 form runs with the 3D z-marching or direct-store kernel (gdx_problem_set_kernel(prob, 0|2, rows)) "
+              "or the fused 3D flux kernel (Q1, at most 2 Gauss points per direction)");
   if (precision != p->precision) gdx::batch_invalidate_problem(p->ctx, p);  // captured kernels change
   p->precision = precision;
   GDX_HIP(hipSetDevice(p->ctx->device));
@@ -433,7 +435,7 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
     // residual_and_jacobian at the stored state: no coefficient or separate RHS phase
     GDX_HIP(hipEventRecord(p->ev[1], st));
     GDX_HIP(hipEventRecord(p->ev[2], st));
-    int rcf = gdx::flux_assemble(p, nullptr, false, st);
+    int rcf = gdx::flux_assemble(p, nullptr, false, precision == GDX_PREC_FP32, st);
     if (rcf) {
       set_error("flux operator launch failed");
       return rcf;
@@ -563,6 +565,10 @@ int gdx_newton(gdx_problem *p, const gdx_newton_opts *o, double *x, gdx_newton_r
               GDX_ERR_ARG, "invalid Newton options");
   GDX_REQUIRE(o->init == GDX_NEWTON_INIT_GIVEN || o->init == GDX_NEWTON_INIT_POISSON, GDX_ERR_ARG,
               "unknown Newton initial guess");
+  GDX_REQUIRE(o->jac_precision == GDX_PREC_FP64 || o->jac_precision == GDX_PREC_FP32, GDX_ERR_ARG,
+              "unknown Jacobian precision");
+  GDX_REQUIRE(o->jac_precision == GDX_PREC_FP64 || gdx::flux_mixed_supported(p), GDX_ERR_UNSUPPORTED,
+              "FP32 Jacobians run in the fused 3D flux kernel (Q1, at most 2 Gauss points per direction)");
   if (gdx_nn *nn = o->precond) {
     GDX_REQUIRE(nn->ctx, GDX_ERR_STATE, "the preconditioner's context was finalized");
     GDX_REQUIRE(nn->ctx == p->ctx, GDX_ERR_ARG, "network and problem belong to different contexts");

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "gdx_device.h"
@@ -40,6 +41,7 @@ struct FluxData {
   double *xs = nullptr;           // state: free values by global id
   double *cm = nullptr;           // cell Jacobians and residuals [NC(NC+1)/2 + NC][ncell]
   double *tab = nullptr;          // reference-cell tables per Gauss point (FluxTab layout)
+  float *tab32 = nullptr;         // the same in FP32, for the Jacobian of the mixed form
   int64_t ncell = 0;
   bool state_set = false;
 };
@@ -109,11 +111,15 @@ __host__ __device__ constexpr int sym_index(int a, int b) {
 }
 
 // cell Jacobian and residual of one cell (the stage-1 arithmetic, shared by the
-// two-launch and the fused forms): J packed upper triangle, then r
-template <int D, int NQ, int MODE>
-__device__ __forceinline__ void flux_cell(const double *__restrict__ tab, const double *__restrict__ ulat, int64_t c0,
-                                          int64_t c1, int64_t c2, const KParams &kp, const FluxParams &fp,
-                                          double (&J)[(1 << D) * ((1 << D) + 1) / 2], double (&R)[1 << D]) {
+// two-launch and the fused forms): J packed upper triangle, then r.  JT is the
+// Jacobian's type: with float (the mixed form) its products and sums over the
+// Gauss points run in FP32 from the FP32 tables; grad u, kappa, gamma and the
+// residual stay FP64 either way.
+template <int D, int NQ, int MODE, class JT = double>
+__device__ __forceinline__ void flux_cell(const double *__restrict__ tab, const float *__restrict__ tab32,
+                                          const double *__restrict__ ulat, int64_t c0, int64_t c1, int64_t c2,
+                                          const KParams &kp, const FluxParams &fp,
+                                          JT (&J)[(1 << D) * ((1 << D) + 1) / 2], double (&R)[1 << D]) {
   using F = FluxTab<D>;
   constexpr int NC = 1 << D, NQD = D == 3 ? NQ * NQ * NQ : NQ * NQ, NJ = NC * (NC + 1) / 2;
   const int64_t sy = kp.nn[0], sz = (int64_t)kp.nn[0] * kp.nn[1];
@@ -122,7 +128,7 @@ __device__ __forceinline__ void flux_cell(const double *__restrict__ tab, const 
 #pragma unroll
   for (int bb = 0; bb < NC; ++bb) uc[bb] = ub[(bb & 1) + sy * ((bb >> 1) & 1) + (D == 3 ? sz * ((bb >> 2) & 1) : 0)];
 #pragma unroll
-  for (int k = 0; k < NJ; ++k) J[k] = 0.0;
+  for (int k = 0; k < NJ; ++k) J[k] = 0;
 #pragma unroll
   for (int a = 0; a < NC; ++a) R[a] = 0.0;
 #pragma unroll 1
@@ -147,14 +153,32 @@ __device__ __forceinline__ void flux_cell(const double *__restrict__ tab, const 
       ga[a] = v;
     }
     const double wk = wd * kap, wg = wd * gam;
+    if constexpr (std::is_same_v<JT, float>) {
+      const float *T32 = tab32 + q * F::QS;
+      const float wkf = (float)wk;
+      float gaf[NC];
 #pragma unroll
-    for (int a = 0; a < NC; ++a) {
-      R[a] = fma(wd, kap * ga[a] - fp.f * T[F::TP + a], R[a]);
-      const double wga = wg * ga[a];
+      for (int a = 0; a < NC; ++a) gaf[a] = (float)ga[a];
 #pragma unroll
-      for (int bb = a; bb < NC; ++bb) {
-        const int k = sym_index<NC>(a, bb);
-        J[k] = fma(wk, T[F::TGAB + k], fma(wga, ga[bb], J[k]));
+      for (int a = 0; a < NC; ++a) {
+        R[a] = fma(wd, kap * ga[a] - fp.f * T[F::TP + a], R[a]);
+        const float wga = (float)(wg * ga[a]);
+#pragma unroll
+        for (int bb = a; bb < NC; ++bb) {
+          const int k = sym_index<NC>(a, bb);
+          J[k] = fmaf(wkf, T32[F::TGAB + k], fmaf(wga, gaf[bb], J[k]));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NC; ++a) {
+        R[a] = fma(wd, kap * ga[a] - fp.f * T[F::TP + a], R[a]);
+        const double wga = wg * ga[a];
+#pragma unroll
+        for (int bb = a; bb < NC; ++bb) {
+          const int k = sym_index<NC>(a, bb);
+          J[k] = fma(wk, T[F::TGAB + k], fma(wga, ga[bb], J[k]));
+        }
       }
     }
   }
@@ -177,7 +201,7 @@ __global__ __launch_bounds__(256) void k_flux_cells(int64_t ncell, const double 
   const int64_t nc0 = kp.ncells[0], nc1 = kp.ncells[1];
   const int64_t c0 = c % nc0, c1 = (c / nc0) % nc1, c2 = D == 3 ? c / (nc0 * nc1) : 0;
   double J[NJ], R[NC];
-  flux_cell<D, NQ, MODE>(tab, ulat, c0, c1, c2, kp, fp, J, R);
+  flux_cell<D, NQ, MODE>(tab, nullptr, ulat, c0, c1, c2, kp, fp, J, R);
 #pragma unroll
   for (int k = 0; k < NJ; ++k) cm[k * ncell + c] = J[k];
 #pragma unroll
@@ -195,19 +219,28 @@ __global__ __launch_bounds__(256) void k_flux_cells(int64_t ncell, const double 
 // loads and stores overlap the other's cell arithmetic (8 x 8 x 4 rows with
 // 405 cells in 146 KB left one workgroup per CU: 0.29 against 0.25 ms at 96^3).
 // The values do not depend on the tile shape (same cell arithmetic, same
-// summation order per row).
-template <int NQ, int MODE, int R2, int R3, int NT>
+// summation order per row).  JT = float is the mixed form: cell Jacobians in
+// FP32 (records of 37 floats, the residuals in FP64 records of their own:
+// 49 KB, three workgroups per CU), the rows still summed in FP64.
+template <int NQ, int MODE, int R2, int R3, int NT, class JT>
 __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tiles, int ntiles,
                                                     const RowInfo *__restrict__ rowinfo,
                                                     const uint8_t *__restrict__ pat_fwd, int pstride,
-                                                    const double *__restrict__ tab, const double *__restrict__ ulat,
-                                                    double *__restrict__ values, double *__restrict__ b, KParams kp,
-                                                    FluxParams fp) {
-  constexpr int D = 3, NC = 8, NJ = 36, NE = NJ + NC, ES = NE + 1;  // odd record stride against bank conflicts
+                                                    const double *__restrict__ tab, const float *__restrict__ tab32,
+                                                    const double *__restrict__ ulat, double *__restrict__ values,
+                                                    double *__restrict__ b, KParams kp, FluxParams fp) {
+  constexpr int D = 3, NC = 8, NJ = 36, NE = NJ + NC;
+  constexpr bool F32 = std::is_same_v<JT, float>;
+  // odd record strides against bank conflicts; FP64: J and r in one record
+  constexpr int ESJ = F32 ? NJ + 1 : NE + 1, ESR = F32 ? NC + 1 : ESJ, OFFR = F32 ? 0 : NJ;
   constexpr int R1 = 8, CX = R1 + 1, CY = R2 + 1, CZ = R3 + 1, NCELL = CX * CY * CZ;
   constexpr int LPR = NT / (R1 * R2 * R3);  // lanes per row
   static_assert(NCELL <= NT && LPR >= 1, "one lane per cell");
-  __shared__ double cj[NCELL * ES];
+  __shared__ JT cj[NCELL * ESJ];
+  __shared__ double cjr_own[F32 ? NCELL * ESR : 1];
+  double *cjr;
+  if constexpr (F32) cjr = cjr_own;
+  else cjr = cj;
   const int tb = tile_of_block(ntiles);
   if (tb >= ntiles) return;
   const Tile t = tiles[tb];
@@ -216,18 +249,22 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
   if (tid < NCELL) {
     const int ix = tid % CX, iy = (tid / CX) % CY, iz = tid / (CX * CY);
     const int64_t cx = t.org[0] - 1 + ix, cy = t.org[1] - 1 + iy, cz = t.org[2] - 1 + iz;
-    double *dst = cj + tid * ES;
+    JT *dst = cj + tid * ESJ;
+    double *dstr = cjr + tid * ESR + OFFR;
     if (ix <= t.ext[0] && iy <= t.ext[1] && iz <= t.ext[2] && cx >= 0 && cx < kp.ncells[0] && cy >= 0 &&
         cy < kp.ncells[1] && cz >= 0 && cz < kp.ncells[2]) {
-      double J[NJ], R[NC];
-      flux_cell<D, NQ, MODE>(tab, ulat, cx, cy, cz, kp, fp, J, R);
+      JT J[NJ];
+      double R[NC];
+      flux_cell<D, NQ, MODE, JT>(tab, tab32, ulat, cx, cy, cz, kp, fp, J, R);
 #pragma unroll
       for (int k = 0; k < NJ; ++k) dst[k] = J[k];
 #pragma unroll
-      for (int a = 0; a < NC; ++a) dst[NJ + a] = R[a];
+      for (int a = 0; a < NC; ++a) dstr[a] = R[a];
     } else {
 #pragma unroll
-      for (int k = 0; k < NE; ++k) dst[k] = 0.0;
+      for (int k = 0; k < NJ; ++k) dst[k] = 0;
+#pragma unroll
+      for (int a = 0; a < NC; ++a) dstr[a] = 0.0;
     }
   }
   __syncthreads();
@@ -250,7 +287,7 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
       if (b0 < 0 || b0 > 1 || b1 < 0 || b1 > 1 || b2 < 0 || b2 > 1) continue;
       const int ai = (NC - 1) - cc, bb = b0 | (b1 << 1) | (b2 << 2);
       const int e = bb < ai ? sym_index<NC>(bb, ai) : sym_index<NC>(ai, bb);
-      acc += cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ES + e];
+      acc += (double)cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ESJ + e];
     }
     const int pos = fwd[sl];
     if (pos < kSlotDirichlet) dstv[pos] = acc;
@@ -260,7 +297,7 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) {
       const int c0 = cc & 1, c1 = (cc >> 1) & 1, c2 = (cc >> 2) & 1;
-      rr += cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ES + NJ + (NC - 1) - cc];
+      rr += cjr[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ESR + OFFR + (NC - 1) - cc];
     }
     b[ri.bidx] = rr;
   }
@@ -345,31 +382,38 @@ __global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tile
 }
 
 template <int D, int NQ, int MODE>
-void launch_flux_m(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
+int launch_flux_m(gdx_problem *p, const FluxData *fd, const FluxParams &fp, bool j32, hipStream_t st) {
   if constexpr (D == 3 && NQ <= 2) {  // (3 Gauss points per direction: the tables leave no room in LDS)
     const int nt = (int)p->tiles.size();
-    if (p->tileR[0] == 8 && p->tileR[1] == kFluxR2 && p->tileR[2] == kFluxR3) {  // the fused form's tile shape
-      hipLaunchKernelGGL((k_flux_fused<NQ, MODE, kFluxR2, kFluxR3, kFluxNT>), dim3(8u * ((nt + 7) / 8)), dim3(kFluxNT), 0, st,
-                         p->dev.tiles, nt, p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->tab, fd->ulat,
-                         p->dev.values, p->dev.b, p->kp, fp);
-      return;
+    if (flux_mixed_supported(p)) {  // the fused form's tile shape
+      if (j32)
+        hipLaunchKernelGGL((k_flux_fused<NQ, MODE, kFluxR2, kFluxR3, kFluxNT, float>), dim3(8u * ((nt + 7) / 8)),
+                           dim3(kFluxNT), 0, st, p->dev.tiles, nt, p->dev.rowinfo, p->dev.pat_fwd,
+                           pattern_stride(p->kp.S), fd->tab, fd->tab32, fd->ulat, p->dev.values, p->dev.b, p->kp, fp);
+      else
+        hipLaunchKernelGGL((k_flux_fused<NQ, MODE, kFluxR2, kFluxR3, kFluxNT, double>), dim3(8u * ((nt + 7) / 8)),
+                           dim3(kFluxNT), 0, st, p->dev.tiles, nt, p->dev.rowinfo, p->dev.pat_fwd,
+                           pattern_stride(p->kp.S), fd->tab, fd->tab32, fd->ulat, p->dev.values, p->dev.b, p->kp, fp);
+      return GDX_OK;
     }
   }
+  if (j32) return GDX_ERR_UNSUPPORTED;  // the mixed form exists in the fused kernel only
   hipLaunchKernelGGL((k_flux_cells<D, NQ, MODE>), dim3((unsigned)((fd->ncell + 255) / 256)), dim3(256), 0, st,
                      fd->ncell, fd->tab, fd->ulat, fd->cm, p->kp, fp);
   const int nt = (int)p->tiles.size();
   hipLaunchKernelGGL((k_flux_rows<D>), dim3(8u * ((nt + 7) / 8)), dim3(256), 0, st, p->dev.tiles, nt,
                      p->dev.rowinfo, p->dev.pat_fwd, pattern_stride(p->kp.S), fd->cm, fd->ncell, p->dev.values,
                      p->dev.b, p->kp);
+  return GDX_OK;
 }
 
 template <int D, int NQ>
-void launch_flux_t(gdx_problem *p, const FluxData *fd, const FluxParams &fp, hipStream_t st) {
+int launch_flux_t(gdx_problem *p, const FluxData *fd, const FluxParams &fp, bool j32, hipStream_t st) {
   switch (fp.mode) {
-    case 1: launch_flux_m<D, NQ, 1>(p, fd, fp, st); break;
-    case 2: launch_flux_m<D, NQ, 2>(p, fd, fp, st); break;
-    case 3: launch_flux_m<D, NQ, 3>(p, fd, fp, st); break;
-    default: launch_flux_m<D, NQ, 0>(p, fd, fp, st); break;
+    case 1: return launch_flux_m<D, NQ, 1>(p, fd, fp, j32, st);
+    case 2: return launch_flux_m<D, NQ, 2>(p, fd, fp, j32, st);
+    case 3: return launch_flux_m<D, NQ, 3>(p, fd, fp, j32, st);
+    default: return launch_flux_m<D, NQ, 0>(p, fd, fp, j32, st);
   }
 }
 
@@ -384,10 +428,14 @@ __global__ void k_fill_zero(int64_t n, double *__restrict__ x) {
 
 bool flux_supported(int dim, int order, int nq) { return order == 1 && (dim == 2 || dim == 3) && nq >= 1 && nq <= 3; }
 
+bool flux_mixed_supported(const gdx_problem *p) {
+  return p->D == 3 && p->NQ <= 2 && p->tileR[0] == 8 && p->tileR[1] == kFluxR2 && p->tileR[2] == kFluxR3;
+}
+
 void flux_free(gdx_problem *p) {
   FluxData *fd = flux_data(p);
   if (!fd) return;
-  void *ptrs[] = {fd->ulat, fd->gid_node, fd->xs, fd->cm, fd->tab};
+  void *ptrs[] = {fd->ulat, fd->gid_node, fd->xs, fd->cm, fd->tab, fd->tab32};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete fd;
@@ -459,14 +507,17 @@ int flux_setup(gdx_problem *p) {
     return GDX_ERR_ALLOC;
   }
   const std::vector<double> tab = flux_tables_host(p->kp, p->D, p->NQ);
-  if (dev_malloc(reinterpret_cast<void **>(&fd->tab), tab.size() * sizeof(double)) != hipSuccess) {
+  const std::vector<float> tab32(tab.begin(), tab.end());
+  if (dev_malloc(reinterpret_cast<void **>(&fd->tab), tab.size() * sizeof(double)) != hipSuccess ||
+      dev_malloc(reinterpret_cast<void **>(&fd->tab32), tab32.size() * sizeof(float)) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_ALLOC;
   }
   if ((p->dev_numbering ? gid_node_device(p, fd->gid_node) != GDX_OK
                          : hipMemcpy(fd->gid_node, gid_node.data(), gid_node.size() * sizeof(long long),
                                      hipMemcpyHostToDevice) != hipSuccess) ||
-      hipMemcpy(fd->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(fd->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(fd->tab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
     flux_free(p);
     return GDX_ERR_HIP;
   }
@@ -497,7 +548,8 @@ int flux_set_state(gdx_problem *p, const double *xh) {
 // Jacobian into the CSR values and residual into b at the state x (device,
 // by global id; nullptr = the stored state).  linear: p = 2, alpha = 1, beta = 0
 // (the Poisson operator with the same source), used for Newton's initial guess.
-int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st) {
+// j32: the mixed form (cell Jacobians in FP32; flux_mixed_supported problems).
+int flux_assemble(gdx_problem *p, const double *x, bool linear, bool j32, hipStream_t st) {
   if (!p->flux) return GDX_ERR_STATE;  // flux_setup runs in gdx_symbolic
   FluxData *fd = flux_data(p);
   const int64_t n = p->ngdofs;
@@ -506,13 +558,14 @@ int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st) 
                        x ? x : fd->xs, fd->ulat);
   const FluxParams fp = flux_params(p->desc, linear);
   if (p->tiles.empty()) return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
-  if (p->D == 3 && p->NQ == 2) launch_flux_t<3, 2>(p, fd, fp, st);
-  else if (p->D == 3 && p->NQ == 1) launch_flux_t<3, 1>(p, fd, fp, st);
-  else if (p->D == 3 && p->NQ == 3) launch_flux_t<3, 3>(p, fd, fp, st);
-  else if (p->D == 2 && p->NQ == 2) launch_flux_t<2, 2>(p, fd, fp, st);
-  else if (p->D == 2 && p->NQ == 1) launch_flux_t<2, 1>(p, fd, fp, st);
-  else if (p->D == 2 && p->NQ == 3) launch_flux_t<2, 3>(p, fd, fp, st);
-  else return GDX_ERR_UNSUPPORTED;
+  int rc = GDX_ERR_UNSUPPORTED;
+  if (p->D == 3 && p->NQ == 2) rc = launch_flux_t<3, 2>(p, fd, fp, j32, st);
+  else if (p->D == 3 && p->NQ == 1) rc = launch_flux_t<3, 1>(p, fd, fp, j32, st);
+  else if (p->D == 3 && p->NQ == 3) rc = launch_flux_t<3, 3>(p, fd, fp, j32, st);
+  else if (p->D == 2 && p->NQ == 2) rc = launch_flux_t<2, 2>(p, fd, fp, j32, st);
+  else if (p->D == 2 && p->NQ == 1) rc = launch_flux_t<2, 1>(p, fd, fp, j32, st);
+  else if (p->D == 2 && p->NQ == 3) rc = launch_flux_t<2, 3>(p, fd, fp, j32, st);
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
 }
 

@@ -286,6 +286,7 @@ int solve_dinv(gdx_problem *p, const double **dinv, int *n);
 int solve_csr_view(gdx_problem *p, const long long **rstart, const int **rlen, const int **cgid);
 void solve_spmv_dev(gdx_problem *p, const double *x, double *y, hipStream_t st);
 bool flux_supported(int dim, int order, int nq);
+bool flux_mixed_supported(const gdx_problem *p);  // the fused 3D kernel (FP32 cell Jacobians)
 int64_t num_cells(const gdx_problem *p);
 int cell_coef_alloc(gdx_problem *p);
 int launch_cell_coefficients(gdx_problem *p, hipStream_t st);
@@ -297,7 +298,7 @@ int flux_setup(gdx_problem *p);
 void flux_free(gdx_problem *p);
 double *flux_state(gdx_problem *p);
 int flux_set_state(gdx_problem *p, const double *xh);
-int flux_assemble(gdx_problem *p, const double *x, bool linear, hipStream_t st);
+int flux_assemble(gdx_problem *p, const double *x, bool linear, bool j32, hipStream_t st);
 int solve_spmv(gdx_problem *p, const double *x, double *y);
 int solve_spmv_timed(gdx_problem *p, int reps, double *ms, double *bytes);
 int solve_pcg(gdx_problem *p, double rtol, int maxiter, double *x, int *iters, double *relres, double *ms);

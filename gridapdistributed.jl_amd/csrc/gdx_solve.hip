@@ -748,7 +748,7 @@ int solve_newton(gdx_problem *p, const gdx_newton_opts *o, double *xh, gdx_newto
   // assemble at xs and return ||r||
   auto assemble = [&](bool linear, double *rnorm) -> int {
     const auto ta = clk::now();
-    int r = flux_assemble(p, xs, linear, st);
+    int r = flux_assemble(p, xs, linear, o->jac_precision == GDX_PREC_FP32, st);
     if (r) return r;
     hipLaunchKernelGGL(k_gather_b, g, b, 0, st, n, s->bidx, p->dev.b, s->b);
     dot2(s, s->b, s->b, nullptr, nullptr, s->sc + 6, st);

@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("GDX_LIBRARY", os.path.join(_PKG_ROOT, "libgdx.so"))
 GDX_OK = 0
 ERRORS = {-1: "GDX_ERR_ARG", -2: "GDX_ERR_HIP", -3: "GDX_ERR_STATE", -4: "GDX_ERR_UNSUPPORTED", -5: "GDX_ERR_ALLOC"}
 SUB_ASSEMBLED_ROWS, FULLY_ASSEMBLED_ROWS = 0, 1
-PREC_FP64 = 0
+PREC_FP64, PREC_FP32 = 0, 1
 KERNEL_HBM_PATTERNS = 0x100  # gdx_problem_set_kernel flag: position patterns read from HBM in every tile
 RHS_CONST, RHS_MANUFACTURED = 0, 1
 OP_DIFFUSION, OP_FLUX, OP_ELASTICITY = 0, 1, 2
@@ -61,7 +61,7 @@ class ProblemDesc(ctypes.Structure):
 class NewtonOpts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiter", ctypes.c_int32),
                 ("lin_rtol", ctypes.c_double), ("lin_maxiter", ctypes.c_int32), ("init", ctypes.c_int32),
-                ("precond", ctypes.c_void_p)]
+                ("precond", ctypes.c_void_p), ("jac_precision", ctypes.c_int32)]
 
 
 class NewtonReport(ctypes.Structure):

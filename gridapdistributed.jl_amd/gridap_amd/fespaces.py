@@ -345,15 +345,17 @@ class GpuProblem:
         return k
 
     def newton(self, x0=None, rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8, lin_maxiter=20000,
-               init=_lib.NEWTON_INIT_GIVEN, precond=None):
+               init=_lib.NEWTON_INIT_GIVEN, precond=None, jac_precision=_lib.PREC_FP64):
         """Newton on the device; returns (x, report dict).  Corrections by
         Jacobi-PCG, or by FPCG with precond's multilevel preconditioner (a
-        LearnedCoarseSolver on the same context)."""
+        LearnedCoarseSolver on the same context).  jac_precision=PREC_FP32
+        assembles the Jacobians with FP32 cell contractions (the residual and
+        the Newton test stay FP64)."""
         x = np.zeros(self.ngdofs) if x0 is None else np.array(x0, dtype=np.float64)
         if x.shape != (self.ngdofs,):
             raise ValueError(f"initial guess must have {self.ngdofs} free values")
         o = _lib.NewtonOpts(float(rtol), float(atol), int(maxiter), float(lin_rtol), int(lin_maxiter), int(init),
-                            None if precond is None else precond.handle)
+                            None if precond is None else precond.handle, int(jac_precision))
         r = _lib.NewtonReport()
         _lib.call("gdx_newton", self.handle, ctypes.byref(o), x.ctypes.data, ctypes.byref(r))
         rep = {k: getattr(r, k) for k, _ in _lib.NewtonReport._fields_}

@@ -57,6 +57,7 @@ end
 struct GdxNewtonOpts
   rtol::Float64; atol::Float64; maxiter::Int32; lin_rtol::Float64; lin_maxiter::Int32; init::Int32
   precond::Ptr{Cvoid}   # gdx_nn* or C_NULL (Jacobi-PCG corrections)
+  jac_precision::Int32  # 0: FP64 Jacobians, 1: FP32 cell contractions (residual FP64)
 end
 struct GdxNewtonReport
   iters::Int32; lin_iters::Int32; converged::Int32; res_ref::Float64; res_final::Float64
@@ -171,16 +172,16 @@ function residual_and_jacobian_hip(p::GdxProblem, ranks, ngdofs, x::Vector{Float
 end
 
 """
-    newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false, precond=C_NULL) -> (x, report)
+    newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false, precond=C_NULL, jac32=false) -> (x, report)
 
 solve(FESolver(NLSolver(method=:newton)), op) of test/PLaplacianTests.jl:54-57
 with device corrections (Jacobi-PCG, or two-grid FPCG when precond is a gdx_nn
 handle on the same context); x is the solution's free values by gid.
 """
 function newton_hip(p::GdxProblem, x0::Vector{Float64}; rtol=1e-10, atol=0.0, maxiter=50, lin_rtol=1e-8,
-                    lin_maxiter=20000, poisson_init=false, precond::Ptr{Cvoid}=C_NULL)
+                    lin_maxiter=20000, poisson_init=false, precond::Ptr{Cvoid}=C_NULL, jac32::Bool=false)
   x = copy(x0)
-  o = GdxNewtonOpts(rtol, atol, maxiter, lin_rtol, lin_maxiter, poisson_init ? 1 : 0, precond)
+  o = GdxNewtonOpts(rtol, atol, maxiter, lin_rtol, lin_maxiter, poisson_init ? 1 : 0, precond, jac32 ? 1 : 0)
   rep = Ref{GdxNewtonReport}()
   GC.@preserve x check(ccall((:gdx_newton, libgdx), Cint,
                              (Ptr{Cvoid}, Ref{GdxNewtonOpts}, Ptr{Float64}, Ref{GdxNewtonReport}), p.h, o, x, rep),

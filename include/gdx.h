@@ -46,7 +46,8 @@ extern "C" {
 
 /* numeric precision of the cell contractions; the CSR values are FP64 */
 #define GDX_PREC_FP64 0
-#define GDX_PREC_FP32 1 /* mixed form: sum-factorised contractions in FP32 (3D, z-marching or direct-store kernel) */
+#define GDX_PREC_FP32 1 /* mixed form: sum-factorised contractions in FP32 (3D, z-marching or direct-store kernel);
+                           for GDX_OP_FLUX the cell Jacobians in FP32 (fused 3D Q1 kernel), residual FP64 */
 
 /* operator kinds */
 #define GDX_OP_DIFFUSION 0 /* affine: a(u,v) and l(v) below */
@@ -228,6 +229,10 @@ typedef struct gdx_newton_opts {
                           gdx_nn_set_multigrid apply), re-attached to every new
                           Jacobian; the problem must fit the network (2n x 2n x 2n
                           Q1 cells). */
+  int32_t jac_precision; /* GDX_PREC_FP64, or GDX_PREC_FP32: the Jacobians' cell
+                            contractions in FP32 (the fused 3D Q1 flux kernel),
+                            rows summed and stored in FP64; the residual, the
+                            Newton test and the corrections' solves stay FP64 */
 } gdx_newton_opts;
 typedef struct gdx_newton_report {
   int32_t iters;       /* Newton corrections applied */

@@ -191,3 +191,96 @@ def test_newton_with_two_grid_fpcg_corrections(gpu_ctx, use_network):
     _, r1 = owned_global(oracle.run(q1).parts, prob.ngdofs)
     assert np.linalg.norm(r1) <= 1e-10 * np.linalg.norm(r0)
     net.close()
+
+
+MIXED_CASES = [flux3d(6, (2, 2, 1)), flux3d((9, 5, 7), (2, 1, 3), p=4.0, alpha=1.0),
+               flux3d(5, (2, 2, 2), strategy="full"), flux3d(8, (2, 2, 2), p=2.5, alpha=0.3, beta=0.7)]
+
+
+@pytest.mark.parametrize("q", MIXED_CASES, ids=lambda q: f"{q['cells']}-{q['parts']}-{q['strategy']}-p{q['flux'][0]}")
+def test_flux_mixed_jacobian(gpu_ctx, q):
+    """gdx_numeric(GDX_PREC_FP32) on a flux problem: cell Jacobians in FP32,
+    rows summed in FP64.  Sparsity and the FP64 residual are bit-identical to
+    the FP64 form; the Jacobian is within 1e-6 of the oracle's (relative to
+    its largest entry)."""
+    from gridap_amd._lib import PREC_FP32
+
+    print("mixed flux case", q["cells"], q["parts"], q["strategy"], q["flux"], flush=True)
+    u = smooth_state(q, 5)
+    qs = dict(q)
+    qs["state"] = u
+    r = oracle.run(qs)
+    op, prob, uh, parts64, b, A = gpu_flux(gpu_ctx, q, u)
+    prob.numeric(PREC_FP32)
+    prob.sync()
+    parts32 = [prob.export_part(k) for k in range(prob.nparts)]
+    worst = 0.0
+    for k, (P, Q, O) in enumerate(zip(parts64, parts32, r.parts)):
+        for key in STRUCT_KEYS:
+            assert np.array_equal(P[key], Q[key]), (k, key)
+        assert np.array_equal(P["b"].view(np.int64), Q["b"].view(np.int64)), k
+        worst = max(worst, rel_err(Q["val"], O["val"]))
+        assert rel_err(P["val"], O["val"]) <= TOL, k
+    print("mixed Jacobian max rel error", worst, flush=True)
+    assert 0.0 < worst <= 1e-6
+    prob.numeric()  # back to FP64: the same values as before, bit for bit
+    prob.sync()
+    for P, Q in zip(parts64, (prob.export_part(k) for k in range(prob.nparts))):
+        assert np.array_equal(P["val"].view(np.int64), Q["val"].view(np.int64))
+
+
+def test_flux_mixed_jacobian_unsupported_forms(gpu_ctx):
+    """The mixed form lives in the fused 3D kernel: 2D and 3 Gauss points per
+    direction are refused (GDX_ERR_UNSUPPORTED), for gdx_numeric and Newton."""
+    from gridap_amd._lib import GdxError, PREC_FP32
+
+    for q in (flux2d(6, (2, 2)), dict(flux3d(4, (1, 1, 1)), qdeg=4)):
+        op = flux_operator_from_workload(gpu_ctx, q)
+        prob = op._problem()
+        with pytest.raises(GdxError) as e:
+            prob.numeric(PREC_FP32)
+        assert e.value.code == -4
+        with pytest.raises(GdxError) as e:
+            prob.newton(None, jac_precision=PREC_FP32)
+        assert e.value.code == -4
+        prob.close()
+
+
+@pytest.mark.parametrize("multilevel", [False, True], ids=["jacobi-pcg", "multilevel-fpcg"])
+def test_newton_with_mixed_jacobians(gpu_ctx, multilevel):
+    """Newton with FP32-contracted Jacobians (residual, test and solves FP64)
+    reaches the FP64 Newton's solution within 1e-8 in at most one more step;
+    the returned state's residual is checked by the oracle."""
+    from gridap_amd import LearnedCoarseSolver
+    from gridap_amd._lib import PREC_FP32
+
+    cells = 32
+    q = W.plaplacian3d_q1(cells, (2, 2, 2), p=3.0, eps=1e-2)
+    op = flux_operator_from_workload(gpu_ctx, q)
+    prob = op._problem()
+    net = None
+    if multilevel:
+        net = LearnedCoarseSolver(gpu_ctx, cells // 2, 3)
+        net.attach(prob)
+        net.set_options(8, use_network=False)
+    kw = dict(rtol=1e-10, maxiter=30, lin_rtol=1e-10, init=NEWTON_INIT_POISSON, precond=net)
+    x64, rep64 = prob.newton(None, **kw)
+    x32, rep32 = prob.newton(None, jac_precision=PREC_FP32, **kw)
+    print("fp64", rep64, "\nfp32 jacobians", rep32, flush=True)
+    assert rep64["converged"] and rep32["converged"]
+    assert rep32["iters"] <= rep64["iters"] + 1
+    assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
+    g2n = np.full(prob.ngdofs, -1, np.int64)
+    for k in range(prob.nparts):
+        P = prob.export_part(k)
+        g2n[P["ldof_gid"]] = P["ldof_gnode"]
+    u = np.zeros(int(np.prod(W.lattice_shape(q))))
+    u[g2n] = x32
+    q0, q1 = dict(q), dict(q)
+    q0["state"], q1["state"] = np.zeros_like(u), u
+    _, r0 = owned_global(oracle.run(q0).parts, prob.ngdofs)
+    _, r1 = owned_global(oracle.run(q1).parts, prob.ngdofs)
+    assert np.linalg.norm(r1) <= 1e-10 * np.linalg.norm(r0)
+    if net is not None:
+        net.close()
+    prob.close()
