@@ -310,7 +310,7 @@ def bench_plaplacian(args):
                                                   "training_steps": C5_TRAIN_STEPS, "training_s": round(t_train, 2)}},
         "roofline": {"bound": "hbm", "achieved": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(8.0 * nnz / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_flux_fused<2,2,4,4,256>", "alg_bytes_per_launch": int(8 * nnz)},
+                     "traffic": traffic, "kernel": "k_flux_fused<2,2,4,4,256,double>", "alg_bytes_per_launch": int(8 * nnz)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_flux()
