@@ -357,76 +357,163 @@ __global__ void k_pack_weights(PackTab T, const uint16_t *__restrict__ thb, uint
   wpk[i] = (co < T.co[l] && k < T.kt[l]) ? thb[T.woff[l] + (int64_t)co * T.kt[l] + src] : (uint16_t)0;
 }
 
-// weight and bias gradient of a layer: per-workgroup partial sums over its
-// tiles, partial[block][W (CO CI 27) | b (CO)]
-template <int CI, int CO>
-__global__ __launch_bounds__(kT) void k_conv_wgrad(int n, const uint16_t *__restrict__ x, const float *__restrict__ g,
-                                                   float *__restrict__ partial) {
-  constexpr int E = CO * CI * 27, EPT = (E + kT - 1) / kT, CH = CI < 8 ? CI : 8;
-  __shared__ float sx[CH * kHV];
-  __shared__ float sg[CO * kT];
-  const int tid = threadIdx.x;
+// weight and bias gradient of a layer on the matrix cores: per workgroup the
+// GEMM  dW[co][ci 27 + t] += g[co][v] x[ci][v + tap t]  over the voxels v of
+// its tiles (8 x 8 x TZ), with a column of ones giving db[co].  M = output
+// channels (blocks of 16), N = (input channel, tap) and the ones column (blocks
+// of 16, every eighth block to each of the 8 waves), K = voxels, four per
+// v_mfma_f32_16x16x4f32: g is FP32 and the BF16 activations are exact in FP32.
+// Lane layout of that instruction (checked on the device): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and returns D[4 (l / 16) + i][l % 16]
+// for i < 4.  Partial sums per workgroup, partial[block][W (CO CI 27) | b
+// (CO)], are reduced across workgroups in a fixed order (k_reduce_partials).
+template <int CI, int CO, int TZ>
+struct WgradMfma {
+  static constexpr int NW = 8, NT = 64 * NW;  // waves and lanes per workgroup
+  static constexpr int KT = CI * 27, NBT = (KT + 1 + 15) / 16, NBW = (NBT + NW - 1) / NW, MB = (CO + 15) / 16;
+  static constexpr int HZ = TZ + 2, HV = 100 * HZ, SX = CI * HV, NV = 64 * TZ;
+  static constexpr int XROWS = CI * HZ * 10, XRU = (XROWS + NT - 1) / NT;      // halo rows of 10, per thread
+  static constexpr int GROWS = MB * 16 * TZ * 8, GRU = (GROWS + NT - 1) / NT;  // g rows of 8, per thread
+};
+
+template <int CI, int CO, int TZ>
+__global__ __launch_bounds__(512) void k_wgrad_mfma(int n, const uint16_t *__restrict__ x, const float *__restrict__ g,
+                                                    float *__restrict__ partial) {
+  using C = WgradMfma<CI, CO, TZ>;
+  __shared__ __align__(16) uint16_t sx[C::SX];
+  __shared__ __align__(16) float sg[C::MB * 16 * C::NV];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ntx = (n + 7) / 8, ntz = (n + TZ - 1) / TZ, ntiles = ntx * ntx * ntz;
   const int64_t n3 = (int64_t)n * n * n;
-  const int ntiles = tiles_of(n);
-  float acc[EPT];
+  const bool rows8 = (n & 7) == 0 && al16(x) && al16(g);
+  // this lane's column in each of the wave's N blocks: halo offset of (ci, tap), -1 the ones column, -2 padding
+  int kof[C::NBW];
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) acc[k] = 0.0f;
-  float bacc = 0.0f;
+  for (int j = 0; j < C::NBW; ++j) {
+    const int nb = wave + C::NW * j, kk = nb * 16 + (lane & 15);
+    const int ci = kk / 27, t = kk % 27;
+    kof[j] = (nb >= C::NBT || kk > C::KT) ? -2
+             : kk == C::KT                 ? -1
+                                           : ci * C::HV + (t / 9) * 100 + ((t / 3) % 3) * 10 + t % 3;
+  }
+  f32x4_t acc[C::MB][C::NBW];
+#pragma unroll
+  for (int mb = 0; mb < C::MB; ++mb)
+#pragma unroll
+    for (int j = 0; j < C::NBW; ++j) acc[mb][j] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    int x0, y0, z0;
-    tile_origin(tile, n, x0, y0, z0);
-    __syncthreads();
-    for (int i = tid; i < CO * kT; i += kT) {
-      const int co = i / kT, v = i % kT;
-      const int gx = x0 + (v & 7), gy = y0 + ((v >> 3) & 7), gz = z0 + (v >> 6);
-      sg[i] = (gx < n && gy < n && gz < n) ? g[co * n3 + ((int64_t)gz * n + gy) * n + gx] : 0.0f;
-    }
-    for (int c0 = 0; c0 < CI; c0 += CH) {
-      __syncthreads();
-      for (int i = tid; i < CH * kHV; i += kT) {
-        const int c = i / kHV, h = i % kHV, hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+    const int x0 = (tile % ntx) * 8, y0 = ((tile / ntx) % ntx) * 8, z0 = (tile / (ntx * ntx)) * TZ;
+    __syncthreads();  // the previous tile's reads are done
+    if (rows8) {
+      // rows of 8 voxels (x0 + 8 <= n): every load of the thread is issued before its LDS stores
+      f32x4_t glo[C::GRU], ghi[C::GRU];
+      u16x8_t mid[C::XRU];
+      uint16_t lf[C::XRU], rt[C::XRU];
+#pragma unroll
+      for (int u = 0; u < C::GRU; ++u) {
+        const int r = tid + C::NT * u, co = r / (TZ * 8), gz = z0 + (r / 8) % TZ, gy = y0 + r % 8;
+        glo[u] = ghi[u] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        if (r < C::GROWS && co < CO && gz < n && gy < n) {
+          const f32x4_t *src = reinterpret_cast<const f32x4_t *>(g + co * n3 + ((int64_t)gz * n + gy) * n + x0);
+          glo[u] = src[0];
+          ghi[u] = src[1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < C::XRU; ++u) {
+        const int r = tid + C::NT * u, c = r / (C::HZ * 10), gz = z0 - 1 + (r / 10) % C::HZ, gy = y0 - 1 + r % 10;
+        mid[u] = u16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+        lf[u] = rt[u] = 0;
+        if (r < C::XROWS && gy >= 0 && gy < n && gz >= 0 && gz < n) {
+          const uint16_t *row = x + (int64_t)c * n3 + ((int64_t)gz * n + gy) * n + x0;
+          mid[u] = *reinterpret_cast<const u16x8_t *>(row);
+          if (x0 > 0) lf[u] = row[-1];
+          if (x0 + 8 < n) rt[u] = row[8];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < C::GRU; ++u) {
+        const int r = tid + C::NT * u;  // g row r: channel r / (8 TZ), voxels (r % (8 TZ)) 8 .. + 7 of the tile
+        if (r < C::GROWS) {
+          f32x4_t *d = reinterpret_cast<f32x4_t *>(sg + (r / (TZ * 8)) * C::NV + (r % (TZ * 8)) * 8);
+          d[0] = glo[u];
+          d[1] = ghi[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < C::XRU; ++u) {
+        const int r = tid + C::NT * u;
+        if (r < C::XROWS) {
+          uint16_t *d = sx + 10 * r;
+          d[0] = lf[u];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) d[1 + j] = mid[u][j];
+          d[9] = rt[u];
+        }
+      }
+    } else {
+      for (int i = tid; i < C::MB * 16 * C::NV; i += C::NT) {
+        const int co = i / C::NV, v = i % C::NV;
+        const int gx = x0 + (v & 7), gy = y0 + ((v >> 3) & 7), gz = z0 + (v >> 6);
+        sg[i] = (co < CO && gx < n && gy < n && gz < n) ? g[co * n3 + ((int64_t)gz * n + gy) * n + gx] : 0.0f;
+      }
+      for (int i = tid; i < C::SX; i += C::NT) {
+        const int c = i / C::HV, h = i % C::HV, hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
         const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-        float v = 0.0f;
-        if (c0 + c < CI && gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
-          v = bf2f(x[(int64_t)(c0 + c) * n3 + ((int64_t)gz * n + gy) * n + gx]);
+        uint16_t v = 0;
+        if (gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
+          v = x[(int64_t)c * n3 + ((int64_t)gz * n + gy) * n + gx];
         sx[i] = v;
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < C::NV; k0 += 4) {
+      const int v = k0 + (lane >> 4);
+      const int voff = (v >> 6) * 100 + ((v >> 3) & 7) * 10 + (v & 7);
+      float a[C::MB];
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int e = tid + kT * k;
-        if (e >= E) break;
-        const int co = e / (CI * 27), ci = (e / 27) % CI, t = e % 27;
-        if (ci < c0 || ci >= c0 + CH) continue;
-        const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-        const float *xs = sx + (ci - c0) * kHV + dz * 100 + dy * 10 + dx;
-        const float *gs = sg + co * kT;
-        float s = 0.0f;
-        for (int v = 0; v < kT; ++v) s = fmaf(xs[(v >> 6) * 100 + ((v >> 3) & 7) * 10 + (v & 7)], gs[v], s);
-        acc[k] += s;
+      for (int mb = 0; mb < C::MB; ++mb) a[mb] = sg[(mb * 16 + (lane & 15)) * C::NV + v];
+#pragma unroll
+      for (int j = 0; j < C::NBW; ++j) {
+        const int ko = kof[j];
+        const float bv = ko >= 0 ? bf2f(sx[ko + voff]) : (ko == -1 ? 1.0f : 0.0f);
+#pragma unroll
+        for (int mb = 0; mb < C::MB; ++mb) acc[mb][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mb], bv, acc[mb][j], 0, 0, 0);
       }
     }
-    if (tid < CO) {
-      float s = 0.0f;
-      for (int v = 0; v < kT; ++v) s += sg[tid * kT + v];
-      bacc += s;
-    }
   }
-  float *dst = partial + (int64_t)blockIdx.x * (E + CO);
+  float *dst = partial + (int64_t)blockIdx.x * (C::KT * CO + CO);
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = tid + kT * k;
-    if (e < E) dst[e] = acc[k];
-  }
-  if (tid < CO) dst[E + tid] = bacc;
+  for (int mb = 0; mb < C::MB; ++mb)
+#pragma unroll
+    for (int j = 0; j < C::NBW; ++j) {
+      const int nb = wave + C::NW * j, kk = nb * 16 + (lane & 15);
+      if (nb >= C::NBT || kk > C::KT) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = mb * 16 + 4 * (lane >> 4) + i;
+        if (co >= CO) continue;
+        if (kk < C::KT) dst[co * C::KT + kk] = acc[mb][j][i];
+        else dst[C::KT * CO + co] = acc[mb][j][i];
+      }
+    }
 }
 
+// out[e] = sum over the nb blocks of partial[b][e], in a fixed order: eight
+// running sums over b = 8 i + j (so eight loads are in flight), then added j = 0..7
 __global__ void k_reduce_partials(int nb, int len, const float *__restrict__ partial, float *__restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= len) return;
-  float s = 0.0f;
-  for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * len + e];
-  out[e] = s;
+  float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  int b = 0;
+  for (; b + 8 <= nb; b += 8)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += partial[(int64_t)(b + j) * len + e];
+  for (int j = 0; b + j < nb; ++j) s[j] += partial[(int64_t)(b + j) * len + e];
+  float t = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) t += s[j];
+  out[e] = t;
 }
 
 // 2x2x2 mean (BF16 -> BF16); n = fine extent
@@ -655,8 +742,16 @@ static void conv_bwd_data(const Net &N, int n, const float *g, int layer, float 
 
 template <int CI, int CO>
 static void conv_wgrad(const Net &N, int n, const uint16_t *x, const float *g, int layer) {
-  const int nb = std::min(tiles_of(n), kPartialBlocks);
-  hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), dim3(nb), dim3(kT), 0, N.st, n, x, g, N.partial);
+  // 8 x 8 x 4 voxel tiles from 32^3 up, 8 x 8 x 1 below (more workgroups on the coarse grids)
+  const int txy = ((n + 7) / 8) * ((n + 7) / 8);
+  int nb;
+  if (n >= 32) {
+    nb = std::min(txy * ((n + 3) / 4), kPartialBlocks);
+    hipLaunchKernelGGL((k_wgrad_mfma<CI, CO, 4>), dim3(nb), dim3(WgradMfma<CI, CO, 4>::NT), 0, N.st, n, x, g, N.partial);
+  } else {
+    nb = std::min(txy * n, kPartialBlocks);
+    hipLaunchKernelGGL((k_wgrad_mfma<CI, CO, 1>), dim3(nb), dim3(WgradMfma<CI, CO, 1>::NT), 0, N.st, n, x, g, N.partial);
+  }
   const int len = CO * CI * 27 + CO;
   hipLaunchKernelGGL(k_reduce_partials, dim3(nblk(len)), dim3(kT), 0, N.st, nb, len, N.partial, N.grad + N.woff[layer]);
 }
