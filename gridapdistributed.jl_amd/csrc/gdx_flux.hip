@@ -245,6 +245,18 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
   if (tb >= ntiles) return;
   const Tile t = tiles[tb];
   const int tid = threadIdx.x;
+  // this lane's stage-2 row: its row-table entry and its pattern entries are
+  // loaded now, so that their latency overlaps the cell arithmetic
+  constexpr int SPL = (27 + LPR - 1) / LPR;  // slots per lane
+  const int r = tid / LPR, q = tid % LPR;
+  const int u1 = r % R1, u2 = (r / R1) % R2, u3 = r / (R1 * R2);
+  const bool rin = u1 < t.ext[0] && u2 < t.ext[1] && u3 < t.ext[2];
+  RowInfo ri{0, 0, -1, 0};
+  if (rin) ri = rowinfo[t.rowinfo_base + row_index(t, t.org[0] + u1, t.org[1] + u2, t.org[2] + u3)];
+  int posv[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j)
+    posv[j] = (ri.pid >= 0 && q + LPR * j < 27) ? pat_fwd[(size_t)ri.pid * pstride + q + LPR * j] : kSlotAbsent;
   // stage 1: the cells the tile's rows touch (zero outside the mesh)
   if (tid < NCELL) {
     const int ix = tid % CX, iy = (tid / CX) % CY, iz = tid / (CX * CY);
@@ -269,15 +281,12 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
   }
   __syncthreads();
   // stage 2: LPR lanes per row split its 27 slots
-  const int r = tid / LPR, q = tid % LPR;
-  const int u1 = r % R1, u2 = (r / R1) % R2, u3 = r / (R1 * R2);
-  if (u1 >= t.ext[0] || u2 >= t.ext[1] || u3 >= t.ext[2]) return;
-  const int g0 = t.org[0] + u1, g1 = t.org[1] + u2, g2 = t.org[2] + u3;
-  const RowInfo ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
-  if (ri.pid < 0) return;
-  const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
+  if (!rin || ri.pid < 0) return;
   double *dstv = values + ri.rowstart;
-  for (int sl = q; sl < 27; sl += LPR) {
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int sl = q + LPR * j;
+    if (sl >= 27) break;
     const int s0 = sl % 3, s1 = (sl / 3) % 3, s2 = sl / 9;
     double acc = 0.0;
 #pragma unroll
@@ -289,7 +298,7 @@ __global__ __launch_bounds__(NT) void k_flux_fused(const Tile *__restrict__ tile
       const int e = bb < ai ? sym_index<NC>(bb, ai) : sym_index<NC>(ai, bb);
       acc += (double)cj[(((u3 + c2) * CY + (u2 + c1)) * CX + (u1 + c0)) * ESJ + e];
     }
-    const int pos = fwd[sl];
+    const int pos = posv[j];
     if (pos < kSlotDirichlet) dstv[pos] = acc;
   }
   if (q == 0) {
@@ -335,6 +344,12 @@ __global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tile
       ri = rowinfo[t.rowinfo_base + row_index(t, g0, g1, g2)];
     }
     const bool live = ri.pid >= 0;
+    // the lane's pattern entries (slots e, e + NC, ...), loaded before the barrier
+    constexpr int SPL = (S + NC - 1) / NC;
+    int posv[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j)
+      posv[j] = (live && e + NC * j < S) ? pat_fwd[(size_t)ri.pid * pstride + e + NC * j] : kSlotAbsent;
     const int c0 = g0 - 1 + e0, c1 = g1 - 1 + e1, c2 = D == 3 ? g2 - 1 + e2 : 0;
     const bool incell = live && c0 >= 0 && c0 < kp.ncells[0] && c1 >= 0 && c1 < kp.ncells[1] &&
                         (D == 2 || (c2 >= 0 && c2 < kp.ncells[2]));
@@ -355,10 +370,12 @@ __global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tile
     if (live) {
       // slot (s0, s1, s2) of the row = sum over the cells c holding both nodes of
       // c's entry for column (s - c), cells in a fixed order
-      const uint8_t *fwd = pat_fwd + (size_t)ri.pid * pstride;
       double *dst = values + ri.rowstart;
       const int base = rl * NC;
-      for (int sl = e; sl < S; sl += NC) {
+#pragma unroll
+      for (int j = 0; j < SPL; ++j) {
+        const int sl = e + NC * j;
+        if (sl >= S) break;
         const int s0 = sl % 3, s1 = (sl / 3) % 3, s2 = D == 3 ? sl / 9 : 0;
         double acc = 0.0;
 #pragma unroll
@@ -367,7 +384,7 @@ __global__ __launch_bounds__(256) void k_flux_rows(const Tile *__restrict__ tile
           if (b0 < 0 || b0 > 1 || b1 < 0 || b1 > 1 || b2 < 0 || b2 > 1) continue;
           acc += part[base + cc][b0 | (b1 << 1) | (b2 << 2)];
         }
-        const int pos = fwd[sl];
+        const int pos = posv[j];
         if (pos < kSlotDirichlet) dst[pos] = acc;
       }
       if (e == 0) {
