@@ -175,16 +175,33 @@ __global__ __launch_bounds__(kB) void k_fine_stencil(int ng, int nf, const long 
     return;
   }
   const int m = (int)(B - A);
-  for (int k = t; k < m; k += kB) {
+  // the thread's entries k = t + kB i: column ids and values, then their
+  // lattice nodes, each group of loads issued together
+  constexpr int IT = (CAP + kB - 1) / kB;
+  int cg[IT];
+  double vv[IT];
+  long long gg[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int k = t + kB * i;
+    cg[i] = k < m ? cgid[A + k] : 0;
+    vv[i] = k < m ? val[A + k] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) gg[i] = t + kB * i < m ? gn[cg[i]] : 0;
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int k = t + kB * i;
+    if (k >= m) break;
     int lo = 0, hi = nr - 1;  // the row holding entry k: the last r with rs[r] - A <= k
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (rs[mid] - A <= k) lo = mid;
       else hi = mid - 1;
     }
-    const int sl = stencil_slot(gn[cgid[A + k]] - an[lo], nf);
+    const int sl = stencil_slot(gg[i] - an[lo], nf);
     if (sl >= 0) {
-      tile[sl][lo] = val[A + k];
+      tile[sl][lo] = vv[i];
       atomicOr(&present[lo], 1u << sl);
     }
   }
@@ -227,17 +244,24 @@ __global__ __launch_bounds__(kB) void k_galerkin_e(int mc, int S, int off, int64
                     Jx + dx <= mc - 1 && Jy + dy <= mc - 1 && Jz + dz <= mc - 1;
   if (live && L == 1) {
     const int nx = dx ? 3 : 7, ny = dy ? 3 : 7, nz = dz ? 3 : 7;
+    // y and x unrolled to their largest count (7) with the counts as
+    // predicates, so that a z plane's loads are issued together; the sums
+    // keep the nested loops' order
     for (int pz = 0; pz < nz; ++pz) {
       const int oaz = kGalPairs[dz + 1][pz][0], obz = kGalPairs[dz + 1][pz][1];
       const double wz = (oaz ? 0.5 : 1.0) * (obz ? 0.5 : 1.0);
       const int ez = 2 * dz + obz - oaz + 1;
       const int64_t az = 2 * Jz + oaz - off;
-      for (int py = 0; py < ny; ++py) {
+#pragma unroll
+      for (int py = 0; py < 7; ++py) {
+        if (py >= ny) break;
         const int oay = kGalPairs[dy + 1][py][0], oby = kGalPairs[dy + 1][py][1];
         const double wyz = wz * (oay ? 0.5 : 1.0) * (oby ? 0.5 : 1.0);
         const int ey = 2 * dy + oby - oay + 1;
         const int64_t ayz = (2 * Jy + oay - off) + (int64_t)S * az;
-        for (int px = 0; px < nx; ++px) {
+#pragma unroll
+        for (int px = 0; px < 7; ++px) {
+          if (px >= nx) break;
           const int oax = kGalPairs[dx + 1][px][0], obx = kGalPairs[dx + 1][px][1];
           const double w = wyz * (oax ? 0.5 : 1.0) * (obx ? 0.5 : 1.0);
           const int ex = 2 * dx + obx - oax + 1;
