@@ -475,6 +475,7 @@ __global__ __launch_bounds__(512) void k_wgrad_mfma(int n, const uint16_t *__res
       for (int mb = 0; mb < C::MB; ++mb) a[mb] = sg[(mb * 16 + (lane & 15)) * C::NV + v];
 #pragma unroll
       for (int j = 0; j < C::NBW; ++j) {
+        if (wave + C::NW * j >= C::NBT) break;  // padding blocks (wave-uniform)
         const int ko = kof[j];
         const float bv = ko >= 0 ? bf2f(sx[ko + voff]) : (ko == -1 ? 1.0f : 0.0f);
 #pragma unroll
