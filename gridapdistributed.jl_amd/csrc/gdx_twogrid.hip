@@ -556,14 +556,19 @@ __global__ void k_prolong_add_c(int mf, const double *__restrict__ ec, double *_
 constexpr int kCoarsestMax = 8;
 constexpr int kCoarsestLds = 6 * 6 * 6;
 constexpr int kCoarsestThreads = 1024;
+__host__ __device__ constexpr int coarsest_lanes(int tot) {
+  int L = 32;
+  while (L > 1 && L * tot > kCoarsestThreads) L >>= 1;
+  return L;
+}
+
+template <int L>
 __global__ __launch_bounds__(kCoarsestThreads) void k_coarsest(int m, double w, int sweeps,
                                                                const double *__restrict__ cstg,
                                                                const double *__restrict__ rhs,
                                                                double *__restrict__ e) {
   __shared__ double ea[512], eb[512], cl[27 * kCoarsestLds];
   const int tot = m * m * m;
-  int L = 32;
-  while (L > 1 && L * tot > kCoarsestThreads) L >>= 1;
   const int t = threadIdx.x, j = t / L, sub = t % L;  // the L lanes of a point share one wave
   int jx = 0, jy = 0, jz = 0;
   const bool live = j < tot && !padding(j, m, jx, jy, jz);
@@ -582,13 +587,26 @@ __global__ __launch_bounds__(kCoarsestThreads) void k_coarsest(int m, double w, 
     const int x = jx + k % 3 - 1, y = jy + (k / 3) % 3 - 1, z = jz + k / 9 - 1;
     if (live && x >= 0 && y >= 0 && z >= 0 && x <= m - 2 && y <= m - 2 && z <= m - 2) nbr |= 1u << k;
   }
+  // this lane's stencil coefficients and neighbour offsets (slots k = sub + L i),
+  // the same every sweep: read once into registers
+  constexpr int KL = (27 + L - 1) / L;
+  double cf[KL];
+  int of[KL];
+#pragma unroll
+  for (int i = 0; i < KL; ++i) {
+    const int k = sub + L * i;
+    const bool use = k < 27 && (nbr >> k & 1u);
+    cf[i] = use ? cst[k * tot + j] : 0.0;
+    of[i] = use ? (k % 3 - 1) + m * ((k / 3) % 3 - 1) + m * m * (k / 9 - 1) : 0;
+  }
   double *a = ea, *b = eb;
   for (int s = 0; s < sweeps; ++s) {
     double part = 0.0;
-    for (int k = sub; k < 27; k += L) {
-      if (!(nbr >> k & 1u)) continue;
-      const int off = (k % 3 - 1) + m * ((k / 3) % 3 - 1) + m * m * (k / 9 - 1);
-      part = fma(cst[k * tot + j], a[j + off], part);
+#pragma unroll
+    for (int i = 0; i < KL; ++i) {
+      const int k = sub + L * i;
+      if (k >= 27 || !(nbr >> k & 1u)) continue;
+      part = fma(cf[i], a[j + of[i]], part);
     }
     for (int o = L / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, L);
     if (sub == 0 && j < tot) b[j] = live ? a[j] + w * (rj - part) / dj : 0.0;
@@ -897,8 +915,16 @@ double *vcycle(TwoGrid *T, size_t l, hipStream_t st) {
   Level &L = T->lv[l];
   const int64_t v = (int64_t)L.m * L.m * L.m;
   if (l + 1 == T->lv.size()) {
-    hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(kCoarsestThreads), 0, st, L.m, kOmegaC, T->coarsest_sweeps, L.cst, L.rhs,
-                       L.e);
+    switch (coarsest_lanes(L.m * L.m * L.m)) {  // m <= 8: 2 to 32 lanes per point
+#define GDX_COARSEST(NL)                                                                                           \
+  case NL:                                                                                                         \
+    hipLaunchKernelGGL(k_coarsest<NL>, dim3(1), dim3(kCoarsestThreads), 0, st, L.m, kOmegaC, T->coarsest_sweeps, \
+                       L.cst, L.rhs, L.e);                                                                         \
+    break;
+      GDX_COARSEST(2) GDX_COARSEST(4) GDX_COARSEST(8) GDX_COARSEST(16) GDX_COARSEST(32)
+#undef GDX_COARSEST
+      default: break;
+    }
     return L.e;
   }
   double *a = L.e, *b = L.e2;
