@@ -70,14 +70,16 @@ def shift(x, dz, dy, dx):
     return out
 
 
-def conv(x, W, b):
-    """3x3x3 zero-padded convolution: y[co] = b[co] + sum_ci sum_t W[co,ci,t] x[ci](. + t)."""
+def conv(x, W, b, acc=np.float64):
+    """3x3x3 zero-padded convolution: y[co] = b[co] + sum_ci sum_t W[co,ci,t] x[ci](. + t),
+    accumulated in acc (FP64; FP32 to measure how FP32 sums move the BF16-stored
+    activations that follow)."""
     co = W.shape[0]
-    y = np.zeros((co,) + x.shape[1:], dtype=np.float64)
+    y = np.zeros((co,) + x.shape[1:], dtype=acc)
     for t, (dz, dy, dx) in enumerate(TAPS):
-        xs = shift(x.astype(np.float64), dz, dy, dx)
-        y += np.einsum("oc,czyx->ozyx", W[:, :, t].astype(np.float64), xs)
-    return y + b.astype(np.float64)[:, None, None, None]
+        xs = shift(x.astype(acc), dz, dy, dx)
+        y += np.einsum("oc,czyx->ozyx", W[:, :, t].astype(acc), xs)
+    return (y + b.astype(acc)[:, None, None, None]).astype(np.float64)
 
 
 def conv_backward(x, W, dy):
@@ -112,10 +114,13 @@ def up_backward(dy):
     return dy.reshape(C, Z // 2, 2, Y // 2, 2, X // 2, 2).sum(axis=(2, 4, 6))
 
 
-def forward(inp, theta, c, round_bf16=True):
+def forward(inp, theta, c, round_bf16=True, acc=np.float64):
     """Forward pass; returns (out, cache).  inp: [cin][n][n][n].
     round_bf16=False evaluates the same network exactly in FP64 (used to pin
-    the backward pass by finite differences)."""
+    the backward pass by finite differences).  acc: the convolutions'
+    accumulation type (FP32: the spread that FP32 sums, as the device forms
+    them, give the BF16-rounded activations; a one-ulp BF16 difference where a
+    sum lies within FP32 rounding of a BF16 rounding boundary)."""
     cin = inp.shape[0]
     if round_bf16:
         L = unpack(bf16(theta), cin, c)
@@ -125,20 +130,20 @@ def forward(inp, theta, c, round_bf16=True):
         st = lambda v: np.asarray(v, dtype=np.float64)
     relu = lambda v: np.maximum(v, 0.0)
     a0 = st(inp)
-    x1a = st(relu(conv(a0, *L[0])))
-    x1 = st(relu(conv(x1a, *L[1])))
+    x1a = st(relu(conv(a0, *L[0], acc=acc)))
+    x1 = st(relu(conv(x1a, *L[1], acc=acc)))
     p1 = st(pool(x1))
-    x2a = st(relu(conv(p1, *L[2])))
-    x2 = st(relu(conv(x2a, *L[3])))
+    x2a = st(relu(conv(p1, *L[2], acc=acc)))
+    x2 = st(relu(conv(x2a, *L[3], acc=acc)))
     p2 = st(pool(x2))
-    x3a = st(relu(conv(p2, *L[4])))
-    x3 = st(relu(conv(x3a, *L[5])))
+    x3a = st(relu(conv(p2, *L[4], acc=acc)))
+    x3 = st(relu(conv(x3a, *L[5], acc=acc)))
     u2 = up(x3)
-    y2a = st(relu(conv(u2, *L[6]) + x2))
-    y2 = st(relu(conv(y2a, *L[7])))
+    y2a = st(relu(conv(u2, *L[6], acc=acc) + x2))
+    y2 = st(relu(conv(y2a, *L[7], acc=acc)))
     u1 = up(y2)
-    y1a = st(relu(conv(u1, *L[8]) + x1))
-    out = conv(y1a, *L[9])
+    y1a = st(relu(conv(u1, *L[8], acc=acc) + x1))
+    out = conv(y1a, *L[9], acc=acc)
     out = out.astype(np.float32) if round_bf16 else out
     cache = dict(L=L, a0=a0, x1a=x1a, x1=x1, p1=p1, x2a=x2a, x2=x2, p2=p2, x3a=x3a, x3=x3, u2=u2, y2a=y2a, y2=y2,
                  u1=u1, y1a=y1a)

@@ -28,7 +28,17 @@ def rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
 
 
-@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2)])
+# n = 8, 16: the thin-tile and generic-staging paths of the MFMA kernels; 32
+# and 64: the 8 x 8 x 2 and 8 x 8 x 4 forward tiles and the 8 x 8 x 4 weight
+# gradient tiles with 16-byte row staging, as the learned workload runs them.
+# The device sums in FP32 and stores BF16 activations, the oracle sums in FP64:
+# where a sum lies within FP32 rounding of a BF16 rounding boundary the stored
+# activation can differ by one BF16 ulp.  With the activations of the 8^3 and
+# 16^3 cases no such flip occurs (agreement to 1e-7); with the larger ones of
+# the 32^3 and 64^3 cases some do, so the bound there is three times the
+# spread between the oracle's own FP32- and FP64-accumulated results (and
+# never below 1e-5) -- a misplaced tap or channel gives errors of order one.
+@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2), (32, 3), (64, 4)])
 def test_forward_backward_match_oracle(gpu_ctx, n, seed):
     print("nn oracle case", n, seed, flush=True)
     rng = np.random.default_rng(seed)
@@ -40,26 +50,33 @@ def test_forward_backward_match_oracle(gpu_ctx, n, seed):
     inp = rng.normal(size=(2, n, n, n)).astype(np.float32)
     out = net.forward(inp)
     ref, cache = NO.forward(inp, theta, C)
-    print("forward rel", rel(out, ref), flush=True)
     dout = rng.normal(size=(1, n, n, n)).astype(np.float32)
     g, dinp = net.backward(dout, want_dinp=True)
     gref, dinp_ref = NO.backward(dout, cache)
+    bf, bg, bd = 1e-5, 1e-5, 1e-5
+    if n >= 32:
+        ref32, cache32 = NO.forward(inp, theta, C, acc=np.float32)
+        g32, dinp32 = NO.backward(dout, cache32)
+        bf = max(1e-5, 3.0 * rel(ref32, ref))
+        bg = max(1e-5, 3.0 * rel(g32, gref))
+        bd = max(1e-5, 3.0 * rel(dinp32, dinp_ref))
+    print("forward rel", rel(out, ref), "bound", bf, flush=True)
     k = 0
     for li, (co, ci) in enumerate(NO.layer_shapes(2, C)):
         m = co * ci * 27
         print("layer", li, "W rel", rel(g[k:k + m], gref[k:k + m]), "b rel", rel(g[k + m:k + m + co], gref[k + m:k + m + co]),
               flush=True)
         k += m + co
-    print("dinp rel", rel(dinp, dinp_ref), flush=True)
-    assert rel(out, ref) < 1e-5
-    assert rel(g, gref) < 1e-5
-    assert rel(dinp, dinp_ref) < 1e-5
-    # layer by layer: every layer's gradient agrees (no layer hidden by the others' size)
-    k = 0
-    for co, ci in NO.layer_shapes(2, C):
-        m = co * ci * 27 + co
-        assert rel(g[k:k + m], gref[k:k + m]) < 1e-5, (co, ci)
-        k += m
+    print("grad rel", rel(g, gref), "bound", bg, "dinp rel", rel(dinp, dinp_ref), "bound", bd, flush=True)
+    assert rel(out, ref) < bf
+    assert rel(g, gref) < bg
+    assert rel(dinp, dinp_ref) < bd
+    if n < 32:  # layer by layer: every layer's gradient agrees (no layer hidden by the others' size)
+        k = 0
+        for co, ci in NO.layer_shapes(2, C):
+            m = co * ci * 27 + co
+            assert rel(g[k:k + m], gref[k:k + m]) < 1e-5, (co, ci)
+            k += m
     net.close()
 
 
