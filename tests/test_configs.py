@@ -3,6 +3,7 @@ workloads; the workload definitions the tests and bench.py use must agree
 with them.  CPU only."""
 import json
 import os
+import re
 
 import numpy as np
 
@@ -39,3 +40,13 @@ def test_c3_c5_match_the_workloads():
     assert e["cells"] == (96, 80, 72) and e["parts"] == (3, 2, 2) and e["qdeg"] == 4
     p5 = W.plaplacian3d_q1(**{k: tuple(v) if isinstance(v, list) else v for k, v in c5["args"].items()})
     assert p5["flux"] == (3.0, 1e-4, 1.0) and p5["cells"] == (96, 96, 96) and p5["rhs_const"] == 1.0
+
+
+def test_c5_network_training_matches_bench():
+    """the network-on form's training length in the bench is the recorded one"""
+    c5 = load("c5.json")
+    src = open(os.path.join(os.path.dirname(CFG), "bench.py")).read()
+    steps = int(re.search(r"^C5_TRAIN_STEPS = (\d+)", src, flags=re.M).group(1))
+    assert f"{steps} Adam steps" in c5["builder_fixed"]["network_on_training"]
+    assert f"k = 0 .. {steps - 1}" in c5["seeds"]["training_step_k"]
+    assert "31 * k + 7" in src and "LearnedCoarseSolver(ctx, n // 2, 0)" in src
