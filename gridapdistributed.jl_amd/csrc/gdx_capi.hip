@@ -75,7 +75,7 @@ int upload(T **dst, const std::vector<T> &src) {
 void free_device(gdx_problem *p) {
   gdx::Device &d = p->dev;
   void *ptrs[] = {d.W, d.F, d.E, d.modes, d.values, d.b, d.rowinfo, d.pat_fwd, d.pat_inv, d.pat_invr, d.pat_len,
-                  d.pat_dir, d.tiles, d.tile_pats, d.cellk, d.csr_rowptr, d.csr_colidx};
+                  d.pat_dir, d.tiles, d.tile_pats, d.cellk, d.cell_lame, d.csr_rowptr, d.csr_colidx};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   d = gdx::Device{};
@@ -239,6 +239,7 @@ void release_problem_device(gdx_problem *p) {
       e = nullptr;
     }
   p->coef_cell = false;
+  p->lame_cell = false;
 }
 }  // namespace gdx
 
@@ -510,18 +511,33 @@ int gdx_set_state(gdx_problem *p, const double *x) {
   return rc;
 }
 
+namespace {
+// the operators with a cell-wise coefficient: diffusion with a constant source
+// (kappa), elasticity (kappa scales lambda and mu)
+bool cell_coef_operator(const gdx_problem *p) {
+  return (p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST) ||
+         p->desc.op_kind == GDX_OP_ELASTICITY;
+}
+
+// a change of coefficient mode changes the captured coefficient or elasticity
+// launch: graphs holding p are dropped (new values in the same buffer replay as
+// they are)
+void set_coef_mode(gdx_problem *p, bool kappa, bool lame) {
+  if (p->coef_cell != kappa || p->lame_cell != lame) gdx::batch_invalidate_problem(p->ctx, p);
+  p->coef_cell = kappa;
+  p->lame_cell = lame;
+}
+}  // namespace
+
 int gdx_set_cell_coefficient(gdx_problem *p, const double *kappa) {
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
-  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
-              "cell-wise kappa needs the diffusion operator with a constant source");
+  GDX_REQUIRE(cell_coef_operator(p), GDX_ERR_UNSUPPORTED,
+              "cell-wise kappa needs the diffusion operator with a constant source, or elasticity");
   GDX_HIP(hipSetDevice(p->ctx->device));
-  // a change of coefficient mode changes the captured coefficient launch: graphs
-  // holding p are dropped (new values in the same buffer replay as they are)
-  if (p->coef_cell != (kappa != nullptr)) gdx::batch_invalidate_problem(p->ctx, p);
   if (!kappa) {
-    p->coef_cell = false;
+    set_coef_mode(p, false, false);
     return GDX_OK;
   }
   const int64_t nc = gdx::num_cells(p);
@@ -530,7 +546,31 @@ int gdx_set_cell_coefficient(gdx_problem *p, const double *kappa) {
   int rc = gdx::cell_coef_alloc(p);
   if (rc) return rc;
   GDX_HIP(hipMemcpy(p->dev.cellk, kappa, nc * sizeof(double), hipMemcpyHostToDevice));
-  p->coef_cell = true;
+  set_coef_mode(p, true, false);
+  return GDX_OK;
+}
+
+int gdx_set_cell_lame(gdx_problem *p, const double *lame) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_ELASTICITY, GDX_ERR_UNSUPPORTED, "cell-wise Lame parameters need elasticity");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  if (!lame) {
+    set_coef_mode(p, false, false);
+    return GDX_OK;
+  }
+  const int64_t nc = gdx::num_cells(p);
+  for (int64_t i = 0; i < nc; ++i) {
+    const double lam = lame[2 * i], mu = lame[2 * i + 1];
+    GDX_REQUIRE(std::isfinite(lam) && std::isfinite(mu) && mu > 0.0 && lam > -2.0 / 3.0 * mu, GDX_ERR_ARG,
+                "each cell needs mu > 0 and lambda > -2 mu / 3, both finite");
+  }
+  if (!p->dev.cell_lame)
+    GDX_REQUIRE(gdx::dev_malloc(reinterpret_cast<void **>(&p->dev.cell_lame), std::max<int64_t>(nc, 1) * 2 *
+                                sizeof(double)) == hipSuccess, GDX_ERR_ALLOC, "out of device memory");
+  GDX_HIP(hipMemcpy(p->dev.cell_lame, lame, nc * 2 * sizeof(double), hipMemcpyHostToDevice));
+  set_coef_mode(p, false, true);
   return GDX_OK;
 }
 
@@ -538,14 +578,17 @@ int gdx_cell_coefficient_grf(gdx_problem *p, uint64_t seed, double corr_len, dou
   GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
-  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION && p->desc.rhs_kind == GDX_RHS_CONST, GDX_ERR_UNSUPPORTED,
-              "cell-wise kappa needs the diffusion operator with a constant source");
+  GDX_REQUIRE(cell_coef_operator(p), GDX_ERR_UNSUPPORTED,
+              "cell-wise kappa needs the diffusion operator with a constant source, or elasticity");
   GDX_REQUIRE(corr_len > 0.0 && log_scale >= 0.0 && log_scale < 50.0, GDX_ERR_ARG, "invalid field parameters");
   GDX_HIP(hipSetDevice(p->ctx->device));
-  if (!p->coef_cell) gdx::batch_invalidate_problem(p->ctx, p);  // the coefficient mode changes
   int rc = gdx::grf_cell_coefficient(p, seed, corr_len, log_scale);
-  if (rc) set_error("random field generation failed");
-  return rc;
+  if (rc) {
+    set_error("random field generation failed");
+    return rc;
+  }
+  set_coef_mode(p, true, false);
+  return GDX_OK;
 }
 
 int gdx_export_cell_coefficient(gdx_problem *p, double *kappa) {

@@ -152,7 +152,6 @@ int grf_cell_coefficient(gdx_problem *p, uint64_t seed, double corr_len, double 
   (void)hipFree(b);
   (void)hipFree(wd);
   if (hipGetLastError() != hipSuccess) rc = GDX_ERR_HIP;
-  if (rc == GDX_OK) p->coef_cell = true;
   return rc;
 }
 

@@ -28,6 +28,20 @@
 // setup), and a lane writes the 3 rows' values at e: every store instruction
 // writes 64 consecutive doubles.  HBM traffic is the 8 B value writes (plus
 // 32 B of row table per node); the arithmetic is ~10 FLOP per stored value.
+//
+// Cell-wise Lame parameters (a piecewise-constant material: (lambda_c, mu_c)
+// per cell, or kappa_c scaling the constants, i.e. a Young's modulus field)
+// break the sum over cells out of the factors: an entry is the sum over the
+// cells c shared by its two nodes of the same expression with cell-local 1D
+// factors (the integrals over the one 1D cell) and (lambda_c, mu_c).  Along
+// each direction a node lies in one cell (a Q2 mid node) or two (a vertex, one
+// at the mesh boundary), its "slots"; the cells of an entry are the slot
+// combinations whose local factors at the column offset are nonzero.  Kernel
+// k_elast_cw: the layout of k_elast, with the node's slot factors
+// (3 directions x 2 slots x 2K+1 offsets x {K,M,P,Q}) and the pairs of its up
+// to 8 cells in LDS, and the loop over slot combinations (1 to 8, uniform over
+// the wave) inside each entry.  Same stores, ~4x the arithmetic of k_elast on
+// Q2 (3.4 slot combinations per node on average).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +95,8 @@ __global__ void k_elast_rows(const RowInfo *__restrict__ rowinfo, long long rinf
 
 struct ElastData {
   double *fac = nullptr;    // [3][nnmax][W][4]: K, M, P, Q at (node, offset + K)
+  double *sfac = nullptr;   // [3][nnmax][2][W][4]: the same per slot (one cell), zero past the slots
+  int *sinfo = nullptr;     // [3][nnmax]: first slot's cell * 4 + slot count (the second slot is the next cell)
   double *mass = nullptr;   // [3][nnmax]
   uint16_t *pinv = nullptr; // [npat][stride]: position -> o0 | o1 << 3 | o2 << 6 | component << 9
   ElastRow *rows = nullptr;
@@ -94,6 +110,7 @@ struct ElastArgs {
   double lam, mu, f[3];
   int nnmax, stride;
   long long nrownodes;
+  int ncell[3];
 };
 
 template <int K, int NPW>
@@ -174,12 +191,136 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastR
   }
 }
 
+// Cell-wise Lame parameters.  CW = 1: coef[c] = kappa_c, (lambda, mu) * kappa_c;
+// CW = 2: coef[2c], coef[2c + 1] = (lambda_c, mu_c).
+template <int K, int NPW, int CW>
+__global__ __launch_bounds__(64 * kWaves) void k_elast_cw(ElastArgs A, const ElastRow *__restrict__ rows,
+                                                          const double *__restrict__ sfac, const int *__restrict__ sinfo,
+                                                          const double *__restrict__ coef,
+                                                          const double *__restrict__ mass,
+                                                          const uint16_t *__restrict__ pinv,
+                                                          double *__restrict__ values, double *__restrict__ b) {
+  constexpr int W = 2 * K + 1;
+  constexpr int NS = 2 * W * 4;                  // slot factors of one direction
+  constexpr int NF = 3 * NS;                     // of one row node
+  constexpr int NL = (NF + 63) / 64;             // loads per lane
+  constexpr int MAXE = (3 * W * W * W + 63) / 64;  // row positions per lane
+  __shared__ __align__(16) double lds[kWaves][NPW][NF + 16];  // factors, then (lambda, mu) of 8 cells
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long node0 = ((long long)blockIdx.x * kWaves + wave) * NPW;
+  if (node0 >= A.nrownodes) return;  // whole waves exit together; no barrier below
+  const int nn = (int)min((long long)NPW, A.nrownodes - node0);
+  ElastRow ri[NPW];
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) ri[j] = rows[node0 + (j < nn ? j : 0)];
+  int si[NPW][3];
+  double fv[NPW][NL], cv[NPW][2];
+  int s3v[NPW][MAXE];
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) si[j][d] = sinfo[d * A.nnmax + ri[j].g[d]];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int r = lane + 64 * l, d = r / NS;
+      const int gd = d == 0 ? ri[j].g[0] : (d == 1 ? ri[j].g[1] : ri[j].g[2]);
+      fv[j][l] = r < NF ? sfac[((size_t)d * A.nnmax + gd) * NS + r % NS] : 0.0;
+    }
+    // lane = s0 + 2 s1 + 4 s2 < 8: the cell of that slot combination (a slot
+    // past the node's count reads the first slot's cell; its factors are zero)
+    cv[j][0] = cv[j][1] = 0.0;
+    if (lane < 8) {
+      long long c = 0;
+#pragma unroll
+      for (int d = 2; d >= 0; --d) {
+        const int sd = (lane >> d) & 1, ns = si[j][d] & 3;
+        c = c * A.ncell[d] + (si[j][d] >> 2) + (sd < ns ? sd : 0);
+      }
+      if (CW == 1) {
+        const double k = coef[c];
+        cv[j][0] = A.lam * k;
+        cv[j][1] = A.mu * k;
+      } else {
+        const double2 lm = reinterpret_cast<const double2 *>(coef)[c];
+        cv[j][0] = lm.x;
+        cv[j][1] = lm.y;
+      }
+    }
+    const uint16_t *pat = pinv + (size_t)ri[j].pid * A.stride;
+#pragma unroll
+    for (int k = 0; k < MAXE; ++k) {
+      const int e = lane + 64 * k;
+      s3v[j][k] = e < ri[j].len ? pat[e] : 0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NPW; ++j)
+    if (j < nn && lane < 3) {
+      const double m = mass[ri[j].g[0]] * mass[A.nnmax + ri[j].g[1]] * mass[2 * A.nnmax + ri[j].g[2]];
+      b[ri[j].bidx + lane] = A.f[lane] * m;
+    }
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+      if (lane + 64 * l < NF) lds[wave][j][lane + 64 * l] = fv[j][l];
+    if (lane < 8) {
+      lds[wave][j][NF + 2 * lane] = cv[j][0];
+      lds[wave][j][NF + 2 * lane + 1] = cv[j][1];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();  // LDS written and read by the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int j = 0; j < NPW; ++j) {
+    if (j >= nn) break;
+    const double *F = lds[wave][j];
+    const double *LM = F + NF;
+    const int len = ri[j].len;
+    const int ns0 = si[j][0] & 3, ns1 = si[j][1] & 3, ns2 = si[j][2] & 3;
+    double *row0 = values + ri[j].rowstart;
+#pragma unroll
+    for (int k = 0; k < MAXE; ++k) {
+      const int e = lane + 64 * k;
+      if (e >= len) break;
+      const int code = s3v[j][k];
+      const int o0 = code & 7, o1 = (code >> 3) & 7, o2 = (code >> 6) & 7, bc = code >> 9;
+      const bool b0 = bc == 0, b1 = bc == 1, b2 = bc == 2;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+      for (int s2 = 0; s2 < ns2; ++s2) {
+        const double2 *f2 = reinterpret_cast<const double2 *>(F + 2 * NS + (s2 * W + o2) * 4);
+        const double2 km2 = f2[0], pq2 = f2[1];
+        for (int s1 = 0; s1 < ns1; ++s1) {
+          const double2 *f1 = reinterpret_cast<const double2 *>(F + NS + (s1 * W + o1) * 4);
+          const double2 km1 = f1[0], pq1 = f1[1];
+          for (int s0 = 0; s0 < ns0; ++s0) {
+            const double2 *f0 = reinterpret_cast<const double2 *>(F + (s0 * W + o0) * 4);
+            const double2 km0 = f0[0], pq0 = f0[1];
+            const double2 lm = reinterpret_cast<const double2 *>(LM)[s0 + 2 * s1 + 4 * s2];
+            const double lam = lm.x, mu = lm.y, lpm = lm.x + lm.y;
+            const double t0 = km0.x * (km1.y * km2.y), t1 = km1.x * (km0.y * km2.y), t2 = km2.x * (km0.y * km1.y);
+            const double lap = mu * (t0 + t1 + t2);
+            const double Qb = b0 ? pq0.y : (b1 ? pq1.y : pq2.y), Pb = b0 ? pq0.x : (b1 ? pq1.x : pq2.x);
+            v0 += b0 ? fma(lpm, t0, lap) : (lam * pq0.x * Qb + mu * pq0.y * Pb) * (b1 ? km2.y : km1.y);
+            v1 += b1 ? fma(lpm, t1, lap) : (lam * pq1.x * Qb + mu * pq1.y * Pb) * (b0 ? km2.y : km0.y);
+            v2 += b2 ? fma(lpm, t2, lap) : (lam * pq2.x * Qb + mu * pq2.y * Pb) * (b0 ? km1.y : km0.y);
+          }
+        }
+      }
+      row0[e] = v0;
+      row0[(size_t)len + e] = v1;
+      row0[(size_t)2 * len + e] = v2;
+    }
+  }
+}
+
 }  // namespace
 
 void elast_free(gdx_problem *p) {
   ElastData *E = ed(p);
   if (!E) return;
-  void *ptrs[] = {E->fac, E->mass, E->pinv, E->rows};
+  void *ptrs[] = {E->fac, E->sfac, E->sinfo, E->mass, E->pinv, E->rows};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete E;
@@ -196,8 +337,39 @@ int elast_setup(gdx_problem *p) {
   for (int d = 0; d < 3; ++d) nnmax = std::max(nnmax, kp.nn[d]);
   if (nnmax > 65535) return GDX_ERR_UNSUPPORTED;  // 16-bit node coordinates in ElastRow
   std::vector<double> fac((size_t)3 * nnmax * W * 4, 0.0), mass((size_t)3 * nnmax, 0.0);
+  std::vector<double> sfac((size_t)3 * nnmax * 2 * W * 4, 0.0);
+  std::vector<int> sinfo((size_t)3 * nnmax, 0);
   for (int d = 0; d < 3; ++d) {
     const double h = kp.h[d];
+    // one cell's K, M, P, Q at local nodes (a, bb)
+    double loc[3][3][4] = {};
+    for (int a = 0; a <= K; ++a)
+      for (int bb = 0; bb <= K; ++bb)
+        for (int q = 0; q < NQ; ++q) {
+          loc[a][bb][0] += kp.qw[q] * kp.dphi[a][q] * kp.dphi[bb][q] / h;
+          loc[a][bb][1] += kp.qw[q] * h * kp.phi[a][q] * kp.phi[bb][q];
+          loc[a][bb][2] += kp.qw[q] * kp.dphi[a][q] * kp.phi[bb][q];
+          loc[a][bb][3] += kp.qw[q] * kp.phi[a][q] * kp.dphi[bb][q];
+        }
+    // slots of node i: the cells holding it, in increasing order, with its local node
+    for (int i = 0; i < kp.nn[d]; ++i) {
+      int cell[2], al[2], ns = 0;
+      if (i % K == 0) {
+        if (i / K - 1 >= 0) cell[ns] = i / K - 1, al[ns++] = K;
+        if (i / K < kp.ncells[d]) cell[ns] = i / K, al[ns++] = 0;
+      } else {
+        cell[ns] = i / K, al[ns++] = i % K;
+      }
+      if (ns == 0) return GDX_ERR_STATE;
+      sinfo[(size_t)d * nnmax + i] = cell[0] * 4 + ns;
+      for (int sl = 0; sl < ns; ++sl)
+        for (int o = -K; o <= K; ++o) {
+          const int bb = al[sl] + o;
+          if (bb < 0 || bb > K) continue;
+          double *f = &sfac[((((size_t)d * nnmax + i) * 2 + sl) * W + (o + K)) * 4];
+          for (int t = 0; t < 4; ++t) f[t] = loc[al[sl]][bb][t];
+        }
+    }
     for (int e = 0; e < kp.ncells[d]; ++e)
       for (int a = 0; a <= K; ++a) {
         const int i = K * e + a;
@@ -264,6 +436,8 @@ int elast_setup(gdx_problem *p) {
     return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
   bool ok = up(reinterpret_cast<void **>(&E->fac), fac.data(), fac.size() * sizeof(double)) &&
+            up(reinterpret_cast<void **>(&E->sfac), sfac.data(), sfac.size() * sizeof(double)) &&
+            up(reinterpret_cast<void **>(&E->sinfo), sinfo.data(), sinfo.size() * sizeof(int)) &&
             up(reinterpret_cast<void **>(&E->mass), mass.data(), mass.size() * sizeof(double)) &&
             up(reinterpret_cast<void **>(&E->pinv), code.data(), code.size() * sizeof(uint16_t)) &&
             up(reinterpret_cast<void **>(&E->rows), rows.data(), rows.size() * sizeof(ElastRow));
@@ -311,10 +485,24 @@ int elast_assemble(gdx_problem *p, hipStream_t st) {
   A.nnmax = E->nnmax;
   A.stride = E->stride;
   A.nrownodes = E->nrownodes;
+  for (int d = 0; d < 3; ++d) A.ncell[d] = (int)p->kp.ncells[d];
   const long long per = (long long)kWaves * kNodesPerWave;
   const long long blocks = (E->nrownodes + per - 1) / per;
   if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
   const dim3 grid((unsigned)blocks), block(64 * kWaves);
+  if (p->coef_cell || p->lame_cell) {
+    const double *coef = p->coef_cell ? p->dev.cellk : p->dev.cell_lame;
+    if (!coef) return GDX_ERR_STATE;
+#define GDX_ELAST_CW(KK, CW)                                                                                       \
+  hipLaunchKernelGGL((k_elast_cw<KK, kNodesPerWave, CW>), grid, block, 0, st, A, E->rows, E->sfac, E->sinfo, coef, \
+                     E->mass, E->pinv, p->dev.values, p->dev.b)
+    if (p->K == 2 && p->coef_cell) GDX_ELAST_CW(2, 1);
+    else if (p->K == 2) GDX_ELAST_CW(2, 2);
+    else if (p->coef_cell) GDX_ELAST_CW(1, 1);
+    else GDX_ELAST_CW(1, 2);
+#undef GDX_ELAST_CW
+    return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+  }
   if (p->K == 2)
     hipLaunchKernelGGL((k_elast<2, kNodesPerWave>), grid, block, 0, st, A, E->rows, E->fac, E->mass, E->pinv,
                        p->dev.values, p->dev.b);

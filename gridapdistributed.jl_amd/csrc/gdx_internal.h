@@ -122,6 +122,7 @@ struct Device {
   Tile *tiles = nullptr;
   int *tile_pats = nullptr;    // per tile: its distinct pattern ids
   double *cellk = nullptr;     // kappa per cell (cell-wise coefficient mode)
+  double *cell_lame = nullptr; // elasticity: (lambda, mu) per cell (cell-wise Lame mode)
   // local CSR of every part, concatenated (device symbolic phase): global row
   // offsets (rows_total + 1 entries) and local column ids
   long long *csr_rowptr = nullptr;
@@ -188,6 +189,8 @@ struct gdx_problem {
   void *flux = nullptr;   // nonlinear operator state (gdx_flux.hip)
   void *elast = nullptr;  // elasticity tables (gdx_elast.hip)
   bool coef_cell = false; // kappa from dev.cellk instead of the Fourier modes
+                          // (elasticity: kappa scales lambda and mu on each cell)
+  bool lame_cell = false; // elasticity: (lambda, mu) from dev.cell_lame
   void *pv = nullptr;     // PRange maps of the device PVectors (gdx_pvector.hip)
   std::vector<gdx_pvector *> pvecs;  // live PVectors on this problem
 };

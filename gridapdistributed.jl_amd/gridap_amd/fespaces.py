@@ -335,6 +335,15 @@ class GpuProblem:
         kappa = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
         _lib.call("gdx_set_cell_coefficient", self.handle, kappa.ctypes.data)
 
+    def set_cell_lame(self, lame):
+        """Elasticity: (lambda, mu) per cell (shape (ncells, 2), x fastest) for the
+        next numeric assembly; None returns to the descriptor's constants."""
+        if lame is None:
+            _lib.call("gdx_set_cell_lame", self.handle, None)
+            return
+        lame = np.ascontiguousarray(lame, dtype=np.float64).reshape(-1, 2)
+        _lib.call("gdx_set_cell_lame", self.handle, lame.ctypes.data)
+
     def cell_coefficient_grf(self, seed, corr_len, log_scale):
         """kappa = exp(log_scale g), g a unit-variance Gaussian random field drawn on the device."""
         _lib.call("gdx_cell_coefficient_grf", self.handle, int(seed), float(corr_len), float(log_scale))

@@ -172,6 +172,25 @@ function residual_and_jacobian_hip(p::GdxProblem, ranks, ngdofs, x::Vector{Float
 end
 
 """
+    set_cell_lame!(p, lame)
+
+Piecewise-constant material of a GDX_OP_ELASTICITY problem: lame is a
+2 x ncells matrix, (lambda_c, mu_c) in column c (cells x fastest), for the
+constitutive law lambda_c tr(eps) I + 2 mu_c eps; `nothing` returns to the
+descriptor's constants.  Takes effect at the next assembly.
+"""
+function set_cell_lame!(p::GdxProblem, lame::Union{Nothing,Matrix{Float64}})
+  if lame === nothing
+    check(ccall((:gdx_set_cell_lame, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), p.h, C_NULL), :gdx_set_cell_lame)
+  else
+    size(lame, 1) == 2 || error("lame must be 2 x ncells")
+    GC.@preserve lame check(ccall((:gdx_set_cell_lame, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), p.h, lame),
+                            :gdx_set_cell_lame)
+  end
+  nothing
+end
+
+"""
     newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false, precond=C_NULL, jac32=false) -> (x, report)
 
 solve(FESolver(NLSolver(method=:newton)), op) of test/PLaplacianTests.jl:54-57

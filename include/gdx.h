@@ -206,6 +206,17 @@ int gdx_pcg(gdx_problem *prob, double rtol, int32_t maxiter, double *xsol, int32
 int gdx_set_cell_coefficient(gdx_problem *prob, const double *kappa);
 int gdx_cell_coefficient_grf(gdx_problem *prob, uint64_t seed, double corr_len, double log_scale);
 int gdx_export_cell_coefficient(gdx_problem *prob, double *kappa);
+/* Piecewise-constant material for GDX_OP_ELASTICITY: the constitutive law
+   sigma(eps) = lambda_c tr(eps) I + 2 mu_c eps on cell c (the Lame parameters of
+   a(u,v) = int( eps(v) : sigma(eps(u)) ) as CellFields,
+   test/MultiFieldTests.jl:25 for the vector-valued space).
+   gdx_set_cell_lame: host array of 2 ncells values, (lambda_c, mu_c) per cell,
+   cells x fastest; each pair needs mu_c > 0 and lambda_c > -2 mu_c / 3; NULL
+   returns to the descriptor's constants.  On an elasticity problem
+   gdx_set_cell_coefficient and gdx_cell_coefficient_grf set kappa_c scaling both
+   constants (a Young's modulus field at fixed Poisson ratio).  The last call of
+   the three decides; it takes effect at the next gdx_numeric. */
+int gdx_set_cell_lame(gdx_problem *prob, const double *lame);
 
 /* Nonlinear operators (GDX_OP_FLUX).  The state uh of
    residual_and_jacobian(op, uh) (test/PLaplacianTests.jl:44-48) is given by its

@@ -80,6 +80,9 @@ typedef struct {
      face x = x0, its edges and vertices included). */
   double lame_lambda, lame_mu;
   double body_force[3];
+  /* op_kind 2: (lambda, mu) per cell (x fastest, 2 values a cell), NULL = the
+     constants above: sigma = lambda_c tr(eps) I + 2 mu_c eps on cell c */
+  const double *cell_lame;
 } orc_desc;
 
 typedef struct {
@@ -115,6 +118,7 @@ typedef struct {
   double *modes;
   double *state;
   double *cell_kappa;
+  double *cell_lame;
   int nparts, K, D, nq;
   int nc;          /* components per node: dim for elasticity, else 1 */
   i64 nn[3];       /* global node counts per dim */
@@ -590,7 +594,12 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
           /* eps(v) : (lambda tr(eps(u)) I + 2 mu eps(u)) with v = phi_a e_ca,
              u = phi_b e_cb:  lambda d_ca(phi_a) d_cb(phi_b)
                             + mu d_cb(phi_a) d_ca(phi_b) + mu [ca == cb] grad(phi_a).grad(phi_b) */
-          const double lam = C->d.lame_lambda, mu = C->d.lame_mu;
+          double lam = C->d.lame_lambda, mu = C->d.lame_mu;
+          if (C->d.cell_lame) {
+            const i64 c = cx + C->d.cells[0] * (cy + C->d.cells[1] * cz);
+            lam = C->d.cell_lame[2 * c];
+            mu = C->d.cell_lame[2 * c + 1];
+          }
           for (int ca = 0; ca < nc; ca++)
             for (int a = 0; a < ncn; a++) {
               const int ia = ca * ncn + a;
@@ -1064,6 +1073,13 @@ void *orc_run(const orc_desc *d, int inplace_reps) {
     memcpy(C->cell_kappa, d->cell_kappa, sizeof(double) * nc);
   }
   C->d.cell_kappa = C->cell_kappa;
+  C->cell_lame = NULL;
+  if (d->cell_lame) {
+    i64 nc = d->cells[0] * (d->dim > 1 ? d->cells[1] : 1) * (d->dim > 2 ? d->cells[2] : 1);
+    C->cell_lame = malloc(sizeof(double) * 2 * nc);
+    memcpy(C->cell_lame, d->cell_lame, sizeof(double) * 2 * nc);
+  }
+  C->d.cell_lame = C->cell_lame;
   double t0 = now_s();
   setup(C);
   double t1 = now_s();
@@ -1139,6 +1155,7 @@ void orc_free(void *h) {
   free(C->modes);
   free(C->state);
   free(C->cell_kappa);
+  free(C->cell_lame);
   free(C);
 }
 

@@ -198,22 +198,68 @@ def assembled_factors_1d(p, d):
     return Km, Mm, Pm, Qm, m
 
 
+def cell_factors_1d(p, d):
+    """The 1D matrices of assembled_factors_1d before the sum over cells: for
+    each cell e along direction d, (K, M, P, Q) of that cell alone (nn_d x nn_d,
+    nonzero on the cell's K + 1 nodes)."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    nq = p["qdeg"] // 2 + 1
+    xq, wq = gauss01(nq)
+    B = lagrange01(K)
+    phi = np.array([nppoly.polyval(xq, c) for c in B])
+    dphi = np.array([nppoly.polyval(xq, nppoly.polyder(c)) for c in B]) / h[d]
+    w = wq * h[d]
+    out = []
+    for e in range(cells[d]):
+        s = slice(K * e, K * e + K + 1)
+        f = [np.zeros((nn[d], nn[d])) for _ in range(4)]
+        f[0][s, s] = np.einsum("aq,bq,q->ab", dphi, dphi, w)
+        f[1][s, s] = np.einsum("aq,bq,q->ab", phi, phi, w)
+        f[2][s, s] = np.einsum("aq,bq,q->ab", dphi, phi, w)
+        f[3][s, s] = np.einsum("aq,bq,q->ab", phi, dphi, w)
+        out.append(f)
+    return out
+
+
 def elasticity_global(p):
     """Linear elasticity on the whole lattice, dof = 3 * lattice node + component
     (x-fastest nodes, no Dirichlet elimination): the matrix as a scipy sparse
-    matrix and the body-force vector.  Built from 1D assembled factors as the sum
-    of Kronecker products of the bilinear form's separable terms,
+    matrix and the body-force vector.  Built from 1D factors as the sum of
+    Kronecker products of the bilinear form's separable terms,
       (a, b) block = lambda d_a(v) d_b(u) + mu d_b(v) d_a(u) + mu [a == b] grad(v).grad(u),
-    an independent route from the C oracle's per-cell 3D quadrature."""
-    import scipy.sparse as sp
-
-    lam, mu = p["lame"]
+    an independent route from the C oracle's per-cell 3D quadrature.  With
+    constant Lame parameters the factors are assembled over the 1D mesh; with
+    p["cell_lame"] ((lambda, mu) per cell, x fastest) every cell adds its own
+    Kronecker terms, weighted by its pair."""
     F = [assembled_factors_1d(p, d) for d in range(3)]
+    m = np.kron(F[2][4], np.kron(F[1][4], F[0][4]))
+    bf = np.asarray(p["body_force"], np.float64)
+    bvec = (m[:, None] * bf[None, :]).ravel()
+    if p.get("cell_lame") is None:
+        lam, mu = p["lame"]
+        KMPQ = [dict(K=F[d][0], M=F[d][1], P=F[d][2], Q=F[d][3]) for d in range(3)]
+        return _elasticity_blocks(KMPQ, lam, mu).tocsr(), bvec
+    cells = [int(c) for c in p["cells"]]
+    lame = np.asarray(p["cell_lame"], np.float64).reshape(-1, 2)
+    C = [cell_factors_1d(p, d) for d in range(3)]
+    A = None
+    for ez in range(cells[2]):
+        for ey in range(cells[1]):
+            for ex in range(cells[0]):
+                lam, mu = lame[ex + cells[0] * (ey + cells[1] * ez)]
+                KMPQ = [dict(zip("KMPQ", C[d][e])) for d, e in enumerate((ex, ey, ez))]
+                t = _elasticity_blocks(KMPQ, lam, mu)
+                A = t if A is None else A + t
+    return A.tocsr(), bvec
+
+
+def _elasticity_blocks(KMPQ, lam, mu):
+    """sum over the 3 x 3 component blocks of the separable terms, with the 1D
+    factors KMPQ[d] = {"K", "M", "P", "Q"}; dof = 3 * node + component"""
+    import scipy.sparse as sp
 
     def kron3(f0, f1, f2):  # x fastest: node = x + nx (y + ny z)
         return sp.kron(sp.csr_matrix(f2), sp.kron(sp.csr_matrix(f1), sp.csr_matrix(f0)))
-
-    KMPQ = [dict(K=F[d][0], M=F[d][1], P=F[d][2], Q=F[d][3]) for d in range(3)]
 
     def term(kinds):  # kinds[d] in K, M, P, Q
         return kron3(KMPQ[0][kinds[0]], KMPQ[1][kinds[1]], KMPQ[2][kinds[2]])
@@ -237,10 +283,7 @@ def elasticity_global(p):
             E = sp.csr_matrix(([1.0], ([a], [b])), shape=(3, 3))
             t = sp.kron(blk, E)
             A = t if A is None else A + t
-    m = np.kron(F[2][4], np.kron(F[1][4], F[0][4]))
-    bf = np.asarray(p["body_force"], np.float64)
-    bvec = (m[:, None] * bf[None, :]).ravel()
-    return A.tocsr(), bvec
+    return A
 
 
 def elasticity_free_dofs(p):

@@ -29,6 +29,7 @@ class OrcDesc(ctypes.Structure):
         ("flux_beta", ctypes.c_double), ("state", ctypes.POINTER(ctypes.c_double)),
         ("cell_kappa", ctypes.POINTER(ctypes.c_double)),
         ("lame_lambda", ctypes.c_double), ("lame_mu", ctypes.c_double), ("body_force", ctypes.c_double * 3),
+        ("cell_lame", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -114,6 +115,13 @@ def make_desc(p):
         d.cell_kappa = ck.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     else:
         d.cell_kappa = None
+    cl = p.get("cell_lame")  # elasticity: (lambda, mu) per cell, x fastest
+    if cl is not None:
+        cl = np.ascontiguousarray(cl, dtype=np.float64).reshape(-1, 2).ravel()
+        d._cl_keep = cl
+        d.cell_lame = cl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    else:
+        d.cell_lame = None
     return d
 
 

@@ -50,3 +50,10 @@ def test_c5_network_training_matches_bench():
     assert f"{steps} Adam steps" in c5["builder_fixed"]["network_on_training"]
     assert f"k = 0 .. {steps - 1}" in c5["seeds"]["training_step_k"]
     assert "31 * k + 7" in src and "LearnedCoarseSolver(ctx, n // 2, 0)" in src
+
+
+def test_c3_cellwise_seed_matches_bench():
+    c3 = load("c3.json")
+    src = open(os.path.join(os.path.dirname(CFG), "bench.py")).read()
+    seed = int(re.search(r"^ELAST_CELLWISE_SEED = (\d+)", src, flags=re.M).group(1))
+    assert c3["seeds"]["elasticity_cellwise_young_modulus"] == seed

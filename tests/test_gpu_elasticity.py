@@ -11,7 +11,12 @@
 - repeated assembly is bit-identical; Jacobi-PCG solves the clamped system
   (checked against scipy on the oracle's matrix);
 - an elasticity problem and Q1 Poisson problems replayed in one hipGraph give
-  the eager result.
+  the eager result;
+- a piecewise-constant material ((lambda_c, mu_c) per cell, or kappa_c scaling
+  the constants): parity with the oracle given the same pairs, within 1e-12;
+  the kappa form is bit-identical to the pairs (lambda kappa_c, mu kappa_c);
+  clearing it returns the constant form bit for bit; a graph captured in the
+  cell-wise mode replays what eager assembly writes, and a mode change drops it.
 """
 import numpy as np
 import pytest
@@ -20,6 +25,7 @@ import micro
 import oracle
 from _helpers import STRUCT_KEYS, owned_global, rel_err
 from gridap_amd import AssemblyBatch, problem_from_workload
+from gridap_amd._lib import GdxError
 from gridap_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -172,3 +178,101 @@ def test_batch_with_poisson_problems_matches_eager(gpu_ctx):
     batch.close()
     for pr in probs:
         pr.close()
+
+
+def random_lame(cells, seed):
+    rng = np.random.default_rng(seed)
+    nc = int(np.prod(cells))
+    return np.stack([rng.uniform(0.2, 3.0, nc), rng.uniform(0.1, 2.0, nc)], axis=1)
+
+
+def parts_values(prob):
+    prob.sync()
+    return [(Q["val"].copy(), Q["b"].copy()) for Q in (prob.export_part(k) for k in range(prob.nparts))]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"q{c['order']}-{c['cells']}-{c['parts']}-{c['strategy']}"
+                         f"-{'clamped' if c['clamped'] else 'free'}")
+def test_cellwise_lame_parity_with_oracle(gpu_ctx, case):
+    print("cell-wise elasticity case", case, flush=True)
+    p = elast(case)
+    lame = random_lame(case["cells"], 7)
+    r = oracle.run(dict(p, cell_lame=lame))
+    assem, prob = gpu_problem(gpu_ctx, p)
+    const = parts_values(prob)
+    prob.set_cell_lame(lame)
+    prob.numeric()
+    prob.sync()
+    for k, P in enumerate(r.parts):
+        Q = prob.export_part(k)
+        for key in STRUCT_KEYS:
+            assert np.array_equal(P[key], Q[key]), (k, key)
+        assert rel_err(Q["val"], P["val"]) <= TOL, k
+        assert rel_err(Q["b"], P["b"]) <= TOL, k
+        no = P["n_own"]
+        assert np.all(Q["val"][Q["rowptr"][no]:] == 0.0) and np.all(Q["b"][no:] == 0.0)
+    # kappa scaling the constants: the pairs (lambda kappa_c, mu kappa_c), bit for bit
+    kap = np.exp(np.random.default_rng(8).normal(size=len(lame)))
+    prob.set_cell_coefficient(kap)
+    prob.numeric()
+    via_kappa = parts_values(prob)
+    prob.set_cell_lame(np.stack([p["lame"][0] * kap, p["lame"][1] * kap], axis=1))
+    prob.numeric()
+    for (v0, b0), (v1, b1) in zip(via_kappa, parts_values(prob)):
+        assert np.array_equal(v0.view(np.int64), v1.view(np.int64)) and np.array_equal(b0, b1)
+    r2 = oracle.run(dict(p, cell_lame=np.stack([p["lame"][0] * kap, p["lame"][1] * kap], axis=1)))
+    for P, (v, bb) in zip(r2.parts, via_kappa):
+        assert rel_err(v, P["val"]) <= TOL
+    # back to the constants
+    prob.set_cell_lame(None)
+    prob.numeric()
+    for (v0, b0), (v1, b1) in zip(const, parts_values(prob)):
+        assert np.array_equal(v0.view(np.int64), v1.view(np.int64)) and np.array_equal(b0, b1)
+    prob.close()
+
+
+def test_cellwise_lame_refusals(gpu_ctx):
+    p = elast(dict(cells=(3, 2, 2), parts=(1, 1, 1), clamped=True, order=2, strategy="sub"))
+    assem, prob = gpu_problem(gpu_ctx, p)
+    for bad in ((1.0, 0.0), (-1.0, 1.0), (np.nan, 1.0), (1.0, np.inf)):
+        lame = np.tile([1.0, 0.5], (12, 1))
+        lame[5] = bad
+        with pytest.raises(GdxError) as e:
+            prob.set_cell_lame(lame)
+        assert e.value.code == -1, bad
+    prob.close()
+    q = W.poisson3d_q1(4, (1, 1, 1))
+    _, pq = gpu_problem(gpu_ctx, q)
+    with pytest.raises(GdxError) as e:
+        pq.set_cell_lame(np.ones((64, 2)))
+    assert e.value.code == -4
+    pq.close()
+
+
+def test_cellwise_lame_graph_replays(gpu_ctx):
+    case = dict(cells=(6, 5, 4), parts=(3, 2, 2), clamped=True, order=2, strategy="sub")
+    p = elast(case)
+    lames = [random_lame(case["cells"], s) for s in (1, 2)]
+    _, prob = gpu_problem(gpu_ctx, p)
+    eager = []
+    for lame in lames:
+        prob.set_cell_lame(lame)
+        prob.numeric()
+        eager.append(parts_values(prob))
+    batch = AssemblyBatch(gpu_ctx)
+    batch.add(prob)
+    batch.autotune(1)
+    batch.capture()
+    for rep in range(4):
+        prob.set_cell_lame(lames[rep % 2])  # same device buffer, new contents
+        batch.launch()
+        batch.sync()
+        for (v0, b0), (v1, b1) in zip(eager[rep % 2], parts_values(prob)):
+            assert np.array_equal(v0.view(np.int64), v1.view(np.int64)) and np.array_equal(b0, b1), rep
+        prob.numeric()  # eager work between replays
+        prob.sync()
+    prob.set_cell_lame(None)  # a mode change drops the graph
+    with pytest.raises(GdxError):
+        batch.launch()
+    batch.close()
+    prob.close()

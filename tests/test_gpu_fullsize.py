@@ -9,6 +9,10 @@ for the nonlinear solve, one oracle residual:
   clamp; the energy of a linear field that vanishes on the clamp,
   |Omega| (lambda tr(e)^2 + 2 mu e:e); sampled rows (extracted with the device
   SpMV) against the Kronecker restatement of the 1D assembled factors;
+- the same with a lognormal Young's modulus field (kappa_c scaling lambda and
+  mu on each cell, drawn on the device): rigid-body modes in the kernel away
+  from the clamp and the linear field's energy
+  sum_c |cell| kappa_c (lambda tr(e)^2 + 2 mu e:e) with the exported field;
 - learned-preconditioner workload, Q1 128^3, lognormal kappa (held-out field):
   FPCG with the multilevel preconditioner and Jacobi-PCG both reach the FP64 true
   residual 1e-8, FPCG in at most half the Jacobi-PCG time (the north-star
@@ -125,6 +129,38 @@ def test_c3_elasticity_full_size(gpu_ctx):
         sc = max(abs(v) for v in ref.values())
         for k in set(ref) | set(got):  # entries the restatement finds zero must be zero to rounding
             assert abs(got.get(k, 0.0) - ref.get(k, 0.0)) <= 1e-12 * sc, (gi, k)
+    prob.close()
+
+
+def test_c3_elasticity_cellwise_full_size(gpu_ctx):
+    p = W.elasticity3d_q2()
+    print("full-size elasticity, cell-wise Young's modulus", p["cells"], p["parts"], flush=True)
+    lam, mu = p["lame"]
+    assem, data = problem_from_workload(gpu_ctx, p)
+    prob = assem._problem_for(data)
+    prob.cell_coefficient_grf(31, 0.1, 1.0)
+    kap = prob.export_cell_coefficient(int(np.prod(p["cells"])))
+    assert kap.min() > 0.0 and kap.max() / kap.min() > 10.0
+    prob.numeric()
+    prob.sync()
+    nn = W.lattice_shape(p)
+    g2n, comp = gid_map(prob)
+    X = micro.node_coords(p, g2n)[1]
+    away = (g2n % nn[0]) >= 3
+    scale = np.abs(prob.spmv(np.where(comp == 0, X[:, 0], 0.0))).max()
+    for a in range(3):
+        assert np.abs(prob.spmv(np.where(comp == a, 1.0, 0.0))[away]).max() <= 1e-12 * scale
+    for i, j in ((0, 1), (1, 2), (0, 2)):
+        rot = np.where(comp == i, -X[:, j], 0.0) + np.where(comp == j, X[:, i], 0.0)
+        assert np.abs(prob.spmv(rot)[away]).max() <= 1e-12 * scale
+    g = np.array([0.3, -0.5, 0.7])
+    u = g[comp] * X[:, 0]
+    e = np.zeros((3, 3))
+    e[0, 0] = g[0]
+    e[0, 1] = e[1, 0] = g[1] / 2
+    e[0, 2] = e[2, 0] = g[2] / 2
+    energy = (lam * np.trace(e) ** 2 + 2.0 * mu * np.sum(e * e)) * kap.sum() / len(kap)  # |Omega| = 1
+    assert abs(u @ prob.spmv(u) - energy) <= 1e-10 * energy
     prob.close()
 
 

@@ -14,6 +14,12 @@ the inverse Jacobian, component-major cell dofs):
 - nnz = 9 prod(8 n_d + 1) for unconstrained Q2 (9 component pairs), the
   closed form of the scope table's elasticity row;
 - the owned rows do not depend on the partition.
+With Lame parameters given per cell (p["cell_lame"], a piecewise-constant
+material): the restatement summed cell by cell gives the same matrix; the
+rigid-body modes stay in the kernel; a linear field stores
+sum_c |cell| (lambda_c tr(e)^2 + 2 mu_c e:e); a pair that is nonzero on one
+cell only couples that cell's nodes and nothing else (pins the x-fastest cell
+index).
 The library's host symbolic phase is compared field by field with the oracle
 (numbering, gids, owners, row/col PRanges with the component-interleaved ghost
 columns, CSR pattern).
@@ -141,3 +147,65 @@ def test_descriptor_errors(host_ctx):
     dO = Measure(Triangulation(model), 4)
     with pytest.raises(NotImplementedError):
         collect_cell_matrix_and_vector(U, V, Integral(Elasticity(1, 1), dO), Integral(BodyForce((0, 0, 1)), dO))
+
+
+def random_lame(cells, seed):
+    rng = np.random.default_rng(seed)
+    nc = int(np.prod(cells))
+    return np.stack([rng.uniform(0.2, 3.0, nc), rng.uniform(0.1, 2.0, nc)], axis=1)
+
+
+@pytest.mark.parametrize("cells,parts,clamped,order", [
+    ((3, 2, 2), (2, 1, 2), True, 2), ((4, 3, 2), (1, 1, 1), False, 2), ((4, 3, 5), (2, 1, 2), True, 1),
+    ((2, 3, 3), (2, 3, 1), False, 1)])
+def test_cellwise_lame_matches_restatement(cells, parts, clamped, order):
+    p = W.elasticity3d_q2(cells=cells, parts=parts, clamped=clamped, order=order, domain=DOM,
+                          body_force=(0.3, -0.2, -1.0))
+    p["cell_lame"] = random_lame(cells, 17 + order)
+    r = oracle.run(p)
+    A, b = owned_global(r.parts, r.ngdofs)
+    g2d = gid_to_latdof(r)
+    Am, bm = micro.elasticity_global(p)
+    Am, bm = Am[g2d][:, g2d], bm[g2d]
+    assert abs(A - Am).max() <= 1e-13 * abs(Am).max()
+    assert np.abs(b - bm).max() <= 1e-13 * np.abs(bm).max()
+    assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+    # the constant pair everywhere is the constant-coefficient form
+    q = dict(p, cell_lame=np.tile(p["lame"], (int(np.prod(cells)), 1)))
+    A1, _ = owned_global(oracle.run(q).parts, r.ngdofs)
+    A0, _ = owned_global(oracle.run(dict(p, cell_lame=None)).parts, r.ngdofs)
+    assert abs(A1 - A0).max() <= 1e-14 * abs(A0).max()
+
+
+def test_cellwise_lame_rigid_modes_energy_and_locality():
+    cells = (3, 2, 4)
+    p = W.elasticity3d_q2(cells=cells, parts=(2, 2, 1), clamped=False, domain=DOM)
+    lame = random_lame(cells, 5)
+    r = oracle.run(dict(p, cell_lame=lame))
+    A, _ = owned_global(r.parts, r.ngdofs)
+    g2d = gid_to_latdof(r)
+    node, comp = g2d // 3, g2d % 3
+    _, X = micro.node_coords(p, node)
+    scale = abs(A).max()
+    for k in range(3):
+        assert np.abs(A @ (comp == k).astype(float)).max() <= 1e-13 * scale
+    for i, j in ((0, 1), (1, 2), (0, 2)):
+        rot = np.where(comp == i, -X[:, j], 0.0) + np.where(comp == j, X[:, i], 0.0)
+        assert np.abs(A @ rot).max() <= 1e-13 * scale
+    G = np.array([[0.3, -0.7, 0.2], [0.5, -0.1, 0.4], [-0.6, 0.9, 0.25]])
+    u = np.einsum("nj,nj->n", G[comp], X)
+    e = 0.5 * (G + G.T)
+    vcell = 1.5 * 1.0 * 0.7 / np.prod(cells)
+    energy = vcell * np.sum(lame[:, 0] * np.trace(e) ** 2 + 2.0 * lame[:, 1] * np.sum(e * e))
+    assert abs(u @ (A @ u) - energy) <= 1e-12 * energy
+    # one cell with a nonzero pair: only that cell's nodes are coupled
+    nn = W.lattice_shape(p)
+    for cx, cy, cz in ((2, 0, 3), (0, 1, 1)):
+        one = np.zeros_like(lame)
+        one[cx + cells[0] * (cy + cells[1] * cz)] = (1.3, 0.6)
+        A1, _ = owned_global(oracle.run(dict(p, cell_lame=one)).parts, r.ngdofs)
+        ix, iy, iz = node % nn[0], (node // nn[0]) % nn[1], node // (nn[0] * nn[1])
+        inside = ((ix >= 2 * cx) & (ix <= 2 * cx + 2) & (iy >= 2 * cy) & (iy <= 2 * cy + 2) &
+                  (iz >= 2 * cz) & (iz <= 2 * cz + 2))
+        rows = np.unique(A1.nonzero()[0])
+        assert len(rows) == 81 and inside[rows].all()
