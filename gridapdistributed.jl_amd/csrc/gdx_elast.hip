@@ -287,14 +287,20 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast_cw(ElastArgs A, const Ela
       const int code = s3v[j][k];
       const int o0 = code & 7, o1 = (code >> 3) & 7, o2 = (code >> 6) & 7, bc = code >> 9;
       const bool b0 = bc == 0, b1 = bc == 1, b2 = bc == 2;
+      // a node in two cells along d (a vertex) meets a column at offset < 0
+      // only in the first, > 0 only in the second: the slots with nonzero
+      // factors for this entry
+      const int lo0 = ns0 == 2 && o0 > K, hi0 = ns0 == 2 && o0 >= K;
+      const int lo1 = ns1 == 2 && o1 > K, hi1 = ns1 == 2 && o1 >= K;
+      const int lo2 = ns2 == 2 && o2 > K, hi2 = ns2 == 2 && o2 >= K;
       double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-      for (int s2 = 0; s2 < ns2; ++s2) {
+      for (int s2 = lo2; s2 <= hi2; ++s2) {
         const double2 *f2 = reinterpret_cast<const double2 *>(F + 2 * NS + (s2 * W + o2) * 4);
         const double2 km2 = f2[0], pq2 = f2[1];
-        for (int s1 = 0; s1 < ns1; ++s1) {
+        for (int s1 = lo1; s1 <= hi1; ++s1) {
           const double2 *f1 = reinterpret_cast<const double2 *>(F + NS + (s1 * W + o1) * 4);
           const double2 km1 = f1[0], pq1 = f1[1];
-          for (int s0 = 0; s0 < ns0; ++s0) {
+          for (int s0 = lo0; s0 <= hi0; ++s0) {
             const double2 *f0 = reinterpret_cast<const double2 *>(F + (s0 * W + o0) * 4);
             const double2 km0 = f0[0], pq0 = f0[1];
             const double2 lm = reinterpret_cast<const double2 *>(LM)[s0 + 2 * s1 + 4 * s2];
