@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = ["csrc/gdx_capi.hip", "csrc/gdx_kernels.hip", "csrc/gdx_march3d.hip", "csrc/gdx_tile3d.hip",
        "csrc/gdx_solve.hip",
        "csrc/gdx_flux.hip", "csrc/gdx_coef.hip",
-       "csrc/gdx_nn.hip", "csrc/gdx_twogrid.hip", "csrc/gdx_batch.hip", "csrc/gdx_elast.hip", "csrc/gdx_pvector.hip",
+       "csrc/gdx_nn.hip", "csrc/gdx_twogrid.hip", "csrc/gdx_batch.hip", "csrc/gdx_elast.hip", "csrc/gdx_mapped.hip",
+       "csrc/gdx_pvector.hip",
        "csrc/gdx_symbolic.cpp", "csrc/gdx_symbolic_dev.hip"]
 OUT = os.path.join(HERE, "libgdx.so")
 

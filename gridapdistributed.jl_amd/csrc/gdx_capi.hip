@@ -231,6 +231,7 @@ void release_problem_device(gdx_problem *p) {
   gdx::solve_free(p);
   gdx::flux_free(p);
   gdx::elast_free(p);
+  gdx::mapped_free(p);
   gdx::pvector_release(p);
   free_device(p);
   for (hipEvent_t &e : p->ev)
@@ -240,6 +241,7 @@ void release_problem_device(gdx_problem *p) {
     }
   p->coef_cell = false;
   p->lame_cell = false;
+  p->mapped = false;
 }
 }  // namespace gdx
 
@@ -300,6 +302,13 @@ int gdx_symbolic(gdx_problem *p) {
     for (gdx_nn *nn : p->ctx->nns)
       if (gdx::twogrid_problem(nn) == p) gdx::twogrid_free(nn);
     free_device(p);
+    // the cell-wise coefficient buffers go with the device buffers: the
+    // problem returns to the descriptor's coefficients
+    p->coef_cell = false;
+    p->lame_cell = false;
+    // and so does a mapped grid's state: the uniform box again
+    gdx::mapped_free(p);
+    p->mapped = false;
     gdx::solve_free(p);  // its row tables are rebuilt from the new structure
     gdx::elast_free(p);
     gdx::flux_free(p);
@@ -409,6 +418,10 @@ extern "C++" namespace gdx {
 int numeric_launches(gdx_problem *p, hipStream_t st) {
   if (p->desc.op_kind == GDX_OP_FLUX) return flux_assemble(p, nullptr, false, p->precision == GDX_PREC_FP32, st);
   if (p->desc.op_kind == GDX_OP_ELASTICITY) return elast_assemble(p, st);
+  if (p->mapped) {
+    int rcm = mapped_assemble(p, st, 0);
+    return rcm ? rcm : mapped_assemble(p, st, 1);
+  }
   int rc = launch_coefficients(p, st);
   if (!rc) rc = launch_rhs(p, st);
   if (!rc) rc = launch_matrix(p, st);
@@ -421,6 +434,7 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
   GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
   GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
   GDX_REQUIRE(precision == GDX_PREC_FP64 || precision == GDX_PREC_FP32, GDX_ERR_ARG, "unknown precision");
+  GDX_REQUIRE(precision == GDX_PREC_FP64 || !p->mapped, GDX_ERR_UNSUPPORTED, "a mapped grid assembles in FP64");
   GDX_REQUIRE(precision == GDX_PREC_FP64 ||
                   (p->D == 3 && p->desc.op_kind == GDX_OP_DIFFUSION && (p->kvariant3d == 0 || p->kvariant3d == 2)) ||
                   (p->desc.op_kind == GDX_OP_FLUX && gdx::flux_mixed_supported(p)),
@@ -452,6 +466,19 @@ int gdx_numeric(gdx_problem *p, int32_t precision) {
     if (rce) {
       set_error("elasticity kernel launch failed");
       return rce;
+    }
+    GDX_HIP(hipEventRecord(p->ev[3], st));
+    return GDX_OK;
+  }
+  if (p->mapped) {
+    // cell records (geometry, coefficients, cell matrices), then the rows
+    int rcm = gdx::mapped_assemble(p, st, 0);
+    GDX_HIP(hipEventRecord(p->ev[1], st));
+    GDX_HIP(hipEventRecord(p->ev[2], st));
+    if (!rcm) rcm = gdx::mapped_assemble(p, st, 1);
+    if (rcm) {
+      set_error("mapped-grid kernel launch failed");
+      return rcm;
     }
     GDX_HIP(hipEventRecord(p->ev[3], st));
     return GDX_OK;
@@ -547,6 +574,32 @@ int gdx_set_cell_coefficient(gdx_problem *p, const double *kappa) {
   if (rc) return rc;
   GDX_HIP(hipMemcpy(p->dev.cellk, kappa, nc * sizeof(double), hipMemcpyHostToDevice));
   set_coef_mode(p, true, false);
+  return GDX_OK;
+}
+
+int gdx_set_vertex_coords(gdx_problem *p, const double *xyz) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->symbolic_done, GDX_ERR_STATE, "gdx_symbolic has not run");
+  GDX_REQUIRE(p->ctx, GDX_ERR_STATE, "host-only problem (created without a context)");
+  GDX_REQUIRE(p->desc.op_kind == GDX_OP_DIFFUSION, GDX_ERR_UNSUPPORTED, "mapped grids are lowered for diffusion only");
+  GDX_HIP(hipSetDevice(p->ctx->device));
+  if (!xyz) {
+    if (p->mapped) gdx::batch_invalidate_problem(p->ctx, p);
+    p->mapped = false;
+    return GDX_OK;
+  }
+  int64_t nv = 1;
+  for (int d = 0; d < p->D; ++d) nv *= p->desc.cells[d] + 1;
+  for (int64_t i = 0; i < nv * p->D; ++i)
+    GDX_REQUIRE(std::isfinite(xyz[i]), GDX_ERR_ARG, "vertex coordinates must be finite");
+  GDX_HIP(hipStreamSynchronize(p->ctx->stream));  // the cell records may be in use
+  int rc = gdx::mapped_set(p, xyz);
+  if (rc) {
+    set_error("setting up the mapped grid's device state failed");
+    return rc;
+  }
+  if (!p->mapped) gdx::batch_invalidate_problem(p->ctx, p);
+  p->mapped = true;
   return GDX_OK;
 }
 

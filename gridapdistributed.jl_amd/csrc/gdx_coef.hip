@@ -85,6 +85,7 @@ int cell_coef_alloc(gdx_problem *p) {
 }
 
 int launch_cell_coefficients(gdx_problem *p, hipStream_t st) {
+  if (!p->dev.cellk) return GDX_ERR_STATE;
   const int64_t nq = (int64_t)p->kp.qn[0] * p->kp.qn[1] * p->kp.qn[2];
   hipLaunchKernelGGL(k_cell_coef, dim3(blocks_for(nq)), dim3(256), 0, st, p->kp, p->dev.cellk, p->dev.W, p->dev.F);
   return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;

@@ -191,6 +191,8 @@ struct gdx_problem {
   bool coef_cell = false; // kappa from dev.cellk instead of the Fourier modes
                           // (elasticity: kappa scales lambda and mu on each cell)
   bool lame_cell = false; // elasticity: (lambda, mu) from dev.cell_lame
+  bool mapped = false;    // diffusion on a mapped grid (vertex coordinates in mapped_data)
+  void *mapped_data = nullptr;  // gdx_mapped.hip
   void *pv = nullptr;     // PRange maps of the device PVectors (gdx_pvector.hip)
   std::vector<gdx_pvector *> pvecs;  // live PVectors on this problem
 };
@@ -297,6 +299,9 @@ int grf_cell_coefficient(gdx_problem *p, uint64_t seed, double corr_len, double 
 int elast_setup(gdx_problem *p);  // factor tables and patterns to HBM
 void elast_free(gdx_problem *p);
 int elast_assemble(gdx_problem *p, hipStream_t st);
+int mapped_set(gdx_problem *p, const double *xyz);  // vertex coordinates; cell records and row list
+void mapped_free(gdx_problem *p);
+int mapped_assemble(gdx_problem *p, hipStream_t st, int stage);  // stage 0: cell records, 1: rows
 int flux_setup(gdx_problem *p);
 void flux_free(gdx_problem *p);
 double *flux_state(gdx_problem *p);

@@ -1,0 +1,484 @@
+// Diffusion on a mapped Cartesian grid: the numeric phase of
+// AffineFEOperator(a, l, U, V) with a(u,v) = int( kappa grad(v).grad(u) ),
+// l(v) = int( f v ) on CartesianDiscreteModel(ranks, parts, domain, cells; map)
+// (src/Geometry.jl:378-388 forwards the map to the CartesianDescriptor): every
+// mesh vertex sits where the map puts it and each cell takes the multilinear
+// (Q1) map of its 2^D vertices, so J, detJ and J^-1 vary over the cell.  The
+// reference integrates each cell with this map (src/FESpaces.jl:794-798); the
+// uniform-box kernels of this library assume J = diag(h) instead.
+//
+// Two launches:
+// - k_mapped_cells: one wave per cell.  The Gauss points' Jacobians, weights
+//   w |detJ| kappa(x_q) and w |detJ| f(x_q) (kappa: the Fourier form at the
+//   mapped point x_q, or kappa per cell), then the physical basis gradients
+//   J^-T grad(phi_a) of every (point, basis function) to LDS, then the cell
+//   matrix's upper triangle and the cell vector by lanes over the entries;
+//   written cell-major to HBM (K_c[a][b], a <= b, then b_c[a]; local nodes
+//   tensor-lexicographic);
+// - k_mapped_rows: lanes per owned free row over its (2K+1)^D stencil slots.
+//   A slot's value is the sum over the cells holding both nodes of their
+//   entry, cells in a fixed order; it goes to the row's position for that slot
+//   or, for a Dirichlet column, into the lifting sum_j A_ij uD(x_j).  The row's
+//   lanes then reduce b = sum_c b_c - lifting in a fixed butterfly order.  No
+//   atomics: repeated assemblies are bit-identical; ghost rows stay zero.
+// HBM traffic per assembly: the cell records (written once, read about twice:
+// each stored upper-triangle entry serves two rows) and the 8 B CSR value
+// writes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gdx_internal.h"
+
+namespace gdx {
+
+namespace {
+
+// One owned free row, in the order of the rows in the values buffer.
+struct MappedRow {
+  long long rowstart;  // first value of the row
+  long long bidx;      // RHS index
+  int pid;             // position pattern
+  unsigned short g[3]; // global lattice coordinates
+  unsigned short pad[3];
+};
+static_assert(sizeof(MappedRow) == 32, "MappedRow is 32 bytes");
+
+struct MappedData {
+  double *vc = nullptr;     // vertex coordinates [nv][D]
+  double *cm = nullptr;     // cell records [ncell][NE]
+  MappedRow *rows = nullptr;
+  long long nrows = 0, ncell = 0;
+  int ne = 0;
+};
+
+MappedData *md(gdx_problem *p) { return reinterpret_cast<MappedData *>(p->mapped_data); }
+
+template <int D, int K>
+struct MShape {
+  static constexpr int NCN = D == 3 ? (K + 1) * (K + 1) * (K + 1) : (K + 1) * (K + 1);  // cell nodes
+  static constexpr int NJ = NCN * (NCN + 1) / 2;                                        // upper triangle
+  static constexpr int NE = NJ + NCN;                                                   // record length
+  static constexpr int NV = 1 << D;                                                     // vertices
+  static constexpr int W = 2 * K + 1;
+  static constexpr int S = D == 3 ? W * W * W : W * W;                                  // stencil slots
+};
+
+// position of (a, b), a <= b, in the row-major upper triangle of an n x n matrix
+__host__ __device__ constexpr int tri_index(int n, int a, int b) { return a * n - a * (a - 1) / 2 + (b - a); }
+
+// the owned free rows of one part's owned node box, at their owned row index
+__global__ void k_mapped_rowlist(const RowInfo *__restrict__ rowinfo, long long rinfo0, long long row_base,
+                                 long long n_own, long long grp0, long long lo0, long long lo1, long long lo2,
+                                 long long n0, long long n1, long long n2, MappedRow *__restrict__ rows,
+                                 unsigned long long *cnt) {
+  const long long li = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (li >= n0 * n1 * n2) return;
+  const RowInfo ri = rowinfo[rinfo0 + li];
+  if (ri.pid < 0) return;
+  const long long o = ri.bidx - row_base;
+  if (o < 0 || o >= n_own) {
+    atomicOr(cnt + 1, 1ull);
+    return;
+  }
+  atomicAdd(cnt, 1ull);  // every owned row must be listed exactly once
+  MappedRow r{};
+  r.rowstart = ri.rowstart;
+  r.bidx = ri.bidx;
+  r.pid = ri.pid;
+  r.g[0] = (unsigned short)(lo0 + li % n0);
+  r.g[1] = (unsigned short)(lo1 + (li / n0) % n1);
+  r.g[2] = (unsigned short)(lo2 + li / (n0 * n1));
+  rows[grp0 + o] = r;
+}
+
+// kappa and f at a physical point (the Fourier form), as the coefficient
+// kernel of the uniform path defines them
+__device__ inline void kappa_f(const KParams &kp, const double *__restrict__ modes, const double x[3], double &kap,
+                               double &f) {
+  double g = 0.0, dg0 = 0.0, dg1 = 0.0, dg2 = 0.0;
+  for (int m = 0; m < kp.nmodes; ++m) {
+    const double *M = modes + 5 * m;
+    const double th = 2.0 * M_PI * (M[0] * x[0] + M[1] * x[1] + M[2] * x[2]) + M[4];
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    g += M[3] * cs;
+    const double s = -M[3] * sn * 2.0 * M_PI;
+    dg0 += s * M[0];
+    dg1 += s * M[1];
+    dg2 += s * M[2];
+  }
+  kap = kp.coef_const * exp(g);
+  f = kp.rhs_kind == GDX_RHS_MANUFACTURED
+          ? -kap * (dg0 * kp.dir_coef[1] + dg1 * kp.dir_coef[2] + dg2 * kp.dir_coef[3])
+          : kp.rhs_const;
+}
+
+// Stage 1: one wave per cell, WPB waves per workgroup.
+template <int D, int K, int NQ, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long ncell, const double *__restrict__ vc,
+                                                           const double *__restrict__ modes,
+                                                           const double *__restrict__ cellk, double *__restrict__ cm) {
+  using Sh = MShape<D, K>;
+  constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // <= 64
+  constexpr int NCN = Sh::NCN, NJ = Sh::NJ, NE = Sh::NE, NV = Sh::NV;
+  __shared__ double G[WPB][NQT][NCN][D];  // physical gradients
+  __shared__ double WK[WPB][NQT], WF[WPB][NQT];
+  __shared__ double T1[3][kMaxK + 1][kMaxQ];  // 1D basis values, derivatives; Gauss points and weights
+  __shared__ double TQ[2][kMaxQ];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < (K + 1) * NQ; i += blockDim.x) {
+    const int a = i / NQ, q = i % NQ;
+    T1[0][a][q] = kp.phi[a][q];
+    T1[1][a][q] = kp.dphi[a][q];
+  }
+  for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
+    TQ[0][i] = kp.qx[i];
+    TQ[1][i] = kp.qw[i];
+  }
+  __syncthreads();  // the only block barrier; waves past the last cell still reach it
+  const long long c = (long long)blockIdx.x * WPB + wave;
+  if (c >= ncell) return;
+  const int n0 = kp.ncells[0], n1 = kp.ncells[1];
+  const long long c0 = c % n0, c1 = (c / n0) % n1, c2 = D == 3 ? c / ((long long)n0 * n1) : 0;
+  const long long nv0 = n0 + 1, nv1 = n1 + 1;
+  // vertex coordinates (every lane reads all 2^D of them: they are few and cached)
+  double X[NV][D];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const long long vi = (c0 + (v & 1)) + nv0 * ((c1 + ((v >> 1) & 1)) + nv1 * (c2 + ((v >> 2) & 1)));
+#pragma unroll
+    for (int d = 0; d < D; ++d) X[v][d] = vc[vi * D + d];
+  }
+  const double kcell = cellk ? kp.coef_const * cellk[c] : 0.0;
+  // Gauss point q = lane: Jacobian, weights, then the gradients of point q
+  double Jit[D][D] = {};  // J^-T
+  if (lane < NQT) {
+    const int qi[3] = {lane % NQ, (lane / NQ) % NQ, D == 3 ? lane / (NQ * NQ) : 0};
+    double xi[3], J[D][D] = {}, x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = TQ[0][qi[d]];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double N = 1.0, dN[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) N *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        dN[e] = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          dN[e] *= d == e ? (((v >> d) & 1) ? 1.0 : -1.0) : (((v >> d) & 1) ? xi[d] : 1.0 - xi[d]);
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        x[d] += N * X[v][d];
+#pragma unroll
+        for (int e = 0; e < D; ++e) J[d][e] += X[v][d] * dN[e];
+      }
+    }
+    double det;
+    if constexpr (D == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      // J^-1 = adj / det; J^-T[d][e] = J^-1[e][d]
+      Jit[0][0] = J[1][1] / det;
+      Jit[1][0] = -J[0][1] / det;
+      Jit[0][1] = -J[1][0] / det;
+      Jit[1][1] = J[0][0] / det;
+    } else {
+      const double a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2],
+                   a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1], a10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                   a11 = J[0][0] * J[2][2] - J[0][2] * J[2][0], a12 = J[0][2] * J[1][0] - J[0][0] * J[1][2],
+                   a20 = J[1][0] * J[2][1] - J[1][1] * J[2][0], a21 = J[0][1] * J[2][0] - J[0][0] * J[2][1],
+                   a22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      det = J[0][0] * a00 + J[0][1] * a10 + J[0][2] * a20;
+      // J^-1[i][j] = a_ij / det; J^-T[d][e] = J^-1[e][d]
+      Jit[0][0] = a00 / det, Jit[0][1] = a10 / det, Jit[0][2] = a20 / det;
+      Jit[1][0] = a01 / det, Jit[1][1] = a11 / det, Jit[1][2] = a21 / det;
+      Jit[2][0] = a02 / det, Jit[2][1] = a12 / det, Jit[2][2] = a22 / det;
+    }
+    double w = fabs(det);
+#pragma unroll
+    for (int d = 0; d < D; ++d) w *= TQ[1][qi[d]];
+    double kap, f;
+    if (cellk) {
+      kap = kcell;
+      f = kp.rhs_const;
+    } else {
+      kappa_f(kp, modes, x, kap, f);
+    }
+    WK[wave][lane] = w * kap;
+    WF[wave][lane] = w * f;
+    // grad(phi_a) = J^-T grad_ref(phi_a) at this point, every a
+    for (int a = 0; a < NCN; ++a) {
+      const int ai[3] = {a % (K + 1), (a / (K + 1)) % (K + 1), D == 3 ? a / ((K + 1) * (K + 1)) : 0};
+      double gr[D];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        gr[e] = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gr[e] *= T1[d == e ? 1 : 0][ai[d]][qi[d]];
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < D; ++e) s += Jit[d][e] * gr[e];
+        G[wave][lane][a][d] = s;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();  // LDS written and read by the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double *out = cm + c * (long long)NE;
+  for (int e = lane; e < NE; e += 64) {
+    double acc = 0.0;
+    if (e < NJ) {
+      int a = 0, r = e;
+      while (r >= NCN - a) r -= NCN - a, ++a;
+      const int b = a + r;
+      for (int q = 0; q < NQT; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s += G[wave][q][a][d] * G[wave][q][b][d];
+        acc += WK[wave][q] * s;
+      }
+    } else {
+      const int a = e - NJ;
+      const int ai[3] = {a % (K + 1), (a / (K + 1)) % (K + 1), D == 3 ? a / ((K + 1) * (K + 1)) : 0};
+      for (int q = 0; q < NQT; ++q) {
+        const int qi[3] = {q % NQ, (q / NQ) % NQ, D == 3 ? q / (NQ * NQ) : 0};
+        double ph = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) ph *= T1[0][ai[d]][qi[d]];
+        acc += WF[wave][q] * ph;
+      }
+    }
+    out[e] = acc;
+  }
+}
+
+// coordinates of global lattice node g on the mapped grid: the mean of the
+// vertices of the edge, face or cell whose midpoint it is (a vertex itself)
+template <int D, int K>
+__device__ inline void node_xyz(const double *__restrict__ vc, const int g[3], long long nv0, long long nv1,
+                                double x[3]) {
+  x[0] = x[1] = x[2] = 0.0;
+  int v0[3], two[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v0[d] = d < D ? g[d] / K : 0;
+    two[d] = d < D && g[d] % K != 0;
+  }
+  const int nb = 1 << (two[0] + two[1] + two[2]);
+  for (int cc = 0; cc < (1 << D); ++cc) {
+    const int s0 = cc & 1, s1 = (cc >> 1) & 1, s2 = (cc >> 2) & 1;
+    if ((s0 && !two[0]) || (s1 && !two[1]) || (s2 && !two[2])) continue;
+    const long long v = (v0[0] + s0) + nv0 * ((v0[1] + s1) + nv1 * (v0[2] + s2));
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] += vc[v * D + d] / nb;
+  }
+}
+
+// Stage 2: LPR lanes per row.
+template <int D, int K>
+__global__ __launch_bounds__(256) void k_mapped_rows(KParams kp, long long nrows, const MappedRow *__restrict__ rows,
+                                                     const uint8_t *__restrict__ pat_fwd, int pstride,
+                                                     const double *__restrict__ cm, const double *__restrict__ vc,
+                                                     double *__restrict__ values, double *__restrict__ b) {
+  using Sh = MShape<D, K>;
+  constexpr int S = Sh::S, W = Sh::W, NE = Sh::NE, NJ = Sh::NJ, NCN = Sh::NCN;
+  constexpr int LPR = S <= 16 ? 16 : (S <= 32 ? 32 : 64);
+  constexpr int SPL = (S + LPR - 1) / LPR;
+  const int q = threadIdx.x % LPR;
+  const long long r = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  MappedRow mr{};
+  mr.pid = -1;
+  if (r < nrows) mr = rows[r];
+  const bool live = mr.pid >= 0;  // lanes of a dead row still join the reduction below
+  const int g[3] = {mr.g[0], mr.g[1], mr.g[2]};
+  int posv[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int sl = q + LPR * j;
+    posv[j] = live && sl < S ? pat_fwd[(size_t)mr.pid * pstride + sl] : kSlotAbsent;
+  }
+  // the row node's cells along each direction ("slots"): first cell cf, its
+  // local node af, count ns; a second slot is cell cf + 1 at local node 0
+  int cf[3] = {0, 0, 0}, af[3] = {0, 0, 0}, ns[3] = {1, 1, 1};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int i = g[d];
+    if (i % K == 0) {
+      const bool left = i / K - 1 >= 0, right = i / K < kp.ncells[d];
+      cf[d] = left ? i / K - 1 : i / K;
+      af[d] = left ? K : 0;
+      ns[d] = (int)left + (int)right;
+    } else {
+      cf[d] = i / K;
+      af[d] = i % K;
+    }
+  }
+  const long long n0 = kp.ncells[0], n1 = kp.ncells[1];
+  double lift = 0.0;
+  double *dst = values + mr.rowstart;
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int sl = q + LPR * j;
+    const int pos = posv[j];
+    if (pos == kSlotAbsent) continue;
+    const int o[3] = {sl % W - K, (sl / W) % W - K, D == 3 ? sl / (W * W) - K : 0};
+    double acc = 0.0;
+    for (int s2 = 0; s2 < ns[2]; ++s2) {
+      const int a2 = s2 ? 0 : af[2], b2 = a2 + o[2];
+      if (D == 3 && (b2 < 0 || b2 > K)) continue;
+      for (int s1 = 0; s1 < ns[1]; ++s1) {
+        const int a1 = s1 ? 0 : af[1], b1 = a1 + o[1];
+        if (b1 < 0 || b1 > K) continue;
+        for (int s0 = 0; s0 < ns[0]; ++s0) {
+          const int a0 = s0 ? 0 : af[0], b0 = a0 + o[0];
+          if (b0 < 0 || b0 > K) continue;
+          const long long cell = (cf[0] + s0) + n0 * ((cf[1] + s1) + n1 * (long long)(D == 3 ? cf[2] + s2 : 0));
+          const int la = a0 + (K + 1) * (a1 + (K + 1) * (D == 3 ? a2 : 0));
+          const int lb = b0 + (K + 1) * (b1 + (K + 1) * (D == 3 ? b2 : 0));
+          acc += cm[cell * NE + (la <= lb ? tri_index(NCN, la, lb) : tri_index(NCN, lb, la))];
+        }
+      }
+    }
+    if (pos < kSlotDirichlet) {
+      dst[pos] = acc;
+    } else {  // Dirichlet column: uD at its node
+      const int gc[3] = {g[0] + o[0], g[1] + o[1], g[2] + o[2]};
+      double x[3];
+      node_xyz<D, K>(vc, gc, n0 + 1, n1 + 1, x);
+      const double ud = kp.dir_coef[0] + kp.dir_coef[1] * x[0] + kp.dir_coef[2] * x[1] + kp.dir_coef[3] * x[2];
+      lift += acc * ud;
+    }
+  }
+  // b = sum over the row node's cells of b_c[local node] - lifting: lane q < 2^D
+  // takes the cell of slot combination q
+  double part = -lift;
+  if (live && q < (1 << D)) {
+    const int s0 = q & 1, s1 = (q >> 1) & 1, s2 = (q >> 2) & 1;
+    if (s0 < ns[0] && s1 < ns[1] && (D == 2 || s2 < ns[2])) {
+      const long long cell = (cf[0] + s0) + n0 * ((cf[1] + s1) + n1 * (long long)(D == 3 ? cf[2] + s2 : 0));
+      const int la = (s0 ? 0 : af[0]) + (K + 1) * ((s1 ? 0 : af[1]) + (K + 1) * (D == 3 ? (s2 ? 0 : af[2]) : 0));
+      part += cm[cell * NE + NJ + la];
+    }
+  }
+#pragma unroll
+  for (int m = LPR / 2; m >= 1; m >>= 1) part += __shfl_xor(part, m, LPR);
+  if (live && q == 0) b[mr.bidx] = part;
+}
+
+template <int D, int K, int NQ>
+int launch_mapped_t(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_stage) {
+  using Sh = MShape<D, K>;
+  constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;
+  constexpr int BYTES = NQT * Sh::NCN * D * 8;
+  constexpr int WPB = BYTES * 4 <= 96 * 1024 ? 4 : (BYTES * 2 <= 96 * 1024 ? 2 : 1);
+  if (!rows_stage) {
+    const long long blocks = (M->ncell + WPB - 1) / WPB;
+    if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((k_mapped_cells<D, K, NQ, WPB>), dim3((unsigned)blocks), dim3(64 * WPB), 0, st, p->kp,
+                       M->ncell, M->vc, p->dev.modes, p->coef_cell ? p->dev.cellk : nullptr, M->cm);
+  } else if (M->nrows > 0) {
+    constexpr int LPR = Sh::S <= 16 ? 16 : (Sh::S <= 32 ? 32 : 64);
+    const long long blocks = (M->nrows * LPR + 255) / 256;
+    if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((k_mapped_rows<D, K>), dim3((unsigned)blocks), dim3(256), 0, st, p->kp, M->nrows, M->rows,
+                       p->dev.pat_fwd, pattern_stride(p->kp.S), M->cm, M->vc, p->dev.values, p->dev.b);
+  }
+  return hipGetLastError() == hipSuccess ? GDX_OK : GDX_ERR_HIP;
+}
+
+template <int D, int K>
+int launch_mapped_k(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_stage) {
+  switch (p->NQ) {
+    case 1: return launch_mapped_t<D, K, 1>(p, M, st, rows_stage);
+    case 2: return launch_mapped_t<D, K, 2>(p, M, st, rows_stage);
+    case 3: return launch_mapped_t<D, K, 3>(p, M, st, rows_stage);
+    case 4: return launch_mapped_t<D, K, 4>(p, M, st, rows_stage);
+  }
+  return GDX_ERR_UNSUPPORTED;
+}
+
+int launch_mapped(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_stage) {
+  if (p->D == 3) return p->K == 2 ? launch_mapped_k<3, 2>(p, M, st, rows_stage) : launch_mapped_k<3, 1>(p, M, st, rows_stage);
+  return p->K == 2 ? launch_mapped_k<2, 2>(p, M, st, rows_stage) : launch_mapped_k<2, 1>(p, M, st, rows_stage);
+}
+
+}  // namespace
+
+void mapped_free(gdx_problem *p) {
+  MappedData *M = md(p);
+  if (!M) return;
+  void *ptrs[] = {M->vc, M->cm, M->rows};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  delete M;
+  p->mapped_data = nullptr;
+}
+
+// vertex coordinates to HBM; the cell records and the row list are made on the
+// first call after a symbolic phase (the numeric phase allocates nothing)
+int mapped_set(gdx_problem *p, const double *xyz) {
+  const int D = p->D, K = p->K;
+  int64_t nv = 1;
+  for (int d = 0; d < D; ++d) nv *= p->kp.ncells[d] + 1;
+  for (int d = 0; d < D; ++d)
+    if (p->kp.nn[d] > 65535) return GDX_ERR_UNSUPPORTED;  // 16-bit node coordinates in MappedRow
+  MappedData *M = md(p);
+  if (!M) {
+    M = new MappedData;
+    p->mapped_data = M;
+    const int ncn = D == 3 ? (K + 1) * (K + 1) * (K + 1) : (K + 1) * (K + 1);
+    M->ne = ncn * (ncn + 1) / 2 + ncn;
+    M->ncell = num_cells(p);
+    std::vector<int64_t> grp0(p->nparts + 1, 0);
+    for (int r = 0; r < p->nparts; ++r) grp0[r + 1] = grp0[r] + p->P[r].n_own;
+    M->nrows = grp0[p->nparts];
+    unsigned long long *cnt = nullptr;  // listed rows, out-of-range flag
+    const size_t rbytes = std::max<int64_t>(M->nrows, 1) * sizeof(MappedRow);
+    bool ok = dev_malloc(reinterpret_cast<void **>(&M->vc), (size_t)nv * D * sizeof(double)) == hipSuccess &&
+              dev_malloc(reinterpret_cast<void **>(&M->cm), (size_t)std::max<int64_t>(M->ncell, 1) * M->ne *
+                                                                sizeof(double)) == hipSuccess &&
+              dev_malloc(reinterpret_cast<void **>(&M->rows), rbytes) == hipSuccess &&
+              dev_malloc(reinterpret_cast<void **>(&cnt), 2 * sizeof(unsigned long long)) == hipSuccess &&
+              hipMemsetAsync(M->rows, 0xFF, rbytes, p->ctx->stream) == hipSuccess &&  // pid -1: no row
+              hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), p->ctx->stream) == hipSuccess;
+    for (int r = 0; r < p->nparts && ok; ++r) {
+      const Part &P = p->P[r];
+      const long long n = P.onb_n[0] * P.onb_n[1] * P.onb_n[2];
+      if (n > 0)
+        hipLaunchKernelGGL(k_mapped_rowlist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->ctx->stream,
+                           p->dev.rowinfo, (long long)P.rowinfo_base, (long long)P.row_base, (long long)P.n_own,
+                           (long long)grp0[r], (long long)P.onb_lo[0], (long long)P.onb_lo[1], (long long)P.onb_lo[2],
+                           (long long)P.onb_n[0], (long long)P.onb_n[1], (long long)P.onb_n[2], M->rows, cnt);
+    }
+    unsigned long long hc[2] = {0, 0};
+    ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(p->ctx->stream) == hipSuccess &&
+         hipMemcpy(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost) == hipSuccess;
+    if (cnt) (void)hipFree(cnt);
+    const bool bad = ok && (hc[1] != 0 || hc[0] != (unsigned long long)M->nrows);
+    if (!ok || bad) {
+      mapped_free(p);
+      return bad ? GDX_ERR_STATE : GDX_ERR_ALLOC;
+    }
+  }
+  if (hipMemcpy(M->vc, xyz, (size_t)nv * D * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return GDX_ERR_HIP;
+  return GDX_OK;
+}
+
+int mapped_assemble(gdx_problem *p, hipStream_t st, int stage) {
+  MappedData *M = md(p);
+  if (!M || !M->vc || !M->cm || !M->rows) return GDX_ERR_STATE;
+  if (p->coef_cell && !p->dev.cellk) return GDX_ERR_STATE;
+  if (!p->coef_cell && p->desc.nmodes > 0 && !p->dev.modes) return GDX_ERR_STATE;
+  return launch_mapped(p, M, st, stage == 1);
+}
+
+}  // namespace gdx

@@ -24,7 +24,7 @@ from .nonlinear import (FEFunction, FEOperator, FESolver, FluxJacobian, FluxResi
 def problem_from_workload(ctx, p, strategy=None):
     """Build model/spaces/forms for a workload dict (gridap_amd.workloads) and
     return (assembler, data) ready for assemble_matrix_and_vector."""
-    model = CartesianDiscreteModel(ctx, p["parts"], p["domain"], p["cells"])
+    model = CartesianDiscreteModel(ctx, p["parts"], p["domain"], p["cells"], vertex_coords=p.get("vertex_coords"))
     if strategy is None:
         strategy = FullyAssembledRows() if p.get("strategy") == "full" else SubAssembledRows()
     if p.get("op") == "elasticity":

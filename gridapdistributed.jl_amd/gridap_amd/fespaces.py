@@ -335,6 +335,15 @@ class GpuProblem:
         kappa = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
         _lib.call("gdx_set_cell_coefficient", self.handle, kappa.ctypes.data)
 
+    def set_vertex_coords(self, xyz):
+        """Mapped grid: coordinates of every mesh vertex (shape (nvertices, dim),
+        x fastest) for the next numeric assembly; None returns to the uniform box."""
+        if xyz is None:
+            _lib.call("gdx_set_vertex_coords", self.handle, None)
+            return
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        _lib.call("gdx_set_vertex_coords", self.handle, xyz.ctypes.data)
+
     def set_cell_lame(self, lame):
         """Elasticity: (lambda, mu) per cell (shape (ncells, 2), x fastest) for the
         next numeric assembly; None returns to the descriptor's constants."""
@@ -441,6 +450,8 @@ class SparseMatrixAssembler:
                 self.problem.close()
             self.problem = GpuProblem(self.V.model.ctx, desc, keep)
             self.problem.symbolic()
+            if self.V.model.vertex_coords is not None:
+                self.problem.set_vertex_coords(self.V.model.vertex_coords)
             self._key = key
         return self.problem
 

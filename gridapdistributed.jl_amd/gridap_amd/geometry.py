@@ -36,20 +36,45 @@ def with_gpu(f, device=0):
 
 
 class CartesianDiscreteModel:
-    """CartesianDiscreteModel(ranks, parts, domain, cells) (src/Geometry.jl:378-388).
+    """CartesianDiscreteModel(ranks, parts, domain, cells; map) (src/Geometry.jl:378-388).
 
     ``domain`` = (x0, x1, y0, y1[, z0, z1]); ``cells`` and ``parts`` per direction.
     Cells are split with the reference's uniform partition (src/PArraysExtras.jl:5-84).
+    ``map`` (optional, the CartesianDescriptor's map) takes an (n, dim) array of
+    points of the uniform box to their images; ``vertex_coords`` gives the mapped
+    vertices directly ((cells_0 + 1) (cells_1 + 1) [(cells_2 + 1)] rows, x fastest).
+    Cells then take the multilinear map of their vertices (diffusion only).
     """
 
-    def __init__(self, ctx, parts, domain, cells):
-        parts, cells, domain = tuple(int(p) for p in parts), tuple(int(c) for c in cells), tuple(map(float, domain))
+    def __init__(self, ctx, parts, domain, cells, map=None, vertex_coords=None):
+        import numpy as np
+
+        parts, cells = tuple(int(p) for p in parts), tuple(int(c) for c in cells)
+        domain = tuple(float(v) for v in domain)
         if len(parts) != len(cells) or len(domain) != 2 * len(cells):
             raise ValueError("parts, cells and domain must describe the same dimension")
         if any(c < p for c, p in zip(cells, parts)):
             raise ValueError("every part needs at least one cell per direction")
         self.ctx, self.parts, self.domain, self.cells = ctx, parts, domain, cells
         self.dim = len(cells)
+        if map is not None and vertex_coords is not None:
+            raise ValueError("give map or vertex_coords, not both")
+        if map is not None:
+            vertex_coords = map(self.uniform_vertices())
+        self.vertex_coords = None
+        if vertex_coords is not None:
+            vc = np.ascontiguousarray(vertex_coords, dtype=np.float64)
+            if vc.shape != (int(np.prod([c + 1 for c in cells])), self.dim):
+                raise ValueError("vertex_coords must have one row of dim coordinates per mesh vertex")
+            self.vertex_coords = vc
+
+    def uniform_vertices(self):
+        """The vertices of the uniform box, (nvertices, dim), x fastest."""
+        import numpy as np
+
+        axes = [np.linspace(self.domain[2 * d], self.domain[2 * d + 1], self.cells[d] + 1) for d in range(self.dim)]
+        grids = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.ravel(order="F") for g in grids], axis=1)
 
     def num_parts(self):
         n = 1

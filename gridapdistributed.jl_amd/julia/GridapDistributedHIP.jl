@@ -15,7 +15,8 @@ Julia binding of libgdx.so (C ABI in include/gdx.h).
   `SparseMatrixCSR{0,Float64,Int}`).
 - The lowered vocabulary: forms built with `HIPForm` from `Diffusion(κ)`,
   `Source(f)`, `Elasticity(λ, μ)`, `BodyForce(f)` on a `CartesianDiscreteModel`
-  with Lagrangian Q1/Q2.  A `HIPForm` is also an ordinary Gridap form (it
+  with Lagrangian Q1/Q2; diffusion also on a model built with a `map` (the
+  mapped vertices go to `gdx_set_vertex_coords`).  A `HIPForm` is also an ordinary Gridap form (it
   carries the Julia closure), so whatever does not lower (`lower` returns
   `nothing`) is assembled by the reference's `DistributedSparseMatrixAssembler`
   unchanged.
@@ -409,13 +410,41 @@ function cartesian_info(model)
   pdesc = model isa GridapDistributed.DistributedDiscreteModel ? model.metadata : nothing
   pdesc isa GridapDistributed.DistributedCartesianDescriptor || return nothing
   desc = pdesc.descriptor
-  (desc.map === identity && !any(desc.isperiodic) && all(pdesc.ghost)) || return nothing
+  (!any(desc.isperiodic) && all(pdesc.ghost)) || return nothing
   D = length(desc.partition)
   D in (2, 3) || return nothing
   origin = Tuple(desc.origin)
   cells = Tuple(desc.partition)
   domain = ntuple(k -> isodd(k) ? origin[(k + 1) ÷ 2] : origin[k ÷ 2] + desc.sizes[k ÷ 2] * cells[k ÷ 2], 2D)
-  (D, cells, Tuple(pdesc.mesh_partition), domain)
+  (D, cells, Tuple(pdesc.mesh_partition), domain, desc.map)
+end
+
+# the descriptor's map applied to every vertex of the uniform box: a D x nv
+# matrix, vertices x fastest (the layout of gdx_set_vertex_coords)
+function mapped_vertices(D, cells, domain, gmap)
+  nv = prod(c + 1 for c in cells)
+  xyz = Matrix{Float64}(undef, D, nv)
+  for (v, I) in enumerate(CartesianIndices(Tuple(c + 1 for c in cells)))
+    x = ntuple(d -> domain[2d - 1] + (domain[2d] - domain[2d - 1]) * (I[d] - 1) / cells[d], D)
+    y = gmap(Point(x))
+    for d in 1:D
+      xyz[d, v] = y[d]
+    end
+  end
+  xyz
+end
+
+"""
+    set_vertex_coords!(p, xyz)
+
+Mapped grid of a diffusion problem: xyz is a D x nv matrix of vertex
+coordinates (vertices x fastest), `nothing` returns to the uniform box.
+"""
+function set_vertex_coords!(p::GdxProblem, xyz::Union{Nothing,Matrix{Float64}})
+  ptr = xyz === nothing ? C_NULL : pointer(xyz)
+  GC.@preserve xyz check(ccall((:gdx_set_vertex_coords, libgdx), Cint, (Ptr{Cvoid}, Ptr{Float64}), p.h, ptr),
+                         :gdx_set_vertex_coords)
+  nothing
 end
 
 pad3(t, v) = ntuple(k -> k <= length(t) ? t[k] : v, 3)
@@ -445,7 +474,7 @@ function lower(a::HIPSparseMatrixAssembler, data::HIPData)
   m.qdeg == l.qdeg || return nothing
   info = cartesian_info(get_background_model(get_triangulation(a.test)))
   info === nothing && return nothing
-  D, cells, parts, domain = info
+  D, cells, parts, domain, gmap = info
   code = strategy_code(get_assembly_strategy(a))
   code === nothing && return nothing
   a.order in (1, 2) || return nothing
@@ -464,6 +493,7 @@ function lower(a::HIPSparseMatrixAssembler, data::HIPData)
   elseif m.integrand isa Elasticity && l.integrand isa BodyForce
     ncomp = 3
     D == 3 || return nothing
+    gmap === identity || return nothing  # mapped grids: diffusion only
     a.dirichlet in (:clamped_x0, :none) || return nothing
     (a.uD.c0 == 0.0 && all(iszero, a.uD.c)) || return nothing
     coef, nmodes, rhs_kind, rhs_const = 1.0, 0, Int32(0), 0.0
@@ -478,17 +508,20 @@ function lower(a::HIPSparseMatrixAssembler, data::HIPData)
   desc = GdxProblemDesc(Int32(D), Int32(a.order), Int32(m.qdeg), code, pad3(Int64.(cells), Int64(1)),
                         pad3(Int32.(parts), Int32(1)), pad6(domain), coef, Int32(nmodes), pointer(modes),
                         rhs_kind, rhs_const, dir_kind, (a.uD.c0, a.uD.c...), op, 0.0, 0.0, 0.0, lam, mu, bf)
-  desc, modes
+  xyz = gmap === identity ? nothing : mapped_vertices(D, cells, domain, gmap)
+  desc, modes, xyz
 end
 
 # the problem of a lowered descriptor: created and run through gdx_symbolic once
 function hip_problem(a::HIPSparseMatrixAssembler, lowered)
-  desc, modes = lowered
+  desc, modes, xyz = lowered
   key = (desc.dim, desc.order, desc.qdeg, desc.strategy, desc.cells, desc.parts, desc.domain, desc.coef_const,
          copy(modes), desc.rhs_kind, desc.rhs_const, desc.dir_kind, desc.dir_coef, desc.op_kind,
-         desc.lame_lambda, desc.lame_mu, desc.body_force)
+         desc.lame_lambda, desc.lame_mu, desc.body_force, xyz === nothing ? nothing : copy(xyz))
   get!(a.problems, key) do
-    GdxProblem(a.ctx, desc, modes)
+    p = GdxProblem(a.ctx, desc, modes)
+    xyz === nothing || set_vertex_coords!(p, xyz)
+    p
   end
 end
 

@@ -218,6 +218,19 @@ int gdx_export_cell_coefficient(gdx_problem *prob, double *kappa);
    the three decides; it takes effect at the next gdx_numeric. */
 int gdx_set_cell_lame(gdx_problem *prob, const double *lame);
 
+/* Mapped Cartesian grid: CartesianDiscreteModel(ranks, parts, domain, cells;
+   map) (src/Geometry.jl:378-388, the map forwarded to the CartesianDescriptor).
+   xyz holds the mapped coordinates of every mesh vertex, dim values each,
+   vertices x fastest over (cells_0 + 1) x (cells_1 + 1) [x (cells_2 + 1)];
+   each cell takes the multilinear (Q1) map of its vertices and a Q2 node sits
+   at that map's image of its reference position (the mean of the vertices of
+   the edge, face or cell whose midpoint it is).  Quadrature uses |det J| at
+   every Gauss point; kappa and f are evaluated at the mapped points, uD at the
+   mapped nodes.  Diffusion only (GDX_ERR_UNSUPPORTED otherwise), FP64; NULL
+   returns to the uniform box.  Call after gdx_symbolic (a new symbolic phase
+   returns to the uniform box); takes effect at the next gdx_numeric. */
+int gdx_set_vertex_coords(gdx_problem *prob, const double *xyz);
+
 /* Nonlinear operators (GDX_OP_FLUX).  The state uh of
    residual_and_jacobian(op, uh) (test/PLaplacianTests.jl:44-48) is given by its
    free values x (host array of length ngdofs, global-id order; NULL = zero);

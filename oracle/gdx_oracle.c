@@ -83,6 +83,11 @@ typedef struct {
   /* op_kind 2: (lambda, mu) per cell (x fastest, 2 values a cell), NULL = the
      constants above: sigma = lambda_c tr(eps) I + 2 mu_c eps on cell c */
   const double *cell_lame;
+  /* mapped Cartesian grid: coordinates of every mesh vertex (dim values each,
+     vertices x fastest over (cells_0 + 1) x (cells_1 + 1) [x (cells_2 + 1)]),
+     NULL = the uniform box.  Cells take the multilinear (Q1) map of their
+     vertices; a Q2 node sits at that map's image of its reference position */
+  const double *vertex_coords;
 } orc_desc;
 
 typedef struct {
@@ -119,6 +124,7 @@ typedef struct {
   double *state;
   double *cell_kappa;
   double *cell_lame;
+  double *vertex_coords;
   int nparts, K, D, nq;
   int nc;          /* components per node: dim for elasticity, else 1 */
   i64 nn[3];       /* global node counts per dim */
@@ -250,6 +256,33 @@ static int is_dirichlet_gnode(const orc_ctx *C, const i64 g[3]) {
 static double gnode_coord(const orc_ctx *C, int d, i64 g) {
   double x0 = C->d.domain[2 * d], x1 = C->d.domain[2 * d + 1];
   return x0 + (x1 - x0) * (double)g / (double)(C->nn[d] - 1);
+}
+
+/* coordinates of global lattice node g: on the uniform box per direction; on a
+   mapped grid the multilinear interpolation of the vertices around it (a vertex
+   itself, or the mean of the 2, 4 or 8 vertices of the edge, face or cell whose
+   midpoint it is) */
+static void node_xyz(const orc_ctx *C, const i64 g[3], double x[3]) {
+  const int D = C->D, K = C->K;
+  x[0] = x[1] = x[2] = 0.0;
+  if (!C->vertex_coords) {
+    for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
+    return;
+  }
+  i64 nv[3] = {1, 1, 1}, v0[3] = {0, 0, 0};
+  int two[3] = {0, 0, 0};
+  for (int d = 0; d < D; d++) {
+    nv[d] = C->d.cells[d] + 1;
+    v0[d] = g[d] / K;
+    two[d] = g[d] % K != 0;
+  }
+  const int nb = 1 << (two[0] + two[1] + two[2]);
+  for (int c = 0; c < 8; c++) {
+    int s[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+    if ((s[0] && !two[0]) || (s[1] && !two[1]) || (s[2] && !two[2])) continue;
+    i64 v = (v0[0] + s[0]) + nv[0] * ((v0[1] + s[1]) + nv[1] * (v0[2] + s[2]));
+    for (int d = 0; d < D; d++) x[d] += C->vertex_coords[D * v + d] / nb;
+  }
 }
 
 /* ---------- local dof numbering (Gridap conforming numbering, restated) ---------- */
@@ -524,7 +557,7 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
   for (int v = 0; v < nv; v++) {
     const int *o = D == 3 ? HEX_V[v] : QUAD_V[v];
     i64 g[3] = {K * (cx + o[0] / 2), K * (cy + o[1] / 2), K * (cz + o[2] / 2)};
-    for (int d = 0; d < 3; d++) vx[v][d] = d < D ? gnode_coord(C, d, g[d]) : 0.0;
+    node_xyz(C, g, vx[v]);
   }
   const int nc = C->nc, ncd = nc * ncn; /* cell dofs: component-major */
   for (int a = 0; a < ncd * ncd; a++) Ac[a] = 0.0;
@@ -703,8 +736,8 @@ static void numeric_loop(orc_ctx *C, part_t *P, int mode, const i64 *fe_to_row, 
       if (FA && dof >= 0 && P->ldof_owner[dof] != P->rank) row[k] = -1; /* ghost rows dropped */
       isdir[k] = dof < 0;
       i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
-      double x[3] = {0, 0, 0};
-      for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
+      double x[3];
+      node_xyz(C, g, x);
       uD[k] = isdir[k] && C->d.op_kind != 2 ? eval_uD(C, x) : 0.0; /* elasticity: clamped, u = 0 */
       ucell[k] = state_at(C, g, isdir[k], uD[k]);
     }
@@ -1080,6 +1113,13 @@ void *orc_run(const orc_desc *d, int inplace_reps) {
     memcpy(C->cell_lame, d->cell_lame, sizeof(double) * 2 * nc);
   }
   C->d.cell_lame = C->cell_lame;
+  C->vertex_coords = NULL;
+  if (d->vertex_coords) {
+    i64 nv = (d->cells[0] + 1) * (d->dim > 1 ? d->cells[1] + 1 : 1) * (d->dim > 2 ? d->cells[2] + 1 : 1);
+    C->vertex_coords = malloc(sizeof(double) * d->dim * nv);
+    memcpy(C->vertex_coords, d->vertex_coords, sizeof(double) * d->dim * nv);
+  }
+  C->d.vertex_coords = C->vertex_coords;
   double t0 = now_s();
   setup(C);
   double t1 = now_s();
@@ -1156,6 +1196,7 @@ void orc_free(void *h) {
   free(C->state);
   free(C->cell_kappa);
   free(C->cell_lame);
+  free(C->vertex_coords);
   free(C);
 }
 
@@ -1177,8 +1218,8 @@ double orc_energy(void *h) {
   FOR_BOX(lo, hi, D, cx, cy, cz) {
     for (int a = 0; a < ncn; a++) {
       i64 g[3] = {K * cx + coff[a][0], K * cy + coff[a][1], K * cz + coff[a][2]};
-      double x[3] = {0, 0, 0};
-      for (int d = 0; d < D; d++) x[d] = gnode_coord(C, d, g[d]);
+      double x[3];
+      node_xyz(C, g, x);
       isdir[a] = C->d.dir_kind == 1 && is_dirichlet_gnode(C, g);
       uD[a] = isdir[a] ? eval_uD(C, x) : 0.0;
       ucell[a] = state_at(C, g, isdir[a], uD[a]);

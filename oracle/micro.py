@@ -293,3 +293,71 @@ def elasticity_free_dofs(p):
     nodes = np.arange(int(np.prod(nn)))
     free = np.ones(len(nodes), bool) if p.get("dirichlet") != "clamped_x0" else (nodes % nn[0]) != 0
     return np.repeat(free, 3)
+
+
+def uniform_vertices(p):
+    """The mesh vertices of the uniform box, (nvertices, dim), x fastest."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    axes = [dom[d, 0] + h[d] * np.arange(cells[d] + 1) for d in range(D)]
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.ravel(order="F") for g in grids], axis=1)
+
+
+def mapped_node_coords(p, gnode):
+    """Coordinates of lattice nodes on a mapped grid (p["vertex_coords"]): the
+    image of the node's reference position under the multilinear map of a cell
+    holding it, i.e. the mean of the vertices of the edge, face or cell whose
+    midpoint it is (a vertex itself)."""
+    D, K, cells, dom, h, nn = _geometry(p)
+    vc = np.asarray(p["vertex_coords"], np.float64).reshape(-1, D)
+    nv = [c + 1 for c in cells]
+    ijk, _ = node_coords(p, gnode)
+    X = np.zeros((len(ijk), 3))
+    for n, g in enumerate(ijk):
+        corners = [[g[d] // K] if g[d] % K == 0 else [g[d] // K, g[d] // K + 1] for d in range(D)]
+        pts = []
+        for c in np.ndindex(*[len(cs) for cs in corners]):
+            v = [corners[d][c[d]] for d in range(D)]
+            idx = v[0] + nv[0] * (v[1] + (nv[1] * v[2] if D == 3 else 0))
+            pts.append(vc[idx])
+        X[n, :D] = np.mean(pts, axis=0)
+    return X
+
+
+def affine_mapped_global(p, B):
+    """Diffusion with constant kappa = coef_const and constant f = rhs_const on the
+    uniform box lattice mapped by x = B X + t (no Dirichlet elimination): the
+    matrix keyed by lattice node and the RHS.  With grad_x = B^-T grad_X,
+      a(phi_i, phi_j) = sum_de C_de int d_d(phi_i) d_e(phi_j) dX,
+      C = kappa |det B| B^-1 B^-T,
+    and int d_d(phi_i) d_e(phi_j) is a Kronecker product of the 1D assembled
+    factors (K along d when d == e; P along d and Q along e otherwise; M along
+    the rest)."""
+    import scipy.sparse as sp
+
+    D = p["dim"]
+    B = np.asarray(B, np.float64)
+    F = [assembled_factors_1d(p, d) for d in range(D)]
+    Binv = np.linalg.inv(B)
+    C = p.get("coef_const", 1.0) * abs(np.linalg.det(B)) * Binv @ Binv.T
+
+    def kron(fs):  # x fastest
+        out = sp.csr_matrix(fs[D - 1])
+        for d in range(D - 2, -1, -1):
+            out = sp.kron(out, sp.csr_matrix(fs[d]))
+        return out
+
+    A = None
+    for d in range(D):
+        for e in range(D):
+            fs = [F[k][1] for k in range(D)]  # M
+            if d == e:
+                fs[d] = F[d][0]  # K
+            else:
+                fs[d], fs[e] = F[d][2], F[e][3]  # P along d, Q along e
+            t = C[d, e] * kron(fs)
+            A = t if A is None else A + t
+    m = F[D - 1][4]
+    for d in range(D - 2, -1, -1):
+        m = np.kron(m, F[d][4])
+    return A.tocsr(), p.get("rhs_const", 0.0) * abs(np.linalg.det(B)) * m

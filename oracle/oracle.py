@@ -30,6 +30,7 @@ class OrcDesc(ctypes.Structure):
         ("cell_kappa", ctypes.POINTER(ctypes.c_double)),
         ("lame_lambda", ctypes.c_double), ("lame_mu", ctypes.c_double), ("body_force", ctypes.c_double * 3),
         ("cell_lame", ctypes.POINTER(ctypes.c_double)),
+        ("vertex_coords", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -122,6 +123,13 @@ def make_desc(p):
         d.cell_lame = cl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     else:
         d.cell_lame = None
+    vc = p.get("vertex_coords")  # mapped grid: dim coordinates per mesh vertex, x fastest
+    if vc is not None:
+        vc = np.ascontiguousarray(vc, dtype=np.float64).ravel()
+        d._vc_keep = vc
+        d.vertex_coords = vc.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    else:
+        d.vertex_coords = None
     return d
 
 

@@ -1,0 +1,99 @@
+"""Diffusion on a mapped Cartesian grid (CartesianDiscreteModel with a map,
+src/Geometry.jl:378-388): pins of the C oracle's general-geometry path, CPU only.
+
+The oracle integrates every cell with the multilinear (Q1) map of its
+vertices (J, |det J| and J^-T at each Gauss point); a Q2 node sits at the
+map's image of its reference position.  What pins it:
+- the identity map (the uniform vertices given explicitly) reproduces the
+  uniform-box assembly;
+- an affine map x = B X + t with a non-symmetric B against an independent
+  numpy route: Kronecker products of the 1D assembled factors weighted by
+  C = kappa |det B| B^-1 B^-T (a transposed J^-1 or a wrong |det J| fails it);
+- the patch test on a randomly perturbed (non-affine) mesh: with constant
+  kappa, f = 0 and linear Dirichlet data u = c0 + c.x, the nodal values of u
+  solve A x = b exactly (linear functions are in the mapped Q1/Q2 spaces and
+  the Gauss rule of degree 2K integrates every term of these rows exactly);
+- the box's volume: with the boundary vertices left in place and f constant,
+  the RHS sums to f |box| (no Dirichlet);
+- the rows of the unconstrained matrix sum to zero.
+"""
+import numpy as np
+import pytest
+
+import micro
+import oracle
+from _helpers import mapped_vertices, owned_global
+from gridap_amd import workloads as W
+
+
+def lattice_perm(r):
+    """global id -> lattice node"""
+    m = np.zeros(r.ngdofs, np.int64)
+    for P in r.parts:
+        m[P["ldof_gid"]] = P["ldof_gnode"]
+    return m
+
+
+def case(dim, order, **kw):
+    if dim == 2:
+        p = W.poisson2d_q1(kw.pop("n", 5), kw.pop("parts", (2, 1)))
+    else:
+        p = W.poisson3d_q1(kw.pop("n", 4), kw.pop("parts", (2, 1, 2)))
+    p = dict(p, order=order, qdeg=2 * order)
+    p.update(kw)
+    return p
+
+
+MODES = np.array([[1, 2, 0, 0.3, 0.1], [0, 1, 3, -0.2, 0.5], [2, 0, 1, 0.25, -0.4]], np.float64)
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_identity_map_is_the_uniform_assembly(dim, order):
+    p = case(dim, order, modes=MODES if dim == 3 else MODES[:, [0, 1, 2, 3, 4]] * [1, 1, 0, 1, 1],
+             rhs="manufactured", uD=(0.5, 1.0, -2.0, 0.7))
+    r0 = oracle.run(p)
+    r1 = oracle.run(dict(p, vertex_coords=micro.uniform_vertices(p)))
+    A0, b0 = owned_global(r0.parts, r0.ngdofs)
+    A1, b1 = owned_global(r1.parts, r1.ngdofs)
+    assert abs(A1 - A0).max() <= 1e-14 * abs(A0).max()
+    assert np.abs(b1 - b0).max() <= 1e-13 * np.abs(b0).max()
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_affine_map_matches_kronecker_restatement(dim, order):
+    p = case(dim, order, dirichlet=False, coef_const=1.7, rhs_const=0.9)
+    B = np.array([[1.2, 0.3, -0.2], [-0.1, 0.8, 0.25], [0.15, -0.3, 1.1]])[:dim, :dim]
+    t = np.array([0.3, -0.2, 0.1])[:dim]
+    X = micro.uniform_vertices(p)
+    r = oracle.run(dict(p, vertex_coords=X @ B.T + t))
+    A, b = owned_global(r.parts, r.ngdofs)
+    g2n = lattice_perm(r)
+    Am, bm = micro.affine_mapped_global(p, B)
+    Am, bm = Am[g2n][:, g2n], bm[g2n]
+    assert abs(A - Am).max() <= 1e-13 * abs(Am).max()
+    assert np.abs(b - bm).max() <= 1e-13 * np.abs(bm).max()
+
+
+@pytest.mark.parametrize("dim,order,parts", [(2, 1, (2, 2)), (2, 2, (1, 2)), (3, 1, (2, 2, 1)), (3, 2, (1, 2, 2))])
+def test_patch_test_on_a_perturbed_mesh(dim, order, parts):
+    uD = (0.4, 1.3, -0.6, 0.9)
+    p = case(dim, order, parts=parts, coef_const=2.3, rhs_const=0.0, uD=uD, n=5 if dim == 2 else 4)
+    p["vertex_coords"] = mapped_vertices(p, amp=0.5, seed=dim + order)
+    r = oracle.run(p)
+    A, b = owned_global(r.parts, r.ngdofs)
+    x = micro.mapped_node_coords(p, lattice_perm(r))
+    u = uD[0] + x @ np.asarray(uD[1:])
+    assert np.abs(A @ u - b).max() <= 1e-12 * np.abs(b).max()
+    # and the map is not affine: the matrix differs from the unperturbed one
+    A0, _ = owned_global(oracle.run(dict(p, vertex_coords=None)).parts, r.ngdofs)
+    assert abs(A - A0).max() > 1e-3 * abs(A0).max()
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (3, 1), (3, 2)])
+def test_volume_and_row_sums(dim, order):
+    p = case(dim, order, dirichlet=False, rhs_const=1.5, modes=MODES, n=4)
+    p["vertex_coords"] = mapped_vertices(p, amp=0.6, seed=3, smooth=0.08)
+    r = oracle.run(p)
+    A, b = owned_global(r.parts, r.ngdofs)
+    assert abs(b.sum() - 1.5) <= 1e-13  # the unit box
+    assert np.abs(A @ np.ones(r.ngdofs)).max() <= 1e-13 * abs(A).max()
