@@ -116,18 +116,40 @@ __device__ inline void kappa_f(const KParams &kp, const double *__restrict__ mod
           : kp.rhs_const;
 }
 
-// Stage 1: one wave per cell, WPB waves per workgroup.
-template <int D, int K, int NQ, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long ncell, const double *__restrict__ vc,
-                                                           const double *__restrict__ modes,
-                                                           const double *__restrict__ cellk, double *__restrict__ cm) {
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// Stage 1: one wave per cell, WPB waves per workgroup.  The cell matrix is
+// G^T diag(w) G with G[(q, d)][a] = (J^-T grad_ref phi_a)_d at Gauss point q:
+// a (NQT D) x NCN matrix, K = NQT D (padded to KD, a multiple of 4) and NCN
+// padded to NT tiles of 16, contracted on v_mfma_f64_16x16x4f64 (lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; its accumulator i is
+// D[4 i + l / 16][l % 16], unlike the FP32 16x16x4 form's D[4 (l / 16) + i]).
+// Only the tiles on and above the diagonal are computed.
+template <int D, int K, int NQ>
+struct CellShape {
+  static constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // Gauss points, <= 64
+  static constexpr int KD = (NQT * D + 3) / 4 * 4;             // contraction length
+  static constexpr int NT = (MShape<D, K>::NCN + 15) / 16;      // 16-wide tiles per side
+  static constexpr int NCN = MShape<D, K>::NCN;
+  static constexpr int BYTES = (KD * NCN + KD + NQT * (D * D + 1)) * 8;  // LDS per wave
+  // waves per workgroup: at most 48 KB of LDS per workgroup (3 or more per CU)
+  static constexpr int WPB = BYTES * 4 <= 48 * 1024 ? 4 : (BYTES * 2 <= 48 * 1024 ? 2 : 1);
+};
+
+template <int D, int K, int NQ>
+__global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cells(
+    KParams kp, long long ncell, const double *__restrict__ vc, const double *__restrict__ modes,
+    const double *__restrict__ cellk, double *__restrict__ cm) {
   using Sh = MShape<D, K>;
-  constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // <= 64
-  constexpr int NCN = Sh::NCN, NJ = Sh::NJ, NE = Sh::NE, NV = Sh::NV;
-  __shared__ double G[WPB][NQT][NCN][D];  // physical gradients
-  __shared__ double WK[WPB][NQT], WF[WPB][NQT];
-  __shared__ double T1[3][kMaxK + 1][kMaxQ];  // 1D basis values, derivatives; Gauss points and weights
-  __shared__ double TQ[2][kMaxQ];
+  using C = CellShape<D, K, NQ>;
+  constexpr int NQT = C::NQT, KD = C::KD, NT = C::NT, WPB = C::WPB;
+  constexpr int NCN = Sh::NCN, NJ = Sh::NJ, NV = Sh::NV;
+  __shared__ double G[WPB][KD][NCN];   // physical gradients, row (q, d), column a
+  __shared__ double WG[WPB][KD];       // w |det J| kappa of row (q, d)'s point
+  __shared__ double JT[WPB][NQT][D * D];  // J^-T per point
+  __shared__ double WF[WPB][NQT];      // w |det J| f
+  __shared__ double T1[2][kMaxK + 1][kMaxQ];  // 1D basis values and derivatives at the Gauss points
+  __shared__ double TQ[2][kMaxQ];             // Gauss points and weights on [0, 1]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < (K + 1) * NQ; i += blockDim.x) {
     const int a = i / NQ, q = i % NQ;
@@ -144,18 +166,15 @@ __global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long
   const int n0 = kp.ncells[0], n1 = kp.ncells[1];
   const long long c0 = c % n0, c1 = (c / n0) % n1, c2 = D == 3 ? c / ((long long)n0 * n1) : 0;
   const long long nv0 = n0 + 1, nv1 = n1 + 1;
-  // vertex coordinates (every lane reads all 2^D of them: they are few and cached)
-  double X[NV][D];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    const long long vi = (c0 + (v & 1)) + nv0 * ((c1 + ((v >> 1) & 1)) + nv1 * (c2 + ((v >> 2) & 1)));
-#pragma unroll
-    for (int d = 0; d < D; ++d) X[v][d] = vc[vi * D + d];
-  }
-  const double kcell = cellk ? kp.coef_const * cellk[c] : 0.0;
-  // Gauss point q = lane: Jacobian, weights, then the gradients of point q
-  double Jit[D][D] = {};  // J^-T
+  // A: Gauss point q = lane: Jacobian of the multilinear map, weights, J^-T
   if (lane < NQT) {
+    double X[NV][D];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const long long vi = (c0 + (v & 1)) + nv0 * ((c1 + ((v >> 1) & 1)) + nv1 * (c2 + ((v >> 2) & 1)));
+#pragma unroll
+      for (int d = 0; d < D; ++d) X[v][d] = vc[vi * D + d];
+    }
     const int qi[3] = {lane % NQ, (lane / NQ) % NQ, D == 3 ? lane / (NQ * NQ) : 0};
     double xi[3], J[D][D] = {}, x[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -179,10 +198,9 @@ __global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long
         for (int e = 0; e < D; ++e) J[d][e] += X[v][d] * dN[e];
       }
     }
-    double det;
+    double det, Jit[D][D];  // J^-T[d][e] = J^-1[e][d] = adj(J)[e][d] / det
     if constexpr (D == 2) {
       det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      // J^-1 = adj / det; J^-T[d][e] = J^-1[e][d]
       Jit[0][0] = J[1][1] / det;
       Jit[1][0] = -J[0][1] / det;
       Jit[0][1] = -J[1][0] / det;
@@ -194,7 +212,6 @@ __global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long
                    a20 = J[1][0] * J[2][1] - J[1][1] * J[2][0], a21 = J[0][1] * J[2][0] - J[0][0] * J[2][1],
                    a22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
       det = J[0][0] * a00 + J[0][1] * a10 + J[0][2] * a20;
-      // J^-1[i][j] = a_ij / det; J^-T[d][e] = J^-1[e][d]
       Jit[0][0] = a00 / det, Jit[0][1] = a10 / det, Jit[0][2] = a20 / det;
       Jit[1][0] = a01 / det, Jit[1][1] = a11 / det, Jit[1][2] = a21 / det;
       Jit[2][0] = a02 / det, Jit[2][1] = a12 / det, Jit[2][2] = a22 / det;
@@ -204,17 +221,30 @@ __global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long
     for (int d = 0; d < D; ++d) w *= TQ[1][qi[d]];
     double kap, f;
     if (cellk) {
-      kap = kcell;
+      kap = kp.coef_const * cellk[c];
       f = kp.rhs_const;
     } else {
       kappa_f(kp, modes, x, kap, f);
     }
-    WK[wave][lane] = w * kap;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      WG[wave][lane * D + d] = w * kap;
+#pragma unroll
+      for (int e = 0; e < D; ++e) JT[wave][lane][d * D + e] = Jit[d][e];
+    }
     WF[wave][lane] = w * f;
-    // grad(phi_a) = J^-T grad_ref(phi_a) at this point, every a
-    for (int a = 0; a < NCN; ++a) {
+  }
+  for (int k = NQT * D + lane; k < KD; k += 64) WG[wave][k] = 0.0;  // padding rows
+  __builtin_amdgcn_wave_barrier();  // LDS written and read by the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // B: G[(q, d)][a] = sum_e J^-T[d][e] grad_ref(phi_a)_e at q; padding rows zero
+  for (int i = lane; i < NQT * NCN; i += 64) {
+    const int q = i / NCN, a = i % NCN;
+    double gr[D];
+    {
+      const int qi[3] = {q % NQ, (q / NQ) % NQ, D == 3 ? q / (NQ * NQ) : 0};
       const int ai[3] = {a % (K + 1), (a / (K + 1)) % (K + 1), D == 3 ? a / ((K + 1) * (K + 1)) : 0};
-      double gr[D];
 #pragma unroll
       for (int e = 0; e < D; ++e) {
         gr[e] = 1.0;
@@ -225,39 +255,48 @@ __global__ __launch_bounds__(64 * WPB) void k_mapped_cells(KParams kp, long long
       for (int d = 0; d < D; ++d) {
         double s = 0.0;
 #pragma unroll
-        for (int e = 0; e < D; ++e) s += Jit[d][e] * gr[e];
-        G[wave][lane][a][d] = s;
+        for (int e = 0; e < D; ++e) s += JT[wave][q][d * D + e] * gr[e];
+        G[wave][q * D + d][a] = s;
       }
     }
   }
-  __builtin_amdgcn_wave_barrier();  // LDS written and read by the same wave
+  for (int i = NQT * D * NCN + lane; i < KD * NCN; i += 64) (&G[wave][0][0])[i] = 0.0;
+  __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  double *out = cm + c * (long long)NE;
-  for (int e = lane; e < NE; e += 64) {
-    double acc = 0.0;
-    if (e < NJ) {
-      int a = 0, r = e;
-      while (r >= NCN - a) r -= NCN - a, ++a;
-      const int b = a + r;
-      for (int q = 0; q < NQT; ++q) {
-        double s = 0.0;
+  double *out = cm + c * (long long)Sh::NE;
+  // C: the tiles on and above the diagonal on the matrix cores
+  const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-        for (int d = 0; d < D; ++d) s += G[wave][q][a][d] * G[wave][q][b][d];
-        acc += WK[wave][q] * s;
+  for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+    for (int tb = ta; tb < NT; ++tb) {
+      f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int k0 = 0; k0 < KD; k0 += 4) {
+        const int k = k0 + lk, ca = 16 * ta + li, cb = 16 * tb + li;  // columns past NCN are zero
+        const double av = ca < NCN ? G[wave][k][ca] * WG[wave][k] : 0.0;
+        const double bv = cb < NCN ? G[wave][k][cb] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
       }
-    } else {
-      const int a = e - NJ;
-      const int ai[3] = {a % (K + 1), (a / (K + 1)) % (K + 1), D == 3 ? a / ((K + 1) * (K + 1)) : 0};
-      for (int q = 0; q < NQT; ++q) {
-        const int qi[3] = {q % NQ, (q / NQ) % NQ, D == 3 ? q / (NQ * NQ) : 0};
-        double ph = 1.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) ph *= T1[0][ai[d]][qi[d]];
-        acc += WF[wave][q] * ph;
+      for (int i = 0; i < 4; ++i) {
+        const int a = 16 * ta + 4 * i + lk, b = 16 * tb + li;
+        if (a <= b && b < NCN) out[tri_index(NCN, a, b)] = acc[i];
       }
     }
-    out[e] = acc;
+  // D: the cell vector
+  for (int a = lane; a < NCN; a += 64) {
+    const int ai[3] = {a % (K + 1), (a / (K + 1)) % (K + 1), D == 3 ? a / ((K + 1) * (K + 1)) : 0};
+    double acc = 0.0;
+    for (int q = 0; q < NQT; ++q) {
+      const int qi[3] = {q % NQ, (q / NQ) % NQ, D == 3 ? q / (NQ * NQ) : 0};
+      double ph = 1.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) ph *= T1[0][ai[d]][qi[d]];
+      acc += WF[wave][q] * ph;
+    }
+    out[NJ + a] = acc;
   }
 }
 
@@ -294,7 +333,11 @@ __global__ __launch_bounds__(256) void k_mapped_rows(KParams kp, long long nrows
   constexpr int LPR = S <= 16 ? 16 : (S <= 32 ? 32 : 64);
   constexpr int SPL = (S + LPR - 1) / LPR;
   const int q = threadIdx.x % LPR;
-  const long long r = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  // workgroups are dealt to the 8 XCDs in turn: XCD x takes the x-th eighth of
+  // the rows, so the cell records its rows share stay in its own L2
+  const long long per = gridDim.x / 8;  // the grid is a multiple of 8
+  const long long blk = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long r = (blk * blockDim.x + threadIdx.x) / LPR;
   MappedRow mr{};
   mr.pid = -1;
   if (r < nrows) mr = rows[r];
@@ -377,17 +420,15 @@ __global__ __launch_bounds__(256) void k_mapped_rows(KParams kp, long long nrows
 template <int D, int K, int NQ>
 int launch_mapped_t(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_stage) {
   using Sh = MShape<D, K>;
-  constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;
-  constexpr int BYTES = NQT * Sh::NCN * D * 8;
-  constexpr int WPB = BYTES * 4 <= 96 * 1024 ? 4 : (BYTES * 2 <= 96 * 1024 ? 2 : 1);
+  constexpr int WPB = CellShape<D, K, NQ>::WPB;
   if (!rows_stage) {
     const long long blocks = (M->ncell + WPB - 1) / WPB;
     if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL((k_mapped_cells<D, K, NQ, WPB>), dim3((unsigned)blocks), dim3(64 * WPB), 0, st, p->kp,
+    hipLaunchKernelGGL((k_mapped_cells<D, K, NQ>), dim3((unsigned)blocks), dim3(64 * WPB), 0, st, p->kp,
                        M->ncell, M->vc, p->dev.modes, p->coef_cell ? p->dev.cellk : nullptr, M->cm);
   } else if (M->nrows > 0) {
     constexpr int LPR = Sh::S <= 16 ? 16 : (Sh::S <= 32 ? 32 : 64);
-    const long long blocks = (M->nrows * LPR + 255) / 256;
+    const long long blocks = ((M->nrows * LPR + 255) / 256 + 7) / 8 * 8;  // a multiple of 8 (XCD order)
     if (blocks > 0x7FFFFFFFLL) return GDX_ERR_UNSUPPORTED;
     hipLaunchKernelGGL((k_mapped_rows<D, K>), dim3((unsigned)blocks), dim3(256), 0, st, p->kp, M->nrows, M->rows,
                        p->dev.pat_fwd, pattern_stride(p->kp.S), M->cm, M->vc, p->dev.values, p->dev.b);

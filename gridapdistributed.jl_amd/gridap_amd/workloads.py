@@ -57,6 +57,29 @@ def poisson3d_q1(n, parts=(1, 1, 1), strategy="sub", modes=None, rhs_const=1.0):
                 dirichlet=True, uD=(0.0, 0.0, 0.0, 0.0))
 
 
+def mapped_vertices(p, amp=0.2, seed=0, smooth=0.0):
+    """Vertex coordinates of a mapped grid for workload p ((nvertices, dim), x
+    fastest): the uniform box's vertices, interior ones moved at random by up
+    to amp / 2 of the cell size per direction (numpy default_rng(seed)), plus a
+    smooth displacement of amplitude smooth that vanishes on the box boundary,
+    so the domain stays the box."""
+    D = p["dim"]
+    cells = list(p["cells"])[:D]
+    dom = np.asarray(list(p["domain"])[:2 * D], np.float64).reshape(D, 2)
+    h = (dom[:, 1] - dom[:, 0]) / np.asarray(cells)
+    idx = np.stack(np.meshgrid(*[np.arange(c + 1) for c in cells], indexing="ij"), -1).reshape(-1, D, order="F")
+    X = dom[:, 0] + idx * h
+    rng = np.random.default_rng(seed)
+    interior = np.all((idx > 0) & (idx < np.asarray(cells)), axis=1)
+    Y = X.copy()
+    Y[interior] += amp * h * rng.uniform(-0.5, 0.5, size=(int(interior.sum()), D))
+    if smooth:
+        s = (X - dom[:, 0]) / (dom[:, 1] - dom[:, 0])
+        bump = np.prod(np.sin(np.pi * s), axis=1)
+        Y += smooth * bump[:, None] * (dom[:, 1] - dom[:, 0]) * np.cos(np.pi * np.arange(1, D + 1) / 3.0)
+    return Y
+
+
 def lame_parameters(E, nu):
     """(lambda, mu) of Young's modulus E and Poisson's ratio nu."""
     return E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))

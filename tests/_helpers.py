@@ -33,25 +33,9 @@ def rel_err(a, b):
     return float(np.abs(a - b).max() / scale) if a.size else 0.0
 
 
-def mapped_vertices(p, amp=0.2, seed=0, smooth=0.0):
-    """Vertex coordinates of a mapped grid for workload p: the uniform box's
-    vertices, interior ones moved by up to amp (in units of the local cell size)
-    at random, plus a smooth displacement of amplitude smooth that vanishes on
-    the box boundary (the domain stays the box)."""
-    import micro
 
-    D = p["dim"]
-    cells = list(p["cells"])[:D]
-    X = micro.uniform_vertices(p)
-    dom = np.asarray(list(p["domain"])[:2 * D], np.float64).reshape(D, 2)
-    h = (dom[:, 1] - dom[:, 0]) / np.asarray(cells)
-    rng = np.random.default_rng(seed)
-    idx = np.stack(np.meshgrid(*[np.arange(c + 1) for c in cells], indexing="ij"), -1).reshape(-1, D, order="F")
-    interior = np.all((idx > 0) & (idx < np.asarray(cells)), axis=1)
-    Y = X.copy()
-    Y[interior] += amp * h * rng.uniform(-0.5, 0.5, size=(interior.sum(), D))
-    if smooth:
-        s = (X - dom[:, 0]) / (dom[:, 1] - dom[:, 0])
-        bump = np.prod(np.sin(np.pi * s), axis=1)
-        Y += smooth * bump[:, None] * (dom[:, 1] - dom[:, 0]) * np.cos(np.pi * np.arange(1, D + 1) / 3.0)
-    return Y
+def mapped_vertices(p, amp=0.2, seed=0, smooth=0.0):
+    """a mapped grid's vertex coordinates (the workload definition's)"""
+    from gridap_amd import workloads as W
+
+    return W.mapped_vertices(p, amp=amp, seed=seed, smooth=smooth)

@@ -57,3 +57,11 @@ def test_c3_cellwise_seed_matches_bench():
     src = open(os.path.join(os.path.dirname(CFG), "bench.py")).read()
     seed = int(re.search(r"^ELAST_CELLWISE_SEED = (\d+)", src, flags=re.M).group(1))
     assert c3["seeds"]["elasticity_cellwise_young_modulus"] == seed
+
+
+def test_c2_mapped_seed_matches_bench():
+    c2 = load("c2.json")
+    src = open(os.path.join(os.path.dirname(CFG), "bench.py")).read()
+    seed = int(re.search(r"^MAPPED_SEED = (\d+)", src, flags=re.M).group(1))
+    assert c2["seeds"]["mapped_grid_vertices"] == seed
+    assert "amp=0.5, seed=MAPPED_SEED, smooth=0.05" in src
