@@ -566,6 +566,11 @@ def bench_mapped(args):
     # second kernel: 8 B per stored value written, every record entry read once
     alg_rows = 8.0 * nnz + rec
     ach = alg_rows / (rows_ms * 1e-3) / 1e9
+    traffic = None  # HBM bytes of the row kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes at 64^3
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_mapped_rows_poisson3d_q2_64_mapped.json")
+    if args.cells == 64 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["traffic_bytes"]
     out = {
         "metric": "3D Q2 assembly Gnnz/s (FP64), mapped grid",
         "value": round(nnz / (ms * 1e-3) / 1e9, 2), "unit": "Gnnz/s", "n_gpus": 1, "steps": args.steps,
@@ -579,7 +584,7 @@ def bench_mapped(args):
                    "cell_record_bytes": rec,
                    "cells_kernel_gflop_s": round(ncell * 27 * 729 * 8.0 / (cells_ms * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_mapped_rows<3,2>",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_mapped_rows<3,2,1>",
                      "alg_bytes_per_launch": int(alg_rows)},
     }
     prob.close()
