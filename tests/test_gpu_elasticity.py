@@ -276,3 +276,25 @@ def test_cellwise_lame_graph_replays(gpu_ctx):
         batch.launch()
     batch.close()
     prob.close()
+
+
+def test_cellwise_lame_solve(gpu_ctx):
+    """Jacobi-PCG on the clamped system with a piecewise-constant material
+    reaches 1e-10 and agrees with scipy's direct solve of the oracle's matrix."""
+    import scipy.sparse.linalg as spla
+
+    case = dict(cells=(8, 6, 5), parts=(2, 2, 1), clamped=True, order=2, strategy="sub")
+    p = elast(case)
+    lame = random_lame(case["cells"], 3)
+    _, prob = gpu_problem(gpu_ctx, p)
+    prob.set_cell_lame(lame)
+    prob.numeric()
+    prob.sync()
+    r = oracle.run(dict(p, cell_lame=lame))
+    A, b = owned_global(r.parts, r.ngdofs)
+    xref = spla.spsolve(A.tocsc(), b)
+    x, it, rr, _ = prob.pcg(1e-10, 20000)
+    print("cell-wise elasticity Jacobi-PCG iterations", it, "relres", rr, flush=True)
+    assert rr <= 1e-10
+    assert np.linalg.norm(x - xref) <= 1e-7 * np.linalg.norm(xref)
+    prob.close()

@@ -191,3 +191,29 @@ def test_refusals(gpu_ctx):
         pe.set_vertex_coords(micro.uniform_vertices(el))
     assert e.value.code == -4
     pe.close()
+
+
+def test_solvers_on_a_mapped_grid(gpu_ctx):
+    """The assembled mapped-grid system through the device solvers: Jacobi-PCG
+    and FPCG with the multilevel preconditioner (Galerkin levels of this matrix)
+    reach 1e-8 and agree with a direct solve of the exported matrix."""
+    import scipy.sparse.linalg as spla
+
+    from gridap_amd import LearnedCoarseSolver, fpcg
+
+    p = case(3, 1, 16, (2, 2, 1), modes=MODES, rhs="manufactured", uD=(0.5, 1.0, -2.0, 0.7))
+    p["vertex_coords"] = mapped_vertices(p, amp=0.5, seed=9, smooth=0.05)
+    prob = gpu_problem(gpu_ctx, p)
+    A, b = owned_global([prob.export_part(k) for k in range(prob.nparts)], prob.ngdofs)
+    xref = spla.spsolve(A.tocsc(), b)
+    x, it, rr, _ = prob.pcg(1e-10, 5000)
+    assert rr <= 1e-10 and np.linalg.norm(x - xref) <= 1e-7 * np.linalg.norm(xref)
+    net = LearnedCoarseSolver(gpu_ctx, 8, 0)
+    net.attach(prob)
+    net.set_options(8, use_network=False)
+    xf, itf, rf, _, fb = fpcg(prob, net, None, 1e-10, 2000)
+    print("mapped 16^3 Q1: Jacobi-PCG", it, "multilevel FPCG", itf, flush=True)
+    assert rf <= 1e-10 and not fb and itf < it
+    assert np.linalg.norm(xf - xref) <= 1e-7 * np.linalg.norm(xref)
+    net.close()
+    prob.close()
