@@ -242,6 +242,8 @@ void release_problem_device(gdx_problem *p) {
   p->coef_cell = false;
   p->lame_cell = false;
   p->mapped = false;
+  p->diff_w = 1.0;
+  p->reac_w = 0.0;
 }
 }  // namespace gdx
 
@@ -309,6 +311,8 @@ int gdx_symbolic(gdx_problem *p) {
     // and so does a mapped grid's state: the uniform box again
     gdx::mapped_free(p);
     p->mapped = false;
+    p->diff_w = 1.0;
+    p->reac_w = 0.0;
     gdx::solve_free(p);  // its row tables are rebuilt from the new structure
     gdx::elast_free(p);
     gdx::flux_free(p);
@@ -586,6 +590,8 @@ int gdx_set_vertex_coords(gdx_problem *p, const double *xyz) {
   if (!xyz) {
     if (p->mapped) gdx::batch_invalidate_problem(p->ctx, p);
     p->mapped = false;
+    p->diff_w = 1.0;  // the uniform kernels assemble the plain diffusion form
+    p->reac_w = 0.0;
     return GDX_OK;
   }
   int64_t nv = 1;
@@ -600,6 +606,19 @@ int gdx_set_vertex_coords(gdx_problem *p, const double *xyz) {
   }
   if (!p->mapped) gdx::batch_invalidate_problem(p->ctx, p);
   p->mapped = true;
+  return GDX_OK;
+}
+
+int gdx_set_reaction(gdx_problem *p, double diffusion_weight, double reaction) {
+  GDX_REQUIRE(p, GDX_ERR_ARG, "problem is NULL");
+  GDX_REQUIRE(p->mapped, GDX_ERR_STATE, "the weighted form runs on the mapped-grid path: call gdx_set_vertex_coords");
+  GDX_REQUIRE(std::isfinite(diffusion_weight) && std::isfinite(reaction) && diffusion_weight >= 0.0 &&
+                  reaction >= 0.0 && (diffusion_weight > 0.0 || reaction > 0.0),
+              GDX_ERR_ARG, "weights must be finite, non-negative and not both zero");
+  // the captured cell kernel takes the weights by value
+  if (diffusion_weight != p->diff_w || reaction != p->reac_w) gdx::batch_invalidate_problem(p->ctx, p);
+  p->diff_w = diffusion_weight;
+  p->reac_w = reaction;
   return GDX_OK;
 }
 

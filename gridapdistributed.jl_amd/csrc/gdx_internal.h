@@ -192,6 +192,7 @@ struct gdx_problem {
                           // (elasticity: kappa scales lambda and mu on each cell)
   bool lame_cell = false; // elasticity: (lambda, mu) from dev.cell_lame
   bool mapped = false;    // diffusion on a mapped grid (vertex coordinates in mapped_data)
+  double diff_w = 1.0, reac_w = 0.0;  // mapped grids: diff_w kappa grad(u).grad(v) + reac_w u v
   void *mapped_data = nullptr;  // gdx_mapped.hip
   void *pv = nullptr;     // PRange maps of the device PVectors (gdx_pvector.hip)
   std::vector<gdx_pvector *> pvecs;  // live PVectors on this problem

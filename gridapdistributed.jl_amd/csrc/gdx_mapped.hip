@@ -107,11 +107,12 @@ typedef double f64x4_t __attribute__((ext_vector_type(4)));
 template <int D, int K, int NQ>
 struct CellShape {
   static constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // Gauss points, <= 64
-  static constexpr int KD = (NQT * D + 3) / 4 * 4;             // contraction length
+  static constexpr int KD = (NQT * D + 3) / 4 * 4;             // contraction length (gradient rows)
+  static constexpr int KR = (NQT * (D + 1) + 3) / 4 * 4;       // with the reaction term's value rows
   static constexpr int NT = (MShape<D, K>::NCN + 15) / 16;      // 16-wide tiles per side
   static constexpr int NCN = MShape<D, K>::NCN;
   static constexpr int SHARED = (NQT * NCN * (D + 1) + 2 * kMaxQ) * 8;  // LDS of the workgroup's tables
-  static constexpr int PERWAVE = NQT * (D * D + 2) * 8;                 // LDS per wave
+  static constexpr int PERWAVE = NQT * (D * D + 3) * 8;                 // LDS per wave
   // waves per workgroup: at most 60 KB of LDS per workgroup
   static constexpr int WPB = SHARED + 4 * PERWAVE <= 60 * 1024 ? 4 : (SHARED + 2 * PERWAVE <= 60 * 1024 ? 2 : 1);
 };
@@ -119,7 +120,7 @@ struct CellShape {
 template <int D, int K, int NQ>
 __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cells(
     KParams kp, long long ncell, const double *__restrict__ vc, const double *__restrict__ modes,
-    const double *__restrict__ cellk, double *__restrict__ cm) {
+    const double *__restrict__ cellk, double dw, double rw, double *__restrict__ cm) {
   using Sh = MShape<D, K>;
   using C = CellShape<D, K, NQ>;
   constexpr int NQT = C::NQT, KD = C::KD, NT = C::NT, WPB = C::WPB;
@@ -128,7 +129,7 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
   __shared__ double PH[NQT][NCN];         // basis values at the Gauss points
   __shared__ double TQ[2][kMaxQ];         // Gauss points and weights on [0, 1]
   __shared__ double JT[WPB][NQT][D * D];  // J^-T per point
-  __shared__ double WQ[WPB][NQT], WF[WPB][NQT];  // w |det J| kappa, w |det J| f
+  __shared__ double WQ[WPB][NQT], WF[WPB][NQT], WM[WPB][NQT];  // w |det J| kappa dw, w |det J| f, w |det J| rw
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
     TQ[0][i] = kp.qx[i];
@@ -247,7 +248,8 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
         for (int d = 0; d < D; ++d)
 #pragma unroll
           for (int e = 0; e < D; ++e) JT[wave][q][d * D + e] = Jit[d][e];
-        WQ[wave][q] = w * kap;
+        WQ[wave][q] = w * kap * dw;
+        WM[wave][q] = w * rw;
         WF[wave][q] = w * f;
       }
     }
@@ -260,8 +262,11 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
     for (int ta = 0; ta < NT; ++ta)
 #pragma unroll
       for (int tb = 0; tb < NT; ++tb) acc[ta][tb] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+    // with a reaction term rw u v, the rows (q, D) of the contraction hold the
+    // basis values, weighted by w |det J| rw
+    const int kend = rw != 0.0 ? C::KR : KD;
 #pragma unroll 3
-    for (int k0 = 0; k0 < KD; k0 += 4) {
+    for (int k0 = 0; k0 < kend; k0 += 4) {
       const int k = k0 + lk;
       double gv[NT], wk = 0.0;
 #pragma unroll
@@ -281,6 +286,14 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
             for (int e = 0; e < D; ++e) sg += jt[e] * GR[qk][a][e];
             gv[t] = sg;
           }
+        }
+      } else if (rw != 0.0 && k < NQT * (D + 1)) {
+        const int qm = k - NQT * D;
+        wk = WM[wave][qm];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int a = 16 * t + li;
+          gv[t] = a < NCN ? PH[qm][a] : 0.0;
         }
       }
 #pragma unroll
@@ -465,7 +478,8 @@ int launch_mapped_t(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_sta
     // take every (WPB x workgroups)-th cell
     const long long blocks = std::min<long long>((M->ncell + WPB - 1) / WPB, kMappedBlocks);
     hipLaunchKernelGGL((k_mapped_cells<D, K, NQ>), dim3((unsigned)std::max<long long>(blocks, 1)), dim3(64 * WPB), 0, st, p->kp,
-                       M->ncell, M->vc, p->dev.modes, p->coef_cell ? p->dev.cellk : nullptr, M->cm);
+                       M->ncell, M->vc, p->dev.modes, p->coef_cell ? p->dev.cellk : nullptr, p->diff_w, p->reac_w,
+                       M->cm);
   } else if (M->nrows > 0) {
     constexpr int LPR = Sh::S <= 16 ? 16 : (Sh::S <= 32 ? 32 : 64);
     const long long groups = (M->nrows + kMappedRowsPerGroup - 1) / kMappedRowsPerGroup;

@@ -17,6 +17,8 @@ from .geometry import CartesianDiscreteModel, DeviceContext, Measure, Triangulat
 from .batch import AssemblyBatch
 from .learned import LearnedCoarseSolver, fpcg
 from .visualization import interpolate, writevtk
+from .multifield import (Block, BlockMultiFieldStyle, BlockPMatrix, BlockPVector, Mass, MultiFieldFESpace,
+                         assemble_block_matrix_and_vector)
 from .nonlinear import (FEFunction, FEOperator, FESolver, FluxJacobian, FluxResidual, NLSolver, PowerLawFlux,
                         jacobian, residual, residual_and_jacobian, residual_and_jacobian_, solve, zero)
 

@@ -28,7 +28,7 @@ EXPORTED = [
     "gdx_export_rows", "gdx_export_cols", "gdx_export_csr", "gdx_export_rhs", "gdx_device_values", "gdx_stream",
     "gdx_sync", "gdx_phase_times", "gdx_owned_nnz", "gdx_spmv", "gdx_pcg", "gdx_set_state", "gdx_newton",
     "gdx_set_cell_coefficient", "gdx_cell_coefficient_grf", "gdx_export_cell_coefficient", "gdx_set_cell_lame",
-    "gdx_set_vertex_coords",
+    "gdx_set_vertex_coords", "gdx_set_reaction",
     "gdx_nn_create", "gdx_nn_destroy", "gdx_nn_num_params", "gdx_nn_save", "gdx_nn_load", "gdx_nn_forward",
     "gdx_nn_backward", "gdx_nn_attach", "gdx_nn_train_step", "gdx_nn_apply", "gdx_nn_apply_noalloc", "gdx_fpcg",
     "gdx_nn_set_options", "gdx_nn_coarse_operator", "gdx_nn_set_multigrid", "gdx_nn_level_operator",
@@ -128,6 +128,7 @@ def lib():
         "gdx_export_cell_coefficient": [vp, vp],
         "gdx_set_cell_lame": [vp, vp],
         "gdx_set_vertex_coords": [vp, vp],
+        "gdx_set_reaction": [vp, ctypes.c_double, ctypes.c_double],
         "gdx_nn_create": [vp, i32, ctypes.c_uint64, P(vp)],
         "gdx_nn_destroy": [vp],
         "gdx_nn_num_params": [vp, P(i64)],

@@ -244,10 +244,12 @@ class GpuProblem:
         _lib.call("gdx_problem_create", ctx.handle, ctypes.byref(desc), ctypes.byref(h))
         self.handle = h
         self.symbolic_done = False
+        self.mapped_coords_set = False  # vertex coordinates on the device (a symbolic phase drops them)
 
     def symbolic(self):
         _lib.call("gdx_symbolic", self.handle)
         self.symbolic_done = True
+        self.mapped_coords_set = False
         n = ctypes.c_int32()
         _lib.call("gdx_num_parts", self.handle, ctypes.byref(n))
         g = ctypes.c_int64()
@@ -340,9 +342,15 @@ class GpuProblem:
         x fastest) for the next numeric assembly; None returns to the uniform box."""
         if xyz is None:
             _lib.call("gdx_set_vertex_coords", self.handle, None)
+            self.mapped_coords_set = False
             return
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         _lib.call("gdx_set_vertex_coords", self.handle, xyz.ctypes.data)
+        self.mapped_coords_set = True
+
+    def set_reaction(self, diffusion_weight, reaction):
+        """Mapped-grid path: assemble diffusion_weight kappa grad(u).grad(v) + reaction u v."""
+        _lib.call("gdx_set_reaction", self.handle, float(diffusion_weight), float(reaction))
 
     def set_cell_lame(self, lame):
         """Elasticity: (lambda, mu) per cell (shape (ncells, 2), x fastest) for the

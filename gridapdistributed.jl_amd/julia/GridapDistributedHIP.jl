@@ -192,6 +192,19 @@ function set_cell_lame!(p::GdxProblem, lame::Union{Nothing,Matrix{Float64}})
 end
 
 """
+    set_reaction!(p, diffusion_weight, reaction)
+
+Diffusion-reaction form on the mapped-grid path (after `set_vertex_coords!`):
+diffusion_weight kappa grad(u).grad(v) + reaction u v, the terms of a
+multi-field block such as u2 v2 or u1 v2 (test/BlockSparseMatrixAssemblersTests.jl:110).
+"""
+function set_reaction!(p::GdxProblem, diffusion_weight::Real, reaction::Real)
+  check(ccall((:gdx_set_reaction, libgdx), Cint, (Ptr{Cvoid}, Float64, Float64), p.h, diffusion_weight, reaction),
+        :gdx_set_reaction)
+  nothing
+end
+
+"""
     newton_hip(p, x0; rtol=1e-10, maxiter=50, lin_rtol=1e-8, poisson_init=false, precond=C_NULL, jac32=false) -> (x, report)
 
 solve(FESolver(NLSolver(method=:newton)), op) of test/PLaplacianTests.jl:54-57

@@ -230,6 +230,14 @@ int gdx_set_cell_lame(gdx_problem *prob, const double *lame);
    returns to the uniform box.  Call after gdx_symbolic (a new symbolic phase
    returns to the uniform box); takes effect at the next gdx_numeric. */
 int gdx_set_vertex_coords(gdx_problem *prob, const double *xyz);
+/* Diffusion-reaction form on the mapped-grid path:
+   a(u,v) = int( diffusion_weight kappa grad(u).grad(v) + reaction u v ), the
+   terms of a multi-field block such as u2 v2 or u1 v2 in
+   test/BlockSparseMatrixAssemblersTests.jl:110-114.  Needs
+   gdx_set_vertex_coords (the uniform box's own vertices for an unmapped mesh);
+   clearing the coordinates or a new symbolic phase returns to (1, 0).  Weights
+   finite, non-negative, not both zero. */
+int gdx_set_reaction(gdx_problem *prob, double diffusion_weight, double reaction);
 
 /* Nonlinear operators (GDX_OP_FLUX).  The state uh of
    residual_and_jacobian(op, uh) (test/PLaplacianTests.jl:44-48) is given by its

@@ -88,6 +88,10 @@ typedef struct {
      NULL = the uniform box.  Cells take the multilinear (Q1) map of their
      vertices; a Q2 node sits at that map's image of its reference position */
   const double *vertex_coords;
+  /* op_kind 0 with weights_set: a(u,v) = int( diffusion_weight kappa grad(u).grad(v)
+     + reaction u v ); without, diffusion_weight = 1 and reaction = 0 */
+  int weights_set;
+  double diffusion_weight, reaction;
 } orc_desc;
 
 typedef struct {
@@ -677,10 +681,12 @@ static void integrate_cell(const orc_ctx *C, i64 cx, i64 cy, i64 cz, int ncn, in
         }
         for (int a = 0; a < ncn; a++) {
           bc[a] += f * phi[a] * dV;
+          const double dw = C->d.weights_set ? C->d.diffusion_weight : 1.0;
+          const double rw = C->d.weights_set ? C->d.reaction : 0.0;
           for (int b = 0; b < ncn; b++) {
             double s = 0.0;
             for (int e = 0; e < D; e++) s += grad[a][e] * grad[b][e];
-            Ac[a * ncn + b] += kap * s * dV;
+            Ac[a * ncn + b] += (dw * kap * s + rw * phi[a] * phi[b]) * dV;
           }
         }
       }

@@ -332,7 +332,8 @@ def affine_mapped_global(p, B):
       C = kappa |det B| B^-1 B^-T,
     and int d_d(phi_i) d_e(phi_j) is a Kronecker product of the 1D assembled
     factors (K along d when d == e; P along d and Q along e otherwise; M along
-    the rest)."""
+    the rest).  With p["weights"] = (w_d, r) the form is
+    w_d kappa grad(u).grad(v) + r u v, the second term r |det B| M x M (x M)."""
     import scipy.sparse as sp
 
     D = p["dim"]
@@ -347,7 +348,8 @@ def affine_mapped_global(p, B):
             out = sp.kron(out, sp.csr_matrix(fs[d]))
         return out
 
-    A = None
+    wd, rw = p.get("weights", (1.0, 0.0))
+    A = rw * abs(np.linalg.det(B)) * kron([F[k][1] for k in range(D)])
     for d in range(D):
         for e in range(D):
             fs = [F[k][1] for k in range(D)]  # M
@@ -355,8 +357,7 @@ def affine_mapped_global(p, B):
                 fs[d] = F[d][0]  # K
             else:
                 fs[d], fs[e] = F[d][2], F[e][3]  # P along d, Q along e
-            t = C[d, e] * kron(fs)
-            A = t if A is None else A + t
+            A = A + wd * C[d, e] * kron(fs)
     m = F[D - 1][4]
     for d in range(D - 2, -1, -1):
         m = np.kron(m, F[d][4])

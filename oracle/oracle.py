@@ -31,6 +31,7 @@ class OrcDesc(ctypes.Structure):
         ("lame_lambda", ctypes.c_double), ("lame_mu", ctypes.c_double), ("body_force", ctypes.c_double * 3),
         ("cell_lame", ctypes.POINTER(ctypes.c_double)),
         ("vertex_coords", ctypes.POINTER(ctypes.c_double)),
+        ("weights_set", ctypes.c_int), ("diffusion_weight", ctypes.c_double), ("reaction", ctypes.c_double),
     ]
 
 
@@ -130,6 +131,12 @@ def make_desc(p):
         d.vertex_coords = vc.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     else:
         d.vertex_coords = None
+    # diffusion-reaction: a(u,v) = int( w_d kappa grad(u).grad(v) + r u v )
+    if "weights" in p:
+        d.weights_set = 1
+        d.diffusion_weight, d.reaction = (float(v) for v in p["weights"])
+    else:
+        d.weights_set, d.diffusion_weight, d.reaction = 0, 1.0, 0.0
     return d
 
 

@@ -15,7 +15,10 @@ map's image of its reference position.  What pins it:
   the Gauss rule of degree 2K integrates every term of these rows exactly);
 - the box's volume: with the boundary vertices left in place and f constant,
   the RHS sums to f |box| (no Dirichlet);
-- the rows of the unconstrained matrix sum to zero.
+- the rows of the unconstrained matrix sum to zero;
+- the diffusion-reaction form w_d kappa grad(u).grad(v) + r u v (the block
+  terms of a multi-field form) against the same affine-map restatement, and
+  the pure reaction form summing to r |Omega|.
 """
 import numpy as np
 import pytest
@@ -97,3 +100,24 @@ def test_volume_and_row_sums(dim, order):
     A, b = owned_global(r.parts, r.ngdofs)
     assert abs(b.sum() - 1.5) <= 1e-13  # the unit box
     assert np.abs(A @ np.ones(r.ngdofs)).max() <= 1e-13 * abs(A).max()
+
+
+@pytest.mark.parametrize("dim,order,weights", [(2, 1, (0.0, 1.0)), (2, 2, (0.7, 2.5)), (3, 1, (1.0, 3.0)),
+                                               (3, 2, (0.0, 1.5))])
+def test_reaction_term_matches_restatement(dim, order, weights):
+    p = case(dim, order, dirichlet=False, coef_const=1.7, rhs_const=0.9, weights=weights)
+    B = np.array([[1.2, 0.3, -0.2], [-0.1, 0.8, 0.25], [0.15, -0.3, 1.1]])[:dim, :dim]
+    X = micro.uniform_vertices(p)
+    r = oracle.run(dict(p, vertex_coords=X @ B.T))
+    A, b = owned_global(r.parts, r.ngdofs)
+    g2n = lattice_perm(r)
+    Am, bm = micro.affine_mapped_global(p, B)
+    Am = Am[g2n][:, g2n]
+    assert abs(A - Am).max() <= 1e-13 * abs(Am).max()
+    if weights[0] == 0.0:  # the mass matrix sums to r |Omega|
+        assert abs(A.sum() - weights[1] * abs(np.linalg.det(B))) <= 1e-13 * weights[1]
+    # on the uniform box, without vertex coordinates, the same weights
+    r0 = oracle.run(p)
+    A0, _ = owned_global(r0.parts, r0.ngdofs)
+    Au, _ = owned_global(oracle.run(dict(p, vertex_coords=X)).parts, r0.ngdofs)
+    assert abs(A0 - Au).max() <= 1e-14 * abs(A0).max()
