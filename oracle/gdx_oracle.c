@@ -37,7 +37,16 @@
  *   in-place numeric   src/Algebra.jl:306-393,530-535 (LocalView +
  *                      create_from_nz(::PSparseMatrix) = assemble!)
  *
- * Parity status of each function is stated in DESIGN.md of the repository.
+ * Parity status.  Pinned by closed forms, invariants, an independent numpy
+ * restatement of the cell integrals (oracle/micro.py) and hand enumeration (the
+ * tests under tests/test_oracle_*.py): values, sparsity keyed by lattice node,
+ * partition, owners and gids, the Q2 numbering rule as written here, mapped
+ * geometry, the reaction term, elasticity, the flux operator.  Parity
+ * unpinned (no Gridap or PartitionedArrays to compare with): whether Gridap's
+ * own Q2 edge/face enumeration is the first-encounter rule restated here,
+ * PartitionedArrays' assembly_neighbors order (senders ascending by rank
+ * here), and explicit-zero retention.  These fix the numbering and order of the
+ * outputs only.
  * All indices in the outputs are 0-based.
  */
 #include <math.h>

@@ -15,7 +15,7 @@ for the nonlinear solve, one oracle residual:
   sum_c |cell| kappa_c (lambda tr(e)^2 + 2 mu e:e) with the exported field;
 - learned-preconditioner workload, Q1 128^3, lognormal kappa (held-out field):
   FPCG with the multilevel preconditioner and Jacobi-PCG both reach the FP64 true
-  residual 1e-8, FPCG in at most half the Jacobi-PCG time (the north-star
+  residual 1e-8, FPCG in at most half the Jacobi-PCG time (the workload's
   target); nnz and free dofs in closed form;
 - p-Laplacian Q1 96^3, parts (4,2,2): Newton with multilevel FPCG corrections
   and with Jacobi-PCG corrections reach 1e-10 and agree; the oracle's residual
