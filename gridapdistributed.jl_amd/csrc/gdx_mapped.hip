@@ -117,7 +117,7 @@ struct CellShape {
   static constexpr int WPB = SHARED + 4 * PERWAVE <= 60 * 1024 ? 4 : (SHARED + 2 * PERWAVE <= 60 * 1024 ? 2 : 1);
 };
 
-template <int D, int K, int NQ>
+template <int D, int K, int NQ, bool REACT>
 __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cells(
     KParams kp, long long ncell, const double *__restrict__ vc, const double *__restrict__ modes,
     const double *__restrict__ cellk, double dw, double rw, double *__restrict__ cm) {
@@ -223,9 +223,10 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
         double g = 0.0, dg0 = 0.0, dg1 = 0.0, dg2 = 0.0;
         for (int m = h; m < kp.nmodes; m += QS) {
           const double *Mo = modes + 5 * m;
-          const double th = 2.0 * M_PI * (Mo[0] * x[0] + Mo[1] * x[1] + Mo[2] * x[2]) + Mo[4];
+          // theta / pi = 2 k.x + phi / pi: sincospi reduces its argument exactly
+          const double tp = 2.0 * (Mo[0] * x[0] + Mo[1] * x[1] + Mo[2] * x[2]) + Mo[4] * M_1_PI;
           double sn, cs;
-          sincos(th, &sn, &cs);
+          sincospi(tp, &sn, &cs);
           g += Mo[3] * cs;
           const double sd = -Mo[3] * sn * 2.0 * M_PI;
           dg0 += sd * Mo[0];
@@ -262,11 +263,11 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
     for (int ta = 0; ta < NT; ++ta)
 #pragma unroll
       for (int tb = 0; tb < NT; ++tb) acc[ta][tb] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-    // with a reaction term rw u v, the rows (q, D) of the contraction hold the
-    // basis values, weighted by w |det J| rw
-    const int kend = rw != 0.0 ? C::KR : KD;
+    // with a reaction term rw u v (REACT), the rows (q, D) of the contraction
+    // hold the basis values, weighted by w |det J| rw
+    constexpr int KEND = REACT ? C::KR : KD;
 #pragma unroll 3
-    for (int k0 = 0; k0 < kend; k0 += 4) {
+    for (int k0 = 0; k0 < KEND; k0 += 4) {
       const int k = k0 + lk;
       double gv[NT], wk = 0.0;
 #pragma unroll
@@ -287,7 +288,7 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
             gv[t] = sg;
           }
         }
-      } else if (rw != 0.0 && k < NQT * (D + 1)) {
+      } else if (REACT && k < NQT * (D + 1)) {
         const int qm = k - NQT * D;
         wk = WM[wave][qm];
 #pragma unroll
@@ -477,9 +478,14 @@ int launch_mapped_t(gdx_problem *p, MappedData *M, hipStream_t st, bool rows_sta
     // a persistent grid: every workgroup builds its tables once, then its waves
     // take every (WPB x workgroups)-th cell
     const long long blocks = std::min<long long>((M->ncell + WPB - 1) / WPB, kMappedBlocks);
-    hipLaunchKernelGGL((k_mapped_cells<D, K, NQ>), dim3((unsigned)std::max<long long>(blocks, 1)), dim3(64 * WPB), 0, st, p->kp,
-                       M->ncell, M->vc, p->dev.modes, p->coef_cell ? p->dev.cellk : nullptr, p->diff_w, p->reac_w,
-                       M->cm);
+    if (p->reac_w != 0.0)
+      hipLaunchKernelGGL((k_mapped_cells<D, K, NQ, true>), dim3((unsigned)std::max<long long>(blocks, 1)),
+                         dim3(64 * WPB), 0, st, p->kp, M->ncell, M->vc, p->dev.modes,
+                         p->coef_cell ? p->dev.cellk : nullptr, p->diff_w, p->reac_w, M->cm);
+    else
+      hipLaunchKernelGGL((k_mapped_cells<D, K, NQ, false>), dim3((unsigned)std::max<long long>(blocks, 1)),
+                         dim3(64 * WPB), 0, st, p->kp, M->ncell, M->vc, p->dev.modes,
+                         p->coef_cell ? p->dev.cellk : nullptr, p->diff_w, 0.0, M->cm);
   } else if (M->nrows > 0) {
     constexpr int LPR = Sh::S <= 16 ? 16 : (Sh::S <= 32 ? 32 : 64);
     const long long groups = (M->nrows + kMappedRowsPerGroup - 1) / kMappedRowsPerGroup;
