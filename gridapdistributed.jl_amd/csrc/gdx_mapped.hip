@@ -404,24 +404,36 @@ __global__ __launch_bounds__(256) void k_mapped_rows(KParams kp, long long nrows
         af[j][d] = i % K;
       }
     }
+    // the record row of the node in each of its cells: entry (a, b) of cell c
+    // is at c NE + a NCN + b, and b = a + lo with lo the column offset's linear
+    // index, so a slot adds lo to a per-cell base (computed once per row)
+    long long base[NC];
+    bool cv[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      const int s0 = cc & 1, s1 = (cc >> 1) & 1, s2 = D == 3 ? (cc >> 2) & 1 : 0;
+      cv[cc] = s0 < ns[j][0] && s1 < ns[j][1] && s2 < ns[j][2];
+      const int a0 = s0 ? 0 : af[j][0], a1 = s1 ? 0 : af[j][1], a2 = s2 ? 0 : af[j][2];
+      const long long cell = (cf[j][0] + s0) + n0 * ((cf[j][1] + s1) + n1 * (long long)(D == 3 ? cf[j][2] + s2 : 0));
+      base[cc] = cell * NE + (long long)(a0 + (K + 1) * (a1 + (K + 1) * (D == 3 ? a2 : 0))) * (NCN + 1);
+    }
     // a combination of the node's cells that does not hold the column reads
     // record 0 and adds +0
 #pragma unroll
     for (int t = 0; t < SPL; ++t) {
       const int sl = q + LPR * t;
-      const int o[3] = {sl % W - K, (sl / W) % W - K, D == 3 ? sl / (W * W) - K : 0};
+      const int o0 = sl % W - K, o1 = (sl / W) % W - K, o2 = D == 3 ? sl / (W * W) - K : 0;
+      const int lo = o0 + (K + 1) * (o1 + (K + 1) * o2);
+      // the column lies in the node's first cell (local index af + o) or its
+      // second (local index o), direction by direction
+      const bool f0 = af[j][0] + o0 >= 0 && af[j][0] + o0 <= K, g0 = o0 >= 0 && o0 <= K;
+      const bool f1 = af[j][1] + o1 >= 0 && af[j][1] + o1 <= K, g1 = o1 >= 0 && o1 <= K;
+      const bool f2 = D == 2 || (af[j][2] + o2 >= 0 && af[j][2] + o2 <= K), g2 = D == 2 || (o2 >= 0 && o2 <= K);
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc) {  // s0 fastest: the order of the sum below
         const int s0 = cc & 1, s1 = (cc >> 1) & 1, s2 = D == 3 ? (cc >> 2) & 1 : 0;
-        const int a0 = s0 ? 0 : af[j][0], a1 = s1 ? 0 : af[j][1], a2 = s2 ? 0 : af[j][2];
-        const int b0 = a0 + o[0], b1 = a1 + o[1], b2 = a2 + o[2];
-        const bool ok = sl < S && s0 < ns[j][0] && s1 < ns[j][1] && s2 < ns[j][2] && b0 >= 0 && b0 <= K && b1 >= 0 &&
-                        b1 <= K && (D == 2 || (b2 >= 0 && b2 <= K));
-        const long long cell =
-            (cf[j][0] + s0) + n0 * ((cf[j][1] + s1) + n1 * (long long)(D == 3 ? cf[j][2] + s2 : 0));
-        const int la = a0 + (K + 1) * (a1 + (K + 1) * (D == 3 ? a2 : 0));
-        const int lb = b0 + (K + 1) * (b1 + (K + 1) * (D == 3 ? b2 : 0));
-        const double x = cm[ok ? cell * NE + la * NCN + lb : 0];
+        const bool ok = sl < S && cv[cc] && (s0 ? g0 : f0) && (s1 ? g1 : f1) && (s2 ? g2 : f2);
+        const double x = cm[ok ? base[cc] + lo : 0];
         v[j][t][cc] = ok ? x : 0.0;
       }
     }
