@@ -103,24 +103,29 @@ def assemble_block_matrix_and_vector(X, Y, terms, sources, strategy=None):
         else:
             rw += t.integrand.sigma
         spec[(t.i, t.j)] = (kap, dw, rw, meas)
-    blocks = [[None] * nf for _ in range(nf)]
-    rhs = [None] * nf
-    probs = {}
-    for i in range(nf):
-        row = sorted(j for (ii, j) in spec if ii == i)
+    rows = [sorted(j for (ii, j) in spec if ii == i) for i in range(nf)]
+    # everything is checked before any block is assembled
+    for i, row in enumerate(rows):
         if not row:
             raise ValueError(f"test field {i} has no term")
         for j in row:
             kap, dw, rw, meas = spec[(i, j)]
-            kappa = kap if kap is not None else ConstantCoefficient(1.0)
-            if isinstance(kappa, FourierExpCoefficient) and rw:
+            if isinstance(kap, FourierExpCoefficient) and rw:
                 raise NotImplementedError("a reaction term beside a Fourier kappa is not lowered")
+            U, V = _space(X.spaces[j]), Y.spaces[i]
+            if U is not V and (U.reffe.order != V.reffe.order or U.dirichlet != V.dirichlet):
+                raise NotImplementedError("the fields of a block share one Lagrangian space")
+    blocks = [[None] * nf for _ in range(nf)]
+    rhs = [None] * nf
+    probs = {}
+    for i in range(nf):
+        row = rows[i]
+        for j in row:
+            kap, dw, rw, meas = spec[(i, j)]
+            kappa = kap if kap is not None else ConstantCoefficient(1.0)
             # the field's source goes with the first block of its row
             src = sources[i] if (j == row[0] and sources[i] is not None) else Source(ConstantSource(0.0))
             U, V = X.spaces[j], Y.spaces[i]
-            if _space(U) is not V and (_space(U).reffe.order != V.reffe.order or
-                                        _space(U).dirichlet != V.dirichlet):
-                raise NotImplementedError("the fields of a block share one Lagrangian space")
             assem = SparseMatrixAssembler(U, V, strategy)
             data = collect_cell_matrix_and_vector(U, V, Integral(Diffusion(kappa), meas), Integral(src, meas), None)
             prob = assem._problem_for(data)
