@@ -10,8 +10,9 @@ and the three-field biform3 / liform3.  It checks that the block assembly and
 the monolithic one agree on b and on A 1, field by field, to 1e-10.  Here every
 block is compared with the oracle's single-field assembly of its own
 diffusion-reaction form (structure bit-identical, values within 1e-12), and
-per field b and A 1 are compared with the oracle's blocks summed to 1e-10 -
-the reference's invariant with the monolithic system taken from the oracle.
+per field b and A 1 are compared with the oracle's blocks summed to 1e-10:
+the reference's per-field invariant, its other side taken from the oracle
+(no monolithic multi-field numbering is built here).
 Also: the diffusion-reaction form alone (gdx_set_reaction) on mapped and
 uniform grids, 2D and 3D, against the oracle; its refusals.
 """
