@@ -111,12 +111,29 @@ def _component_of(gnode):
 
 
 def node_coords(model, order):
-    """Coordinates (x fastest) of the global node lattice of order `order`."""
+    """Coordinates (x fastest) of the global node lattice of order `order`.  On a
+    mapped model a node sits at its cells' multilinear map of its reference
+    position: a vertex at its mapped coordinates, a Q2 edge, face or cell
+    midpoint at the mean of that entity's mapped vertices."""
     D = model.dim
     shp = [order * c + 1 for c in model.cells]
-    axes = [np.linspace(model.domain[2 * d], model.domain[2 * d + 1], shp[d]) for d in range(D)]
-    grids = np.meshgrid(*axes, indexing="ij")
-    return np.stack([g.ravel(order="F") for g in grids], axis=1)
+    if model.vertex_coords is None:
+        axes = [np.linspace(model.domain[2 * d], model.domain[2 * d + 1], shp[d]) for d in range(D)]
+        grids = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.ravel(order="F") for g in grids], axis=1)
+    idx = np.stack(np.meshgrid(*[np.arange(n) for n in shp], indexing="ij"), -1).reshape(-1, D, order="F")
+    v0, two = idx // order, (idx % order) != 0
+    nv = [c + 1 for c in model.cells]
+    acc = np.zeros((len(idx), D))
+    cnt = np.zeros(len(idx))
+    for s in np.ndindex(*([2] * D)):
+        s = np.asarray(s)
+        ok = np.all((s == 0) | two, axis=1)
+        v = np.minimum(v0 + s, np.asarray(nv) - 1)
+        flat = v[:, 0] + (v[:, 1] * nv[0] if D > 1 else 0) + (v[:, 2] * nv[0] * nv[1] if D > 2 else 0)
+        acc += np.where(ok[:, None], model.vertex_coords[flat], 0.0)
+        cnt += ok
+    return acc / cnt[:, None]
 
 
 def interpolate(f, space):

@@ -71,3 +71,22 @@ def test_block_forms_refuse_what_is_not_lowered(host_ctx):
     with pytest.raises(NotImplementedError):  # two diffusion terms in one block
         assemble_block_matrix_and_vector(X, Y, [Block(0, 0, Diffusion(), dO), Block(0, 0, Diffusion(), dO),
                                                 Block(1, 1, Mass(1.0), dO)], src)
+
+
+def test_vtk_node_coordinates_on_a_mapped_model(host_ctx):
+    """VTK points and interpolation use the mapped node coordinates: vertices
+    where the map puts them, Q2 midpoints at the mean of their entity's
+    vertices (the rule of the oracle's mapped geometry)"""
+    import micro
+    from gridap_amd.visualization import node_coords
+
+    p = W.poisson3d_q1(3, (1, 1, 1))
+    vc = W.mapped_vertices(p, amp=0.5, seed=2, smooth=0.05)
+    m = CartesianDiscreteModel(host_ctx, (1, 1, 1), (0, 1, 0, 1, 0, 1), (3, 3, 3), vertex_coords=vc)
+    assert np.array_equal(node_coords(m, 1), vc)
+    x2 = node_coords(m, 2)
+    q = dict(p, order=2, vertex_coords=vc)
+    ref = micro.mapped_node_coords(q, np.arange(7 ** 3))
+    assert np.abs(x2 - ref[:, :3]).max() <= 1e-15
+    m0 = CartesianDiscreteModel(host_ctx, (1, 1, 1), (0, 1, 0, 1, 0, 1), (3, 3, 3))
+    assert np.abs(node_coords(m0, 2) - micro.node_coords(dict(p, order=2), np.arange(7 ** 3))[1]).max() <= 1e-15
