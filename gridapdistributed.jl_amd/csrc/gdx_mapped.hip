@@ -103,7 +103,8 @@ typedef double f64x4_t __attribute__((ext_vector_type(4)));
 // the lane's two G values from LDS and feeds every tile pair to
 // v_mfma_f64_16x16x4f64 (lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
 // its accumulator i is D[4 i + l / 16][l % 16], unlike the FP32 16x16x4 form's
-// D[4 (l / 16) + i]).  The full matrix is stored, rows contiguous.
+// D[4 (l / 16) + i]).  Only the tiles on and above the diagonal are formed;
+// the full matrix is stored, rows contiguous.
 template <int D, int K, int NQ>
 struct CellShape {
   static constexpr int NQT = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // Gauss points, <= 64
@@ -300,18 +301,23 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
 #pragma unroll
       for (int ta = 0; ta < NT; ++ta)
 #pragma unroll
-        for (int tb = 0; tb < NT; ++tb)
+        for (int tb = ta; tb < NT; ++tb)
           acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(wk * gv[ta], gv[tb], acc[ta][tb], 0, 0, 0);
     }
+    // the cell matrix is symmetric: the tiles below the diagonal are the
+    // transposes of those above it, written from the same accumulators
     double *out = cm + c * (long long)Sh::NE;
 #pragma unroll
     for (int ta = 0; ta < NT; ++ta)
 #pragma unroll
-      for (int tb = 0; tb < NT; ++tb)
+      for (int tb = ta; tb < NT; ++tb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int a = 16 * ta + 4 * i + lk, b = 16 * tb + li;
-          if (a < NCN && b < NCN) out[a * NCN + b] = acc[ta][tb][i];
+          if (a < NCN && b < NCN) {
+            out[a * NCN + b] = acc[ta][tb][i];
+            if (ta != tb) out[b * NCN + a] = acc[ta][tb][i];
+          }
         }
     // D: the cell vector
     for (int a = lane; a < NCN; a += 64) {
