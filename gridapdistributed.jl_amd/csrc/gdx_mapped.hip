@@ -56,6 +56,8 @@ struct MappedData {
 MappedData *md(gdx_problem *p) { return reinterpret_cast<MappedData *>(p->mapped_data); }
 
 constexpr long long kMappedBlocks = 2048;  // workgroups of the persistent cell kernel (8 per CU)
+constexpr int kLdsModes = 32;             // Fourier modes each cell-kernel workgroup stages in LDS
+static_assert(kLdsModes % 2 == 0, "the lanes of a Gauss point split the modes in pairs");
 
 template <int D, int K>
 struct MShape {
@@ -112,7 +114,7 @@ struct CellShape {
   static constexpr int KR = (NQT * (D + 1) + 3) / 4 * 4;       // with the reaction term's value rows
   static constexpr int NT = (MShape<D, K>::NCN + 15) / 16;      // 16-wide tiles per side
   static constexpr int NCN = MShape<D, K>::NCN;
-  static constexpr int SHARED = (NQT * NCN * (D + 1) + 2 * kMaxQ) * 8;  // LDS of the workgroup's tables
+  static constexpr int SHARED = (NQT * NCN * (D + 1) + 2 * kMaxQ + 5 * kLdsModes) * 8;  // LDS of the workgroup's tables
   static constexpr int PERWAVE = NQT * (D * D + 3) * 8;                 // LDS per wave
   // waves per workgroup: at most 60 KB of LDS per workgroup
   static constexpr int WPB = SHARED + 4 * PERWAVE <= 60 * 1024 ? 4 : (SHARED + 2 * PERWAVE <= 60 * 1024 ? 2 : 1);
@@ -129,6 +131,7 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
   __shared__ double GR[NQT][NCN][D];      // reference gradients at the Gauss points
   __shared__ double PH[NQT][NCN];         // basis values at the Gauss points
   __shared__ double TQ[2][kMaxQ];         // Gauss points and weights on [0, 1]
+  __shared__ double MO[kLdsModes][5];     // the first Fourier modes (the others are read from HBM)
   __shared__ double JT[WPB][NQT][D * D];  // J^-T per point
   __shared__ double WQ[WPB][NQT], WF[WPB][NQT], WM[WPB][NQT];  // w |det J| kappa dw, w |det J| f, w |det J| rw
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -136,6 +139,8 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
     TQ[0][i] = kp.qx[i];
     TQ[1][i] = kp.qw[i];
   }
+  const int nml = cellk || !modes ? 0 : min(kp.nmodes, kLdsModes);
+  for (int i = threadIdx.x; i < 5 * nml; i += blockDim.x) MO[i / 5][i % 5] = modes[i];
   for (int i = threadIdx.x; i < NQT * NCN; i += blockDim.x) {
     const int q = i / NCN, a = i % NCN;
     const int qi[3] = {q % NQ, (q / NQ) % NQ, D == 3 ? q / (NQ * NQ) : 0};
@@ -222,8 +227,7 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
         // kappa = coef exp(g), g = sum_m a_m cos(2 pi k_m.x + phi_m); the lanes
         // of a point take every other mode and add the partner's sums
         double g = 0.0, dg0 = 0.0, dg1 = 0.0, dg2 = 0.0;
-        for (int m = h; m < kp.nmodes; m += QS) {
-          const double *Mo = modes + 5 * m;
+        auto mode = [&](const double *Mo) {
           // theta / pi = 2 k.x + phi / pi: sincospi reduces its argument exactly
           const double tp = 2.0 * (Mo[0] * x[0] + Mo[1] * x[1] + Mo[2] * x[2]) + Mo[4] * M_1_PI;
           double sn, cs;
@@ -233,7 +237,11 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
           dg0 += sd * Mo[0];
           dg1 += sd * Mo[1];
           dg2 += sd * Mo[2];
-        }
+        };
+        // nml is kLdsModes (a multiple of QS) or all the modes, so the second
+        // loop continues each lane's sequence h, h + QS, ...
+        for (int m = h; m < nml; m += QS) mode(MO[m]);
+        for (int m = nml + h; m < kp.nmodes; m += QS) mode(modes + 5 * m);
         if (QS == 2) {
           g += __shfl_xor(g, 32);
           dg0 += __shfl_xor(dg0, 32);

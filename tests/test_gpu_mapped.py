@@ -3,7 +3,7 @@
 - parity with the oracle given the same vertex coordinates: structure
   bit-identical, values and RHS within 1e-12 relative, ghost rows zero
   (2D/3D, Q1/Q2, both strategies, Fourier kappa with the manufactured source
-  and Dirichlet lifting, cell-wise kappa, no Dirichlet);
+  and Dirichlet lifting, 37 Fourier modes, cell-wise kappa, no Dirichlet);
 - repeated assembly is bit-identical; clearing the map, or a new symbolic
   phase, returns to the uniform assembly bit for bit;
 - at 3D Q2 32^3, parts (2,2,2): the patch test (nodal values of a linear u
@@ -107,6 +107,23 @@ def test_cellwise_kappa_on_a_mapped_grid(gpu_ctx):
     prob.set_cell_coefficient(kap)
     prob.numeric()
     prob.sync()
+    for k, P in enumerate(r.parts):
+        Q = prob.export_part(k)
+        assert rel_err(Q["val"], P["val"]) <= TOL and rel_err(Q["b"], P["b"]) <= TOL, k
+    prob.close()
+
+
+def test_many_fourier_modes(gpu_ctx):
+    """37 modes: the cell kernel stages the first 32 in LDS and reads the rest
+    from HBM, the two lanes of a Gauss point continuing their alternation"""
+    rng = np.random.default_rng(17)
+    modes = np.column_stack([rng.integers(0, 4, (37, 3)), rng.uniform(-0.08, 0.08, 37),
+                             rng.uniform(-np.pi, np.pi, 37)]).astype(np.float64)
+    p = case(3, 2, 4, (2, 1, 1), modes=modes, rhs="manufactured", uD=(0.5, 1.0, -2.0, 0.7))
+    p["vertex_coords"] = mapped_vertices(p, amp=0.4, seed=8)
+    print("mapped case 37 modes", flush=True)
+    r = oracle.run(p)
+    prob = gpu_problem(gpu_ctx, p)
     for k, P in enumerate(r.parts):
         Q = prob.export_part(k)
         assert rel_err(Q["val"], P["val"]) <= TOL and rel_err(Q["b"], P["b"]) <= TOL, k
