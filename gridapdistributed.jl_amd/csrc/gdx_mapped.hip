@@ -201,10 +201,11 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
       double det, Jit[D][D];  // J^-T[d][e] = J^-1[e][d] = adj(J)[e][d] / det
       if constexpr (D == 2) {
         det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        Jit[0][0] = J[1][1] / det;
-        Jit[1][0] = -J[0][1] / det;
-        Jit[0][1] = -J[1][0] / det;
-        Jit[1][1] = J[0][0] / det;
+        const double id = 1.0 / det;
+        Jit[0][0] = J[1][1] * id;
+        Jit[1][0] = -J[0][1] * id;
+        Jit[0][1] = -J[1][0] * id;
+        Jit[1][1] = J[0][0] * id;
       } else {
         const double a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2],
                      a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1], a10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
@@ -212,9 +213,10 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
                      a20 = J[1][0] * J[2][1] - J[1][1] * J[2][0], a21 = J[0][1] * J[2][0] - J[0][0] * J[2][1],
                      a22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
         det = J[0][0] * a00 + J[0][1] * a10 + J[0][2] * a20;
-        Jit[0][0] = a00 / det, Jit[0][1] = a10 / det, Jit[0][2] = a20 / det;
-        Jit[1][0] = a01 / det, Jit[1][1] = a11 / det, Jit[1][2] = a21 / det;
-        Jit[2][0] = a02 / det, Jit[2][1] = a12 / det, Jit[2][2] = a22 / det;
+        const double id = 1.0 / det;  // one division, nine products
+        Jit[0][0] = a00 * id, Jit[0][1] = a10 * id, Jit[0][2] = a20 * id;
+        Jit[1][0] = a01 * id, Jit[1][1] = a11 * id, Jit[1][2] = a21 * id;
+        Jit[2][0] = a02 * id, Jit[2][1] = a12 * id, Jit[2][2] = a22 * id;
       }
       double w = fabs(det);
 #pragma unroll
