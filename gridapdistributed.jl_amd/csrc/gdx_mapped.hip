@@ -118,10 +118,15 @@ struct CellShape {
   static constexpr int PERWAVE = NQT * (D * D + 3) * 8;                 // LDS per wave
   // waves per workgroup: at most 60 KB of LDS per workgroup
   static constexpr int WPB = SHARED + 4 * PERWAVE <= 60 * 1024 ? 4 : (SHARED + 2 * PERWAVE <= 60 * 1024 ? 2 : 1);
+  // waves per SIMD asked of the register allocator: 4 up to 27 Gauss points
+  // (the 3D Q2 instances then spill 20-96 bytes per lane and still run
+  // faster: 3 to 5 workgroups per CU instead of 2)
+  static constexpr int OCC = NQT <= 27 ? 4 : 1;
 };
 
 template <int D, int K, int NQ, bool REACT>
-__global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cells(
+__global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB))
+__attribute__((amdgpu_waves_per_eu(CellShape<D, K, NQ>::OCC))) void k_mapped_cells(
     KParams kp, long long ncell, const double *__restrict__ vc, const double *__restrict__ modes,
     const double *__restrict__ cellk, double dw, double rw, double *__restrict__ cm) {
   using Sh = MShape<D, K>;
@@ -134,7 +139,9 @@ __global__ __launch_bounds__((64 * CellShape<D, K, NQ>::WPB)) void k_mapped_cell
   __shared__ double MO[kLdsModes][5];     // the first Fourier modes (the others are read from HBM)
   __shared__ double JT[WPB][NQT][D * D];  // J^-T per point
   __shared__ double WQ[WPB][NQT], WF[WPB][NQT], WM[WPB][NQT];  // w |det J| kappa dw, w |det J| f, w |det J| rw
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is wave-uniform: so are the cell index, its lattice
+  // coordinates and vertex addresses (scalar arithmetic and scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
     TQ[0][i] = kp.qx[i];
     TQ[1][i] = kp.qw[i];
