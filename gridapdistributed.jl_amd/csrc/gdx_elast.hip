@@ -122,7 +122,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast(ElastArgs A, const ElastR
   constexpr int NF = 3 * W * 4;                  // factors of one row node
   constexpr int MAXE = (3 * W * W * W + 63) / 64;  // row positions per lane
   __shared__ __align__(16) double lds[kWaves][NPW][NF];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave-uniform
   const long long node0 = ((long long)blockIdx.x * kWaves + wave) * NPW;
   if (node0 >= A.nrownodes) return;  // whole waves exit together; no barrier below
   const int nn = (int)min((long long)NPW, A.nrownodes - node0);
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_elast_cw(ElastArgs A, const Ela
   constexpr int NL = (NF + 63) / 64;             // loads per lane
   constexpr int MAXE = (3 * W * W * W + 63) / 64;  // row positions per lane
   __shared__ __align__(16) double lds[kWaves][NPW][NF + 16];  // factors, then (lambda, mu) of 8 cells
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wave-uniform
   const long long node0 = ((long long)blockIdx.x * kWaves + wave) * NPW;
   if (node0 >= A.nrownodes) return;  // whole waves exit together; no barrier below
   const int nn = (int)min((long long)NPW, A.nrownodes - node0);

@@ -119,8 +119,8 @@ struct CellShape {
   // waves per workgroup: at most 60 KB of LDS per workgroup
   static constexpr int WPB = SHARED + 4 * PERWAVE <= 60 * 1024 ? 4 : (SHARED + 2 * PERWAVE <= 60 * 1024 ? 2 : 1);
   // waves per SIMD asked of the register allocator: 4 up to 27 Gauss points
-  // (the 3D Q2 instances then spill 20-96 bytes per lane and still run
-  // faster: 3 to 5 workgroups per CU instead of 2)
+  // (some 3D instances then spill 20-96 bytes per lane; the 3D Q2 one with
+  // 27 points still runs faster with twice the waves)
   static constexpr int OCC = NQT <= 27 ? 4 : 1;
 };
 
